@@ -1,0 +1,203 @@
+/*
+ * kfun3d.h -- C-ABI of the MI355X (gfx950) pseudo-transient Newton-Krylov
+ * path of KFUN3D.
+ *
+ * Drop-in boundary.  The entry points below carry the names, argument
+ * meanings and call order of the reference:
+ *   inc/core_kernel.h:6-22  ComputeA, ComputeTimeStep, ComputeFlux,
+ *                           ComputeForces, ComputeNumericalILU,
+ *                           ComputeSparseTriangularSolve
+ *   src/kernel.c:24,47      iJacVec, Kernel
+ *   inc/utils.h:18-28       Compute2ndNorm, ComputeAXPY, ComputeNewAXPY,
+ *                           Normalize
+ *   src/mesh.c:22           Setup
+ * and the GEOMETRY container keeps the field layout of inc/geometry.h:4-106.
+ *
+ * Memory convention (the one thing a caller of the reference must know):
+ *   - every `double *` vector argument of the entry points (q, r, x, y, w, b,
+ *     the three GRADIENT arrays) is a DEVICE pointer of length 4N
+ *     (AoS [p,u,v,w] per vertex, as in the reference);
+ *   - g->q->q and g->n->cdt are DEVICE arrays, allocated by Setup;
+ *   - every other GEOMETRY array (edges, normals, LSQ terms, boundary lists,
+ *     coordinates) is HOST memory exactly as the reference reads it; Setup
+ *     keeps a device mirror of each inside g->matrix;
+ *   - g->matrix is the opaque device context (the reference stores its
+ *     fun3d::Matrix there, src/mesh.c:121);
+ *   - Compute2ndNorm and Normalize return host doubles (they synchronize);
+ *     ComputeForces writes a host double[3].
+ *
+ * Errors: the reference returns void and asserts.  Here every HIP failure and
+ * every invalid argument aborts with file/line on stderr, and kf_last_error()
+ * reports non-finite pivots / norms detected on the device.
+ *
+ * Threading: one host thread per GEOMETRY; all work of one GEOMETRY is issued
+ * on one HIP stream (kf_stream()).  Distinct GEOMETRYs are independent.
+ */
+#ifndef KFUN3D_H
+#define KFUN3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- GEOMETRY: same layout as inc/geometry.h:4-106 ---------------------- */
+
+struct xyzn { double *x0, *x1, *x2, *x3; };   /* unit normal + dual-face area */
+struct xyz  { double *x0, *x1, *x2; };
+typedef struct xyz GRADIENT;                   /* d/dx, d/dy, d/dz of (p,u,v,w) */
+struct edge { unsigned int *n0, *n1; };
+struct face { unsigned int *n0, *n1, *n2; };
+struct weights { struct xyz *w0, *w1; };       /* LSQ terms seen from n0 / n1 */
+struct etbl { unsigned long int sz; struct edge *eptr; struct xyzn *xyzn; struct weights *w; };
+struct ntbl { unsigned long int sz; struct xyz *xyz; unsigned int *part; double *cdt; };
+struct stbl { unsigned long int sz; unsigned int *i; };
+struct bntbl { unsigned long int sz; unsigned int *nptr; struct xyz *xyz; };
+struct bftbl { unsigned long int sz; struct face *fptr; };
+struct btbl { struct bntbl *s; struct bntbl *f; struct bftbl *fc; };
+struct ctbl { unsigned int b; unsigned int b2; unsigned long int sz; };
+struct qtbl { double *q; };
+
+typedef struct geometry {
+  struct etbl *e;  /* edges (sorted by n0, n0 < n1), normals, LSQ terms */
+  struct ntbl *n;  /* vertex coordinates, owner part (all 0), cdt (device) */
+  struct btbl *b;  /* solid nodes, free nodes, solid faces (colour order) */
+  struct ctbl *c;  /* b = 4, b2 = 16, sz = 4N */
+  struct stbl *s;  /* edge ranges per owner: {0, E} (one owner: the GPU) */
+  struct stbl *t;  /* solid-face colour offsets */
+  struct qtbl *q;  /* state q[4N] (device) */
+  void *matrix;    /* device context */
+} GEOMETRY;
+
+/* ---- reference entry points (inc/core_kernel.h:6-22) --------------------- */
+
+/* src/fill.c:881.  A := analytic first-order Jacobian + diag(cdt) + BC blocks,
+ * built from g->q->q and g->n->cdt into the context's BSR 4x4 storage. */
+void ComputeA(GEOMETRY *g);
+/* src/timestep.c:156.  g->n->cdt := local pseudo-time-step term / cfl. */
+void ComputeTimeStep(const double cfl, GEOMETRY *g);
+/* src/flux.c:782.  r := R(q) (LSQ gradient, Roe edge flux, slip wall,
+ * free-stream nodes); grad := LSQ gradient of q.  q, grad->x*, r: device. */
+void ComputeFlux(const GEOMETRY *g, const double *q, GRADIENT *grad, double *r);
+/* src/forces.c:101.  forces[0..2] (host) := lift, drag, moment. */
+void ComputeForces(const GEOMETRY *g, double *forces);
+/* src/ilu.c:271.  In-place block ILU(0) of A (L\U with D^-1 on the diagonal).
+ * ComputeA must precede every call, as in the reference. */
+void ComputeNumericalILU(GEOMETRY *g);
+/* src/sptrsv.c:86.  x := U^-1 L^-1 b with the factors of the last ILU.
+ * b, x: device, must not alias. */
+void ComputeSparseTriangularSolve(const GEOMETRY *g, const double *b, double *x);
+
+/* ---- src/kernel.c --------------------------------------------------------- */
+
+/* src/kernel.c:24.  y := ((R(q + h x/|x|) - r) + cdt (w - q)) |x| / h with
+ * h = 1.490116119384766e-08 * norm; w receives the perturbed state and grad
+ * its gradient.  All vectors device. */
+void iJacVec(const GEOMETRY *g, const double *x, const double *r, const double norm,
+             GRADIENT *grad, double *w, double *y);
+/* src/kernel.c:47.  The pseudo-transient Newton-Krylov loop with the options
+ * of kf_default_options() (reference profile) unless kf_set_options() ran. */
+void Kernel(GEOMETRY *g);
+
+/* ---- inc/utils.h:18-28 (device vectors, host scalars) ---------------------- */
+
+double Compute2ndNorm(const size_t sz, const double *v);
+void ComputeAXPY(const size_t sz, const double a, const double *x, double *y);
+void ComputeNewAXPY(const size_t sz, const double a, const double *x, const double *y, double *w);
+double Normalize(const size_t sz, double *x);
+
+/* ---- src/mesh.c:22 --------------------------------------------------------- */
+
+/* Reads a big-endian .msh file (layout of src/mesh.c, src/edges.c,
+ * src/nodes.c, src/boundaries.c), runs the host preprocessing of the
+ * reference (edge sort src/index.c, LSQ weights src/weights.c, solid-face
+ * colouring src/sbface_coloring.c, CSR + level sets matrix/init.cpp), sets the
+ * free-stream state and uploads everything into a new device context. */
+void Setup(const char *fpath, GEOMETRY *g);
+
+/* ---- extensions (no reference counterpart) ---------------------------------- */
+
+typedef enum {
+  KF_PREC_FP64 = 0,     /* Jacobian, ILU factors and triangular solves in FP64 */
+  KF_PREC_FP32 = 1,     /* ... in FP32 (residual, Arnoldi, update stay FP64)   */
+  KF_PREC_BF16 = 2      /* factors stored in BF16, FP32 arithmetic             */
+} kf_precision;
+
+typedef enum {
+  KF_ORTHO_MGS = 0,     /* modified Gram-Schmidt, as src/kernel.c:129-139      */
+  KF_ORTHO_CGS2 = 1     /* batched classical Gram-Schmidt, two passes          */
+} kf_ortho;
+
+typedef struct kf_options {
+  double alpha_deg;     /* angle of attack (reference 3, inc/phy.h:8-9)       */
+  double cfl_init;      /* CFL_INIT (reference 50, src/kernel.c:19)           */
+  double cfl_max;       /* CLF_MAX  (reference 1e5)                           */
+  int    ts_max;        /* TS_MAX   (reference 15)                            */
+  double fnorm_ratio;   /* stop when fnorm_init/fnorm >= this (reference 1e7,
+                           evaluated in double, not through uint32)          */
+  int    gmres_restart; /* GMRES_RESTART (30, at most 30)                     */
+  double gmres_rtol;    /* GMRES_RELATIVE_TOLERANCE (0.01)                    */
+  int    gmres_max_it;  /* GMRES_MAX_IT (10000)                               */
+  int    first_order;   /* 1: gradients forced to zero (first-order flux)     */
+  kf_precision prec;    /* preconditioner precision                          */
+  kf_ortho ortho;       /* orthogonalization                                  */
+  int    verbose;       /* print the reference's per-step lines              */
+} kf_options;
+
+typedef struct kf_step_record {
+  int    step;          /* pseudo-time step index                             */
+  double cfl;
+  double fnorm;         /* |R(q)| before the update                           */
+  int    krylov;        /* GMRES iterations of this step                      */
+  double forces[3];     /* lift, drag, moment after the update                */
+} kf_step_record;
+
+kf_options kf_default_options(void);              /* reference profile */
+void kf_set_options(GEOMETRY *g, const kf_options *o);
+kf_options kf_get_options(const GEOMETRY *g);
+/* Kernel() with an explicit record buffer; returns the number of steps run. */
+int  kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec);
+
+/* Device-resident copies / uploads of a state vector (4N doubles). */
+void kf_set_state(GEOMETRY *g, const double *q_host);
+void kf_get_state(const GEOMETRY *g, double *q_host);
+void kf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+void kf_memcpy_h2d(void *dst, const void *src, size_t bytes);
+void *kf_device_alloc(size_t bytes);
+void kf_device_free(void *p);
+void kf_synchronize(const GEOMETRY *g);
+void *kf_stream(const GEOMETRY *g);               /* the context's hipStream_t */
+
+/* Factored BSR blocks (after ComputeNumericalILU) or the Jacobian (after
+ * ComputeA) in FP64, nnzb x 16 row-major blocks in CSR order of
+ * kf_pattern(); and the pattern itself (host copies). */
+size_t kf_nnzb(const GEOMETRY *g);
+void kf_pattern(const GEOMETRY *g, unsigned int *rowptr, unsigned int *col, unsigned int *diag);
+void kf_get_blocks(const GEOMETRY *g, double *blocks_host);
+
+/* Synthetic Kuhn-lattice mesh: nx*ny*nz vertices on [0,3]x[0,1]x[0,1],
+ * interior jitter +-0.15 h (splitmix64, seed), bottom slip wall with the
+ * bump z = 0.1 sin^2(pi (x-1)) on 1<=x<=2, other faces free stream. */
+int  kf_write_lattice_msh(const char *fpath, int nx, int ny, int nz, uint64_t seed);
+
+/* Setup with explicit options (Setup uses kf_default_options()). */
+void kf_setup_ex(const char *fpath, GEOMETRY *g, const kf_options *o);
+/* Host part of Setup only (reader + preprocessing, g->matrix = NULL). */
+void kf_setup_host(const char *fpath, GEOMETRY *g);
+/* Compute2ndNorm / ComputeAXPY / ComputeNewAXPY / Normalize take no
+ * GEOMETRY (inc/utils.h); they run on the context bound last (Setup binds). */
+void kf_bind(GEOMETRY *g);
+
+/* Releases everything Setup allocated, host and device. */
+void kf_geometry_free(GEOMETRY *g);
+
+/* Non-zero when the device flagged a non-finite norm or a singular pivot. */
+int  kf_last_error(const GEOMETRY *g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KFUN3D_H */

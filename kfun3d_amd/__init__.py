@@ -1,0 +1,351 @@
+"""kfun3d_amd -- MI355X (gfx950) pseudo-transient Newton-Krylov path of KFUN3D.
+
+The product is the C-ABI shared library ``libkfun3d.so`` (include/kfun3d.h):
+the reference's entry points (ComputeFlux, ComputeTimeStep, ComputeA,
+ComputeNumericalILU, ComputeSparseTriangularSolve, ComputeForces, iJacVec,
+Kernel, Compute2ndNorm, ComputeAXPY, ComputeNewAXPY, Normalize, Setup) backed
+by hand-written HIP kernels.  This module is the ctypes binding used by the
+tests and the benchmark; it computes nothing itself.  Loading fails loudly
+when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libkfun3d.so")
+
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
+ORTHO = {"mgs": 0, "cgs2": 1}
+
+
+class Xyzn(C.Structure):
+    _fields_ = [("x0", C.POINTER(C.c_double)), ("x1", C.POINTER(C.c_double)),
+                ("x2", C.POINTER(C.c_double)), ("x3", C.POINTER(C.c_double))]
+
+
+class Xyz(C.Structure):
+    _fields_ = [("x0", C.POINTER(C.c_double)), ("x1", C.POINTER(C.c_double)), ("x2", C.POINTER(C.c_double))]
+
+
+class Edge(C.Structure):
+    _fields_ = [("n0", C.POINTER(C.c_uint)), ("n1", C.POINTER(C.c_uint))]
+
+
+class Face(C.Structure):
+    _fields_ = [("n0", C.POINTER(C.c_uint)), ("n1", C.POINTER(C.c_uint)), ("n2", C.POINTER(C.c_uint))]
+
+
+class Weights(C.Structure):
+    _fields_ = [("w0", C.POINTER(Xyz)), ("w1", C.POINTER(Xyz))]
+
+
+class Etbl(C.Structure):
+    _fields_ = [("sz", C.c_ulong), ("eptr", C.POINTER(Edge)), ("xyzn", C.POINTER(Xyzn)), ("w", C.POINTER(Weights))]
+
+
+class Ntbl(C.Structure):
+    _fields_ = [("sz", C.c_ulong), ("xyz", C.POINTER(Xyz)), ("part", C.POINTER(C.c_uint)),
+                ("cdt", C.POINTER(C.c_double))]
+
+
+class Stbl(C.Structure):
+    _fields_ = [("sz", C.c_ulong), ("i", C.POINTER(C.c_uint))]
+
+
+class Bntbl(C.Structure):
+    _fields_ = [("sz", C.c_ulong), ("nptr", C.POINTER(C.c_uint)), ("xyz", C.POINTER(Xyz))]
+
+
+class Bftbl(C.Structure):
+    _fields_ = [("sz", C.c_ulong), ("fptr", C.POINTER(Face))]
+
+
+class Btbl(C.Structure):
+    _fields_ = [("s", C.POINTER(Bntbl)), ("f", C.POINTER(Bntbl)), ("fc", C.POINTER(Bftbl))]
+
+
+class Ctbl(C.Structure):
+    _fields_ = [("b", C.c_uint), ("b2", C.c_uint), ("sz", C.c_ulong)]
+
+
+class Qtbl(C.Structure):
+    _fields_ = [("q", C.POINTER(C.c_double))]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("e", C.POINTER(Etbl)), ("n", C.POINTER(Ntbl)), ("b", C.POINTER(Btbl)), ("c", C.POINTER(Ctbl)),
+                ("s", C.POINTER(Stbl)), ("t", C.POINTER(Stbl)), ("q", C.POINTER(Qtbl)), ("matrix", C.c_void_p)]
+
+
+class Options(C.Structure):
+    _fields_ = [("alpha_deg", C.c_double), ("cfl_init", C.c_double), ("cfl_max", C.c_double),
+                ("ts_max", C.c_int), ("fnorm_ratio", C.c_double), ("gmres_restart", C.c_int),
+                ("gmres_rtol", C.c_double), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
+                ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int)]
+
+
+class StepRecord(C.Structure):
+    _fields_ = [("step", C.c_int), ("cfl", C.c_double), ("fnorm", C.c_double), ("krylov", C.c_int),
+                ("forces", C.c_double * 3)]
+
+
+GP = C.POINTER(Geometry)
+DP = C.c_void_p   # device pointers travel as plain addresses
+
+
+def _load():
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError("kfun3d_amd: %s is not built (run python -m kfun3d_amd.build)" % LIBPATH)
+    lib = C.CDLL(LIBPATH)
+    sig = {
+        "Setup": (None, [C.c_char_p, GP]),
+        "kf_setup_ex": (None, [C.c_char_p, GP, C.POINTER(Options)]),
+        "kf_setup_host": (None, [C.c_char_p, GP]),
+        "kf_geometry_free": (None, [GP]),
+        "kf_default_options": (Options, []),
+        "kf_set_options": (None, [GP, C.POINTER(Options)]),
+        "kf_get_options": (Options, [GP]),
+        "kf_kernel": (C.c_int, [GP, C.POINTER(StepRecord), C.c_int]),
+        "kf_set_state": (None, [GP, C.c_void_p]),
+        "kf_get_state": (None, [GP, C.c_void_p]),
+        "kf_memcpy_d2h": (None, [C.c_void_p, DP, C.c_size_t]),
+        "kf_memcpy_h2d": (None, [DP, C.c_void_p, C.c_size_t]),
+        "kf_device_alloc": (DP, [C.c_size_t]),
+        "kf_device_free": (None, [DP]),
+        "kf_synchronize": (None, [GP]),
+        "kf_stream": (C.c_void_p, [GP]),
+        "kf_nnzb": (C.c_size_t, [GP]),
+        "kf_pattern": (None, [GP, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "kf_get_blocks": (None, [GP, C.c_void_p]),
+        "kf_write_lattice_msh": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_uint64]),
+        "kf_last_error": (C.c_int, [GP]),
+        "kf_bind": (None, [GP]),
+        "ComputeFlux": (None, [GP, DP, C.POINTER(Xyz), DP]),
+        "ComputeTimeStep": (None, [C.c_double, GP]),
+        "ComputeA": (None, [GP]),
+        "ComputeForces": (None, [GP, C.POINTER(C.c_double)]),
+        "ComputeNumericalILU": (None, [GP]),
+        "ComputeSparseTriangularSolve": (None, [GP, DP, DP]),
+        "iJacVec": (None, [GP, DP, DP, C.c_double, C.POINTER(Xyz), DP, DP]),
+        "Kernel": (None, [GP]),
+        "Compute2ndNorm": (C.c_double, [C.c_size_t, DP]),
+        "ComputeAXPY": (None, [C.c_size_t, C.c_double, DP, DP]),
+        "ComputeNewAXPY": (None, [C.c_size_t, C.c_double, DP, DP, DP]),
+        "Normalize": (C.c_double, [C.c_size_t, DP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+# exported C symbols the header declares (checked by the CPU tests)
+ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "ComputeNumericalILU",
+               "ComputeSparseTriangularSolve", "iJacVec", "Kernel", "Compute2ndNorm", "ComputeAXPY",
+               "ComputeNewAXPY", "Normalize", "Setup", "kf_default_options", "kf_set_options", "kf_get_options",
+               "kf_kernel", "kf_set_state", "kf_get_state", "kf_memcpy_d2h", "kf_memcpy_h2d", "kf_device_alloc",
+               "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_pattern", "kf_get_blocks",
+               "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
+               "kf_bind"]
+
+
+def options(**kw):
+    o = lib().kf_default_options()
+    for k, v in kw.items():
+        if k == "prec":
+            v = PREC[v] if isinstance(v, str) else v
+        if k == "ortho":
+            v = ORTHO[v] if isinstance(v, str) else v
+        setattr(o, k, v)
+    return o
+
+
+def config_options(**kw):
+    """Options of the synthetic configurations: alpha 3.06 deg, CFL ramp from
+    10, stop at a 1e-10 residual drop."""
+    base = dict(alpha_deg=3.06, cfl_init=10.0, ts_max=500, fnorm_ratio=1e10)
+    base.update(kw)
+    return options(**base)
+
+
+def write_lattice(path, nx, ny, nz, seed):
+    rc = lib().kf_write_lattice_msh(path.encode(), nx, ny, nz, seed)
+    if rc != 0:
+        raise RuntimeError("kf_write_lattice_msh failed (%d)" % rc)
+    return path
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    ct = {np.float64: C.c_double, np.uint32: C.c_uint}[dtype]
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(n,)).copy()
+
+
+class DeviceVec:
+    """A device buffer of float64 (4N for a state vector)."""
+
+    def __init__(self, n, host=None):
+        self.n = int(n)
+        self.ptr = lib().kf_device_alloc(8 * max(self.n, 1))
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        h = np.ascontiguousarray(host, dtype=np.float64)
+        assert h.size == self.n
+        lib().kf_memcpy_h2d(self.ptr, h.ctypes.data, 8 * self.n)
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=np.float64)
+        lib().kf_memcpy_d2h(out.ctypes.data, self.ptr, 8 * self.n)
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().kf_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Mesh:
+    """A GEOMETRY built by Setup (device=True) or by its host part only."""
+
+    def __init__(self, path, opts=None, device=True):
+        self.path = path
+        self.g = Geometry()
+        self.device = device
+        if device:
+            o = opts if opts is not None else lib().kf_default_options()
+            lib().kf_setup_ex(path.encode(), C.byref(self.g), C.byref(o))
+        else:
+            lib().kf_setup_host(path.encode(), C.byref(self.g))
+        self.N = int(self.g.n.contents.sz)
+        self.E = int(self.g.e.contents.sz)
+        self.sz = 4 * self.N
+
+    @property
+    def ref(self):
+        return C.byref(self.g)
+
+    def close(self):
+        if self.g.e:
+            lib().kf_geometry_free(C.byref(self.g))
+
+    # ---- host arrays (as the reference's Setup leaves them) ----
+    def host_arrays(self):
+        g = self.g
+        e, n, b = g.e.contents, g.n.contents, g.b.contents
+        E, N = self.E, self.N
+        s, f, fc = b.s.contents, b.f.contents, b.fc.contents
+        t = g.t.contents
+        out = {
+            "e_n0": _arr(e.eptr.contents.n0, E, np.uint32), "e_n1": _arr(e.eptr.contents.n1, E, np.uint32),
+            "e_x0": _arr(e.xyzn.contents.x0, E, np.float64), "e_x1": _arr(e.xyzn.contents.x1, E, np.float64),
+            "e_x2": _arr(e.xyzn.contents.x2, E, np.float64), "e_x3": _arr(e.xyzn.contents.x3, E, np.float64),
+            "w0_x0": _arr(e.w.contents.w0.contents.x0, E, np.float64),
+            "w0_x1": _arr(e.w.contents.w0.contents.x1, E, np.float64),
+            "w0_x2": _arr(e.w.contents.w0.contents.x2, E, np.float64),
+            "w1_x0": _arr(e.w.contents.w1.contents.x0, E, np.float64),
+            "w1_x1": _arr(e.w.contents.w1.contents.x1, E, np.float64),
+            "w1_x2": _arr(e.w.contents.w1.contents.x2, E, np.float64),
+            "n_x0": _arr(n.xyz.contents.x0, N, np.float64), "n_x1": _arr(n.xyz.contents.x1, N, np.float64),
+            "n_x2": _arr(n.xyz.contents.x2, N, np.float64),
+            "fc_n0": _arr(fc.fptr.contents.n0, fc.sz, np.uint32), "fc_n1": _arr(fc.fptr.contents.n1, fc.sz, np.uint32),
+            "fc_n2": _arr(fc.fptr.contents.n2, fc.sz, np.uint32),
+            "t_i": _arr(t.i, t.sz + 1, np.uint32),
+            "s_nptr": _arr(s.nptr, s.sz, np.uint32), "s_x0": _arr(s.xyz.contents.x0, s.sz, np.float64),
+            "s_x1": _arr(s.xyz.contents.x1, s.sz, np.float64), "s_x2": _arr(s.xyz.contents.x2, s.sz, np.float64),
+            "f_nptr": _arr(f.nptr, f.sz, np.uint32), "f_x0": _arr(f.xyz.contents.x0, f.sz, np.float64),
+            "f_x1": _arr(f.xyz.contents.x1, f.sz, np.float64), "f_x2": _arr(f.xyz.contents.x2, f.sz, np.float64),
+        }
+        return out
+
+    # ---- device entry points ----
+    def set_options(self, opts):
+        lib().kf_set_options(self.ref, C.byref(opts))
+
+    def state(self):
+        q = np.empty(self.sz)
+        lib().kf_get_state(self.ref, q.ctypes.data)
+        return q
+
+    def set_state(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        lib().kf_set_state(self.ref, q.ctypes.data)
+
+    def flux(self, q_dev, grad_devs, r_dev):
+        grad = Xyz(C.cast(grad_devs[0].ptr, C.POINTER(C.c_double)), C.cast(grad_devs[1].ptr, C.POINTER(C.c_double)),
+                   C.cast(grad_devs[2].ptr, C.POINTER(C.c_double)))
+        lib().ComputeFlux(self.ref, q_dev.ptr, C.byref(grad), r_dev.ptr)
+
+    def timestep(self, cfl):
+        lib().ComputeTimeStep(cfl, self.ref)
+
+    def cdt(self):
+        out = np.empty(self.N)
+        lib().kf_synchronize(self.ref)
+        lib().kf_memcpy_d2h(out.ctypes.data, C.cast(self.g.n.contents.cdt, C.c_void_p), 8 * self.N)
+        return out
+
+    def fill(self):
+        lib().ComputeA(self.ref)
+
+    def ilu(self):
+        lib().ComputeNumericalILU(self.ref)
+
+    def sptrsv(self, b_dev, x_dev):
+        lib().ComputeSparseTriangularSolve(self.ref, b_dev.ptr, x_dev.ptr)
+
+    def forces(self):
+        f = (C.c_double * 3)()
+        lib().ComputeForces(self.ref, f)
+        return np.array(list(f))
+
+    def ijacvec(self, x_dev, r_dev, qnorm, grad_devs, w_dev, y_dev):
+        grad = Xyz(C.cast(grad_devs[0].ptr, C.POINTER(C.c_double)), C.cast(grad_devs[1].ptr, C.POINTER(C.c_double)),
+                   C.cast(grad_devs[2].ptr, C.POINTER(C.c_double)))
+        lib().iJacVec(self.ref, x_dev.ptr, r_dev.ptr, qnorm, C.byref(grad), w_dev.ptr, y_dev.ptr)
+
+    def blocks(self):
+        out = np.empty(int(lib().kf_nnzb(self.ref)) * 16)
+        lib().kf_get_blocks(self.ref, out.ctypes.data)
+        return out.reshape(-1, 4, 4)
+
+    def pattern(self):
+        nnzb = int(lib().kf_nnzb(self.ref))
+        rp = np.empty(self.N + 1, dtype=np.uint32)
+        col = np.empty(nnzb, dtype=np.uint32)
+        dg = np.empty(self.N, dtype=np.uint32)
+        lib().kf_pattern(self.ref, rp.ctypes.data, col.ctypes.data, dg.ctypes.data)
+        return rp, col, dg
+
+    def kernel(self, max_rec=1000):
+        rec = (StepRecord * max_rec)()
+        n = lib().kf_kernel(self.ref, rec, max_rec)
+        return [dict(step=r.step, cfl=r.cfl, fnorm=r.fnorm, krylov=r.krylov, forces=list(r.forces))
+                for r in rec[:min(n, max_rec)]]
+
+    def last_error(self):
+        return lib().kf_last_error(self.ref)
+
+    def synchronize(self):
+        lib().kf_synchronize(self.ref)

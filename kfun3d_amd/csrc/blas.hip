@@ -1,0 +1,305 @@
+/*
+ * blas.hip -- BLAS-1 of src/utils.c:53-123 and the FD Jacobian-vector
+ * combination of src/kernel.c:24-45 on gfx950.
+ *
+ * Reductions are deterministic: a fixed grid of KF_NRED blocks walks the
+ * vector with a fixed stride, each block reduces through LDS in a fixed
+ * tree, and one block combines the KF_NRED partials in a fixed tree.  The
+ * sum order differs from the reference's one-thread loop (it is a tree), the
+ * result is the same on every run.  Scalars stay on the device (d.scal) so
+ * that dependent vector updates need no host round trip.
+ */
+#include "kf_dev.h"
+
+#define RB 256
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ __launch_bounds__(RB) void k_dot_part(size_t n, const double *__restrict__ x, const double *__restrict__ y,
+                                                 double *__restrict__ part)
+{
+  __shared__ double sh[RB];
+  double acc = 0.0;
+  const size_t stride = (size_t)gridDim.x * RB;
+  for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += stride) acc += x[i] * y[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RB) void k_red_final(const double *__restrict__ part, double *__restrict__ out, int take_sqrt)
+{
+  __shared__ double sh[RB];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < KF_NRED; i += RB) acc += part[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
+}
+
+void kf_dot_dev(kf_ctx *c, size_t n, const double *x, const double *y, int slot)
+{
+  hipLaunchKernelGGL(k_dot_part, dim3(KF_NRED), dim3(RB), 0, c->stream, n, x, y, c->d.red);
+  KF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, c->d.red, c->d.scal + slot, 0);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_nrm2_dev(kf_ctx *c, size_t n, const double *x, int slot)
+{
+  hipLaunchKernelGGL(k_dot_part, dim3(KF_NRED), dim3(RB), 0, c->stream, n, x, x, c->d.red);
+  KF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, c->d.red, c->d.scal + slot, 1);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out)
+{
+  KF_CHECK(hipMemcpyAsync(c->hscal + slot, c->d.scal + slot, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < count; i++) out[i] = c->hscal[slot + i];
+}
+
+/* y = a x + y with the reference's ax-then-add rounding (src/utils.c:79-83) */
+__global__ __launch_bounds__(RB) void k_axpy(size_t n, double a, const double *__restrict__ x, double *__restrict__ y)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const double ax = a * x[i];
+  y[i] = ax + y[i];
+}
+
+__global__ __launch_bounds__(RB) void k_newaxpy(size_t n, double a, const double *__restrict__ x,
+                                                const double *__restrict__ y, double *__restrict__ w)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const double ax = a * x[i];
+  w[i] = ax + y[i];
+}
+
+/* x *= 1/s, s = scal[slot] (src/utils.c:117-118: the reciprocal, then the product) */
+__global__ __launch_bounds__(RB) void k_scale_recip(size_t n, double *__restrict__ x, const double *__restrict__ s)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const double f = 1.0 / *s;
+  x[i] *= f;
+}
+
+/* y = (-h) x + y, h = scal[slot] (the MGS update, src/kernel.c:138) */
+__global__ __launch_bounds__(RB) void k_axpy_slot(size_t n, const double *__restrict__ s, const double *__restrict__ x,
+                                                  double *__restrict__ y)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const double a = -*s;
+  const double ax = a * x[i];
+  y[i] = ax + y[i];
+}
+
+/* w = (h / |x|) x + q (src/kernel.c:27-30) */
+__global__ __launch_bounds__(RB) void k_fd_perturb(size_t n, double h, const double *__restrict__ xnorm,
+                                                   const double *__restrict__ x, const double *__restrict__ q,
+                                                   double *__restrict__ w)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const double a = h / *xnorm;
+  const double ax = a * x[i];
+  w[i] = ax + q[i];
+}
+
+/* y = ((y - r) + cdt (w - q)) |x| / h (src/kernel.c:33-44) */
+__global__ __launch_bounds__(RB) void k_fd_combine(uint32_t N, double h, const double *__restrict__ xnorm,
+                                                   const double *__restrict__ r, const double *__restrict__ w,
+                                                   const double *__restrict__ q, const double *__restrict__ cdt,
+                                                   double *__restrict__ y)
+{
+  const size_t idx = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (idx >= 4 * (size_t)N) return;
+  const double a = *xnorm / h;
+  const double cd = cdt[idx >> 2];
+  y[idx] = (y[idx] - r[idx] + cd * (w[idx] - q[idx])) * a;
+}
+
+__global__ __launch_bounds__(RB) void k_fill_zero(size_t n, double *__restrict__ x)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+void kf_axpy(kf_ctx *c, size_t n, double a, const double *x, double *y)
+{
+  if (!n) return;
+  hipLaunchKernelGGL(k_axpy, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, a, x, y);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_newaxpy(kf_ctx *c, size_t n, double a, const double *x, const double *y, double *w)
+{
+  if (!n) return;
+  hipLaunchKernelGGL(k_newaxpy, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, a, x, y, w);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_scale_recip(kf_ctx *c, size_t n, double *x, int slot)
+{
+  if (!n) return;
+  hipLaunchKernelGGL(k_scale_recip, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, x, c->d.scal + slot);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_axpy_slot(kf_ctx *c, size_t n, int slot, const double *x, double *y)
+{
+  if (!n) return;
+  hipLaunchKernelGGL(k_axpy_slot, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, c->d.scal + slot, x, y);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_fd_perturb(kf_ctx *c, const double *x, double h, int slot_xnorm, double *w)
+{
+  const size_t n = 4 * (size_t)c->d.N;
+  hipLaunchKernelGGL(k_fd_perturb, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, h, c->d.scal + slot_xnorm,
+                     x, c->q, w);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_fd_combine(kf_ctx *c, const double *r, const double *w, double h, int slot_xnorm, double *y)
+{
+  const size_t n = 4 * (size_t)c->d.N;
+  hipLaunchKernelGGL(k_fd_combine, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, c->d.N, h,
+                     c->d.scal + slot_xnorm, r, w, c->q, c->cdt, y);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_fill_zero(kf_ctx *c, size_t n, double *x)
+{
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_zero, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, x);
+  KF_LAUNCH_CHECK();
+}
+
+/* ---- batched classical Gram-Schmidt: h_v = V_v . w for all v in one pass
+ * over w, then w -= sum_v h_v V_v (one more pass); deterministic trees ---- */
+
+#define CGS_MAXV 32
+#define CGS_BLOCKS 512
+
+__global__ __launch_bounds__(RB) void k_cgs_dots(size_t n, const double *__restrict__ vb, int nv,
+                                                 const double *__restrict__ w, double *__restrict__ part)
+{
+  __shared__ double sh[RB];
+  double acc[CGS_MAXV];
+#pragma unroll
+  for (int v = 0; v < CGS_MAXV; v++) acc[v] = 0.0;
+  const size_t stride = (size_t)gridDim.x * RB;
+  for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += stride) {
+    const double wi = w[i];
+#pragma unroll
+    for (int v = 0; v < CGS_MAXV; v++)
+      if (v < nv) acc[v] += vb[(size_t)v * n + i] * wi;
+  }
+  for (int v = 0; v < nv; v++) {
+    const double s = block_sum(acc[v], sh);
+    if (threadIdx.x == 0) part[(size_t)v * CGS_BLOCKS + blockIdx.x] = s;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_cgs_final(const double *__restrict__ part, int nv, double *__restrict__ out)
+{
+  __shared__ double sh[RB];
+  for (int v = 0; v < nv; v++) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < CGS_BLOCKS; i += RB) acc += part[(size_t)v * CGS_BLOCKS + i];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[v] = s;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_cgs_update(size_t n, const double *__restrict__ vb, int nv,
+                                                   const double *__restrict__ h, double *__restrict__ w)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int v = 0; v < nv; v++) s += h[v] * vb[(size_t)v * n + i];
+  w[i] = w[i] - s;
+}
+
+/* ---- C wrappers for the host driver --------------------------------------- */
+
+extern "C" {
+
+void kfs_nrm2(GEOMETRY *g, size_t n, const double *x, int slot) { kf_nrm2_dev(kf_ctx_of(g), n, x, slot); }
+void kfs_dot(GEOMETRY *g, size_t n, const double *x, const double *y, int slot) { kf_dot_dev(kf_ctx_of(g), n, x, y, slot); }
+void kfs_fetch(GEOMETRY *g, int slot, int count, double *out) { kf_scal_fetch(kf_ctx_of(g), slot, count, out); }
+void kfs_axpy(GEOMETRY *g, size_t n, double a, const double *x, double *y) { kf_axpy(kf_ctx_of(g), n, a, x, y); }
+void kfs_newaxpy(GEOMETRY *g, size_t n, double a, const double *x, const double *y, double *w)
+{
+  kf_newaxpy(kf_ctx_of(g), n, a, x, y, w);
+}
+void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot) { kf_scale_recip(kf_ctx_of(g), n, x, slot); }
+void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y) { kf_axpy_slot(kf_ctx_of(g), n, slot, x, y); }
+void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w) { kf_fd_perturb(kf_ctx_of(g), x, h, slot, w); }
+void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int slot, double *y)
+{
+  kf_fd_combine(kf_ctx_of(g), r, w, h, slot, y);
+}
+void kfs_zero(GEOMETRY *g, size_t n, double *x) { kf_fill_zero(kf_ctx_of(g), n, x); }
+
+void kfs_cgs_dots(GEOMETRY *g, size_t n, const double *vbase, int nv, const double *w, int slot)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (nv > CGS_MAXV || KF_NRED < CGS_BLOCKS * 1) abort();
+  double *part = c->d.eres;   /* free between residual evaluations: 4E >= 32 * 512 doubles */
+  if (4 * (size_t)c->d.E < (size_t)CGS_MAXV * CGS_BLOCKS) {
+    fprintf(stderr, "kf: mesh too small for the batched Gram-Schmidt scratch\n");
+    abort();
+  }
+  hipLaunchKernelGGL(k_cgs_dots, dim3(CGS_BLOCKS), dim3(RB), 0, c->stream, n, vbase, nv, w, part);
+  KF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cgs_final, dim3(1), dim3(RB), 0, c->stream, part, nv, c->d.scal + slot);
+  KF_LAUNCH_CHECK();
+}
+
+void kfs_cgs_update(GEOMETRY *g, size_t n, const double *vbase, int nv, int slot, double *w)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  hipLaunchKernelGGL(k_cgs_update, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, vbase, nv,
+                     c->d.scal + slot, w);
+  KF_LAUNCH_CHECK();
+}
+
+void kfs_work(GEOMETRY *g, int nbasis, double **grad3, double **r, double **w, double **t, double **x, double **vecs)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  const size_t sz = 4 * (size_t)c->d.N;
+  double **bufs[7] = {&c->gx, &c->gy, &c->gz, &c->r, &c->w, &c->t, &c->xk};
+  for (int k = 0; k < 7; k++)
+    if (!*bufs[k]) {
+      KF_CHECK(hipMalloc((void **)bufs[k], sz * sizeof(double)));
+      KF_CHECK(hipMemset(*bufs[k], 0, sz * sizeof(double)));
+    }
+  if (c->nvecs < nbasis) {
+    if (c->vecs) KF_CHECK(hipFree(c->vecs));
+    KF_CHECK(hipMalloc((void **)&c->vecs, (size_t)nbasis * sz * sizeof(double)));
+    KF_CHECK(hipMemset(c->vecs, 0, (size_t)nbasis * sz * sizeof(double)));
+    c->nvecs = nbasis;
+  }
+  grad3[0] = c->gx; grad3[1] = c->gy; grad3[2] = c->gz;
+  *r = c->r; *w = c->w; *t = c->t; *x = c->xk; *vecs = c->vecs;
+}
+
+}  // extern "C"

@@ -1,0 +1,441 @@
+/*
+ * ctx.hip -- device context (g->matrix) and the C-ABI entry points that
+ * replace the reference's inc/core_kernel.h, inc/utils.h and src/mesh.c:22.
+ */
+#include <math.h>
+#include <string.h>
+
+#include "kf_dev.h"
+
+template <typename T>
+static T *dalloc(size_t n)
+{
+  void *p = NULL;
+  KF_CHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+  return (T *)p;
+}
+
+template <typename T>
+static T *dupload(const T *h, size_t n)
+{
+  T *p = dalloc<T>(n);
+  if (n) KF_CHECK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+/* rows of each level cut into groups of at most KF_GROUP */
+static uint32_t *level_groups(const uint32_t *lptr, uint32_t nlev, uint32_t *ngrp)
+{
+  size_t n = 0;
+  for (uint32_t l = 0; l < nlev; l++) n += (lptr[l + 1] - lptr[l] + KF_GROUP - 1) / KF_GROUP;
+  uint32_t *g = (uint32_t *)kf_xmalloc((n + 1) * sizeof(uint32_t));
+  size_t k = 0;
+  for (uint32_t l = 0; l < nlev; l++)
+    for (uint32_t s = lptr[l]; s < lptr[l + 1]; s += KF_GROUP) g[k++] = s;
+  g[k] = lptr[nlev];
+  *ngrp = (uint32_t)n;
+  return g;
+}
+
+extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o)
+{
+  kf_ctx *c = new kf_ctx();
+  memset((void *)c, 0, sizeof(*c));
+  KF_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->opt = *o;
+  kf_dv &d = c->d;
+  const size_t N = p->N, E = p->E;
+  if (N >= (1u << 31) || E >= (1u << 30) || p->nnzb >= (1u << 31)) {
+    fprintf(stderr, "kf: mesh too large for 32-bit indices\n");
+    abort();
+  }
+  d.N = (uint32_t)N;
+  d.E = (uint32_t)E;
+  d.S = (uint32_t)g->b->fc->sz;
+  d.NS = (uint32_t)g->b->s->sz;
+  d.NF = (uint32_t)g->b->f->sz;
+  kf_free_stream(o->alpha_deg, &d.U, &d.V);
+  c->nnzb = p->nnzb;
+
+  d.n0 = dupload(g->e->eptr->n0, E);
+  d.n1 = dupload(g->e->eptr->n1, E);
+  d.enx = dupload(g->e->xyzn->x0, E);
+  d.eny = dupload(g->e->xyzn->x1, E);
+  d.enz = dupload(g->e->xyzn->x2, E);
+  d.ea = dupload(g->e->xyzn->x3, E);
+  d.x = dupload(g->n->xyz->x0, N);
+  d.y = dupload(g->n->xyz->x1, N);
+  d.z = dupload(g->n->xyz->x2, N);
+
+  /* sliced ELL of the incidence lists (slice height 64) */
+  const uint32_t nslice = (uint32_t)((N + 63) / 64);
+  d.nslice = nslice;
+  uint32_t *sptr = (uint32_t *)kf_xmalloc((nslice + 1) * sizeof(uint32_t));
+  uint32_t *deg = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
+  size_t rows = 0;
+  for (uint32_t s = 0; s < nslice; s++) {
+    uint32_t len = 0;
+    for (size_t v = (size_t)s * 64; v < N && v < (size_t)s * 64 + 64; v++) {
+      deg[v] = p->vptr[v + 1] - p->vptr[v];
+      if (deg[v] > len) len = deg[v];
+    }
+    sptr[s] = (uint32_t)rows;
+    rows += len;
+  }
+  sptr[nslice] = (uint32_t)rows;
+  const size_t ne = rows * 64;
+  uint32_t *nbr = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
+  uint32_t *ent = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
+  uint32_t *blk = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
+  double *wx = (double *)kf_xcalloc(ne + 1, sizeof(double));
+  double *wy = (double *)kf_xcalloc(ne + 1, sizeof(double));
+  double *wz = (double *)kf_xcalloc(ne + 1, sizeof(double));
+  const uint32_t *n0 = g->e->eptr->n0, *n1 = g->e->eptr->n1;
+  for (size_t v = 0; v < N; v++) {
+    const size_t base = (size_t)sptr[v >> 6] * 64 + (v & 63);
+    for (uint32_t k = 0; k < deg[v]; k++) {
+      const uint32_t s = p->vptr[v] + k, en = p->vent[s], e = en >> 1;
+      const size_t i = base + (size_t)k * 64;
+      ent[i] = en;
+      blk[i] = p->vblk[s];
+      if (en & 1u) {
+        nbr[i] = n0[e];
+        wx[i] = g->e->w->w1->x0[e]; wy[i] = g->e->w->w1->x1[e]; wz[i] = g->e->w->w1->x2[e];
+      } else {
+        nbr[i] = n1[e];
+        wx[i] = g->e->w->w0->x0[e]; wy[i] = g->e->w->w0->x1[e]; wz[i] = g->e->w->w0->x2[e];
+      }
+    }
+  }
+  d.sell_ptr = dupload(sptr, nslice + 1);
+  d.deg = dupload(deg, N);
+  d.sell_nbr = dupload(nbr, ne);
+  d.sell_ent = dupload(ent, ne);
+  d.sell_blk = dupload(blk, ne);
+  d.sell_wx = dupload(wx, ne);
+  d.sell_wy = dupload(wy, ne);
+  d.sell_wz = dupload(wz, ne);
+  free(sptr); free(deg); free(nbr); free(ent); free(blk); free(wx); free(wy); free(wz);
+
+  /* boundary */
+  d.sf0 = dupload(g->b->fc->fptr->n0, d.S);
+  d.sf1 = dupload(g->b->fc->fptr->n1, d.S);
+  d.sf2 = dupload(g->b->fc->fptr->n2, d.S);
+  d.ncolor = (uint32_t)g->t->sz;
+  d.color = dupload(g->t->i, (size_t)d.ncolor + 1);
+  d.sfptr = dupload(p->sfptr, N + 1);
+  d.sfent = dupload(p->sfent, 3 * (size_t)d.S);
+  d.snx = dupload(g->b->s->xyz->x0, d.NS);
+  d.sny = dupload(g->b->s->xyz->x1, d.NS);
+  d.snz = dupload(g->b->s->xyz->x2, d.NS);
+  d.snptr = dupload(p->snptr, N + 1);
+  d.snent = dupload(p->snent, d.NS);
+  d.fnx = dupload(g->b->f->xyz->x0, d.NF);
+  d.fny = dupload(g->b->f->xyz->x1, d.NF);
+  d.fnz = dupload(g->b->f->xyz->x2, d.NF);
+  d.fnptr = dupload(p->fnptr, N + 1);
+  d.fnent = dupload(p->fnent, d.NF);
+
+  /* BSR pattern and level schedule */
+  d.rowptr = dupload(p->rowptr, N + 1);
+  d.col = dupload(p->col, p->nnzb);
+  d.diag = dupload(p->diag, N);
+  for (size_t i = 0; i < N; i++)
+    if (p->rowptr[i + 1] - p->rowptr[i] > c->maxrow) c->maxrow = p->rowptr[i + 1] - p->rowptr[i];
+  c->nlev_l = p->nlev_l;
+  c->nlev_u = p->nlev_u;
+  c->lrows = dupload(p->lrows, N);
+  c->urows = dupload(p->urows, N);
+  uint32_t *lg = level_groups(p->lptr, p->nlev_l, &c->nlgrp);
+  uint32_t *ug = level_groups(p->uptr, p->nlev_u, &c->nugrp);
+  c->lgrp = dupload(lg, c->nlgrp + 1);
+  c->ugrp = dupload(ug, c->nugrp + 1);
+  free(lg); free(ug);
+  c->rdone = dalloc<uint32_t>(N);
+  KF_CHECK(hipMemset(c->rdone, 0, N * sizeof(uint32_t)));
+  c->ticket = dalloc<uint32_t>(4);
+  KF_CHECK(hipMemset(c->ticket, 0, 4 * sizeof(uint32_t)));
+  c->a64 = dalloc<double>(16 * p->nnzb);
+  KF_CHECK(hipMemset(c->a64, 0, 16 * p->nnzb * sizeof(double)));
+
+  /* state and work */
+  c->q = dalloc<double>(4 * N);
+  c->cdt = dalloc<double>(N);
+  KF_CHECK(hipMemset(c->cdt, 0, N * sizeof(double)));
+  d.eres = dalloc<double>(4 * E + 8);
+  d.red = dalloc<double>(KF_NRED);
+  d.scal = dalloc<double>(KF_NSCAL);
+  KF_CHECK(hipMemset(d.scal, 0, KF_NSCAL * sizeof(double)));
+  d.err = dalloc<uint32_t>(1);
+  KF_CHECK(hipMemset(d.err, 0, sizeof(uint32_t)));
+  KF_CHECK(hipHostMalloc((void **)&c->hscal, KF_NSCAL * sizeof(double), hipHostMallocDefault));
+
+  /* free-stream state, src/guess.c:8-20 */
+  double *qh = (double *)kf_xmalloc(4 * N * sizeof(double));
+  for (size_t i = 0; i < N; i++) {
+    qh[4 * i + 0] = 1.0; qh[4 * i + 1] = d.U; qh[4 * i + 2] = d.V; qh[4 * i + 3] = 0.0;
+  }
+  KF_CHECK(hipMemcpy(c->q, qh, 4 * N * sizeof(double), hipMemcpyHostToDevice));
+  free(qh);
+
+  g->q->q = c->q;
+  g->n->cdt = c->cdt;
+  g->matrix = c;
+}
+
+static void kf_unbind_ctx(kf_ctx *c);
+
+static void dfree(void *p)
+{
+  if (p) KF_CHECK(hipFree(p));
+}
+
+extern "C" void kf_ctx_destroy(GEOMETRY *g)
+{
+  if (!g->matrix) return;
+  kf_ctx *c = (kf_ctx *)g->matrix;
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  kf_dv &d = c->d;
+  void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
+                  d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
+                  d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
+                  d.red, d.scal, d.err, c->lrows, c->urows, c->lgrp, c->ugrp, c->rdone, c->ticket, c->a64, c->a32,
+                  c->abf, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
+  KF_CHECK(hipHostFree(c->hscal));
+  KF_CHECK(hipStreamDestroy(c->stream));
+  kf_unbind_ctx(c);
+  delete c;
+  g->matrix = NULL;
+  if (g->q) g->q->q = NULL;
+  if (g->n) g->n->cdt = NULL;
+}
+
+/* ---- options ---------------------------------------------------------------- */
+
+extern "C" kf_options kf_default_options(void)
+{
+  kf_options o;
+  memset(&o, 0, sizeof(o));
+  o.alpha_deg = 3.0;        /* inc/phy.h:8-9 */
+  o.cfl_init = 50.0;        /* src/kernel.c:19 */
+  o.cfl_max = 100000.0;     /* src/kernel.c:20 */
+  o.ts_max = 15;            /* src/kernel.c:21 */
+  o.fnorm_ratio = 1e7;      /* src/kernel.c:22 */
+  o.gmres_restart = 30;     /* src/kernel.c:15 */
+  o.gmres_rtol = 0.01;      /* src/kernel.c:14 */
+  o.gmres_max_it = 10000;   /* src/kernel.c:17 */
+  o.first_order = 0;
+  o.prec = KF_PREC_FP64;
+  o.ortho = KF_ORTHO_MGS;
+  o.verbose = 0;
+  return o;
+}
+
+static void set_free_stream_state(kf_ctx *c);
+
+extern "C" void kf_set_options(GEOMETRY *g, const kf_options *o)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (o->gmres_restart < 1 || o->gmres_restart > 31) {
+    fprintf(stderr, "kf: gmres_restart must be in [1, 31]\n");
+    abort();
+  }
+  const int alpha_changed = o->alpha_deg != c->opt.alpha_deg;
+  c->opt = *o;
+  if (alpha_changed) {
+    kf_free_stream(o->alpha_deg, &c->d.U, &c->d.V);
+    set_free_stream_state(c);
+  }
+}
+
+extern "C" kf_options kf_get_options(const GEOMETRY *g) { return kf_ctx_of(g)->opt; }
+
+static void set_free_stream_state(kf_ctx *c)
+{
+  const size_t N = c->d.N;
+  double *qh = (double *)kf_xmalloc(4 * N * sizeof(double));
+  for (size_t i = 0; i < N; i++) {
+    qh[4 * i + 0] = 1.0; qh[4 * i + 1] = c->d.U; qh[4 * i + 2] = c->d.V; qh[4 * i + 3] = 0.0;
+  }
+  KF_CHECK(hipMemcpyAsync(c->q, qh, 4 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  free(qh);
+}
+
+/* ---- setup ------------------------------------------------------------------ */
+
+extern "C" void kf_setup_host(const char *fpath, GEOMETRY *g)
+{
+  memset(g, 0, sizeof(*g));
+  kf_read_msh(fpath, g);
+  kf_host_setup(g);
+}
+
+extern "C" void kf_bind(GEOMETRY *g);
+
+extern "C" void kf_setup_ex(const char *fpath, GEOMETRY *g, const kf_options *o)
+{
+  kf_setup_host(fpath, g);
+  kf_prep p;
+  kf_build_prep(g, &p);
+  kf_ctx_create(g, &p, o);
+  kf_free_prep(&p);
+  kf_bind(g);
+}
+
+extern "C" void Setup(const char *fpath, GEOMETRY *g)
+{
+  kf_options o = kf_default_options();
+  kf_setup_ex(fpath, g, &o);
+}
+
+extern "C" void kf_geometry_free(GEOMETRY *g)
+{
+  kf_ctx_destroy(g);
+  kf_free_host_geometry(g);
+}
+
+extern "C" int kf_last_error(const GEOMETRY *g)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  uint32_t e = 0;
+  KF_CHECK(hipMemcpyAsync(&e, c->d.err, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  return (int)e;
+}
+
+/* ---- memory helpers ---------------------------------------------------------- */
+
+extern "C" void kf_set_state(GEOMETRY *g, const double *q_host)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  KF_CHECK(hipMemcpyAsync(c->q, q_host, 4 * (size_t)c->d.N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  KF_CHECK(hipStreamSynchronize(c->stream));
+}
+
+extern "C" void kf_get_state(const GEOMETRY *g, double *q_host)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  KF_CHECK(hipMemcpyAsync(q_host, c->q, 4 * (size_t)c->d.N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KF_CHECK(hipStreamSynchronize(c->stream));
+}
+
+extern "C" void kf_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+  KF_CHECK(hipDeviceSynchronize());
+  KF_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+
+extern "C" void kf_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+  KF_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  KF_CHECK(hipDeviceSynchronize());
+}
+
+extern "C" void *kf_device_alloc(size_t bytes)
+{
+  void *p = NULL;
+  KF_CHECK(hipMalloc(&p, bytes ? bytes : 8));
+  KF_CHECK(hipMemset(p, 0, bytes ? bytes : 8));
+  return p;
+}
+
+extern "C" void kf_device_free(void *p) { dfree(p); }
+extern "C" void kf_synchronize(const GEOMETRY *g) { KF_CHECK(hipStreamSynchronize(kf_ctx_of(g)->stream)); }
+extern "C" void *kf_stream(const GEOMETRY *g) { return (void *)kf_ctx_of(g)->stream; }
+extern "C" size_t kf_nnzb(const GEOMETRY *g) { return kf_ctx_of(g)->nnzb; }
+
+extern "C" void kf_pattern(const GEOMETRY *g, unsigned int *rowptr, unsigned int *col, unsigned int *diag)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  if (rowptr) KF_CHECK(hipMemcpy(rowptr, c->d.rowptr, ((size_t)c->d.N + 1) * 4, hipMemcpyDeviceToHost));
+  if (col) KF_CHECK(hipMemcpy(col, c->d.col, c->nnzb * 4, hipMemcpyDeviceToHost));
+  if (diag) KF_CHECK(hipMemcpy(diag, c->d.diag, (size_t)c->d.N * 4, hipMemcpyDeviceToHost));
+}
+
+extern "C" void kf_get_blocks(const GEOMETRY *g, double *blocks_host)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  if (c->opt.prec == KF_PREC_FP64) {
+    KF_CHECK(hipMemcpy(blocks_host, c->a64, c->nnzb * 16 * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    float *tmp = (float *)kf_xmalloc(c->nnzb * 16 * sizeof(float));
+    KF_CHECK(hipMemcpy(tmp, c->a32, c->nnzb * 16 * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < c->nnzb * 16; i++) blocks_host[i] = tmp[i];
+    free(tmp);
+  }
+}
+
+/* ---- reference entry points ---------------------------------------------------- */
+
+extern "C" void ComputeFlux(const GEOMETRY *g, const double *q, GRADIENT *grad, double *r)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  kf_launch_flux(c, q, grad->x0, grad->x1, grad->x2, r);
+}
+
+extern "C" void ComputeTimeStep(const double cfl, GEOMETRY *g) { kf_launch_timestep(kf_ctx_of(g), cfl); }
+
+extern "C" void ComputeForces(const GEOMETRY *g, double *forces) { kf_launch_forces(kf_ctx_of(g), forces); }
+
+extern "C" void ComputeA(GEOMETRY *g) { kf_launch_fill(kf_ctx_of(g)); }
+
+extern "C" void ComputeNumericalILU(GEOMETRY *g) { kf_launch_ilu(kf_ctx_of(g)); }
+
+extern "C" void ComputeSparseTriangularSolve(const GEOMETRY *g, const double *b, double *x)
+{
+  if (b == x) {
+    fprintf(stderr, "kf: ComputeSparseTriangularSolve: b and x must not alias\n");
+    abort();
+  }
+  kf_launch_sptrsv(kf_ctx_of(g), b, x);
+}
+
+/* inc/utils.h:18-28.  The reference's BLAS-1 take no GEOMETRY; these run on
+ * the context of the most recent Setup (or kf_bind). */
+static kf_ctx *g_current = NULL;
+
+extern "C" void kf_bind(GEOMETRY *g) { g_current = kf_ctx_of(g); }
+
+static void kf_unbind_ctx(kf_ctx *c)
+{
+  if (g_current == c) g_current = NULL;
+}
+
+static kf_ctx *current(void)
+{
+  if (!g_current) {
+    fprintf(stderr, "kf: no device context bound (call Setup or kf_bind first)\n");
+    abort();
+  }
+  return g_current;
+}
+
+extern "C" double Compute2ndNorm(const size_t sz, const double *v)
+{
+  kf_ctx *c = current();
+  double out;
+  kf_nrm2_dev(c, sz, v, 0);
+  kf_scal_fetch(c, 0, 1, &out);
+  return out;
+}
+
+extern "C" void ComputeAXPY(const size_t sz, const double a, const double *x, double *y) { kf_axpy(current(), sz, a, x, y); }
+
+extern "C" void ComputeNewAXPY(const size_t sz, const double a, const double *x, const double *y, double *w)
+{
+  kf_newaxpy(current(), sz, a, x, y, w);
+}
+
+extern "C" double Normalize(const size_t sz, double *x)
+{
+  kf_ctx *c = current();
+  double out;
+  kf_nrm2_dev(c, sz, x, 0);
+  kf_scale_recip(c, sz, x, 0);
+  kf_scal_fetch(c, 0, 1, &out);
+  return out;
+}

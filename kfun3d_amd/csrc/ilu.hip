@@ -1,0 +1,178 @@
+/*
+ * ilu.hip -- block ILU(0) of the BSR Jacobian, in place: strictly lower
+ * blocks become L (unit diagonal implied), upper blocks U, and the diagonal
+ * holds inv(U(i,i)), the storage the reference's ilu_Edmond leaves in A
+ * (matrix/ilu.cpp:49-138) and its triangular solve expects.
+ *
+ * The reference's one-thread sweep is sequential ILU(0) in natural row
+ * order: for each row i and each column k of its pattern, ascending,
+ *   a = A(i,k) - sum_{m < min(i,k)} L(i,m) U(m,k)   (m ascending)
+ *   L(i,k) = a inv(U(k,k))  if k < i,   U(i,k) = a  otherwise.
+ * Here row i runs on one wavefront as soon as the rows of its lower
+ * pattern are final (level schedule of matrix/init.cpp:194-232, sync-free:
+ * see sync.h).  The row is staged in LDS; for each lower block m ascending
+ * 16 lanes form L(i,m) = a(i,m) inv(U(m,m)) and then 4 x 16 lanes subtract
+ * L(i,m) U(m,k) from every matching block of the row, so each block sees
+ * its updates in ascending m, in the reference's block-product order
+ * (matrix/init.cpp:105-116).  16 lanes then form the cofactor inverse of the
+ * diagonal block, one lane per entry, with the expressions of
+ * matrix/inverse.cpp:195-352.
+ */
+#include "sync.h"
+
+/* matrix/inverse.cpp:200-326: entry e of the adjugate is
+ * s0*(a b c) s1*(...) ... six signed triple products summed left to right */
+__constant__ uint8_t kf_cof_idx[16][6][3] = {
+  {{5, 10, 15}, {5, 11, 14}, {9, 6, 15}, {9, 7, 14}, {13, 6, 11}, {13, 7, 10}},
+  {{1, 10, 15}, {1, 11, 14}, {9, 2, 15}, {9, 3, 14}, {13, 2, 11}, {13, 3, 10}},
+  {{1, 6, 15}, {1, 7, 14}, {5, 2, 15}, {5, 3, 14}, {13, 2, 7}, {13, 3, 6}},
+  {{1, 6, 11}, {1, 7, 10}, {5, 2, 11}, {5, 3, 10}, {9, 2, 7}, {9, 3, 6}},
+  {{4, 10, 15}, {4, 11, 14}, {8, 6, 15}, {8, 7, 14}, {12, 6, 11}, {12, 7, 10}},
+  {{0, 10, 15}, {0, 11, 14}, {8, 2, 15}, {8, 3, 14}, {12, 2, 11}, {12, 3, 10}},
+  {{0, 6, 15}, {0, 7, 14}, {4, 2, 15}, {4, 3, 14}, {12, 2, 7}, {12, 3, 6}},
+  {{0, 6, 11}, {0, 7, 10}, {4, 2, 11}, {4, 3, 10}, {8, 2, 7}, {8, 3, 6}},
+  {{4, 9, 15}, {4, 11, 13}, {8, 5, 15}, {8, 7, 13}, {12, 5, 11}, {12, 7, 9}},
+  {{0, 9, 15}, {0, 11, 13}, {8, 1, 15}, {8, 3, 13}, {12, 1, 11}, {12, 3, 9}},
+  {{0, 5, 15}, {0, 7, 13}, {4, 1, 15}, {4, 3, 13}, {12, 1, 7}, {12, 3, 5}},
+  {{0, 5, 11}, {0, 7, 9}, {4, 1, 11}, {4, 3, 9}, {8, 1, 7}, {8, 3, 5}},
+  {{4, 9, 14}, {4, 10, 13}, {8, 5, 14}, {8, 6, 13}, {12, 5, 10}, {12, 6, 9}},
+  {{0, 9, 14}, {0, 10, 13}, {8, 1, 14}, {8, 2, 13}, {12, 1, 10}, {12, 2, 9}},
+  {{0, 5, 14}, {0, 6, 13}, {4, 1, 14}, {4, 2, 13}, {12, 1, 6}, {12, 2, 5}},
+  {{0, 5, 10}, {0, 6, 9}, {4, 1, 10}, {4, 2, 9}, {8, 1, 6}, {8, 2, 5}}};
+/* bit t set: term t is subtracted (term 0: its first factor is negated) */
+__constant__ uint8_t kf_cof_neg[16] = {0x26, 0x19, 0x26, 0x19, 0x19, 0x26, 0x19, 0x26,
+                                      0x26, 0x19, 0x26, 0x19, 0x19, 0x26, 0x19, 0x26};
+
+#define ILU_MAXROW 128
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict__ rows, uint32_t nrows,
+                                            uint32_t *ticket, uint32_t base, uint32_t *rdone, uint32_t epoch,
+                                            T *a)
+{
+  __shared__ uint32_t scol[ILU_MAXROW];
+  __shared__ T sa[ILU_MAXROW * 16];
+  const uint32_t lane = threadIdx.x;
+  for (;;) {
+    const uint32_t t = next_ticket(ticket, base);
+    if (t >= nrows) break;
+    const uint32_t i = rows[t];
+    const uint32_t j0 = d.rowptr[i], j1 = d.rowptr[i + 1], jd = d.diag[i];
+    const uint32_t L = j1 - j0, nlow = jd - j0, pd = jd - j0;
+    for (uint32_t k = lane; k < L; k += 64) scol[k] = d.col[j0 + k];
+    for (uint32_t k = lane; k < 16 * L; k += 64) sa[k] = a[16 * (size_t)j0 + k];
+    __syncthreads();
+    for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err);
+    __syncthreads();
+
+    const uint32_t e = lane & 15, er = e >> 2, ec = e & 3, grp = lane >> 4;
+    for (uint32_t p = 0; p < nlow; p++) {
+      const uint32_t m = scol[p];
+      /* L(i,m) = a(i,m) inv(U(m,m)) */
+      T lv = T(0);
+      if (lane < 16) {
+        const T *dm = a + 16 * (size_t)d.diag[m];
+#pragma unroll
+        for (int k = 0; k < 4; k++) lv += sa[16 * p + 4 * er + k] * dev_load(dm + 4 * k + ec);
+      }
+      __syncthreads();
+      if (lane < 16) sa[16 * p + e] = lv;
+      __syncthreads();
+      /* a(i,k) -= L(i,m) U(m,k) for the upper blocks (m,k) of row m in row i */
+      const uint32_t m1 = d.rowptr[m + 1];
+      for (uint32_t jm = d.diag[m] + 1 + grp; jm < m1; jm += 4) {
+        const uint32_t kc = d.col[jm];
+        uint32_t lo = 0, hi = L;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (scol[mid] < kc) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < L && scol[lo] == kc) {
+          const T *um = a + 16 * (size_t)jm;
+          T s = T(0);
+#pragma unroll
+          for (int k = 0; k < 4; k++) s += sa[16 * p + 4 * er + k] * dev_load(um + 4 * k + ec);
+          sa[16 * lo + e] = sa[16 * lo + e] - s;
+        }
+      }
+      __syncthreads();
+    }
+
+    /* diagonal: cofactor inverse, one lane per entry (lanes 16..63 mirror
+     * lanes 0..15 so that the shuffles below read defined values) */
+    {
+      const T *s = sa + 16 * pd;
+      const uint8_t neg = kf_cof_neg[e];
+      T acc = s[kf_cof_idx[e][0][0]];
+      if (neg & 1u) acc = -acc;
+      acc = acc * s[kf_cof_idx[e][0][1]] * s[kf_cof_idx[e][0][2]];
+#pragma unroll
+      for (int tt = 1; tt < 6; tt++) {
+        const T pr = s[kf_cof_idx[e][tt][0]] * s[kf_cof_idx[e][tt][1]] * s[kf_cof_idx[e][tt][2]];
+        acc = (neg & (1u << tt)) ? acc - pr : acc + pr;
+      }
+      const T d0 = __shfl(acc, 0, 16), d4 = __shfl(acc, 4, 16), d8 = __shfl(acc, 8, 16), d12 = __shfl(acc, 12, 16);
+      T det = s[0] * d0 + s[1] * d4 + s[2] * d8 + s[3] * d12;
+      det = T(1) / det;
+      acc *= det;
+      __syncthreads();
+      if (lane < 16) {
+        if (!isfinite((double)acc)) atomicOr(d.err, KF_ERR_NONFINITE_PIVOT);
+        sa[16 * pd + e] = acc;
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < 16 * L; k += 64) dev_store(a + 16 * (size_t)j0 + k, sa[k]);
+    stores_done();
+    __syncthreads();
+    if (lane == 0) dev_store(rdone + i, epoch);
+  }
+}
+
+static uint32_t persistent_waves(uint32_t work)
+{
+  const uint32_t cap = 2048;
+  return work < cap ? (work ? work : 1) : cap;
+}
+
+template <typename T>
+static void launch_ilu(kf_ctx *c, T *a)
+{
+  const uint32_t nw = persistent_waves(c->d.N);
+  const uint32_t epoch = ++c->epoch;
+  hipLaunchKernelGGL(k_ilu<T>, dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
+                     c->ticket_base[2], c->rdone, epoch, a);
+  KF_LAUNCH_CHECK();
+  c->ticket_base[2] += c->d.N + nw;
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(size_t n, const float *__restrict__ src, uint16_t *__restrict__ dst)
+{
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  /* round to nearest even */
+  uint32_t u = __float_as_uint(src[i]);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7FFFu + lsb;
+  dst[i] = (uint16_t)(u >> 16);
+}
+
+void kf_launch_ilu(kf_ctx *c)
+{
+  if (c->maxrow > ILU_MAXROW) {
+    fprintf(stderr, "kf: ILU supports at most %d blocks per row (mesh has %u)\n", ILU_MAXROW, c->maxrow);
+    abort();
+  }
+  if (c->opt.prec == KF_PREC_FP64) {
+    launch_ilu<double>(c, c->a64);
+  } else {
+    launch_ilu<float>(c, c->a32);
+    if (c->opt.prec == KF_PREC_BF16) {
+      const size_t n = 16 * c->nnzb;
+      if (!c->abf) KF_CHECK(hipMalloc((void **)&c->abf, n * sizeof(uint16_t)));
+      hipLaunchKernelGGL(k_to_bf16, dim3(kf_blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->a32, c->abf);
+      KF_LAUNCH_CHECK();
+    }
+  }
+}

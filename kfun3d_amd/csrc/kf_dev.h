@@ -1,0 +1,153 @@
+/*
+ * kf_dev.h -- device context and launch helpers of the gfx950 path.
+ *
+ * HBM layout (one context per GEOMETRY, built once by kf_ctx_create):
+ *   edges     SoA in edge order: n0[E], n1[E] (u32); nx, ny, nz, area[E] (f64)
+ *   vertices  x[N], y[N], z[N] (f64); state q[4N] and every Krylov vector AoS
+ *             (p,u,v,w) per vertex = one 32-byte segment per vertex
+ *   incidence sliced-ELL with slice height 64 (one wavefront): vertex v of
+ *             slice s = v / 64, lane l = v % 64, owns entries
+ *             (sell_ptr[s] + k) * 64 + l for k < deg[v], ascending edge id
+ *             (the order the reference's one-thread edge loops accumulate in):
+ *               sell_nbr  neighbour vertex              (u32)
+ *               sell_ent  edge << 1 | side              (u32, side 1: v is n1)
+ *               sell_blk  BSR block of (v, neighbour)   (u32)
+ *               sell_w*   LSQ term of the edge seen from v, 3 x f64 SoA
+ *   BSR       rowptr[N+1], col[nnzb], diag[N]; blocks of 16 values row-major,
+ *             FP64 (a64) or FP32 (a32); BF16 factor copy (abf)
+ *   levels    rows in level order of the lower (L) / upper (U) triangle,
+ *             cut into groups of at most 16 rows of one level
+ */
+#ifndef KF_DEV_H
+#define KF_DEV_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "../host/kf_host.h"
+
+#define KF_CHECK(call)                                                                       \
+  do {                                                                                       \
+    hipError_t kf_e_ = (call);                                                               \
+    if (kf_e_ != hipSuccess) {                                                               \
+      fprintf(stderr, "kf: HIP error %s at %s:%d: %s\n", hipGetErrorName(kf_e_), __FILE__,  \
+              __LINE__, hipGetErrorString(kf_e_));                                           \
+      abort();                                                                               \
+    }                                                                                        \
+  } while (0)
+
+#define KF_LAUNCH_CHECK() KF_CHECK(hipGetLastError())
+
+#define KF_NRED 1024          /* partial sums of a deterministic reduction */
+#define KF_NSCAL 256          /* device scalar slots */
+#define KF_GROUP 16           /* rows per triangular-solve group */
+
+/* device error word bits */
+#define KF_ERR_NONFINITE_PIVOT 1u
+#define KF_ERR_SPIN_LIMIT 2u
+
+/* everything a kernel reads, passed by value */
+struct kf_dv {
+  uint32_t N, E, S, NS, NF, nslice;
+  double U, V;
+  uint32_t *n0, *n1;
+  double *enx, *eny, *enz, *ea;
+  double *x, *y, *z;
+  uint32_t *sell_ptr, *deg;
+  uint32_t *sell_nbr, *sell_ent, *sell_blk;
+  double *sell_wx, *sell_wy, *sell_wz;
+  uint32_t *sf0, *sf1, *sf2;
+  uint32_t *color, ncolor;    /* solid-face colour offsets */
+  uint32_t *sfptr, *sfent;
+  double *snx, *sny, *snz;
+  uint32_t *snptr, *snent;
+  double *fnx, *fny, *fnz;
+  uint32_t *fnptr, *fnent;
+  uint32_t *rowptr, *col, *diag;
+  double *eres;
+  double *red, *scal;
+  uint32_t *err;
+};
+
+struct kf_ctx {
+  kf_dv d;
+  hipStream_t stream;
+  kf_options opt;
+  size_t nnzb;
+  uint32_t maxrow;          /* most blocks in one BSR row */
+  uint32_t nlev_l, nlev_u;
+  /* level schedule */
+  uint32_t *lrows, *urows, *lgrp, *ugrp;
+  uint32_t nlgrp, nugrp;
+  uint32_t *rdone;          /* per-row completion epoch */
+  uint32_t *ticket;         /* monotonically increasing work tickets */
+  uint32_t ticket_base[4];
+  uint32_t epoch;
+  /* BSR values */
+  double *a64;
+  float *a32;
+  uint16_t *abf;
+  /* state */
+  double *q, *cdt;
+  double *hscal;            /* pinned host mirror of d.scal */
+  /* Kernel() work vectors */
+  double *gx, *gy, *gz, *r, *w, *t, *xk, *vecs;
+  int nvecs;
+  float *f32a, *f32b;       /* single-precision triangular-solve buffers */
+};
+
+static inline kf_ctx *kf_ctx_of(const GEOMETRY *g)
+{
+  if (!g || !g->matrix) {
+    fprintf(stderr, "kf: GEOMETRY has no device context (Setup was not called)\n");
+    abort();
+  }
+  return (kf_ctx *)g->matrix;
+}
+
+/* Workgroups are dispatched round robin over the 8 XCDs.  This remap gives
+ * every XCD a contiguous run of logical blocks -- a contiguous slab of
+ * vertices -- so that neighbour gathers mostly hit that XCD's own L2. */
+__device__ __forceinline__ uint32_t kf_xcd_block()
+{
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t per = (nb + 7u) / 8u;
+  const uint32_t x = b % 8u, i = b / 8u;
+  const uint32_t full = (nb % 8u) ? nb % 8u : 8u;
+  return x < full ? x * per + i : full * per + (x - full) * (per - 1u) + i;
+}
+
+static inline uint32_t kf_blocks_for(size_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+static inline void kf_check_aligned(const void *p, const char *what)
+{
+  if (((uintptr_t)p) & 31u) {
+    fprintf(stderr, "kf: %s must be 32-byte aligned (one vertex = 4 doubles)\n", what);
+    abort();
+  }
+}
+
+/* launches (defined in the .hip files) */
+void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r);
+void kf_launch_timestep(kf_ctx *c, double cfl);
+void kf_launch_forces(kf_ctx *c, double *forces_host);
+void kf_launch_fill(kf_ctx *c);
+void kf_launch_ilu(kf_ctx *c);
+void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x);
+
+/* BLAS-1 on the context stream; reductions are deterministic (fixed
+ * partition, fixed tree) and leave their result in d.scal[slot] */
+void kf_dot_dev(kf_ctx *c, size_t n, const double *x, const double *y, int slot);
+void kf_nrm2_dev(kf_ctx *c, size_t n, const double *x, int slot);
+void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out);
+void kf_axpy(kf_ctx *c, size_t n, double a, const double *x, double *y);
+void kf_newaxpy(kf_ctx *c, size_t n, double a, const double *x, const double *y, double *w);
+void kf_scale_recip(kf_ctx *c, size_t n, double *x, int slot);
+void kf_axpy_slot(kf_ctx *c, size_t n, int slot, const double *x, double *y);
+void kf_fd_perturb(kf_ctx *c, const double *x, double h, int slot_xnorm, double *w);
+void kf_fd_combine(kf_ctx *c, const double *r, const double *w, double h, int slot_xnorm, double *y);
+void kf_fill_zero(kf_ctx *c, size_t n, double *x);
+
+#endif

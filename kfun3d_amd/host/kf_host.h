@@ -1,0 +1,93 @@
+/*
+ * kf_host.h -- host-side internals shared by the C preprocessing, the C
+ * solver driver and the HIP device context.
+ */
+#ifndef KF_HOST_H
+#define KF_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/kfun3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void *kf_xmalloc(size_t bytes);
+void *kf_xcalloc(size_t n, size_t size);
+
+/* Derived host arrays the device context is built from.  All orders are
+ * the reference's: incidences ascend in edge (resp. face, node-list) index,
+ * which is the order the reference's one-thread loops accumulate in. */
+typedef struct kf_prep {
+  size_t N, E, nnzb;
+  /* BSR 4x4 pattern: CSR with sorted columns (matrix/init.cpp:7-74) */
+  uint32_t *rowptr, *col, *diag;
+  /* vertex -> incident edges in ascending edge id; ent = edge << 1 | side,
+   * side 0: the vertex is n0 of the edge, side 1: it is n1 */
+  uint32_t *vptr, *vent;
+  /* BSR block of (vertex, neighbour) for each incidence entry */
+  uint32_t *vblk;
+  /* level sets of the lower / upper triangle (matrix/init.cpp:194-274) */
+  uint32_t nlev_l, nlev_u;
+  uint32_t *lptr, *lrows, *uptr, *urows;
+  /* vertex -> (solid face << 2 | corner), solid-node entries, free-node
+   * entries, each ascending */
+  uint32_t *sfptr, *sfent, *snptr, *snent, *fnptr, *fnent;
+  /* U and V of the free stream, computed as the reference's
+   * cos/sin(alpha / (180 / M_PI)) */
+  double U, V;
+} kf_prep;
+
+/* .msh reader (src/mesh.c:22-154 without the device part) */
+void kf_read_msh(const char *fpath, GEOMETRY *g);
+/* host preprocessing of src/mesh.c:113-153 with one owner */
+void kf_host_setup(GEOMETRY *g);
+/* derived arrays for the device */
+void kf_build_prep(const GEOMETRY *g, kf_prep *p);
+void kf_free_prep(kf_prep *p);
+void kf_free_host_geometry(GEOMETRY *g);
+void kf_free_lsq(GEOMETRY *g);
+
+/* restatements of the reference's host setup steps */
+void kf_index_sort(size_t sz, const uint32_t *key, uint32_t *perm);       /* src/index.c:90-112 */
+void kf_color_solid_faces(GEOMETRY *g);                                    /* src/sbface_coloring.c:11-93 */
+void kf_lsq_weights(GEOMETRY *g);                                          /* src/weights.c:299-398 */
+void kf_free_stream(double alpha_deg, double *U, double *V);               /* inc/phy.h:8-9 */
+
+/* device context (HIP side) */
+void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o);
+void kf_ctx_destroy(GEOMETRY *g);
+
+/* Device vector operations for the C driver (blas.hip).  Reductions leave
+ * their result in a device scalar slot; kfs_fetch copies slots to the host
+ * (and synchronizes). */
+enum {
+  KFS_SLOT_API = 0,      /* Compute2ndNorm / Normalize */
+  KFS_SLOT_XNORM = 1,    /* |x| of iJacVec */
+  KFS_SLOT_H = 2,        /* Hessenberg column: slots 2 .. 2+restart+1 */
+  KFS_SLOT_FNORM = 40,
+  KFS_SLOT_QNORM = 41,
+  KFS_SLOT_BETA = 42
+};
+void kfs_nrm2(GEOMETRY *g, size_t n, const double *x, int slot);
+void kfs_dot(GEOMETRY *g, size_t n, const double *x, const double *y, int slot);
+void kfs_fetch(GEOMETRY *g, int slot, int count, double *out);
+void kfs_axpy(GEOMETRY *g, size_t n, double a, const double *x, double *y);
+void kfs_newaxpy(GEOMETRY *g, size_t n, double a, const double *x, const double *y, double *w);
+void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot);
+void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y);
+void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w);
+void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int slot, double *y);
+void kfs_zero(GEOMETRY *g, size_t n, double *x);
+void kfs_cgs_dots(GEOMETRY *g, size_t n, const double *vbase, int nv, const double *w, int slot);
+void kfs_cgs_update(GEOMETRY *g, size_t n, const double *vbase, int nv, int slot, double *w);
+/* Kernel() work vectors: grad x/y/z, r, w, t, x, and nbasis Krylov vectors */
+void kfs_work(GEOMETRY *g, int nbasis, double **grad3, double **r, double **w, double **t, double **x, double **vecs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KF_HOST_H */
