@@ -196,6 +196,13 @@ void kf_geometry_free(GEOMETRY *g);
 /* Non-zero when the device flagged a non-finite norm or a singular pivot. */
 int  kf_last_error(const GEOMETRY *g);
 
+/* Per-kernel timing with HIP events on the context stream.  kf_profile(g, 1)
+ * clears and starts it; kf_profile_get returns the total milliseconds and the
+ * launch count of one kernel class ("grad", "edge_flux", "residual",
+ * "timestep", "fill", "ilu", "trsv_lower", "trsv_upper"); -1 if unknown. */
+void kf_profile(GEOMETRY *g, int on);
+int  kf_profile_get(GEOMETRY *g, const char *name, double *total_ms, long *count);
+
 #ifdef __cplusplus
 }
 #endif

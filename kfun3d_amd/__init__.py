@@ -122,6 +122,8 @@ def _load():
         "kf_write_lattice_msh": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_uint64]),
         "kf_last_error": (C.c_int, [GP]),
         "kf_bind": (None, [GP]),
+        "kf_profile": (None, [GP, C.c_int]),
+        "kf_profile_get": (C.c_int, [GP, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
         "ComputeFlux": (None, [GP, DP, C.POINTER(Xyz), DP]),
         "ComputeTimeStep": (None, [C.c_double, GP]),
         "ComputeA": (None, [GP]),
@@ -346,6 +348,18 @@ class Mesh:
 
     def last_error(self):
         return lib().kf_last_error(self.ref)
+
+    def profile(self, on):
+        lib().kf_profile(self.ref, 1 if on else 0)
+
+    def profile_get(self, name):
+        """(total milliseconds, launches) of one kernel class since profile(True)"""
+        ms, n = C.c_double(), C.c_long()
+        if lib().kf_profile_get(self.ref, name.encode(), C.byref(ms), C.byref(n)) != 0:
+            raise KeyError(name)
+        return ms.value, n.value
+
+    PROFILED = ["grad", "edge_flux", "residual", "timestep", "fill", "ilu", "trsv_lower", "trsv_upper"]
 
     def synchronize(self):
         lib().kf_synchronize(self.ref)

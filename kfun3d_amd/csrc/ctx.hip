@@ -439,3 +439,66 @@ extern "C" double Normalize(const size_t sz, double *x)
   kf_scal_fetch(c, 0, 1, &out);
   return out;
 }
+
+/* ---- per-kernel timing with HIP events on the context stream ----------------- */
+
+static const char *kf_prof_names[KP_N] = {"grad", "edge_flux", "residual", "timestep", "fill", "ilu",
+                                          "trsv_lower", "trsv_upper"};
+
+static void prof_drain(kf_ctx *c)
+{
+  kf_prof &p = c->prof;
+  if (!p.used) return;
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < p.used; s++) {
+    float ms = 0.f;
+    KF_CHECK(hipEventElapsedTime(&ms, p.ev[2 * s], p.ev[2 * s + 1]));
+    p.total_ms[p.id[s]] += ms;
+    p.count[p.id[s]]++;
+  }
+  p.used = 0;
+}
+
+int kf_prof_begin(kf_ctx *c, int id)
+{
+  kf_prof &p = c->prof;
+  if (!p.on) return -1;
+  if (p.used == KF_PROF_POOL) prof_drain(c);
+  const int s = p.used++;
+  p.id[s] = id;
+  KF_CHECK(hipEventRecord(p.ev[2 * s], c->stream));
+  return s;
+}
+
+void kf_prof_end(kf_ctx *c, int slot)
+{
+  if (slot < 0) return;
+  KF_CHECK(hipEventRecord(c->prof.ev[2 * slot + 1], c->stream));
+}
+
+/* enable (1) / disable (0) kernel timing; enabling clears the totals */
+extern "C" void kf_profile(GEOMETRY *g, int on)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  kf_prof &p = c->prof;
+  if (on && !p.ev[0])
+    for (int i = 0; i < 2 * KF_PROF_POOL; i++) KF_CHECK(hipEventCreate(&p.ev[i]));
+  prof_drain(c);
+  if (on)
+    for (int k = 0; k < KP_N; k++) { p.total_ms[k] = 0.0; p.count[k] = 0; }
+  p.on = on;
+}
+
+/* total milliseconds and launch count of one kernel class since kf_profile(g, 1) */
+extern "C" int kf_profile_get(GEOMETRY *g, const char *name, double *total_ms, long *count)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  prof_drain(c);
+  for (int k = 0; k < KP_N; k++)
+    if (!strcmp(name, kf_prof_names[k])) {
+      *total_ms = c->prof.total_ms[k];
+      *count = c->prof.count[k];
+      return 0;
+    }
+  return -1;
+}

@@ -256,6 +256,7 @@ static void ensure_f32(kf_ctx *c)
 void kf_launch_fill(kf_ctx *c)
 {
   const uint32_t nb = kf_blocks_for(c->d.N, 128);
+  const int s = kf_prof_begin(c, KP_FILL);
   if (c->opt.prec == KF_PREC_FP64) {
     hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(128), 0, c->stream, c->d, c->q, c->cdt, c->a64);
   } else {
@@ -263,4 +264,5 @@ void kf_launch_fill(kf_ctx *c)
     hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(128), 0, c->stream, c->d, c->q, c->cdt, c->a32);
   }
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
 }

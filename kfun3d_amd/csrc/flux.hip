@@ -284,13 +284,19 @@ void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *
   kf_check_aligned(q, "q"); kf_check_aligned(r, "r");
   kf_check_aligned(gx, "grad->x0"); kf_check_aligned(gy, "grad->x1"); kf_check_aligned(gz, "grad->x2");
   const kf_dv &d = c->d;
+  int s = kf_prof_begin(c, KP_GRAD);
   hipLaunchKernelGGL(k_grad, dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q, gx, gy, gz,
                      c->opt.first_order);
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
+  s = kf_prof_begin(c, KP_EDGE);
   hipLaunchKernelGGL(k_edge_flux, dim3(kf_blocks_for(d.E, 256)), dim3(256), 0, c->stream, d, q, gx, gy, gz, d.eres);
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
+  s = kf_prof_begin(c, KP_RES);
   hipLaunchKernelGGL(k_residual, dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q, d.eres, r);
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
 }
 
 /* ---- local time step: src/timestep.c:58-152 as a per-vertex gather ------- */
@@ -338,8 +344,10 @@ __global__ __launch_bounds__(256) void k_timestep(kf_dv d, const double *__restr
 
 void kf_launch_timestep(kf_ctx *c, double cfl)
 {
+  const int s = kf_prof_begin(c, KP_TSTEP);
   hipLaunchKernelGGL(k_timestep, dim3(kf_blocks_for(c->d.N, 256)), dim3(256), 0, c->stream, c->d, c->q, cfl, c->cdt);
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
 }
 
 /* ---- wall forces: src/forces.c:27-98 --------------------------------------

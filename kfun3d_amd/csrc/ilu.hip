@@ -53,6 +53,7 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
   __shared__ uint32_t scol[ILU_MAXROW];
   __shared__ T sa[ILU_MAXROW * 16];
   const uint32_t lane = threadIdx.x;
+  bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
     if (t >= nrows) break;
@@ -62,7 +63,9 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
     for (uint32_t k = lane; k < L; k += 64) scol[k] = d.col[j0 + k];
     for (uint32_t k = lane; k < 16 * L; k += 64) sa[k] = a[16 * (size_t)j0 + k];
     __syncthreads();
-    for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err);
+    for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err, aborted);
+    aborted = __any(aborted);
+    consume_fence();
     __syncthreads();
 
     const uint32_t e = lane & 15, er = e >> 2, ec = e & 3, grp = lane >> 4;
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
       if (lane < 16) {
         const T *dm = a + 16 * (size_t)d.diag[m];
 #pragma unroll
-        for (int k = 0; k < 4; k++) lv += sa[16 * p + 4 * er + k] * dev_load(dm + 4 * k + ec);
+        for (int k = 0; k < 4; k++) lv += sa[16 * p + 4 * er + k] * dm[4 * k + ec];
       }
       __syncthreads();
       if (lane < 16) sa[16 * p + e] = lv;
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
           const T *um = a + 16 * (size_t)jm;
           T s = T(0);
 #pragma unroll
-          for (int k = 0; k < 4; k++) s += sa[16 * p + 4 * er + k] * dev_load(um + 4 * k + ec);
+          for (int k = 0; k < 4; k++) s += sa[16 * p + 4 * er + k] * um[4 * k + ec];
           sa[16 * lo + e] = sa[16 * lo + e] - s;
         }
       }
@@ -123,10 +126,10 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
       }
     }
     __syncthreads();
-    for (uint32_t k = lane; k < 16 * L; k += 64) dev_store(a + 16 * (size_t)j0 + k, sa[k]);
-    stores_done();
+    for (uint32_t k = lane; k < 16 * L; k += 64) a[16 * (size_t)j0 + k] = sa[k];
+    publish_fence();
     __syncthreads();
-    if (lane == 0) dev_store(rdone + i, epoch);
+    if (lane == 0) flag_store(rdone + i, epoch);
   }
 }
 
@@ -141,9 +144,11 @@ static void launch_ilu(kf_ctx *c, T *a)
 {
   const uint32_t nw = persistent_waves(c->d.N);
   const uint32_t epoch = ++c->epoch;
+  const int s = kf_prof_begin(c, KP_ILU);
   hipLaunchKernelGGL(k_ilu<T>, dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
                      c->ticket_base[2], c->rdone, epoch, a);
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
   c->ticket_base[2] += c->d.N + nw;
 }
 

@@ -71,8 +71,22 @@ struct kf_dv {
   uint32_t *err;
 };
 
+/* per-kernel HIP-event timing on the context stream (kf_profile) */
+enum kf_prof_id { KP_GRAD, KP_EDGE, KP_RES, KP_TSTEP, KP_FILL, KP_ILU, KP_TRSV_L, KP_TRSV_U, KP_N };
+#define KF_PROF_POOL 512
+
+struct kf_prof {
+  int on;
+  int used;
+  hipEvent_t ev[2 * KF_PROF_POOL];
+  int id[KF_PROF_POOL];
+  double total_ms[KP_N];
+  long count[KP_N];
+};
+
 struct kf_ctx {
   kf_dv d;
+  kf_prof prof;
   hipStream_t stream;
   kf_options opt;
   size_t nnzb;
@@ -128,6 +142,10 @@ static inline void kf_check_aligned(const void *p, const char *what)
     abort();
   }
 }
+
+/* event pair around one launch; no-ops unless profiling is on (ctx.hip) */
+int kf_prof_begin(kf_ctx *c, int id);
+void kf_prof_end(kf_ctx *c, int slot);
 
 /* launches (defined in the .hip files) */
 void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r);

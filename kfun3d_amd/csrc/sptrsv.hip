@@ -25,32 +25,34 @@ __global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__re
                                                    const TM *__restrict__ a, const double *__restrict__ b, TV *xv)
 {
   const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
+  bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
     if (t >= ngrp) break;
     const uint32_t g0 = grp[t], g1 = grp[t + 1];
     const bool active = g0 + qd < g1;
     const uint32_t r = active ? rows[g0 + qd] : 0;
+    const uint32_t j0 = active ? d.rowptr[r] : 0, jd = active ? d.diag[r] : 0;
+    /* the quad's four lanes share the waits on the row's lower neighbours */
+    for (uint32_t j = j0 + c; j < jd; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted);
+    aborted = __any(aborted);
+    consume_fence();
     if (active) {
       TV xr = (TV)b[4 * (size_t)r + c];
-      const uint32_t jd = d.diag[r];
-      for (uint32_t j = d.rowptr[r]; j < jd; j++) {
-        const uint32_t col = d.col[j];
-        wait_row(rdone, col, epoch, d.err);
-        const TV *xc = xv + 4 * (size_t)col;
-        const TV x0 = dev_load(xc), x1 = dev_load(xc + 1), x2 = dev_load(xc + 2), x3 = dev_load(xc + 3);
+      for (uint32_t j = j0; j < jd; j++) {
+        const TV *xc = xv + 4 * (size_t)d.col[j];
         const size_t o = 16 * (size_t)j + 4 * c;
         TV s = TV(0);
-        s += (TV)mat_ld(a, o) * x0;
-        s += (TV)mat_ld(a, o + 1) * x1;
-        s += (TV)mat_ld(a, o + 2) * x2;
-        s += (TV)mat_ld(a, o + 3) * x3;
+        s += (TV)mat_ld(a, o) * xc[0];
+        s += (TV)mat_ld(a, o + 1) * xc[1];
+        s += (TV)mat_ld(a, o + 2) * xc[2];
+        s += (TV)mat_ld(a, o + 3) * xc[3];
         xr = xr - s;
       }
-      dev_store(xv + 4 * (size_t)r + c, xr);
+      xv[4 * (size_t)r + c] = xr;
     }
-    stores_done();
-    if (active && c == 0) dev_store(rdone + r, epoch);
+    publish_fence();
+    if (active && c == 0) flag_store(rdone + r, epoch);
   }
 }
 
@@ -61,29 +63,28 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
                                                    const TM *__restrict__ a, TV *xv, double *__restrict__ xout)
 {
   const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
+  bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
     if (t >= ngrp) break;
     const uint32_t g0 = grp[t], g1 = grp[t + 1];
     const bool active = g0 + qd < g1;
     const uint32_t r = active ? rows[g0 + qd] : 0;
+    const uint32_t jd = active ? d.diag[r] : 0, j1 = active ? d.rowptr[r + 1] : 0;
+    for (uint32_t j = jd + 1 + c; j < j1; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted);
+    aborted = __any(aborted);
+    consume_fence();
     TV xr = TV(0);
-    uint32_t jd = 0;
     if (active) {
       xr = xv[4 * (size_t)r + c];
-      jd = d.diag[r];
-      const uint32_t j1 = d.rowptr[r + 1];
       for (uint32_t j = jd + 1; j < j1; j++) {
-        const uint32_t col = d.col[j];
-        wait_row(rdone, col, epoch, d.err);
-        const TV *xc = xv + 4 * (size_t)col;
-        const TV x0 = dev_load(xc), x1 = dev_load(xc + 1), x2 = dev_load(xc + 2), x3 = dev_load(xc + 3);
+        const TV *xc = xv + 4 * (size_t)d.col[j];
         const size_t o = 16 * (size_t)j + 4 * c;
         TV s = TV(0);
-        s += (TV)mat_ld(a, o) * x0;
-        s += (TV)mat_ld(a, o + 1) * x1;
-        s += (TV)mat_ld(a, o + 2) * x2;
-        s += (TV)mat_ld(a, o + 3) * x3;
+        s += (TV)mat_ld(a, o) * xc[0];
+        s += (TV)mat_ld(a, o + 1) * xc[1];
+        s += (TV)mat_ld(a, o + 2) * xc[2];
+        s += (TV)mat_ld(a, o + 3) * xc[3];
         xr = xr - s;
       }
     }
@@ -96,11 +97,11 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
       s += (TV)mat_ld(a, o + 1) * y1;
       s += (TV)mat_ld(a, o + 2) * y2;
       s += (TV)mat_ld(a, o + 3) * y3;
-      dev_store(xv + 4 * (size_t)r + c, s);
+      xv[4 * (size_t)r + c] = s;
       if ((void *)xout != (void *)xv) xout[4 * (size_t)r + c] = (double)s;
     }
-    stores_done();
-    if (active && c == 0) dev_store(rdone + r, epoch);
+    publish_fence();
+    if (active && c == 0) flag_store(rdone + r, epoch);
   }
 }
 
@@ -110,14 +111,18 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *xv, double 
   const uint32_t nwl = c->nlgrp < 2048 ? (c->nlgrp ? c->nlgrp : 1) : 2048;
   const uint32_t nwu = c->nugrp < 2048 ? (c->nugrp ? c->nugrp : 1) : 2048;
   const uint32_t el = ++c->epoch;
+  int s = kf_prof_begin(c, KP_TRSV_L);
   hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64), 0, c->stream, c->d, c->lrows, c->lgrp, c->nlgrp,
                      c->ticket + 0, c->ticket_base[0], c->rdone, el, a, b, xv);
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
   c->ticket_base[0] += c->nlgrp + nwl;
   const uint32_t eu = ++c->epoch;
+  s = kf_prof_begin(c, KP_TRSV_U);
   hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64), 0, c->stream, c->d, c->urows, c->ugrp, c->nugrp,
                      c->ticket + 1, c->ticket_base[1], c->rdone, eu, a, xv, x);
   KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
   c->ticket_base[1] += c->nugrp + nwu;
 }
 

@@ -5,51 +5,61 @@
  * Work is handed out by an atomic ticket in level order, so a wavefront only
  * ever waits for rows owned by tickets issued before its own, i.e. by
  * wavefronts that are already running: no wait can deadlock, whatever the
- * dispatch order of workgroups.  Every wait is additionally bounded
- * (KF_SPIN_LIMIT polls); on expiry the wave records KF_ERR_SPIN_LIMIT in the
- * context's error word and moves on, so every wavefront reaches the end of
- * the kernel.
+ * dispatch order of workgroups.
  *
- * Visibility across the 8 XCDs (each with its own L2): the values one row
- * publishes for others are written with device-scope (sc1) stores, the
- * producer waits for their completion (s_waitcnt vmcnt(0)) and only then
- * stores the row's completion epoch (sc1); consumers poll the epoch with
- * device-scope loads and read the published values with device-scope loads.
- * This avoids the full-L2 write-back / invalidate that a generic agent-scope
- * release/acquire fence costs on gfx950.
+ * Visibility across the 8 XCDs follows the agent-scope release/acquire of
+ * the HIP memory model, whose gfx950 lowering keeps the per-XCD L2 caches
+ * coherent: the producer stores its results with ordinary stores, executes
+ * an agent-scope release fence (L2 write-back of dirty lines, then a wait for
+ * completion) and only then stores the row's completion epoch; a consumer
+ * polls the epoch with agent-scope atomic loads, executes an agent-scope
+ * acquire fence (cache invalidate) once its rows are complete, and then reads
+ * the published values with ordinary loads.
+ *
+ * Every wait is bounded.  A wave whose poll limit expires records
+ * KF_ERR_SPIN_LIMIT in the context's error word and stops waiting for the
+ * rest of the launch (`aborted`), so each wave can time out at most once and
+ * every wavefront reaches the end of the kernel: a failure surfaces as an
+ * error code, never as a hang.
  */
 #ifndef KF_SYNC_H
 #define KF_SYNC_H
 
 #include "kf_dev.h"
 
-#define KF_SPIN_LIMIT (1u << 22)
+#define KF_SPIN_LIMIT (1u << 20)
 
-template <typename T>
-__device__ __forceinline__ T dev_load(const T *p)
+__device__ __forceinline__ uint32_t flag_load(const uint32_t *p)
 {
-  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename T>
-__device__ __forceinline__ void dev_store(T *p, T v)
+__device__ __forceinline__ void flag_store(uint32_t *p, uint32_t v)
 {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* all stores issued so far by this wave have completed */
-__device__ __forceinline__ void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+/* all results this wave stored so far are visible agent-wide before any
+ * flag stored after this call (wave-uniform) */
+__device__ __forceinline__ void publish_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
-/* wait until rdone[row] == epoch; returns false if the poll limit expired */
-__device__ __forceinline__ bool wait_row(const uint32_t *rdone, uint32_t row, uint32_t epoch, uint32_t *err)
+/* values published before the flags this wave has seen are visible to its
+ * subsequent ordinary loads (wave-uniform) */
+__device__ __forceinline__ void consume_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+/* wait until rdone[row] == epoch; false if this wave gave up */
+__device__ __forceinline__ bool wait_row(const uint32_t *rdone, uint32_t row, uint32_t epoch, uint32_t *err,
+                                         bool &aborted)
 {
+  if (aborted) return false;
   uint32_t n = 0;
-  while (dev_load(rdone + row) != epoch) {
+  while (flag_load(rdone + row) != epoch) {
     if (++n > KF_SPIN_LIMIT) {
       atomicOr(err, KF_ERR_SPIN_LIMIT);
+      aborted = true;
       return false;
     }
-    __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_s_sleep(2);
   }
   return true;
 }

@@ -1,0 +1,164 @@
+"""Pins of the oracle that need no reference: closed forms and identities the
+mathematics fixes.  Each is chosen so that a plausible slip (a dropped term,
+a wrong sign or index, a transposed block) fails it."""
+import numpy as np
+import pytest
+
+import kfun3d_amd as K
+import oracle as O
+from conftest import lattice
+
+
+@pytest.fixture(scope="module")
+def small(mesh_dir):
+    m = K.Mesh(lattice(mesh_dir, 7, 6, 5, 3), device=False)
+    yield m
+    m.close()
+
+
+def freestream(N, alpha=3.0):
+    a = alpha / (180 / np.pi)
+    return np.tile(np.array([1.0, np.cos(a), np.sin(a), 0.0]), N)
+
+
+def interior_rows(m):
+    """vertices none of whose neighbours is a boundary vertex"""
+    h = m.host_arrays()
+    bnd = np.zeros(m.N, bool)
+    bnd[h["s_nptr"]] = True
+    bnd[h["f_nptr"]] = True
+    near = bnd.copy()
+    np.logical_or.at(near, h["e_n0"], bnd[h["e_n1"]])
+    np.logical_or.at(near, h["e_n1"], bnd[h["e_n0"]])
+    return np.where(~near)[0], np.where(~bnd)[0]
+
+
+def test_geometric_closure(small):
+    h = small.host_arrays()
+    c = np.zeros((small.N, 3))
+    nA = np.stack([h["e_x0"] * h["e_x3"], h["e_x1"] * h["e_x3"], h["e_x2"] * h["e_x3"]], 1)
+    np.add.at(c, h["e_n0"], nA)
+    np.add.at(c, h["e_n1"], -nA)
+    np.add.at(c, h["s_nptr"], np.stack([h["s_x0"], h["s_x1"], h["s_x2"]], 1))
+    np.add.at(c, h["f_nptr"], np.stack([h["f_x0"], h["f_x1"], h["f_x2"]], 1))
+    assert np.abs(c).max() < 1e-14
+
+
+def test_freestream_preserved_in_interior(small):
+    orc = O.Oracle(small)
+    r, _, _, _ = orc.flux(freestream(small.N))
+    orc.close()
+    _, interior = interior_rows(small)
+    assert np.abs(r.reshape(-1, 4)[interior]).max() < 1e-14
+
+
+def test_lsq_gradient_exact_on_linear_fields(small):
+    h = small.host_arrays()
+    x, y, z = h["n_x0"], h["n_x1"], h["n_x2"]
+    slopes = np.array([[0.3, -1.2, 0.7], [2.0, 0.5, -0.25], [-0.4, 0.1, 1.5], [1.0, -2.0, 0.125]])
+    q = np.stack([0.5 + s[0] * x + s[1] * y + s[2] * z for s in slopes], 1).ravel()
+    orc = O.Oracle(small)
+    _, gx, gy, gz = orc.flux(q)
+    orc.close()
+    for c in range(4):
+        assert np.allclose(gx.reshape(-1, 4)[:, c], slopes[c, 0], atol=1e-11)
+        assert np.allclose(gy.reshape(-1, 4)[:, c], slopes[c, 1], atol=1e-11)
+        assert np.allclose(gz.reshape(-1, 4)[:, c], slopes[c, 2], atol=1e-11)
+
+
+def _dense(rowptr, col, blocks, N):
+    A = np.zeros((4 * N, 4 * N))
+    for i in range(N):
+        for j in range(rowptr[i], rowptr[i + 1]):
+            A[4 * i:4 * i + 4, 4 * col[j]:4 * col[j] + 4] = blocks[j]
+    return A
+
+
+def test_jacobian_matches_central_differences_at_uniform_state(small):
+    """At a uniform state the Roe matrix |A| multiplies q_R - q_L = 0, so the
+    frozen-|A| first-order Jacobian is the exact derivative of the first-order
+    residual on rows whose stencil avoids the boundary terms."""
+    q = freestream(small.N)
+    orc = O.Oracle(small, first_order=1)
+    orc.set_state(q)
+    orc.timestep(1e300)          # cdt ~ 0: A is the flux Jacobian alone
+    orc.fill()
+    rp, col = orc.pattern()
+    A = _dense(rp, col, orc.blocks(), small.N)
+    d = np.random.default_rng(0).standard_normal(small.sz)
+    eps = 1e-6
+    rp_, _, _, _ = orc.flux(q + eps * d)
+    rm_, _, _, _ = orc.flux(q - eps * d)
+    fd = (rp_ - rm_) / (2 * eps)
+    orc.close()
+    rows, _ = interior_rows(small)
+    sel = (4 * rows[:, None] + np.arange(4)).ravel()
+    Ad = A @ d
+    assert np.abs(Ad[sel] - fd[sel]).max() <= 1e-7 * np.abs(fd[sel]).max()
+
+
+def test_ilu0_defining_property_and_triangular_solve(small):
+    orc = O.Oracle(small, ilu_textbook=1)
+    q = freestream(small.N) + 0.05 * np.random.default_rng(1).standard_normal(small.sz)
+    orc.set_state(q)
+    orc.timestep(20.0)
+    orc.fill()
+    rp, col = orc.pattern()
+    A = _dense(rp, col, orc.blocks(), small.N)
+    orc.ilu()
+    F = orc.blocks()
+    N = small.N
+    L = np.eye(4 * N)
+    U = np.zeros((4 * N, 4 * N))
+    for i in range(N):
+        for j in range(rp[i], rp[i + 1]):
+            c = col[j]
+            blk = F[j]
+            if c < i:
+                L[4 * i:4 * i + 4, 4 * c:4 * c + 4] = blk
+            elif c == i:
+                U[4 * i:4 * i + 4, 4 * c:4 * c + 4] = np.linalg.inv(blk)
+            else:
+                U[4 * i:4 * i + 4, 4 * c:4 * c + 4] = blk
+    LU = L @ U
+    mask = _dense(rp, col, np.ones((len(col), 4, 4)), N) > 0
+    assert np.abs((LU - A)[mask]).max() <= 1e-12 * np.abs(A).max()
+    b = np.random.default_rng(2).standard_normal(small.sz)
+    x = orc.sptrsv(b)
+    orc.close()
+    assert np.abs(LU @ x - b).max() <= 1e-10 * np.abs(b).max()
+
+
+def test_reference_ilu_close_to_textbook(small):
+    """ilu_Edmond with one thread is ILU(0) up to the (a - d) + d rounding of
+    its copy side effects."""
+    q = freestream(small.N) + 0.05 * np.random.default_rng(3).standard_normal(small.sz)
+    out = []
+    for tb in (0, 1):
+        orc = O.Oracle(small, ilu_textbook=tb)
+        orc.set_state(q)
+        orc.timestep(20.0)
+        orc.fill()
+        orc.ilu()
+        out.append(orc.blocks())
+        orc.close()
+    assert np.abs(out[0] - out[1]).max() <= 1e-12 * np.abs(out[1]).max()
+
+
+def test_inverse_matches_numpy():
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        a = rng.standard_normal((4, 4)) + 4 * np.eye(4)
+        assert np.allclose(O.inv4(a), np.linalg.inv(a), rtol=1e-12, atol=1e-14)
+
+
+def test_wall_forces_vanish_at_unit_pressure(small):
+    orc = O.Oracle(small)
+    q = freestream(small.N)
+    q[0::4] = 1.0
+    orc.set_state(q)
+    assert np.array_equal(orc.forces(), np.zeros(3))
+    q[0::4] = 1.5
+    orc.set_state(q)
+    assert np.abs(orc.forces()).max() > 0
+    orc.close()
