@@ -1,0 +1,178 @@
+"""Benchmark of the MI355X pseudo-transient Newton-Krylov path.
+
+Workload (the configuration the metric is quoted on, one GPU): the 144x48x48
+Kuhn lattice, seed 1 (331,776 vertices, 2,258,399 edges), second-order
+residual, plain FP64 GPU path: block ILU(0), GMRES(30) with modified
+Gram-Schmidt, CFL ramp from 10, stop at a 1e-10 drop of the nonlinear
+residual.  One "step" is one complete solve from the free stream; ``value``
+is its wall-clock in seconds (mesh already resident in HBM).
+
+The roofline object is for the kernel class that takes the most time in
+the timed solves (HIP events on the context stream, kf_profile); its
+algorithmic bytes per launch are the compulsory traffic of that kernel
+(formulas in ALG_BYTES).  cpu_baseline runs the reference's own CPU path
+(oracle/_ref/kref, built from the reference's sources) on the host cores
+for a bounded sample -- per-call times of ComputeFlux, ComputeA,
+ComputeNumericalILU, ComputeSparseTriangularSolve and one BLAS-1 pass on the
+same mesh -- and extrapolates the solve with the GPU run's call counts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--prec fp64|fp32|bf16]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "wall-clock to 1e-10 residual drop, mixed/learned vs plain FP64 GPU; flux GF/s"
+LATTICE = (144, 48, 48)
+SEED = 1
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_GFLOPS = 78600.0
+
+
+def alg_bytes(kind, N, E, S, Nf, nnzb, s):
+    """Compulsory HBM bytes of one launch (each array moved once)."""
+    return {
+        # idx 8E, normal+area 32E, LSQ terms 48E; xyz 24N, q 32N, r 32N, grad written + read 2*96N
+        "residual": 88 * E + 280 * N,
+        "grad": 56 * E + 128 * N,                 # nbr 8E + terms 48E; q 32N, grad 96N
+        "edge_flux": 72 * E + 152 * N,            # ends 8E, normal 32E, flux out 32E; xyz 24N, q 32N, grad 96N
+        "residual_gather": 40 * E + 32 * N,       # ent 8E, flux in 32E; r 32N
+        "timestep": 40 * E + 40 * N,
+        "fill": 40 * E + 40 * N + 16 * s * nnzb,
+        "ilu": 2 * 16 * s * nnzb + 4 * nnzb,
+        "trsv_lower": E * (16 * s + 4) + 64 * N,  # lower blocks + cols; b in, x out
+        "trsv_upper": (E + N) * (16 * s + 4) + 64 * N,
+    }[kind]
+
+
+PROFILE_TO_KIND = {"grad": "grad", "edge_flux": "edge_flux", "residual": "residual_gather", "timestep": "timestep",
+                   "fill": "fill", "ilu": "ilu", "trsv_lower": "trsv_lower", "trsv_upper": "trsv_upper"}
+
+
+def cpu_baseline(mesh_path, sizes, counts, host_threads):
+    kref = os.path.join(ROOT, "oracle", "_ref", "kref")
+    if not os.path.exists(kref):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(host_threads), OMP_PROC_BIND="close")
+    reps = 3
+    t0 = time.time()
+    out = subprocess.run([kref, "time", mesh_path, str(host_threads), str(reps)], stdout=subprocess.PIPE, text=True,
+                         env=env, timeout=900, check=True).stdout
+    wall = time.time() - t0
+    t = json.loads(out.strip().splitlines()[-1])
+    est = (counts["flux"] * t["flux_s"] + counts["trsv"] * t["sptrsv_s"] + counts["fill"] * (t["fill_s"] + t["ilu_s"]) +
+           counts["blas_passes"] * t["blas_pass_s"])
+    return {"value": est, "unit": "s", "cores": host_threads, "kind": "reference",
+            "sample": ("reference src/*.c + matrix/*.cpp (gcc -O3 -fopenmp, no FMA) on the same %dx%dx%d mesh, %d "
+                       "threads, contiguous owner partition: per-call ComputeFlux %.4f s, ComputeSparseTriangularSolve "
+                       "%.4f s, ComputeA %.4f s, ComputeNumericalILU %.4f s, one BLAS-1 pass %.5f s (%d reps, %.1f s "
+                       "wall); value = those times x the GPU solve's call counts (%d residuals, %d solves, %d "
+                       "fills, %d BLAS passes)" % (sizes + (host_threads, t["flux_s"], t["sptrsv_s"], t["fill_s"],
+                                                           t["ilu_s"], t["blas_pass_s"], reps, wall, counts["flux"],
+                                                           counts["trsv"], counts["fill"], counts["blas_passes"])))}
+
+
+def blas_passes(hist, restart):
+    """4N-vector passes of the driver's BLAS-1 per solve (MGS: a dot and an
+    axpy per basis vector, plus norms, scalings, the FD combine and update)."""
+    n = 0
+    for h in hist:
+        k = h["krylov"]
+        cycles = max(1, (k + restart - 1) // restart)
+        for j in range(k):
+            jj = j % restart
+            n += 2 * (jj + 1) + 2 + 4          # MGS dots + axpys, norm + scale, iJacVec norm/perturb/combine
+        n += cycles * 3 + k + 4                # restart residual, x += V y, update, |r|, |q|
+    return n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the path's multi-GPU sharding is a design, "
+              "not an implementation); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import kfun3d_amd as K
+    tmp = tempfile.mkdtemp(prefix="kfbench_")
+    mesh = os.path.join(tmp, "c2.msh")
+    K.write_lattice(mesh, *LATTICE, SEED)
+    opts = K.config_options(prec=args.prec)
+    m = K.Mesh(mesh, opts)
+    q0 = m.state()
+    h = m.host_arrays()
+    N, E, S, Nf = m.N, m.E, len(h["fc_n0"]), len(h["f_nptr"])
+    nnzb = N + 2 * E
+    s_bytes = {"fp64": 8, "fp32": 4, "bf16": 2}[args.prec]
+
+    for _ in range(args.warmup):
+        m.set_state(q0)
+        m.kernel()
+    m.synchronize()
+    times, hist = [], None
+    m.profile(True)
+    for _ in range(args.steps):
+        m.set_state(q0)
+        m.synchronize()
+        t0 = time.perf_counter()
+        hist = m.kernel()
+        m.synchronize()
+        times.append(time.perf_counter() - t0)
+    prof = {k: m.profile_get(k) for k in K.Mesh.PROFILED}
+    m.profile(False)
+    assert m.last_error() == 0, "device error word set"
+    drop = hist[0]["fnorm"] / hist[-1]["fnorm"]
+    value = max(times) if len(times) == 1 else sorted(times)[len(times) // 2]
+
+    # dominant kernel and its roofline
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, n = prof[dom]
+    kind = PROFILE_TO_KIND[dom]
+    s_mat = 8 if kind not in ("fill", "ilu", "trsv_lower", "trsv_upper") else s_bytes
+    ab = alg_bytes(kind, N, E, S, Nf, nnzb, s_mat)
+    avg_s = ms / n / 1e3
+    achieved = ab / avg_s / 1e9
+    flux_s = sum(prof[k][0] / max(prof[k][1], 1) for k in ("grad", "edge_flux", "residual")) / 1e3
+    flux_flops = 360 * E + 48 * S + 193 * Nf
+    counts = {"flux": prof["edge_flux"][1] // max(args.steps, 1), "trsv": prof["trsv_lower"][1] // max(args.steps, 1),
+              "fill": prof["fill"][1] // max(args.steps, 1), "blas_passes": blas_passes(hist, opts.gmres_restart)}
+    host_threads = min(16, len(os.sched_getaffinity(0)))
+    cpu = None if args.no_cpu else cpu_baseline(mesh, LATTICE, counts, host_threads)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64" if args.prec == "fp64" else "f64 residual/Arnoldi, %s preconditioner" % args.prec,
+        "data": "synthetic",
+        "config": {"workload": "144x48x48 Kuhn lattice seed 1 (331,776 vertices), second order, %s preconditioner, "
+                               "GMRES(30) MGS, CFL ramp from 10, solve from free stream to a 1e-10 residual drop"
+                               % args.prec,
+                   "lattice": list(LATTICE), "seed": SEED, "vertices": N, "edges": E},
+        "pseudo_steps": len(hist), "krylov_total": sum(x["krylov"] for x in hist), "residual_drop": drop,
+        "flux_gflops": flux_flops / flux_s / 1e9, "flux_fp64_frac": flux_flops / flux_s / 1e9 / FP64_PEAK_GFLOPS,
+        "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
+        "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": ab,
+                     "avg_launch_ms": ms / n},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    m.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -144,6 +144,12 @@ typedef struct kf_options {
   kf_precision prec;    /* preconditioner precision                          */
   kf_ortho ortho;       /* orthogonalization                                  */
   int    verbose;       /* print the reference's per-step lines              */
+  int    ilu_exact;     /* FP64: 1 = the reference's one-thread sweep bit for
+                           bit (matrix/ilu.cpp:49-138, including the rounding
+                           of its L/Ut copy side effects); 0 = textbook ILU(0) */
+  int    reduce_seq;    /* 1 = norms and dot products summed in index order
+                           like the reference's one-thread loops (one
+                           wavefront; for parity runs); 0 = fixed-order tree */
 } kf_options;
 
 typedef struct kf_step_record {

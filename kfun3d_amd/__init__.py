@@ -83,7 +83,8 @@ class Options(C.Structure):
     _fields_ = [("alpha_deg", C.c_double), ("cfl_init", C.c_double), ("cfl_max", C.c_double),
                 ("ts_max", C.c_int), ("fnorm_ratio", C.c_double), ("gmres_restart", C.c_int),
                 ("gmres_rtol", C.c_double), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
-                ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int)]
+                ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int), ("ilu_exact", C.c_int),
+                ("reduce_seq", C.c_int)]
 
 
 class StepRecord(C.Structure):

@@ -46,21 +46,50 @@ __global__ __launch_bounds__(RB) void k_red_final(const double *__restrict__ par
   if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
 }
 
-void kf_dot_dev(kf_ctx *c, size_t n, const double *x, const double *y, int slot)
+/* Index-order sum, for parity runs: the reference's one-thread loops add
+ * x[0] y[0], x[1] y[1], ... in that order (src/utils.c:61-62,
+ * src/kernel.c:133-134) into an OpenMP reduction copy that starts at zero and
+ * is then added to 0.  One wavefront forms 64 products at a time and adds
+ * them lane by lane; every lane carries the same running sum. */
+__device__ __forceinline__ double lane_value(double v, int k)
 {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__global__ __launch_bounds__(64) void k_dot_seq(size_t n, const double *__restrict__ x, const double *__restrict__ y,
+                                                double *__restrict__ out, int take_sqrt)
+{
+  const int lane = threadIdx.x;
+  double acc = 0.0;
+  for (size_t base = 0; base < n; base += 64) {
+    const size_t i = base + lane;
+    const double p = i < n ? x[i] * y[i] : 0.0;
+    const int cnt = (int)(n - base < 64 ? n - base : 64);
+    for (int k = 0; k < cnt; k++) acc = acc + lane_value(p, k);
+  }
+  acc = 0.0 + acc;
+  if (lane == 0) *out = take_sqrt ? sqrt(acc) : acc;
+}
+
+static void dot_any(kf_ctx *c, size_t n, const double *x, const double *y, int slot, int take_sqrt)
+{
+  if (c->opt.reduce_seq) {
+    hipLaunchKernelGGL(k_dot_seq, dim3(1), dim3(64), 0, c->stream, n, x, y, c->d.scal + slot, take_sqrt);
+    KF_LAUNCH_CHECK();
+    return;
+  }
   hipLaunchKernelGGL(k_dot_part, dim3(KF_NRED), dim3(RB), 0, c->stream, n, x, y, c->d.red);
   KF_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, c->d.red, c->d.scal + slot, 0);
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, c->d.red, c->d.scal + slot, take_sqrt);
   KF_LAUNCH_CHECK();
 }
 
-void kf_nrm2_dev(kf_ctx *c, size_t n, const double *x, int slot)
-{
-  hipLaunchKernelGGL(k_dot_part, dim3(KF_NRED), dim3(RB), 0, c->stream, n, x, x, c->d.red);
-  KF_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, c->d.red, c->d.scal + slot, 1);
-  KF_LAUNCH_CHECK();
-}
+void kf_dot_dev(kf_ctx *c, size_t n, const double *x, const double *y, int slot) { dot_any(c, n, x, y, slot, 0); }
+
+void kf_nrm2_dev(kf_ctx *c, size_t n, const double *x, int slot) { dot_any(c, n, x, x, slot, 1); }
 
 void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out)
 {

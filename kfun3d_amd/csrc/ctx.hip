@@ -44,6 +44,13 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   KF_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->opt = *o;
   kf_dv &d = c->d;
+  /* launch-shape knobs of the level-scheduled kernels (tuning only) */
+  const char *ev;
+  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 2048;
+  c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 2048;
+  d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
+  if (c->trsv_waves < 1) c->trsv_waves = 1;
+  if (c->ilu_waves < 1) c->ilu_waves = 1;
   const size_t N = p->N, E = p->E;
   if (N >= (1u << 31) || E >= (1u << 30) || p->nnzb >= (1u << 31)) {
     fprintf(stderr, "kf: mesh too large for 32-bit indices\n");
@@ -200,7 +207,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->urows, c->lgrp, c->ugrp, c->rdone, c->ticket, c->a64, c->a32,
-                  c->abf, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b};
+                  c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));
   KF_CHECK(hipStreamDestroy(c->stream));
@@ -229,6 +236,8 @@ extern "C" kf_options kf_default_options(void)
   o.prec = KF_PREC_FP64;
   o.ortho = KF_ORTHO_MGS;
   o.verbose = 0;
+  o.ilu_exact = 1;
+  o.reduce_seq = 0;
   return o;
 }
 

@@ -45,14 +45,37 @@ __constant__ uint8_t kf_cof_neg[16] = {0x26, 0x19, 0x26, 0x19, 0x19, 0x26, 0x19,
 
 #define ILU_MAXROW 128
 
+/* d = A B for 4x4 row-major blocks, entry (r, c) of lane e = 4 r + c, with the
+ * reference's summation order (matrix/init.cpp:105-116) */
 template <typename T>
+__device__ __forceinline__ T blk_mul(const T *A, const T *B, uint32_t r, uint32_t c)
+{
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < 4; k++) s += A[4 * r + k] * B[4 * k + c];
+  return s;
+}
+
+/* EXACT: the reference's one-thread sweep bit for bit.  Each block it forms
+ * ends with the product of the last common pair of its merge (matrix/ilu.cpp:
+ * 71-86): subtracted, then added back, which rounds.  For a lower block (i,k)
+ * that pair is L_old(i,k) U(k,k), where L_old is what the L copy held before
+ * this sweep -- the previous sweep's L(i,k), or, for the first lower slot of
+ * row i, the Jacobian block A(i, k_last) that matrix/matrix.cpp:145 copies
+ * there during the fill (k_last = the row's largest lower column, the last
+ * lower block the fill touches); for an upper block it is I A(i,k).  lold
+ * keeps the previous sweep's L blocks (BSR positions), ud the non-inverted
+ * U(k,k). */
+template <typename T, bool EXACT>
 __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict__ rows, uint32_t nrows,
                                             uint32_t *ticket, uint32_t base, uint32_t *rdone, uint32_t epoch,
-                                            T *a)
+                                            T *a, T *lold, T *ud)
 {
   __shared__ uint32_t scol[ILU_MAXROW];
   __shared__ T sa[ILU_MAXROW * 16];
+  __shared__ T sq[16];
   const uint32_t lane = threadIdx.x;
+  const uint32_t e = lane & 15, er = e >> 2, ec = e & 3, grp = lane >> 4;
   bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
@@ -63,21 +86,33 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
     for (uint32_t k = lane; k < L; k += 64) scol[k] = d.col[j0 + k];
     for (uint32_t k = lane; k < 16 * L; k += 64) sa[k] = a[16 * (size_t)j0 + k];
     __syncthreads();
-    for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err, aborted);
+    for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err, aborted, d.spin_sleep);
     aborted = __any(aborted);
     consume_fence();
     __syncthreads();
 
-    const uint32_t e = lane & 15, er = e >> 2, ec = e & 3, grp = lane >> 4;
     for (uint32_t p = 0; p < nlow; p++) {
       const uint32_t m = scol[p];
+      const T *dm = a + 16 * (size_t)d.diag[m];
+      if (EXACT) {
+        /* a(i,m) = (a - L_old U(m,m)) + L_old U(m,m) */
+        const T *lo = p == 0 ? a + 16 * ((size_t)j0 + nlow - 1) : lold + 16 * ((size_t)j0 + p);
+        if (lane < 16) sq[e] = lo[e];
+        __syncthreads();
+        T dq = T(0);
+        if (lane < 16) dq = blk_mul(sq, ud + 16 * (size_t)m, er, ec);
+        __syncthreads();
+        if (lane < 16) {
+          T v = sa[16 * p + e];
+          v = T(-1) * dq + v;
+          v = T(1) * dq + v;
+          sa[16 * p + e] = v;
+        }
+        __syncthreads();
+      }
       /* L(i,m) = a(i,m) inv(U(m,m)) */
       T lv = T(0);
-      if (lane < 16) {
-        const T *dm = a + 16 * (size_t)d.diag[m];
-#pragma unroll
-        for (int k = 0; k < 4; k++) lv += sa[16 * p + 4 * er + k] * dm[4 * k + ec];
-      }
+      if (lane < 16) lv = blk_mul(sa + 16 * p, dm, er, ec);
       __syncthreads();
       if (lane < 16) sa[16 * p + e] = lv;
       __syncthreads();
@@ -92,14 +127,27 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
           else hi = mid;
         }
         if (lo < L && scol[lo] == kc) {
-          const T *um = a + 16 * (size_t)jm;
-          T s = T(0);
-#pragma unroll
-          for (int k = 0; k < 4; k++) s += sa[16 * p + 4 * er + k] * um[4 * k + ec];
+          const T s = blk_mul(sa + 16 * p, a + 16 * (size_t)jm, er, ec);
           sa[16 * lo + e] = sa[16 * lo + e] - s;
         }
       }
       __syncthreads();
+    }
+
+    if (EXACT) {
+      /* upper blocks: a = (a - I A(i,k)) + I A(i,k), A = the fill's block */
+      for (uint32_t p = pd + grp; p < L; p += 4) {
+        const T *a0 = a + 16 * ((size_t)j0 + p);
+        T dq = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) dq += (er == (uint32_t)k ? T(1) : T(0)) * a0[4 * k + ec];
+        T v = sa[16 * p + e];
+        v = T(-1) * dq + v;
+        v = T(1) * dq + v;
+        sa[16 * p + e] = v;
+      }
+      __syncthreads();
+      if (lane < 16) ud[16 * (size_t)i + e] = sa[16 * pd + e];
     }
 
     /* diagonal: cofactor inverse, one lane per entry (lanes 16..63 mirror
@@ -127,26 +175,27 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
     }
     __syncthreads();
     for (uint32_t k = lane; k < 16 * L; k += 64) a[16 * (size_t)j0 + k] = sa[k];
+    if (EXACT)
+      for (uint32_t k = lane; k < 16 * nlow; k += 64) lold[16 * (size_t)j0 + k] = sa[k];
     publish_fence();
     __syncthreads();
     if (lane == 0) flag_store(rdone + i, epoch);
   }
 }
 
-static uint32_t persistent_waves(uint32_t work)
+static uint32_t persistent_waves(uint32_t work, uint32_t cap)
 {
-  const uint32_t cap = 2048;
   return work < cap ? (work ? work : 1) : cap;
 }
 
-template <typename T>
-static void launch_ilu(kf_ctx *c, T *a)
+template <typename T, bool EXACT>
+static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
 {
-  const uint32_t nw = persistent_waves(c->d.N);
+  const uint32_t nw = persistent_waves(c->d.N, c->ilu_waves);
   const uint32_t epoch = ++c->epoch;
   const int s = kf_prof_begin(c, KP_ILU);
-  hipLaunchKernelGGL(k_ilu<T>, dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
-                     c->ticket_base[2], c->rdone, epoch, a);
+  hipLaunchKernelGGL((k_ilu<T, EXACT>), dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
+                     c->ticket_base[2], c->rdone, epoch, a, lold, ud);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   c->ticket_base[2] += c->d.N + nw;
@@ -170,9 +219,18 @@ void kf_launch_ilu(kf_ctx *c)
     abort();
   }
   if (c->opt.prec == KF_PREC_FP64) {
-    launch_ilu<double>(c, c->a64);
+    if (c->opt.ilu_exact) {
+      if (!c->lold) {
+        KF_CHECK(hipMalloc((void **)&c->lold, 16 * c->nnzb * sizeof(double)));
+        KF_CHECK(hipMemset(c->lold, 0, 16 * c->nnzb * sizeof(double)));
+        KF_CHECK(hipMalloc((void **)&c->ud, 16 * (size_t)c->d.N * sizeof(double)));
+      }
+      launch_ilu<double, true>(c, c->a64, c->lold, c->ud);
+    } else {
+      launch_ilu<double, false>(c, c->a64, NULL, NULL);
+    }
   } else {
-    launch_ilu<float>(c, c->a32);
+    launch_ilu<float, false>(c, c->a32, NULL, NULL);
     if (c->opt.prec == KF_PREC_BF16) {
       const size_t n = 16 * c->nnzb;
       if (!c->abf) KF_CHECK(hipMalloc((void **)&c->abf, n * sizeof(uint16_t)));

@@ -51,6 +51,7 @@
 /* everything a kernel reads, passed by value */
 struct kf_dv {
   uint32_t N, E, S, NS, NF, nslice;
+  uint32_t spin_sleep;          /* s_sleep(1) repeats between two polls */
   double U, V;
   uint32_t *n0, *n1;
   double *enx, *eny, *enz, *ea;
@@ -91,6 +92,7 @@ struct kf_ctx {
   kf_options opt;
   size_t nnzb;
   uint32_t maxrow;          /* most blocks in one BSR row */
+  uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch */
   uint32_t nlev_l, nlev_u;
   /* level schedule */
   uint32_t *lrows, *urows, *lgrp, *ugrp;
@@ -103,6 +105,8 @@ struct kf_ctx {
   double *a64;
   float *a32;
   uint16_t *abf;
+  double *lold;             /* previous sweep's L blocks (exact ILU) */
+  double *ud;               /* non-inverted U(i,i) (exact ILU) */
   /* state */
   double *q, *cdt;
   double *hscal;            /* pinned host mirror of d.scal */

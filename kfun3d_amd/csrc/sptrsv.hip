@@ -34,7 +34,7 @@ __global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__re
     const uint32_t r = active ? rows[g0 + qd] : 0;
     const uint32_t j0 = active ? d.rowptr[r] : 0, jd = active ? d.diag[r] : 0;
     /* the quad's four lanes share the waits on the row's lower neighbours */
-    for (uint32_t j = j0 + c; j < jd; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted);
+    for (uint32_t j = j0 + c; j < jd; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted, d.spin_sleep);
     aborted = __any(aborted);
     consume_fence();
     if (active) {
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
     const bool active = g0 + qd < g1;
     const uint32_t r = active ? rows[g0 + qd] : 0;
     const uint32_t jd = active ? d.diag[r] : 0, j1 = active ? d.rowptr[r + 1] : 0;
-    for (uint32_t j = jd + 1 + c; j < j1; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted);
+    for (uint32_t j = jd + 1 + c; j < j1; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted, d.spin_sleep);
     aborted = __any(aborted);
     consume_fence();
     TV xr = TV(0);
@@ -108,8 +108,8 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
 template <typename TM, typename TV>
 static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *xv, double *x)
 {
-  const uint32_t nwl = c->nlgrp < 2048 ? (c->nlgrp ? c->nlgrp : 1) : 2048;
-  const uint32_t nwu = c->nugrp < 2048 ? (c->nugrp ? c->nugrp : 1) : 2048;
+  const uint32_t nwl = c->nlgrp < c->trsv_waves ? (c->nlgrp ? c->nlgrp : 1) : c->trsv_waves;
+  const uint32_t nwu = c->nugrp < c->trsv_waves ? (c->nugrp ? c->nugrp : 1) : c->trsv_waves;
   const uint32_t el = ++c->epoch;
   int s = kf_prof_begin(c, KP_TRSV_L);
   hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64), 0, c->stream, c->d, c->lrows, c->lgrp, c->nlgrp,

@@ -49,7 +49,7 @@ __device__ __forceinline__ void consume_fence() { __builtin_amdgcn_fence(__ATOMI
 
 /* wait until rdone[row] == epoch; false if this wave gave up */
 __device__ __forceinline__ bool wait_row(const uint32_t *rdone, uint32_t row, uint32_t epoch, uint32_t *err,
-                                         bool &aborted)
+                                         bool &aborted, uint32_t sleep)
 {
   if (aborted) return false;
   uint32_t n = 0;
@@ -59,7 +59,7 @@ __device__ __forceinline__ bool wait_row(const uint32_t *rdone, uint32_t row, ui
       aborted = true;
       return false;
     }
-    __builtin_amdgcn_s_sleep(2);
+    for (uint32_t z = 0; z < sleep; z++) __builtin_amdgcn_s_sleep(1);
   }
   return true;
 }

@@ -20,7 +20,7 @@ class OrOpts(C.Structure):
     _fields_ = [("alpha_deg", C.c_double), ("cfl_init", C.c_double), ("cfl_max", C.c_double),
                 ("fnorm_ratio", C.c_double), ("gmres_rtol", C.c_double), ("ts_max", C.c_int),
                 ("gmres_restart", C.c_int), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
-                ("ilu_textbook", C.c_int)]
+                ("ilu_textbook", C.c_int), ("reduce_order", C.c_int)]
 
 
 class OrStep(C.Structure):
@@ -69,7 +69,7 @@ def lib():
 def reference_options(**kw):
     """The reference's compile-time constants (inc/phy.h, src/kernel.c)."""
     o = dict(alpha_deg=3.0, cfl_init=50.0, cfl_max=1e5, fnorm_ratio=1e7, gmres_rtol=0.01, ts_max=15,
-             gmres_restart=30, gmres_max_it=10000, first_order=0, ilu_textbook=0)
+             gmres_restart=30, gmres_max_it=10000, first_order=0, ilu_textbook=0, reduce_order=0)
     o.update(kw)
     return o
 

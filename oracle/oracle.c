@@ -901,6 +901,47 @@ double or_norm(size_t n, const double *v)
   return sqrt(s);
 }
 
+/* the HIP path's reduction order: 1024 blocks x 256 threads, each thread a
+ * strided sum, an LDS tree per block, then a 256-thread strided sum of the
+ * 1024 partials and one more tree (a check of the device, not the reference) */
+static double tree256(double *sh)
+{
+  for (int s = 128; s > 0; s >>= 1)
+    for (int t = 0; t < s; t++) sh[t] = sh[t] + sh[t + s];
+  return sh[0];
+}
+
+static double dot_blocked(size_t n, const double *x, const double *y)
+{
+  static double part[1024];
+  double sh[256];
+  const size_t stride = 1024 * 256;
+  for (int b = 0; b < 1024; b++) {
+    for (int t = 0; t < 256; t++) {
+      double acc = 0.0;
+      for (size_t i = (size_t)b * 256 + t; i < n; i += stride) acc += x[i] * y[i];
+      sh[t] = acc;
+    }
+    part[b] = tree256(sh);
+  }
+  for (int t = 0; t < 256; t++) {
+    double acc = 0.0;
+    for (int i = t; i < 1024; i += 256) acc += part[i];
+    sh[t] = acc;
+  }
+  return tree256(sh);
+}
+
+static double ctx_dot(const or_ctx *c, size_t n, const double *x, const double *y)
+{
+  if (c->o.reduce_order) return dot_blocked(n, x, y);
+  double s = 0.0;
+  for (size_t i = 0; i < n; i++) s += x[i] * y[i];
+  return s;
+}
+
+static double ctx_norm(const or_ctx *c, size_t n, const double *v) { return sqrt(ctx_dot(c, n, v, v)); }
+
 static void axpy_n(size_t n, double a, const double *x, double *y)
 {
   for (size_t i = 0; i < n; i++) { const double ax = a * x[i]; y[i] = ax + y[i]; }
@@ -911,9 +952,9 @@ static void newaxpy_n(size_t n, double a, const double *x, const double *y, doub
   for (size_t i = 0; i < n; i++) { const double ax = a * x[i]; w[i] = ax + y[i]; }
 }
 
-static double normalize_n(size_t n, double *x)
+static double normalize_n(const or_ctx *c, size_t n, double *x)
 {
-  const double nrm = or_norm(n, x);
+  const double nrm = ctx_norm(c, n, x);
   const double s = 1.0 / nrm;
   for (size_t i = 0; i < n; i++) x[i] *= s;
   return nrm;
@@ -925,7 +966,7 @@ void or_ijacvec(or_ctx *c, const double *x, const double *r, double norm, double
                 double *w, double *y)
 {
   const size_t sz = 4 * c->N;
-  const double xnorm = or_norm(sz, x);
+  const double xnorm = ctx_norm(c, sz, x);
   const double h = FD_EPS * norm;
   newaxpy_n(sz, (h / xnorm), x, c->q, w);
   or_flux(c, w, gx, gy, gz, y);
@@ -953,7 +994,7 @@ int or_kernel(or_ctx *c, or_step *rec, int max_rec)
   int its = 0, nrec = 0;
   while (its < c->o.ts_max) {
     or_flux(c, c->q, c->grad[0], c->grad[1], c->grad[2], r);
-    const double fnorm = or_norm(sz, r);
+    const double fnorm = ctx_norm(c, sz, r);
     if (its == 0) fnorm_init = fnorm;
     else {
       double cfl0 = 1.1 * c->o.cfl_init * fnorm_init;
@@ -962,7 +1003,7 @@ int or_kernel(or_ctx *c, or_step *rec, int max_rec)
     }
     or_timestep(c, cfl);
     or_fill(c);
-    const double qnorm = sqrt(1.0 + or_norm(sz, c->q));
+    const double qnorm = sqrt(1.0 + ctx_norm(c, sz, c->q));
     memset(x, 0, sz * sizeof(double));
     or_ilu(c);
     or_sptrsv(c, r, vecs[0]);
@@ -974,7 +1015,7 @@ int or_kernel(or_ctx *c, or_step *rec, int max_rec)
         newaxpy_n(sz, -1.0, t, r, w);
         or_sptrsv(c, w, vecs[0]);
       }
-      rs[0] = normalize_n(sz, vecs[0]);
+      rs[0] = normalize_n(c, sz, vecs[0]);
       tol = (total == 0) ? c->o.gmres_rtol * rs[0] : tol;
       int it = 0;
       while (it < restart) {
@@ -982,12 +1023,10 @@ int or_kernel(or_ctx *c, or_step *rec, int max_rec)
         or_sptrsv(c, t, vecs[1 + it]);
         double *hh = ghh + (size_t)it * (nb + 1);
         for (int v = 0; v < it + 1; v++) {
-          double dot = 0.0;
-          for (size_t i = 0; i < sz; i++) dot += vecs[it + 1][i] * vecs[v][i];
-          hh[v] = dot;
+          hh[v] = ctx_dot(c, sz, vecs[it + 1], vecs[v]);
           axpy_n(sz, -hh[v], vecs[v], vecs[it + 1]);
         }
-        hh[it + 1] = normalize_n(sz, vecs[1 + it]);
+        hh[it + 1] = normalize_n(c, sz, vecs[1 + it]);
         for (int i = 0; i < it; i++) {
           const double tt = *hh;
           *hh = cc[i] * *hh + ss[i] * *(hh + 1);

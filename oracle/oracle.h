@@ -49,6 +49,8 @@ typedef struct or_opts {
   double alpha_deg, cfl_init, cfl_max, fnorm_ratio, gmres_rtol;
   int ts_max, gmres_restart, gmres_max_it, first_order;
   int ilu_textbook;     /* 0: one-thread ilu_Edmond with its copy quirks */
+  int reduce_order;     /* 0: one-thread sums (reference); 1: the HIP path's
+                           fixed blocked tree (1024 x 256 strided, two trees) */
 } or_opts;
 
 typedef struct or_step {

@@ -246,8 +246,12 @@ int main(int argc, char **argv)
     t0 = now();
     for (int k = 0; k < reps; k++) ComputeSparseTriangularSolve(&g, r, x);
     double t_trsv = (now() - t0) / reps;
-    printf("{\"threads\": %u, \"flux_s\": %.9g, \"fill_s\": %.9g, \"ilu_s\": %.9g, \"sptrsv_s\": %.9g, \"reps\": %d}\n", T,
-           t_flux, t_fill, t_ilu, t_trsv, reps);
+    /* one BLAS-1 pass over a state-length vector: ComputeAXPY (src/utils.c:69-87) */
+    t0 = now();
+    for (int k = 0; k < 10 * reps; k++) ComputeAXPY(sz, 1e-3, r, x);
+    double t_blas = (now() - t0) / (10 * reps);
+    printf("{\"threads\": %u, \"flux_s\": %.9g, \"fill_s\": %.9g, \"ilu_s\": %.9g, \"sptrsv_s\": %.9g, "
+           "\"blas_pass_s\": %.9g, \"reps\": %d}\n", T, t_flux, t_fill, t_ilu, t_trsv, t_blas, reps);
     return 0;
   }
   g_out = argv[3];
