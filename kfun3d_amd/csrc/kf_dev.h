@@ -98,6 +98,14 @@ struct kf_ctx {
   uint32_t *lrows, *urows, *lgrp, *ugrp;
   uint32_t nlgrp, nugrp;
   uint32_t *rdone;          /* per-row completion epoch */
+  /* triangular solves by contiguous row ranges of KF_GROUP rows */
+  uint8_t *rlvl_l, *rlvl_u; /* local level of each row inside its range */
+  uint8_t *rdep_l, *rdep_u; /* local depth of each range */
+  uint32_t *rord_l, *rord_u; /* ranges in ticket order */
+  uint32_t nrange;
+  double *ty64;             /* forward-sweep output (FP64) */
+  uint64_t tcall;           /* triangular solves so far (sentinel numbering) */
+  uint64_t ystate[2];       /* call whose forward sentinel y holds (FP64, FP32) */
   uint32_t *ticket;         /* monotonically increasing work tickets */
   uint32_t ticket_base[4];
   uint32_t epoch;
@@ -113,7 +121,7 @@ struct kf_ctx {
   /* Kernel() work vectors */
   double *gx, *gy, *gz, *r, *w, *t, *xk, *vecs;
   int nvecs;
-  float *f32a, *f32b;       /* single-precision triangular-solve buffers */
+  float *f32a, *f32b;       /* single-precision forward / backward outputs */
 };
 
 static inline kf_ctx *kf_ctx_of(const GEOMETRY *g)

@@ -1,139 +1,395 @@
 /*
  * sptrsv.hip -- x = U^-1 L^-1 b with the in-place ILU(0) factors
- * (matrix/sptrsv.cpp:34-73, level scheduling of matrix/init.cpp:194-274).
+ * (matrix/sptrsv.cpp:34-73; the reference schedules rows by level sets,
+ * matrix/init.cpp:194-274).
  *
- * A quad of lanes owns one block row (lane c computes component c); a
- * wavefront owns up to 16 rows of one level, handed out by ticket in level
- * order (sync.h).  Per row, the block products are summed in ascending column
- * order exactly as the reference:
+ * Work unit: a contiguous range of 16 rows -- [16 g, 16 g + 16) for the
+ * forward sweep, the 16 rows ending at N - 1 - 16 g (taken in decreasing
+ * order) for the backward sweep -- owned by one wavefront, a quad of lanes
+ * per row (lane c computes component c).  A row depends only on rows of its
+ * own range or of ranges with a smaller index.  Ranges are handed out by
+ * ticket in level order of that range DAG (host: range_order), so a wave
+ * only ever waits for ranges whose tickets were issued before its own, to
+ * waves that are running: no wait can deadlock.
+ *
+ * Hand-off without flags or fences.  Every result slot is preset to a
+ * signalling NaN that names the sweep and the call (the sentinel); a
+ * producer stores its results with agent-scope atomic stores, a consumer
+ * loads the values it needs with agent-scope atomic loads and retries the
+ * ones that still hold the sentinel.  Only single-location coherence is
+ * relied on -- the value read is the value used -- so no release/acquire
+ * (L2 write-back and invalidate on the 8 XCDs) is needed, and the data
+ * arrives in the same round trip that signals it.  Arithmetic never yields a
+ * signalling NaN and every stored result is canonicalised, so a result never
+ * equals a sentinel.  Sentinels are set up for free: the forward waves preset
+ * the backward output of their rows, the backward waves reset the forward
+ * output of their rows for the next call.
+ *
+ *   external dependences (rows of earlier ranges): polled as above;
+ *   internal dependences (rows of the same range): resolved inside the wave
+ *     through LDS, row by row in the local level order computed on the host
+ *     (rlvl / rdep).
+ *
+ * Columns and factor blocks of the first KF_TRSV_PF off-diagonal entries of
+ * every row are loaded into registers as soon as the wave takes its ticket.
+ *
+ * Per row the block products are summed in ascending column order exactly
+ * as the reference:
  *   forward   x_r = b_r;  x_r -= A(r,j) x_j  for j < r           (dgemv, then axpy -1)
  *   backward  x_r -= A(r,j) x_j  for j > r;  x_r = inv(D_r) x_r
- * so the FP64 result is bit-identical.  In the single-precision modes the
- * factors are FP32 or BF16 (converted to FP32 on load), the vectors FP32,
- * and the result is widened to FP64 on the last store.
+ * Columns of earlier forward ranges are smaller than those of the row's own
+ * range, columns of earlier backward ranges larger, so the forward sweep sums
+ * its external entries first and the backward sweep last, and the FP64
+ * result is bit-identical.  In the single-precision modes the factors are
+ * FP32 or BF16 (converted to FP32 on load), the vectors FP32, and the result
+ * is widened to FP64 on the last store.
+ *
+ * Every wait is bounded (KF_SPIN_LIMIT polls): a lane that gives up records
+ * KF_ERR_SPIN_LIMIT and stops waiting for the rest of the launch.
  */
 #include "sync.h"
+
+static_assert(KF_GROUP == 16, "one quad of lanes per row of a 16-row range");
+
+#define KF_TRSV_PF 8   /* off-diagonal entries per row held in registers */
 
 __device__ __forceinline__ double mat_ld(const double *a, size_t i) { return a[i]; }
 __device__ __forceinline__ float mat_ld(const float *a, size_t i) { return a[i]; }
 __device__ __forceinline__ float mat_ld(const uint16_t *a, size_t i) { return __uint_as_float((uint32_t)a[i] << 16); }
 
-template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ rows,
-                                                   const uint32_t *__restrict__ grp, uint32_t ngrp, uint32_t *ticket,
-                                                   uint32_t base, uint32_t *rdone, uint32_t epoch,
-                                                   const TM *__restrict__ a, const double *__restrict__ b, TV *xv)
+/* sentinels: signalling NaNs (quiet bit clear, top payload bit set so the
+ * payload is never zero) carrying the low bits of the sweep number k */
+template <typename TV> struct sentinel;
+template <> struct sentinel<double> {
+  typedef unsigned long long bits;
+  static __host__ __device__ bits of(uint64_t k) { return 0x7ff4000000000000ull | (k & 0x0003ffffffffffffull); }
+  static __device__ bits raw(double v) { return (bits)__double_as_longlong(v); }
+  static __device__ double val(bits b) { return __longlong_as_double((long long)b); }
+};
+template <> struct sentinel<float> {
+  typedef unsigned int bits;
+  static __host__ __device__ bits of(uint64_t k) { return 0x7fa00000u | (uint32_t)(k & 0x1fffffu); }
+  static __device__ bits raw(float v) { return __float_as_uint(v); }
+  static __device__ float val(bits b) { return __uint_as_float(b); }
+};
+
+template <typename TV>
+__device__ __forceinline__ TV ld_coherent(const TV *p)
 {
+  return __hip_atomic_load(const_cast<TV *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename TV>
+__device__ __forceinline__ void st_coherent(TV *p, TV v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* s = A(j) row c . x  (dgemv row; summed from zero in column order) */
+template <typename TV>
+__device__ __forceinline__ TV row_dot(const TV m[4], TV x0, TV x1, TV x2, TV x3)
+{
+  TV s = TV(0);
+  s += m[0] * x0;
+  s += m[1] * x1;
+  s += m[2] * x2;
+  s += m[3] * x3;
+  return s;
+}
+
+/* value of lane I of this lane's quad (DPP quad_perm broadcast; the quad's
+ * four lanes are active together) */
+template <int I>
+__device__ __forceinline__ float quad_get(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), I * 0x55, 0xf, 0xf, false));
+}
+
+template <int I>
+__device__ __forceinline__ double quad_get(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), I * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), I * 0x55, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <typename TM, typename TV>
+__device__ __forceinline__ void ld_row(const TM *__restrict__ a, uint32_t j, uint32_t c, TV m[4])
+{
+  const size_t o = 16 * (size_t)j + 4 * c;
+  m[0] = (TV)mat_ld(a, o);
+  m[1] = (TV)mat_ld(a, o + 1);
+  m[2] = (TV)mat_ld(a, o + 2);
+  m[3] = (TV)mat_ld(a, o + 3);
+}
+
+/* the four components of row `col` of a vector being produced in this
+ * launch, once none of them holds the sentinel; false if the lane gave up */
+template <typename TV>
+__device__ __forceinline__ bool wait_vec(const TV *v, uint32_t col, typename sentinel<TV>::bits s, TV x[4],
+                                         uint32_t *err, uint32_t sleep)
+{
+  const TV *p = v + 4 * (size_t)col;
+  for (uint32_t n = 0;; n++) {
+    x[0] = ld_coherent(p); x[1] = ld_coherent(p + 1); x[2] = ld_coherent(p + 2); x[3] = ld_coherent(p + 3);
+    if (sentinel<TV>::raw(x[0]) != s && sentinel<TV>::raw(x[1]) != s && sentinel<TV>::raw(x[2]) != s &&
+        sentinel<TV>::raw(x[3]) != s)
+      return true;
+    if (n >= KF_SPIN_LIMIT) {
+      atomicOr(err, KF_ERR_SPIN_LIMIT);
+      return false;
+    }
+    for (uint32_t z = 0; z < sleep; z++) __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+/* Loads x of the prefetched entries j0 + k in [jlo, jhi) into xe, retrying
+ * those that still hold the sentinel; all loads of one round are in flight
+ * together.  Returns false if the lane gave up. */
+template <typename TV>
+__device__ __forceinline__ bool wait_prefetched(const TV *v, const uint32_t cp[KF_TRSV_PF], uint32_t j0,
+                                                uint32_t jlo, uint32_t jhi, typename sentinel<TV>::bits s,
+                                                TV xe[KF_TRSV_PF][4], uint32_t *err, uint32_t sleep)
+{
+  uint32_t pend = 0;
+#pragma unroll
+  for (int k = 0; k < KF_TRSV_PF; k++)
+    if (j0 + k >= jlo && j0 + k < jhi) pend |= 1u << k;
+  for (uint32_t n = 0; pend; n++) {
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++)
+      if (pend & (1u << k)) {
+        const TV *p = v + 4 * (size_t)cp[k];
+        xe[k][0] = ld_coherent(p); xe[k][1] = ld_coherent(p + 1);
+        xe[k][2] = ld_coherent(p + 2); xe[k][3] = ld_coherent(p + 3);
+      }
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++)
+      if ((pend & (1u << k)) && sentinel<TV>::raw(xe[k][0]) != s && sentinel<TV>::raw(xe[k][1]) != s &&
+          sentinel<TV>::raw(xe[k][2]) != s && sentinel<TV>::raw(xe[k][3]) != s)
+        pend &= ~(1u << k);
+    if (!pend) break;
+    if (n >= KF_SPIN_LIMIT) {
+      atomicOr(err, KF_ERR_SPIN_LIMIT);
+      return false;
+    }
+    for (uint32_t z = 0; z < sleep; z++) __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+/* forward sweep: y = L^-1 b; presets the backward output x of its rows to
+ * the backward sentinel */
+template <typename TM, typename TV>
+__global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ ord,
+                                                   const uint8_t *__restrict__ lvl, const uint8_t *__restrict__ depth,
+                                                   uint32_t nrange, uint32_t *ticket, uint32_t base,
+                                                   typename sentinel<TV>::bits sy, typename sentinel<TV>::bits sx,
+                                                   const TM *__restrict__ a, const double *__restrict__ b, TV *y,
+                                                   TV *xnext)
+{
+  __shared__ TV xs[64];
   const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
   bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
-    if (t >= ngrp) break;
-    const uint32_t g0 = grp[t], g1 = grp[t + 1];
-    const bool active = g0 + qd < g1;
-    const uint32_t r = active ? rows[g0 + qd] : 0;
+    if (t >= nrange) break;
+    const uint32_t g = ord[t];
+    const uint32_t lo = KF_GROUP * g, r = lo + qd;
+    const bool active = r < d.N;
     const uint32_t j0 = active ? d.rowptr[r] : 0, jd = active ? d.diag[r] : 0;
-    /* the quad's four lanes share the waits on the row's lower neighbours */
-    for (uint32_t j = j0 + c; j < jd; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted, d.spin_sleep);
-    aborted = __any(aborted);
-    consume_fence();
-    if (active) {
-      TV xr = (TV)b[4 * (size_t)r + c];
-      for (uint32_t j = j0; j < jd; j++) {
-        const TV *xc = xv + 4 * (size_t)d.col[j];
-        const size_t o = 16 * (size_t)j + 4 * c;
-        TV s = TV(0);
-        s += (TV)mat_ld(a, o) * xc[0];
-        s += (TV)mat_ld(a, o + 1) * xc[1];
-        s += (TV)mat_ld(a, o + 2) * xc[2];
-        s += (TV)mat_ld(a, o + 3) * xc[3];
-        xr = xr - s;
-      }
-      xv[4 * (size_t)r + c] = xr;
+    const uint32_t my = active ? lvl[r] : 0xffu, dep = depth[g];
+    uint32_t cp[KF_TRSV_PF];
+    TV mp[KF_TRSV_PF][4];
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++) {
+      cp[k] = j0 + k < jd ? d.col[j0 + k] : 0xffffffffu;
+      if (j0 + k < jd) ld_row(a, j0 + k, c, mp[k]);
     }
-    publish_fence();
-    if (active && c == 0) flag_store(rdone + r, epoch);
+    TV xr = active ? (TV)b[4 * (size_t)r + c] : TV(0);
+    if (active) xnext[4 * (size_t)r + c] = sentinel<TV>::val(sx);
+    /* first entry inside the range */
+    uint32_t jx = j0;
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++)
+      if (cp[k] < lo) jx = j0 + k + 1;
+    if (jx == j0 + KF_TRSV_PF)
+      while (jx < jd && d.col[jx] < lo) jx++;
+    /* external entries, ascending */
+    TV xe[KF_TRSV_PF][4];
+    if (!aborted) aborted = !wait_prefetched(y, cp, j0, j0, jx, sy, xe, d.err, d.spin_sleep);
+    aborted = __any(aborted);   /* a wave gives up at most once per launch */
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++)
+      if (j0 + k < jx) xr = xr - row_dot(mp[k], xe[k][0], xe[k][1], xe[k][2], xe[k][3]);
+    for (uint32_t j = j0 + KF_TRSV_PF; j < jx; j++) {
+      TV m[4], xc[4];
+      ld_row(a, j, c, m);
+      if (!aborted) aborted = !wait_vec(y, d.col[j], sy, xc, d.err, d.spin_sleep);
+      xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
+    }
+    /* internal entries, level by level through LDS */
+    for (uint32_t l = 0; l < dep; l++) {
+      if (my == l) {
+#pragma unroll
+        for (int k = 0; k < KF_TRSV_PF; k++)
+          if (j0 + k >= jx && j0 + k < jd) {
+            const TV *xc = xs + 4 * (cp[k] - lo);
+            xr = xr - row_dot(mp[k], xc[0], xc[1], xc[2], xc[3]);
+          }
+        for (uint32_t j = jx > j0 + KF_TRSV_PF ? jx : j0 + KF_TRSV_PF; j < jd; j++) {
+          TV m[4];
+          ld_row(a, j, c, m);
+          const TV *xc = xs + 4 * (d.col[j] - lo);
+          xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
+        }
+        xr = __builtin_canonicalize(xr);
+        xs[lane] = xr;
+      }
+      __syncthreads();
+    }
+    if (active) st_coherent(y + 4 * (size_t)r + c, xr);
   }
 }
 
+/* backward sweep: x = U^-1 y (y from the forward sweep); resets y of its rows
+ * to the forward sentinel of the next call */
 template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ rows,
-                                                   const uint32_t *__restrict__ grp, uint32_t ngrp, uint32_t *ticket,
-                                                   uint32_t base, uint32_t *rdone, uint32_t epoch,
-                                                   const TM *__restrict__ a, TV *xv, double *__restrict__ xout)
+__global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ ord,
+                                                   const uint8_t *__restrict__ lvl, const uint8_t *__restrict__ depth,
+                                                   uint32_t nrange, uint32_t *ticket, uint32_t base,
+                                                   typename sentinel<TV>::bits sx, typename sentinel<TV>::bits synext,
+                                                   const TM *__restrict__ a, TV *y, TV *x, double *__restrict__ xout)
 {
+  __shared__ TV xs[64];
   const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
+  const uint32_t last = d.N - 1;
   bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
-    if (t >= ngrp) break;
-    const uint32_t g0 = grp[t], g1 = grp[t + 1];
-    const bool active = g0 + qd < g1;
-    const uint32_t r = active ? rows[g0 + qd] : 0;
-    const uint32_t jd = active ? d.diag[r] : 0, j1 = active ? d.rowptr[r + 1] : 0;
-    for (uint32_t j = jd + 1 + c; j < j1; j += 4) wait_row(rdone, d.col[j], epoch, d.err, aborted, d.spin_sleep);
+    if (t >= nrange) break;
+    const uint32_t g = ord[t];
+    const uint32_t hi = last - KF_GROUP * g;
+    const bool active = qd <= hi;
+    const uint32_t r = active ? hi - qd : 0;
+    const uint32_t jd = active ? d.diag[r] : 0, j0 = jd + 1, j1 = active ? d.rowptr[r + 1] : 0;
+    const uint32_t my = active ? lvl[r] : 0xffu, dep = depth[g];
+    uint32_t cp[KF_TRSV_PF];
+    TV mp[KF_TRSV_PF][4], dg[4];
+    if (active) ld_row(a, jd, c, dg);
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++) {
+      cp[k] = j0 + k < j1 ? d.col[j0 + k] : 0;
+      if (j0 + k < j1) ld_row(a, j0 + k, c, mp[k]);
+    }
+    TV x0 = TV(0);
+    if (active) {
+      x0 = y[4 * (size_t)r + c];
+      y[4 * (size_t)r + c] = sentinel<TV>::val(synext);
+    }
+    /* first entry beyond the range */
+    uint32_t jy = j0;
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++)
+      if (j0 + k < j1 && cp[k] <= hi) jy = j0 + k + 1;
+    if (jy == j0 + KF_TRSV_PF)
+      while (jy < j1 && d.col[jy] <= hi) jy++;
+    /* products of the prefetched external entries, ahead of the serial part */
+    TV xe[KF_TRSV_PF][4], se[KF_TRSV_PF];
+    if (!aborted) aborted = !wait_prefetched(x, cp, j0, jy, j1, sx, xe, d.err, d.spin_sleep);
     aborted = __any(aborted);
-    consume_fence();
-    TV xr = TV(0);
-    if (active) {
-      xr = xv[4 * (size_t)r + c];
-      for (uint32_t j = jd + 1; j < j1; j++) {
-        const TV *xc = xv + 4 * (size_t)d.col[j];
-        const size_t o = 16 * (size_t)j + 4 * c;
-        TV s = TV(0);
-        s += (TV)mat_ld(a, o) * xc[0];
-        s += (TV)mat_ld(a, o + 1) * xc[1];
-        s += (TV)mat_ld(a, o + 2) * xc[2];
-        s += (TV)mat_ld(a, o + 3) * xc[3];
-        xr = xr - s;
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++)
+      if (j0 + k >= jy && j0 + k < j1) se[k] = row_dot(mp[k], xe[k][0], xe[k][1], xe[k][2], xe[k][3]);
+    TV res = TV(0);
+    for (uint32_t l = 0; l < dep; l++) {
+      if (my == l) {   /* the quad's four lanes take this branch together */
+        TV xr = x0;
+#pragma unroll
+        for (int k = 0; k < KF_TRSV_PF; k++)
+          if (j0 + k < jy) {
+            const TV *xc = xs + 4 * (hi - cp[k]);
+            xr = xr - row_dot(mp[k], xc[0], xc[1], xc[2], xc[3]);
+          }
+        for (uint32_t j = j0 + KF_TRSV_PF; j < jy; j++) {
+          TV m[4];
+          ld_row(a, j, c, m);
+          const TV *xc = xs + 4 * (hi - d.col[j]);
+          xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
+        }
+#pragma unroll
+        for (int k = 0; k < KF_TRSV_PF; k++)
+          if (j0 + k >= jy && j0 + k < j1) xr = xr - se[k];
+        for (uint32_t j = jy > j0 + KF_TRSV_PF ? jy : j0 + KF_TRSV_PF; j < j1; j++) {
+          TV m[4], xc[4];
+          ld_row(a, j, c, m);
+          if (!aborted) aborted = !wait_vec(x, d.col[j], sx, xc, d.err, d.spin_sleep);
+          xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
+        }
+        /* x_r = inv(D_r) x_r: the quad exchanges its four components */
+        const TV y0 = quad_get<0>(xr), y1 = quad_get<1>(xr), y2 = quad_get<2>(xr), y3 = quad_get<3>(xr);
+        res = __builtin_canonicalize(row_dot(dg, y0, y1, y2, y3));
+        xs[lane] = res;
       }
+      __syncthreads();
     }
-    /* x_r = inv(D_r) x_r: the quad exchanges its four components */
-    const TV y0 = __shfl(xr, 0, 4), y1 = __shfl(xr, 1, 4), y2 = __shfl(xr, 2, 4), y3 = __shfl(xr, 3, 4);
     if (active) {
-      const size_t o = 16 * (size_t)jd + 4 * c;
-      TV s = TV(0);
-      s += (TV)mat_ld(a, o) * y0;
-      s += (TV)mat_ld(a, o + 1) * y1;
-      s += (TV)mat_ld(a, o + 2) * y2;
-      s += (TV)mat_ld(a, o + 3) * y3;
-      xv[4 * (size_t)r + c] = s;
-      if ((void *)xout != (void *)xv) xout[4 * (size_t)r + c] = (double)s;
+      st_coherent(x + 4 * (size_t)r + c, res);
+      if ((void *)xout != (void *)x) xout[4 * (size_t)r + c] = (double)res;
     }
-    publish_fence();
-    if (active && c == 0) flag_store(rdone + r, epoch);
   }
 }
 
-template <typename TM, typename TV>
-static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *xv, double *x)
+template <typename TV>
+__global__ void k_fill_bits(TV *v, size_t n, typename sentinel<TV>::bits s)
 {
-  const uint32_t nwl = c->nlgrp < c->trsv_waves ? (c->nlgrp ? c->nlgrp : 1) : c->trsv_waves;
-  const uint32_t nwu = c->nugrp < c->trsv_waves ? (c->nugrp ? c->nugrp : 1) : c->trsv_waves;
-  const uint32_t el = ++c->epoch;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = sentinel<TV>::val(s);
+}
+
+/* forward output y (held between calls, preset to the next call's forward
+ * sentinel by the previous backward sweep) and backward output x */
+template <typename TM, typename TV>
+static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslot, TV *x, double *xout)
+{
+  const uint32_t nr = c->nrange;
+  const uint32_t nw = nr < c->trsv_waves ? nr : c->trsv_waves;
+  const uint64_t k = c->tcall;
+  const size_t n = 4 * (size_t)c->d.N;
+  if (c->ystate[yslot] != k) {
+    hipLaunchKernelGGL((k_fill_bits<TV>), dim3(kf_blocks_for(n, 256)), dim3(256), 0, c->stream, y, n,
+                       sentinel<TV>::of(2 * k));
+    KF_LAUNCH_CHECK();
+  }
   int s = kf_prof_begin(c, KP_TRSV_L);
-  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64), 0, c->stream, c->d, c->lrows, c->lgrp, c->nlgrp,
-                     c->ticket + 0, c->ticket_base[0], c->rdone, el, a, b, xv);
+  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nw), dim3(64), 0, c->stream, c->d, c->rord_l, c->rlvl_l, c->rdep_l,
+                     nr, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k), sentinel<TV>::of(2 * k + 1), a, b,
+                     y, x);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  c->ticket_base[0] += c->nlgrp + nwl;
-  const uint32_t eu = ++c->epoch;
+  c->ticket_base[0] += nr + nw;
   s = kf_prof_begin(c, KP_TRSV_U);
-  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64), 0, c->stream, c->d, c->urows, c->ugrp, c->nugrp,
-                     c->ticket + 1, c->ticket_base[1], c->rdone, eu, a, xv, x);
+  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nw), dim3(64), 0, c->stream, c->d, c->rord_u, c->rlvl_u, c->rdep_u,
+                     nr, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1), sentinel<TV>::of(2 * k + 2),
+                     a, y, x, xout);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  c->ticket_base[1] += c->nugrp + nwu;
+  c->ticket_base[1] += nr + nw;
+  c->tcall = k + 1;
+  c->ystate[yslot] = k + 1;
 }
 
 void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x)
 {
   kf_check_aligned(b, "b"); kf_check_aligned(x, "x");
+  const size_t n = 4 * (size_t)c->d.N;
   if (c->opt.prec == KF_PREC_FP64) {
-    launch_trsv<double, double>(c, c->a64, b, x, x);
+    if (!c->ty64) KF_CHECK(hipMalloc((void **)&c->ty64, n * sizeof(double)));
+    launch_trsv<double, double>(c, c->a64, b, c->ty64, 0, x, x);
     return;
   }
-  if (!c->f32a) KF_CHECK(hipMalloc((void **)&c->f32a, 4 * (size_t)c->d.N * sizeof(float)));
-  if (c->opt.prec == KF_PREC_FP32) launch_trsv<float, float>(c, c->a32, b, c->f32a, x);
-  else launch_trsv<uint16_t, float>(c, c->abf, b, c->f32a, x);
+  if (!c->f32a) KF_CHECK(hipMalloc((void **)&c->f32a, n * sizeof(float)));
+  if (!c->f32b) KF_CHECK(hipMalloc((void **)&c->f32b, n * sizeof(float)));
+  if (c->opt.prec == KF_PREC_FP32) launch_trsv<float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x);
+  else launch_trsv<uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x);
 }
