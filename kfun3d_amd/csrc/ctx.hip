@@ -212,6 +212,13 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   d.diag = dupload(p->diag, N);
   for (size_t i = 0; i < N; i++)
     if (p->rowptr[i + 1] - p->rowptr[i] > c->maxrow) c->maxrow = p->rowptr[i + 1] - p->rowptr[i];
+  for (size_t i = 0; i < N; i++) {
+    uint32_t u = 0;
+    const uint32_t nlow = p->diag[i] - p->rowptr[i];
+    for (uint32_t j = p->rowptr[i]; j < p->diag[i]; j++) u += p->rowptr[p->col[j] + 1] - p->diag[p->col[j]] - 1;
+    if (u > c->ilu_umax) c->ilu_umax = u;
+    if (3 * nlow + u > c->ilu_smax) c->ilu_smax = 3 * nlow + u;
+  }
   c->nlev_l = p->nlev_l;
   c->nlev_u = p->nlev_u;
   c->lrows = dupload(p->lrows, N);

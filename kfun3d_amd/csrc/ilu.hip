@@ -183,6 +183,217 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
   }
 }
 
+/* Shared-memory layout of k_ilu_staged, sized on the host from the mesh:
+ * maxrow blocks per row, umax = most upper entries over the rows m of one
+ * row's lower pattern, smax = most staged blocks of one row. */
+struct ilu_layout {
+  uint32_t scol, sa, sdg, soff, sjm, slo, sslot, smjm, stage, bytes;
+  __host__ __device__ ilu_layout(uint32_t maxrow, uint32_t umax, uint32_t smax, uint32_t tsize)
+  {
+    uint32_t o = 0;
+    auto take = [&o](uint32_t n) { const uint32_t r = o; o += (n + 15u) & ~15u; return r; };
+    scol = take(4 * maxrow);
+    sa = take(16 * tsize * maxrow);
+    sdg = take(4 * maxrow);
+    soff = take(4 * (maxrow + 1));
+    sjm = take(4 * umax);
+    slo = take(umax);
+    sslot = take(2 * umax);
+    smjm = take(4 * smax);
+    stage = take(16 * tsize * smax);
+    bytes = o;
+  }
+};
+
+/* k_ilu with every block the row reads from other rows staged in LDS by one
+ * round of loads after the wait, instead of fetching them row by row of the
+ * lower pattern.  Before the wait (read-only pattern data and the row's own
+ * blocks only): for each lower block m the upper entries (m, k) of row m and
+ * the position of column k in row i (or none); the L_old blocks.  After the
+ * acquire: inv(U(m,m)), U(m,m) and the matching U(m,k) blocks, all loads in
+ * flight together.  Arithmetic and its order are those of k_ilu. */
+template <typename T, bool EXACT>
+__global__ __launch_bounds__(64) void k_ilu_staged(kf_dv d, const uint32_t *__restrict__ rows, uint32_t nrows,
+                                                   uint32_t *ticket, uint32_t base, uint32_t *rdone, uint32_t epoch,
+                                                   T *a, T *lold, T *ud, uint32_t maxrow, uint32_t umax,
+                                                   uint32_t smax)
+{
+  extern __shared__ __align__(16) unsigned char ilu_lds[];
+  const ilu_layout lay(maxrow, umax, smax, sizeof(T));
+  uint32_t *scol = (uint32_t *)(ilu_lds + lay.scol);
+  T *sa = (T *)(ilu_lds + lay.sa);
+  uint32_t *sdg = (uint32_t *)(ilu_lds + lay.sdg);
+  uint32_t *soff = (uint32_t *)(ilu_lds + lay.soff);
+  uint32_t *sjm = (uint32_t *)(ilu_lds + lay.sjm);
+  uint8_t *slo = (uint8_t *)(ilu_lds + lay.slo);
+  uint16_t *sslot = (uint16_t *)(ilu_lds + lay.sslot);
+  uint32_t *smjm = (uint32_t *)(ilu_lds + lay.smjm);
+  T *stage = (T *)(ilu_lds + lay.stage);
+  const uint32_t lane = threadIdx.x;
+  const uint32_t e = lane & 15, er = e >> 2, ec = e & 3, grp = lane >> 4;
+  bool aborted = false;
+  for (;;) {
+    const uint32_t t = next_ticket(ticket, base);
+    if (t >= nrows) break;
+    const uint32_t i = rows[t];
+    const uint32_t j0 = d.rowptr[i], j1 = d.rowptr[i + 1], jd = d.diag[i];
+    const uint32_t L = j1 - j0, nlow = jd - j0, pd = jd - j0;
+    /* staged slots: [0, nlow) inv(U(m,m)); EXACT: [nlow, 2 nlow) U(m,m),
+     * [2 nlow, 3 nlow) L_old; then the matching U(m,k) */
+    const uint32_t mbase = EXACT ? 3 * nlow : nlow;
+    for (uint32_t k = lane; k < L; k += 64) scol[k] = d.col[j0 + k];
+    for (uint32_t k = lane; k < 16 * L; k += 64) sa[k] = a[16 * (size_t)j0 + k];
+    __syncthreads();
+    for (uint32_t p = lane; p < nlow; p += 64) {
+      const uint32_t m = scol[p], dg = d.diag[m];
+      sdg[p] = dg;
+      soff[p + 1] = d.rowptr[m + 1] - dg - 1;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      soff[0] = 0;
+      for (uint32_t p = 0; p < nlow; p++) soff[p + 1] += soff[p];
+    }
+    __syncthreads();
+    const uint32_t U = soff[nlow];
+    uint32_t nmatch = 0;
+    for (uint32_t b0 = 0; b0 < U; b0 += 64) {
+      const uint32_t idx = b0 + lane;
+      bool hit = false;
+      uint32_t lo = 0, jm = 0;
+      if (idx < U) {
+        uint32_t pl = 0, ph = nlow - 1;   /* last p with soff[p] <= idx */
+        while (pl < ph) {
+          const uint32_t mid = (pl + ph + 1) >> 1;
+          if (soff[mid] <= idx) pl = mid;
+          else ph = mid - 1;
+        }
+        jm = sdg[pl] + 1 + (idx - soff[pl]);
+        const uint32_t kc = d.col[jm];
+        uint32_t l0 = 0, l1 = L;
+        while (l0 < l1) {
+          const uint32_t mid = (l0 + l1) >> 1;
+          if (scol[mid] < kc) l0 = mid + 1;
+          else l1 = mid;
+        }
+        lo = l0;
+        hit = l0 < L && scol[l0] == kc;
+        sjm[idx] = jm;
+      }
+      const uint64_t mask = __ballot(hit);
+      const uint32_t slot = mbase + nmatch + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      if (idx < U) {
+        slo[idx] = hit ? (uint8_t)lo : (uint8_t)0xff;
+        sslot[idx] = hit ? (uint16_t)slot : (uint16_t)0xffff;
+        if (hit) smjm[slot] = jm;
+      }
+      nmatch += (uint32_t)__popcll(mask);
+    }
+    const uint32_t nst = mbase + nmatch;
+    if (EXACT)
+      for (uint32_t k = lane; k < 16 * nlow; k += 64) {
+        const uint32_t p = k >> 4, ee = k & 15;
+        stage[16 * (2 * nlow + p) + ee] =
+            p == 0 ? a[16 * ((size_t)j0 + nlow - 1) + ee] : lold[16 * ((size_t)j0 + p) + ee];
+      }
+    __syncthreads();
+    for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err, aborted, d.spin_sleep);
+    aborted = __any(aborted);
+    consume_fence();
+    /* one round of loads: slots other than L_old */
+    {
+      const uint32_t npost = (EXACT ? 2 * nlow : nlow) + nmatch;
+#pragma unroll 4
+      for (uint32_t k = lane; k < 16 * npost; k += 64) {
+        uint32_t sl = k >> 4;
+        const uint32_t ee = k & 15;
+        if (EXACT && sl >= 2 * nlow) sl += nlow;   /* skip the L_old slots */
+        const T *src;
+        if (sl < nlow) src = a + 16 * (size_t)sdg[sl];
+        else if (EXACT && sl < 2 * nlow) src = ud + 16 * (size_t)scol[sl - nlow];
+        else src = a + 16 * (size_t)smjm[sl];
+        stage[16 * sl + ee] = src[ee];
+      }
+    }
+    __syncthreads();
+
+    for (uint32_t p = 0; p < nlow; p++) {
+      if (EXACT) {
+        /* a(i,m) = (a - L_old U(m,m)) + L_old U(m,m) */
+        if (lane < 16) {
+          const T dq = blk_mul(stage + 16 * (2 * nlow + p), stage + 16 * (nlow + p), er, ec);
+          T v = sa[16 * p + e];
+          v = T(-1) * dq + v;
+          v = T(1) * dq + v;
+          sa[16 * p + e] = v;
+        }
+        __syncthreads();
+      }
+      /* L(i,m) = a(i,m) inv(U(m,m)) */
+      T lv = T(0);
+      if (lane < 16) lv = blk_mul(sa + 16 * p, stage + 16 * p, er, ec);
+      __syncthreads();
+      if (lane < 16) sa[16 * p + e] = lv;
+      __syncthreads();
+      /* a(i,k) -= L(i,m) U(m,k) for the upper blocks (m,k) of row m in row i */
+      for (uint32_t idx = soff[p] + grp; idx < soff[p + 1]; idx += 4) {
+        const uint32_t lo = slo[idx];
+        if (lo != 0xffu) {
+          const T s = blk_mul(sa + 16 * p, stage + 16 * (size_t)sslot[idx], er, ec);
+          sa[16 * lo + e] = sa[16 * lo + e] - s;
+        }
+      }
+      __syncthreads();
+    }
+    (void)nst;
+
+    if (EXACT) {
+      /* upper blocks: a = (a - I A(i,k)) + I A(i,k), A = the fill's block */
+      for (uint32_t p = pd + grp; p < L; p += 4) {
+        const T *a0 = a + 16 * ((size_t)j0 + p);
+        T dq = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) dq += (er == (uint32_t)k ? T(1) : T(0)) * a0[4 * k + ec];
+        T v = sa[16 * p + e];
+        v = T(-1) * dq + v;
+        v = T(1) * dq + v;
+        sa[16 * p + e] = v;
+      }
+      __syncthreads();
+      if (lane < 16) ud[16 * (size_t)i + e] = sa[16 * pd + e];
+    }
+
+    {
+      const T *s = sa + 16 * pd;
+      const uint8_t neg = kf_cof_neg[e];
+      T acc = s[kf_cof_idx[e][0][0]];
+      if (neg & 1u) acc = -acc;
+      acc = acc * s[kf_cof_idx[e][0][1]] * s[kf_cof_idx[e][0][2]];
+#pragma unroll
+      for (int tt = 1; tt < 6; tt++) {
+        const T pr = s[kf_cof_idx[e][tt][0]] * s[kf_cof_idx[e][tt][1]] * s[kf_cof_idx[e][tt][2]];
+        acc = (neg & (1u << tt)) ? acc - pr : acc + pr;
+      }
+      const T d0 = __shfl(acc, 0, 16), d4 = __shfl(acc, 4, 16), d8 = __shfl(acc, 8, 16), d12 = __shfl(acc, 12, 16);
+      T det = s[0] * d0 + s[1] * d4 + s[2] * d8 + s[3] * d12;
+      det = T(1) / det;
+      acc *= det;
+      __syncthreads();
+      if (lane < 16) {
+        if (!isfinite((double)acc)) atomicOr(d.err, KF_ERR_NONFINITE_PIVOT);
+        sa[16 * pd + e] = acc;
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < 16 * L; k += 64) a[16 * (size_t)j0 + k] = sa[k];
+    if (EXACT)
+      for (uint32_t k = lane; k < 16 * nlow; k += 64) lold[16 * (size_t)j0 + k] = sa[k];
+    publish_fence();
+    __syncthreads();
+    if (lane == 0) flag_store(rdone + i, epoch);
+  }
+}
+
 static uint32_t persistent_waves(uint32_t work, uint32_t cap)
 {
   return work < cap ? (work ? work : 1) : cap;
@@ -193,9 +404,15 @@ static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
 {
   const uint32_t nw = persistent_waves(c->d.N, c->ilu_waves);
   const uint32_t epoch = ++c->epoch;
+  const ilu_layout lay(c->maxrow, c->ilu_umax, c->ilu_smax, sizeof(T));
   const int s = kf_prof_begin(c, KP_ILU);
-  hipLaunchKernelGGL((k_ilu<T, EXACT>), dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
-                     c->ticket_base[2], c->rdone, epoch, a, lold, ud);
+  if (lay.bytes <= 64 * 1024)
+    hipLaunchKernelGGL((k_ilu_staged<T, EXACT>), dim3(nw), dim3(64), lay.bytes, c->stream, c->d, c->lrows, c->d.N,
+                       c->ticket + 2, c->ticket_base[2], c->rdone, epoch, a, lold, ud, c->maxrow, c->ilu_umax,
+                       c->ilu_smax);
+  else
+    hipLaunchKernelGGL((k_ilu<T, EXACT>), dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
+                       c->ticket_base[2], c->rdone, epoch, a, lold, ud);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   c->ticket_base[2] += c->d.N + nw;

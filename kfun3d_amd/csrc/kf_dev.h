@@ -92,6 +92,8 @@ struct kf_ctx {
   kf_options opt;
   size_t nnzb;
   uint32_t maxrow;          /* most blocks in one BSR row */
+  uint32_t ilu_umax;        /* most upper entries of the rows one ILU row reads */
+  uint32_t ilu_smax;        /* most blocks one ILU row stages */
   uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch */
   uint32_t nlev_l, nlev_u;
   /* level schedule */
