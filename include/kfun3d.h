@@ -196,6 +196,15 @@ void kf_setup_host(const char *fpath, GEOMETRY *g);
  * GEOMETRY (inc/utils.h); they run on the context bound last (Setup binds). */
 void kf_bind(GEOMETRY *g);
 
+/* Host-side schedule of the triangular solves (for tests and tuning): ranges
+ * of R rows (1, 2, 4, 8 or 16), grouped into tasks of 16 / R ranges of one
+ * level of the range DAG.  For the forward (upper = 0) or backward sweep,
+ * returns the number of tasks and, when cap covers them, writes the ranges of
+ * each task (16 / R per task, ~0u = none), each task's local depth and each
+ * row's local level (N entries).  Works on a host-only GEOMETRY. */
+size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, unsigned *task, unsigned char *tdep,
+                         unsigned char *lvl, size_t cap);
+
 /* Releases everything Setup allocated, host and device. */
 void kf_geometry_free(GEOMETRY *g);
 

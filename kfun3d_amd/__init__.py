@@ -125,6 +125,7 @@ def _load():
         "kf_bind": (None, [GP]),
         "kf_profile": (None, [GP, C.c_int]),
         "kf_profile_get": (C.c_int, [GP, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
+        "kf_trsv_plan_host": (C.c_size_t, [GP, C.c_int, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
         "ComputeFlux": (None, [GP, DP, C.POINTER(Xyz), DP]),
         "ComputeTimeStep": (None, [C.c_double, GP]),
         "ComputeA": (None, [GP]),
@@ -162,7 +163,7 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "kf_kernel", "kf_set_state", "kf_get_state", "kf_memcpy_d2h", "kf_memcpy_h2d", "kf_device_alloc",
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
-               "kf_bind"]
+               "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host"]
 
 
 def options(**kw):
@@ -332,6 +333,16 @@ class Mesh:
         out = np.empty(int(lib().kf_nnzb(self.ref)) * 16)
         lib().kf_get_blocks(self.ref, out.ctypes.data)
         return out.reshape(-1, 4, 4)
+
+    def trsv_plan(self, upper, rows):
+        """Host-side schedule of the triangular solves: (task table of shape
+        (ntask, 16 // rows), each task's local depth, each row's local level)."""
+        nt = int(lib().kf_trsv_plan_host(self.ref, int(upper), rows, None, None, None, 0))
+        task = np.empty(nt * (16 // rows), dtype=np.uint32)
+        tdep = np.empty(nt, dtype=np.uint8)
+        lvl = np.empty(self.N, dtype=np.uint8)
+        lib().kf_trsv_plan_host(self.ref, int(upper), rows, task.ctypes.data, tdep.ctypes.data, lvl.ctypes.data, nt)
+        return task.reshape(nt, 16 // rows), tdep, lvl
 
     def pattern(self):
         nnzb = int(lib().kf_nnzb(self.ref))

@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkfun3d.so")
 ARCH = os.environ.get("KF_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRCS = ["gen.c", "msh.c", "setup.c", "solver.c"]
+HOST_SRCS = ["gen.c", "msh.c", "setup.c", "plan.c", "solver.c"]
 HIP_SRCS = ["ctx.hip", "flux.hip", "blas.hip", "fill.hip", "ilu.hip", "sptrsv.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",

@@ -23,69 +23,6 @@ static T *dupload(const T *h, size_t n)
   return p;
 }
 
-/* Local level of every row inside its contiguous range of KF_GROUP rows:
- * forward ranges [16 g, 16 g + 16), backward ranges counted from the last
- * row down.  A row's level is 0 when no column of its lower (forward) or
- * upper (backward) part falls inside its own range, else one more than the
- * deepest such column; depth[g] = deepest level + 1. */
-static void range_levels(const kf_prep *p, int upper, uint8_t *lvl, uint8_t *depth)
-{
-  const size_t N = p->N;
-  const size_t ng = (N + KF_GROUP - 1) / KF_GROUP;
-  for (size_t g = 0; g < ng; g++) {
-    size_t lo, hi;   /* rows [lo, hi] of range g */
-    if (!upper) { lo = g * KF_GROUP; hi = lo + KF_GROUP - 1 < N - 1 ? lo + KF_GROUP - 1 : N - 1; }
-    else { hi = N - 1 - g * KF_GROUP; lo = hi + 1 >= KF_GROUP ? hi + 1 - KF_GROUP : 0; }
-    uint8_t dmax = 0;
-    for (size_t t = 0; t <= hi - lo; t++) {
-      const size_t r = upper ? hi - t : lo + t;
-      const uint32_t j0 = upper ? p->diag[r] + 1 : p->rowptr[r], j1 = upper ? p->rowptr[r + 1] : p->diag[r];
-      uint8_t l = 0;
-      for (uint32_t j = j0; j < j1; j++) {
-        const uint32_t cidx = p->col[j];
-        if (cidx >= lo && cidx <= hi && lvl[cidx] + 1 > l) l = (uint8_t)(lvl[cidx] + 1);
-      }
-      lvl[r] = l;
-      if (l + 1 > dmax) dmax = (uint8_t)(l + 1);
-    }
-    depth[g] = dmax;
-  }
-}
-
-/* Ticket order of the ranges: by level of the range DAG (a range's level is
- * one more than the deepest range it reads from), ties by range index.  Any
- * topological order keeps the hand-off deadlock-free; level order keeps the
- * wavefronts in flight on ranges whose inputs are about to complete. */
-static uint32_t *range_order(const kf_prep *p, int upper)
-{
-  const size_t N = p->N;
-  const size_t ng = (N + KF_GROUP - 1) / KF_GROUP;
-  uint32_t *lev = (uint32_t *)kf_xcalloc(ng, sizeof(uint32_t));
-  uint32_t maxlev = 0;
-  for (size_t g = 0; g < ng; g++) {
-    uint32_t l = 0;
-    for (size_t t = 0; t < KF_GROUP; t++) {
-      if (g * KF_GROUP + t >= N) break;
-      const size_t r = upper ? N - 1 - (g * KF_GROUP + t) : g * KF_GROUP + t;
-      const uint32_t j0 = upper ? p->diag[r] + 1 : p->rowptr[r], j1 = upper ? p->rowptr[r + 1] : p->diag[r];
-      for (uint32_t j = j0; j < j1; j++) {
-        const size_t h = upper ? (N - 1 - p->col[j]) / KF_GROUP : p->col[j] / KF_GROUP;
-        if (h != g && lev[h] + 1 > l) l = lev[h] + 1;
-      }
-    }
-    lev[g] = l;
-    if (l > maxlev) maxlev = l;
-  }
-  uint32_t *cnt = (uint32_t *)kf_xcalloc(maxlev + 2, sizeof(uint32_t));
-  for (size_t g = 0; g < ng; g++) cnt[lev[g] + 1]++;
-  for (uint32_t l = 0; l <= maxlev; l++) cnt[l + 1] += cnt[l];
-  uint32_t *ord = (uint32_t *)kf_xmalloc(ng * sizeof(uint32_t));
-  for (size_t g = 0; g < ng; g++) ord[cnt[lev[g]]++] = (uint32_t)g;
-  free(cnt);
-  free(lev);
-  return ord;
-}
-
 /* rows of each level cut into groups of at most KF_GROUP */
 static uint32_t *level_groups(const uint32_t *lptr, uint32_t nlev, uint32_t *ngrp)
 {
@@ -231,22 +168,25 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   c->rdone = dalloc<uint32_t>(N);
   KF_CHECK(hipMemset(c->rdone, 0, N * sizeof(uint32_t)));
   {
-    const size_t ng = (N + KF_GROUP - 1) / KF_GROUP;
-    uint8_t *lv = (uint8_t *)kf_xmalloc(N + 1), *dp = (uint8_t *)kf_xmalloc(ng + 1);
-    range_levels(p, 0, lv, dp);
-    c->rlvl_l = dupload(lv, N);
-    c->rdep_l = dupload(dp, ng);
-    range_levels(p, 1, lv, dp);
-    c->rlvl_u = dupload(lv, N);
-    c->rdep_u = dupload(dp, ng);
-    free(lv); free(dp);
-    uint32_t *ord = range_order(p, 0);
-    c->rord_l = dupload(ord, ng);
-    free(ord);
-    ord = range_order(p, 1);
-    c->rord_u = dupload(ord, ng);
-    free(ord);
-    c->nrange = (uint32_t)ng;
+    /* triangular solves: ranges of R rows (tuning knob, a power of two
+     * up to 16) */
+    uint32_t R = (ev = getenv("KF_TRSV_ROWS")) ? (uint32_t)atoi(ev) : 4;
+    if (R != 1 && R != 2 && R != 4 && R != 8 && R != 16) R = 4;
+    c->trsv_R = R;
+    c->trsv_S = KF_GROUP / R;
+    kf_trsv_plan t;
+    kf_plan_trsv(p, 0, R, &t);
+    c->rlvl_l = dupload(t.lvl, N);
+    c->task_l = dupload(t.task, (size_t)t.ntask * t.S);
+    c->tdep_l = dupload(t.tdep, t.ntask);
+    c->ntask_l = t.ntask;
+    kf_free_trsv_plan(&t);
+    kf_plan_trsv(p, 1, R, &t);
+    c->rlvl_u = dupload(t.lvl, N);
+    c->task_u = dupload(t.task, (size_t)t.ntask * t.S);
+    c->tdep_u = dupload(t.tdep, t.ntask);
+    c->ntask_u = t.ntask;
+    kf_free_trsv_plan(&t);
     c->ystate[0] = c->ystate[1] = ~(uint64_t)0;
   }
   c->ticket = dalloc<uint32_t>(4);
@@ -296,7 +236,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->urows, c->lgrp, c->ugrp, c->rdone,
-                  c->rlvl_l, c->rlvl_u, c->rdep_l, c->rdep_u, c->rord_l, c->rord_u, c->ty64, c->ticket, c->a64, c->a32,
+                  c->rlvl_l, c->rlvl_u, c->task_l, c->task_u, c->tdep_l, c->tdep_u, c->ty64, c->ticket, c->a64, c->a32,
                   c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));

@@ -100,11 +100,13 @@ struct kf_ctx {
   uint32_t *lrows, *urows, *lgrp, *ugrp;
   uint32_t nlgrp, nugrp;
   uint32_t *rdone;          /* per-row completion epoch */
-  /* triangular solves by contiguous row ranges of KF_GROUP rows */
-  uint8_t *rlvl_l, *rlvl_u; /* local level of each row inside its range */
-  uint8_t *rdep_l, *rdep_u; /* local depth of each range */
-  uint32_t *rord_l, *rord_u; /* ranges in ticket order */
-  uint32_t nrange;
+  /* triangular solves: contiguous ranges of trsv_R rows, trsv_S = 16 / R
+   * ranges of one DAG level per task (one wavefront) */
+  uint32_t trsv_R, trsv_S;
+  uint8_t *rlvl_l, *rlvl_u;  /* local level of each row inside its range */
+  uint32_t *task_l, *task_u; /* ranges of each task, in ticket order */
+  uint8_t *tdep_l, *tdep_u;  /* local depth of each task */
+  uint32_t ntask_l, ntask_u;
   double *ty64;             /* forward-sweep output (FP64) */
   uint64_t tcall;           /* triangular solves so far (sentinel numbering) */
   uint64_t ystate[2];       /* call whose forward sentinel y holds (FP64, FP32) */

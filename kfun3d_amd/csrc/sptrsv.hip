@@ -3,14 +3,15 @@
  * (matrix/sptrsv.cpp:34-73; the reference schedules rows by level sets,
  * matrix/init.cpp:194-274).
  *
- * Work unit: a contiguous range of 16 rows -- [16 g, 16 g + 16) for the
- * forward sweep, the 16 rows ending at N - 1 - 16 g (taken in decreasing
- * order) for the backward sweep -- owned by one wavefront, a quad of lanes
- * per row (lane c computes component c).  A row depends only on rows of its
- * own range or of ranges with a smaller index.  Ranges are handed out by
- * ticket in level order of that range DAG (host: range_order), so a wave
- * only ever waits for ranges whose tickets were issued before its own, to
- * waves that are running: no wait can deadlock.
+ * Work unit: a contiguous range of R rows (R = 1, 2, 4, 8 or 16) --
+ * [R g, R g + R) for the forward sweep, the R rows ending at N - 1 - R g
+ * (taken in decreasing order) for the backward sweep.  A row depends only on
+ * rows of its own range or of ranges with a smaller index.  A task is up to
+ * 16 / R ranges of one level of that range DAG, owned by one wavefront with
+ * a quad of lanes per row (lane c computes component c); ranges of one level
+ * are independent.  Tasks are handed out by ticket in level order (host:
+ * range_tasks), so a wave only ever waits for ranges whose tickets were
+ * issued before its own, to waves that are running: no wait can deadlock.
  *
  * Hand-off without flags or fences.  Every result slot is preset to a
  * signalling NaN that names the sweep and the call (the sentinel); a
@@ -28,7 +29,7 @@
  *   external dependences (rows of earlier ranges): polled as above;
  *   internal dependences (rows of the same range): resolved inside the wave
  *     through LDS, row by row in the local level order computed on the host
- *     (rlvl / rdep).
+ *     (rlvl; the task's depth tdep).
  *
  * Columns and factor blocks of the first KF_TRSV_PF off-diagonal entries of
  * every row are loaded into registers as soon as the wave takes its ticket.
@@ -49,7 +50,7 @@
  */
 #include "sync.h"
 
-static_assert(KF_GROUP == 16, "one quad of lanes per row of a 16-row range");
+static_assert(KF_GROUP == 16, "one quad of lanes per row, 16 rows per task");
 
 #define KF_TRSV_PF 8   /* off-diagonal entries per row held in registers */
 
@@ -181,9 +182,10 @@ __device__ __forceinline__ bool wait_prefetched(const TV *v, const uint32_t cp[K
 /* forward sweep: y = L^-1 b; presets the backward output x of its rows to
  * the backward sentinel */
 template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ ord,
-                                                   const uint8_t *__restrict__ lvl, const uint8_t *__restrict__ depth,
-                                                   uint32_t nrange, uint32_t *ticket, uint32_t base,
+__global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ task, uint32_t S,
+                                                   uint32_t lg2R, const uint8_t *__restrict__ lvl,
+                                                   const uint8_t *__restrict__ tdep, uint32_t ntask,
+                                                   uint32_t *ticket, uint32_t base,
                                                    typename sentinel<TV>::bits sy, typename sentinel<TV>::bits sx,
                                                    const TM *__restrict__ a, const double *__restrict__ b, TV *y,
                                                    TV *xnext)
@@ -193,12 +195,12 @@ __global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__re
   bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
-    if (t >= nrange) break;
-    const uint32_t g = ord[t];
-    const uint32_t lo = KF_GROUP * g, r = lo + qd;
-    const bool active = r < d.N;
+    if (t >= ntask) break;
+    const uint32_t slot = qd >> lg2R, g = task[(size_t)t * S + slot];
+    const uint32_t lo = g << lg2R, r = lo + (qd & ((1u << lg2R) - 1u)), sq = slot << lg2R;
+    const bool active = g != ~0u && r < d.N;
     const uint32_t j0 = active ? d.rowptr[r] : 0, jd = active ? d.diag[r] : 0;
-    const uint32_t my = active ? lvl[r] : 0xffu, dep = depth[g];
+    const uint32_t my = active ? lvl[r] : 0xffu, dep = tdep[t];
     uint32_t cp[KF_TRSV_PF];
     TV mp[KF_TRSV_PF][4];
 #pragma unroll
@@ -234,13 +236,13 @@ __global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__re
 #pragma unroll
         for (int k = 0; k < KF_TRSV_PF; k++)
           if (j0 + k >= jx && j0 + k < jd) {
-            const TV *xc = xs + 4 * (cp[k] - lo);
+            const TV *xc = xs + 4 * (sq + cp[k] - lo);
             xr = xr - row_dot(mp[k], xc[0], xc[1], xc[2], xc[3]);
           }
         for (uint32_t j = jx > j0 + KF_TRSV_PF ? jx : j0 + KF_TRSV_PF; j < jd; j++) {
           TV m[4];
           ld_row(a, j, c, m);
-          const TV *xc = xs + 4 * (d.col[j] - lo);
+          const TV *xc = xs + 4 * (sq + d.col[j] - lo);
           xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
         }
         xr = __builtin_canonicalize(xr);
@@ -255,9 +257,10 @@ __global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__re
 /* backward sweep: x = U^-1 y (y from the forward sweep); resets y of its rows
  * to the forward sentinel of the next call */
 template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ ord,
-                                                   const uint8_t *__restrict__ lvl, const uint8_t *__restrict__ depth,
-                                                   uint32_t nrange, uint32_t *ticket, uint32_t base,
+__global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ task, uint32_t S,
+                                                   uint32_t lg2R, const uint8_t *__restrict__ lvl,
+                                                   const uint8_t *__restrict__ tdep, uint32_t ntask,
+                                                   uint32_t *ticket, uint32_t base,
                                                    typename sentinel<TV>::bits sx, typename sentinel<TV>::bits synext,
                                                    const TM *__restrict__ a, TV *y, TV *x, double *__restrict__ xout)
 {
@@ -267,13 +270,15 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
   bool aborted = false;
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
-    if (t >= nrange) break;
-    const uint32_t g = ord[t];
-    const uint32_t hi = last - KF_GROUP * g;
-    const bool active = qd <= hi;
-    const uint32_t r = active ? hi - qd : 0;
+    if (t >= ntask) break;
+    const uint32_t slot = qd >> lg2R, g = task[(size_t)t * S + slot], kr = qd & ((1u << lg2R) - 1u);
+    const uint32_t sq = slot << lg2R;
+    const bool valid = g != ~0u;
+    const uint32_t hi = valid ? last - (g << lg2R) : 0;
+    const bool active = valid && kr <= hi;
+    const uint32_t r = active ? hi - kr : 0;
     const uint32_t jd = active ? d.diag[r] : 0, j0 = jd + 1, j1 = active ? d.rowptr[r + 1] : 0;
-    const uint32_t my = active ? lvl[r] : 0xffu, dep = depth[g];
+    const uint32_t my = active ? lvl[r] : 0xffu, dep = tdep[t];
     uint32_t cp[KF_TRSV_PF];
     TV mp[KF_TRSV_PF][4], dg[4];
     if (active) ld_row(a, jd, c, dg);
@@ -308,13 +313,13 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
 #pragma unroll
         for (int k = 0; k < KF_TRSV_PF; k++)
           if (j0 + k < jy) {
-            const TV *xc = xs + 4 * (hi - cp[k]);
+            const TV *xc = xs + 4 * (sq + hi - cp[k]);
             xr = xr - row_dot(mp[k], xc[0], xc[1], xc[2], xc[3]);
           }
         for (uint32_t j = j0 + KF_TRSV_PF; j < jy; j++) {
           TV m[4];
           ld_row(a, j, c, m);
-          const TV *xc = xs + 4 * (hi - d.col[j]);
+          const TV *xc = xs + 4 * (sq + hi - d.col[j]);
           xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
         }
 #pragma unroll
@@ -352,8 +357,9 @@ __global__ void k_fill_bits(TV *v, size_t n, typename sentinel<TV>::bits s)
 template <typename TM, typename TV>
 static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslot, TV *x, double *xout)
 {
-  const uint32_t nr = c->nrange;
-  const uint32_t nw = nr < c->trsv_waves ? nr : c->trsv_waves;
+  const uint32_t lg2R = (uint32_t)__builtin_ctz(c->trsv_R);
+  const uint32_t nwl = c->ntask_l < c->trsv_waves ? c->ntask_l : c->trsv_waves;
+  const uint32_t nwu = c->ntask_u < c->trsv_waves ? c->ntask_u : c->trsv_waves;
   const uint64_t k = c->tcall;
   const size_t n = 4 * (size_t)c->d.N;
   if (c->ystate[yslot] != k) {
@@ -362,19 +368,19 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
     KF_LAUNCH_CHECK();
   }
   int s = kf_prof_begin(c, KP_TRSV_L);
-  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nw), dim3(64), 0, c->stream, c->d, c->rord_l, c->rlvl_l, c->rdep_l,
-                     nr, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k), sentinel<TV>::of(2 * k + 1), a, b,
-                     y, x);
+  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64), 0, c->stream, c->d, c->task_l, c->trsv_S, lg2R,
+                     c->rlvl_l, c->tdep_l, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
+                     sentinel<TV>::of(2 * k + 1), a, b, y, x);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  c->ticket_base[0] += nr + nw;
+  c->ticket_base[0] += c->ntask_l + nwl;
   s = kf_prof_begin(c, KP_TRSV_U);
-  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nw), dim3(64), 0, c->stream, c->d, c->rord_u, c->rlvl_u, c->rdep_u,
-                     nr, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1), sentinel<TV>::of(2 * k + 2),
-                     a, y, x, xout);
+  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64), 0, c->stream, c->d, c->task_u, c->trsv_S, lg2R,
+                     c->rlvl_u, c->tdep_u, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
+                     sentinel<TV>::of(2 * k + 2), a, y, x, xout);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  c->ticket_base[1] += nr + nw;
+  c->ticket_base[1] += c->ntask_u + nwu;
   c->tcall = k + 1;
   c->ystate[yslot] = k + 1;
 }

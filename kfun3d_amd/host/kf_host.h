@@ -40,6 +40,17 @@ typedef struct kf_prep {
   double U, V;
 } kf_prep;
 
+/* schedule of the level-scheduled triangular solves (plan.c) */
+typedef struct kf_trsv_plan {
+  uint32_t R, S;       /* rows per range, ranges per task (16 / R) */
+  uint32_t ntask;
+  uint8_t *lvl;        /* [N] local level of each row inside its range */
+  uint32_t *task;      /* [ntask * S] ranges of each task, ~0u = none */
+  uint8_t *tdep;       /* [ntask] local depth of each task */
+} kf_trsv_plan;
+void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, kf_trsv_plan *out);
+void kf_free_trsv_plan(kf_trsv_plan *t);
+
 /* .msh reader (src/mesh.c:22-154 without the device part) */
 void kf_read_msh(const char *fpath, GEOMETRY *g);
 /* host preprocessing of src/mesh.c:113-153 with one owner */

@@ -127,6 +127,34 @@ def test_ilu_and_sptrsv(pair):
     fresh.close()
 
 
+@pytest.mark.parametrize("rows", [1, 2, 16])
+def test_sptrsv_other_schedules(mesh_a, rows, monkeypatch):
+    """The triangular solve is bit-identical to the oracle for every range
+    size of its schedule (the default is 4 rows per range)."""
+    monkeypatch.setenv("KF_TRSV_ROWS", str(rows))
+    m = K.Mesh(mesh_a, K.options())
+    try:
+        q = perturbed_state(m.N, 3.0, 15)
+        b = np.random.default_rng(16).standard_normal(m.sz)
+        m.set_state(q)
+        m.timestep(50.0)
+        m.fill()
+        m.ilu()
+        bd, xd = K.DeviceVec(m.sz, b), K.DeviceVec(m.sz)
+        m.sptrsv(bd, xd)
+        assert m.last_error() == 0
+        orc = O.Oracle(m)
+        orc.set_state(q)
+        orc.timestep(50.0)
+        orc.fill()
+        orc.ilu()
+        xe = orc.sptrsv(b)
+        orc.close()
+        assert _bitwise(xd.numpy(), xe), _describe(xd.numpy(), xe)
+    finally:
+        m.close()
+
+
 def test_ilu_textbook(pair):
     """ilu_exact=0: textbook ILU(0), bit for bit against the oracle's textbook
     sweep, and within 1e-10 of the reference's sweep."""
