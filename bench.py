@@ -61,7 +61,7 @@ def cpu_baseline(mesh_path, sizes, counts, host_threads):
     if not os.path.exists(kref):
         return None
     env = dict(os.environ, OMP_NUM_THREADS=str(host_threads), OMP_PROC_BIND="close")
-    reps = 30   # about 10 s of CPU work on 16 threads
+    reps = 100   # a few seconds of CPU work on 16 threads
     t0 = time.time()
     out = subprocess.run([kref, "time", mesh_path, str(host_threads), str(reps)], stdout=subprocess.PIPE, text=True,
                          env=env, timeout=900, check=True).stdout
