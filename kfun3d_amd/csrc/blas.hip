@@ -91,6 +91,63 @@ void kf_dot_dev(kf_ctx *c, size_t n, const double *x, const double *y, int slot)
 
 void kf_nrm2_dev(kf_ctx *c, size_t n, const double *x, int slot) { dot_any(c, n, x, x, slot, 1); }
 
+/* One modified Gram-Schmidt step fused with its neighbours (src/kernel.c:
+ * 129-141): h = <w, v> from the KF_NRED partials of the previous pass,
+ * reduced in k_red_final's tree (every block forms the same value; block 0
+ * stores it), then w = (-h) v + w, and the partials of <w, u> for the next
+ * step (u = the next basis vector, or w itself for the closing norm).  The
+ * grid and the element order of each thread are k_dot_part's, so the
+ * partials are bit-identical to an unfused dot after an unfused update. */
+__global__ __launch_bounds__(RB) void k_mgs_step(size_t n, const double *__restrict__ part_in, double *hslot,
+                                                 const double *v, double *w, const double *u,
+                                                 double *__restrict__ part_out)
+{
+  __shared__ double sh[RB];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < KF_NRED; i += RB) acc += part_in[i];
+  const double h = block_sum(acc, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hslot = h;
+  __syncthreads();   /* every thread has read sh[0] before sh is reused */
+  const double a = -h;
+  const bool self = u == w;
+  acc = 0.0;
+  const size_t stride = (size_t)gridDim.x * RB;
+  for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += stride) {
+    const double ax = a * v[i];
+    const double wn = ax + w[i];
+    w[i] = wn;
+    acc += wn * (self ? wn : u[i]);
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = s;
+}
+
+/* w orthogonalised against nv basis vectors vbase + k stride by modified
+ * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv].  Index-order
+ * sums (reduce_seq) keep the unfused dot / update sequence. */
+void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
+{
+  if (c->opt.reduce_seq) {
+    for (int k = 0; k < nv; k++) {
+      dot_any(c, n, w, vbase + (size_t)k * stride, slot + k, 0);
+      kf_axpy_slot(c, n, slot + k, vbase + (size_t)k * stride, w);
+    }
+    dot_any(c, n, w, w, slot + nv, 1);
+    return;
+  }
+  double *part[2] = {c->d.red, c->d.red + KF_NRED};
+  hipLaunchKernelGGL(k_dot_part, dim3(KF_NRED), dim3(RB), 0, c->stream, n, w, vbase, part[0]);
+  KF_LAUNCH_CHECK();
+  for (int k = 0; k < nv; k++) {
+    const double *u = k + 1 < nv ? vbase + (size_t)(k + 1) * stride : w;
+    hipLaunchKernelGGL(k_mgs_step, dim3(KF_NRED), dim3(RB), 0, c->stream, n, part[k & 1], c->d.scal + slot + k,
+                       vbase + (size_t)k * stride, w, u, part[(k + 1) & 1]);
+    KF_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, part[nv & 1], c->d.scal + slot + nv, 1);
+  KF_LAUNCH_CHECK();
+}
+
 void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out)
 {
   KF_CHECK(hipMemcpyAsync(c->hscal + slot, c->d.scal + slot, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -281,6 +338,10 @@ void kfs_newaxpy(GEOMETRY *g, size_t n, double a, const double *x, const double 
 }
 void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot) { kf_scale_recip(kf_ctx_of(g), n, x, slot); }
 void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y) { kf_axpy_slot(kf_ctx_of(g), n, slot, x, y); }
+void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
+{
+  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot);
+}
 void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w) { kf_fd_perturb(kf_ctx_of(g), x, h, slot, w); }
 void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int slot, double *y)
 {

@@ -199,7 +199,7 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   c->cdt = dalloc<double>(N);
   KF_CHECK(hipMemset(c->cdt, 0, N * sizeof(double)));
   d.eres = dalloc<double>(4 * E + 8);
-  d.red = dalloc<double>(KF_NRED);
+  d.red = dalloc<double>(2 * KF_NRED);   /* two sets of partial sums (fused MGS) */
   d.scal = dalloc<double>(KF_NSCAL);
   KF_CHECK(hipMemset(d.scal, 0, KF_NSCAL * sizeof(double)));
   d.err = dalloc<uint32_t>(1);

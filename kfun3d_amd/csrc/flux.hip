@@ -378,22 +378,43 @@ __global__ __launch_bounds__(256) void k_force_terms(kf_dv d, const double *__re
 }
 
 /* src/forces.c:33 reduces each colour into a zero-initialised private sum
- * that is then added to the running total */
-__global__ void k_force_sum(kf_dv d, const double *__restrict__ ft, double *__restrict__ out)
+ * that is then added to the running total.  The sums are sequential in face
+ * order, as the reference's; the workgroup stages 256 faces' terms in LDS
+ * with coalesced loads and one lane adds them up. */
+__global__ __launch_bounds__(256) void k_force_sum(kf_dv d, const double *__restrict__ ft, double *__restrict__ out)
 {
-  if (threadIdx.x != 0) return;
-  double lift = 0.0, drag = 0.0, momn = 0.0;
-  for (uint32_t cl = 0; cl < d.ncolor; cl++) {
-    double l = 0.0, dr = 0.0, m = 0.0;
-    for (uint32_t j = d.color[cl]; j < d.color[cl + 1]; j++) {
-      const double *o = ft + 6 * (size_t)j;
-      l = l - o[0] + o[1];
-      dr = dr + o[2] + o[3];
-      m = m + o[4] - o[5];
+  __shared__ double s[6 * 256];
+  const uint32_t nc = d.ncolor;
+  const uint32_t j0 = nc ? d.color[0] : 0, j1 = nc ? d.color[nc] : 0;
+  double lift = 0.0, drag = 0.0, momn = 0.0, l = 0.0, dr = 0.0, m = 0.0;
+  uint32_t cl = 0, cend = nc ? d.color[1] : 0;
+  for (uint32_t b = j0; b < j1; b += 256) {
+    const uint32_t n = j1 - b < 256u ? j1 - b : 256u;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 6 * n; k += 256) s[k] = ft[6 * (size_t)b + k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t i = 0; i < n; i++) {
+        while (b + i == cend) {   /* colour cl ends before face b + i */
+          lift = lift + l; drag = drag + dr; momn = momn + m;
+          l = 0.0; dr = 0.0; m = 0.0;
+          cl++;
+          cend = d.color[cl + 1];
+        }
+        const double *o = s + 6 * i;
+        l = l - o[0] + o[1];
+        dr = dr + o[2] + o[3];
+        m = m + o[4] - o[5];
+      }
     }
-    lift = lift + l; drag = drag + dr; momn = momn + m;
   }
-  out[0] = lift; out[1] = drag; out[2] = momn;
+  if (threadIdx.x == 0) {
+    for (; cl < nc; cl++) {   /* the last colour, then any empty ones */
+      lift = lift + l; drag = drag + dr; momn = momn + m;
+      l = 0.0; dr = 0.0; m = 0.0;
+    }
+    out[0] = lift; out[1] = drag; out[2] = momn;
+  }
 }
 
 void kf_launch_forces(kf_ctx *c, double *forces_host)
@@ -409,7 +430,7 @@ void kf_launch_forces(kf_ctx *c, double *forces_host)
     hipLaunchKernelGGL(k_force_terms, dim3(kf_blocks_for(d.S, 256)), dim3(256), 0, c->stream, d, c->q, ft);
     KF_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(64), 0, c->stream, d, ft, d.scal + (KF_NSCAL - 3));
+  hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(256), 0, c->stream, d, ft, d.scal + (KF_NSCAL - 3));
   KF_LAUNCH_CHECK();
   kf_scal_fetch(c, KF_NSCAL - 3, 3, forces_host);
 }

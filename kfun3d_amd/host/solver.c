@@ -107,11 +107,7 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
           hh[it + 1] = h2[it + 1];
         } else {
           /* modified Gram-Schmidt, src/kernel.c:129-141 */
-          for (int v = 0; v <= it; v++) {
-            kfs_dot(g, sz, VEC(it + 1), VEC(v), KFS_SLOT_H + v);
-            kfs_axpy_slot(g, sz, KFS_SLOT_H + v, VEC(v), VEC(it + 1));
-          }
-          kfs_nrm2(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
+          kfs_mgs(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H);
           kfs_scale_recip(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
           kfs_fetch(g, KFS_SLOT_H, it + 2, hh);
         }
