@@ -93,6 +93,80 @@ def blas_passes(hist, restart):
     return n
 
 
+def mixed_precision(K, m64, mesh, hist64):
+    """The same solve with the FP32 and the BF16-factor preconditioner
+    (residual, Arnoldi, least squares and update stay FP64): wall-clock
+    (one warm-up solve, one timed), Krylov count and force error against the
+    plain FP64 path just timed."""
+    import numpy as np
+    f64 = np.array(hist64[-1]["forces"])
+    k64 = sum(x["krylov"] for x in hist64)
+    out = {}
+    for prec in ("fp32", "bf16"):
+        m = K.Mesh(mesh, K.config_options(prec=prec))
+        q0 = m.state()
+        m.kernel()
+        m.set_state(q0)
+        m.synchronize()
+        t0 = time.perf_counter()
+        h = m.kernel()
+        m.synchronize()
+        t = time.perf_counter() - t0
+        assert m.last_error() == 0, "device error word set (%s)" % prec
+        f = np.array(h[-1]["forces"])
+        out[prec] = {"seconds": t, "speedup_vs_fp64": None, "pseudo_steps": len(h),
+                     "krylov_total": sum(x["krylov"] for x in h), "krylov_fp64": k64,
+                     "residual_drop": h[0]["fnorm"] / h[-1]["fnorm"],
+                     "force_rel_err_vs_fp64": (np.abs(f - f64) / np.abs(f64)).max()}
+        m.close()
+    return out
+
+
+COPY_BYTES = 1 << 28
+
+
+def copy_probe(reps):
+    """reps device-to-device copies of COPY_BYTES (the runtime's copy kernel):
+    the reference point that scales FETCH_SIZE / WRITE_SIZE to bytes."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(COPY_BYTES)) == 0
+    for _ in range(max(reps, 1)):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(COPY_BYTES), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"probe": "copy", "bytes_per_copy": COPY_BYTES, "copies": max(reps, 1)}))
+
+
+def trsv_probe(K, m, reps, warmup):
+    """The C2 Jacobian, its ILU(0) and reps triangular solves (x = U^-1 L^-1 b),
+    for counter passes on the dominant kernels."""
+    import numpy as np
+    m.timestep(10.0)
+    m.fill()
+    m.ilu()
+    b = K.DeviceVec(m.sz, np.random.default_rng(0).standard_normal(m.sz))
+    x = K.DeviceVec(m.sz)
+    for _ in range(warmup):
+        m.sptrsv(b, x)
+    m.profile(True)
+    for _ in range(max(reps, 1)):
+        m.sptrsv(b, x)
+    m.synchronize()
+    prof = {k: m.profile_get(k) for k in ("trsv_lower", "trsv_upper")}
+    assert m.last_error() == 0
+    print(json.dumps({"probe": "trsv", "solves": max(reps, 1),
+                      "kernel_ms": {k: v[0] / v[1] for k, v in prof.items() if v[1]}}))
+    m.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,12 +174,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32/BF16-preconditioner comparison")
+    ap.add_argument("--kernel", default="solve", choices=["solve", "trsv", "copy"],
+                    help="solve: the benchmark; trsv / copy: short runs for hardware-counter passes (the "
+                         "triangular solves on the same mesh; a device-to-device copy of known size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path's multi-GPU sharding is a design, "
               "not an implementation); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.kernel == "copy":
+        return copy_probe(args.steps)
     import kfun3d_amd as K
     tmp = tempfile.mkdtemp(prefix="kfbench_")
     mesh = os.path.join(tmp, "c2.msh")
@@ -118,6 +198,8 @@ def main():
     nnzb = N + 2 * E
     s_bytes = {"fp64": 8, "fp32": 4, "bf16": 2}[args.prec]
 
+    if args.kernel == "trsv":
+        return trsv_probe(K, m, args.steps, args.warmup)
     for _ in range(args.warmup):
         m.set_state(q0)
         m.kernel()
@@ -151,6 +233,7 @@ def main():
               "fill": prof["fill"][1] // max(args.steps, 1), "blas_passes": blas_passes(hist, opts.gmres_restart)}
     host_threads = min(16, len(os.sched_getaffinity(0)))
     cpu = None if args.no_cpu else cpu_baseline(mesh, LATTICE, counts, host_threads)
+    mixed = None if (args.no_mixed or args.prec != "fp64") else mixed_precision(K, m, mesh, hist)
 
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -169,7 +252,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": ab,
                      "avg_launch_ms": ms / n},
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
+    if mixed:
+        for v in mixed.values():
+            v["speedup_vs_fp64"] = value / v["seconds"]
     print(json.dumps(out))
     m.close()
 

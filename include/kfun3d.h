@@ -130,6 +130,12 @@ typedef enum {
   KF_ORTHO_CGS2 = 1     /* batched classical Gram-Schmidt, two passes          */
 } kf_ortho;
 
+typedef enum {
+  KF_KRYLOV_GMRES = 0,  /* left-preconditioned restarted GMRES, src/kernel.c:102-200 */
+  KF_KRYLOV_FGMRES = 1  /* right-preconditioned flexible GMRES: z_j = M(v_j),
+                           x += Z y; converges on the unpreconditioned residual */
+} kf_krylov;
+
 typedef struct kf_options {
   double alpha_deg;     /* angle of attack (reference 3, inc/phy.h:8-9)       */
   double cfl_init;      /* CFL_INIT (reference 50, src/kernel.c:19)           */
@@ -150,6 +156,7 @@ typedef struct kf_options {
   int    reduce_seq;    /* 1 = norms and dot products summed in index order
                            like the reference's one-thread loops (one
                            wavefront; for parity runs); 0 = fixed-order tree */
+  kf_krylov krylov;     /* Krylov method (the reference: left GMRES)        */
 } kf_options;
 
 typedef struct kf_step_record {
@@ -200,10 +207,22 @@ void kf_bind(GEOMETRY *g);
  * of R rows (1, 2, 4, 8 or 16), grouped into tasks of 16 / R ranges of one
  * level of the range DAG.  For the forward (upper = 0) or backward sweep,
  * returns the number of tasks and, when cap covers them, writes the ranges of
- * each task (16 / R per task, ~0u = none), each task's local depth and each
- * row's local level (N entries).  Works on a host-only GEOMETRY. */
+ * each task (16 / R per task, ~0u = none), each task's local depth, each
+ * row's local level (N entries) and, if rowtab is given, per task and row
+ * slot the row and its two block bounds (16 x 3 per task).  Works on a
+ * host-only GEOMETRY. */
 size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, unsigned *task, unsigned char *tdep,
-                         unsigned char *lvl, size_t cap);
+                         unsigned char *lvl, unsigned *rowtab, size_t cap);
+
+/* Host-side schedule of the ILU(0) factorisation (for tests): returns the
+ * number of update pairs and, when cap covers them, writes per ticket (rows
+ * in level order) the record {row, first block, blocks, lower blocks, first
+ * pair, pairs, 0, 0} (8 N entries), per BSR position of a lower block (i, m)
+ * the diagonal position of row m and the offset of its first pair (nnzb
+ * entries each), and per pair the block (m, k) of row m and the position of
+ * column k in row i.  Works on a host-only GEOMETRY. */
+size_t kf_ilu_plan_host(const GEOMETRY *g, unsigned *rec, unsigned *ldiag, unsigned *moff, unsigned *mjm,
+                        unsigned short *mlo, size_t cap);
 
 /* Releases everything Setup allocated, host and device. */
 void kf_geometry_free(GEOMETRY *g);

@@ -18,6 +18,7 @@ LIBPATH = os.path.join(HERE, "libkfun3d.so")
 
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
 ORTHO = {"mgs": 0, "cgs2": 1}
+KRYLOV = {"gmres": 0, "fgmres": 1}
 
 
 class Xyzn(C.Structure):
@@ -84,7 +85,7 @@ class Options(C.Structure):
                 ("ts_max", C.c_int), ("fnorm_ratio", C.c_double), ("gmres_restart", C.c_int),
                 ("gmres_rtol", C.c_double), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
                 ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int), ("ilu_exact", C.c_int),
-                ("reduce_seq", C.c_int)]
+                ("reduce_seq", C.c_int), ("krylov", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -125,7 +126,9 @@ def _load():
         "kf_bind": (None, [GP]),
         "kf_profile": (None, [GP, C.c_int]),
         "kf_profile_get": (C.c_int, [GP, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
-        "kf_trsv_plan_host": (C.c_size_t, [GP, C.c_int, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+        "kf_trsv_plan_host": (C.c_size_t, [GP, C.c_int, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_size_t]),
+        "kf_ilu_plan_host": (C.c_size_t, [GP, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
         "ComputeFlux": (None, [GP, DP, C.POINTER(Xyz), DP]),
         "ComputeTimeStep": (None, [C.c_double, GP]),
         "ComputeA": (None, [GP]),
@@ -134,6 +137,8 @@ def _load():
         "ComputeSparseTriangularSolve": (None, [GP, DP, DP]),
         "iJacVec": (None, [GP, DP, DP, C.c_double, C.POINTER(Xyz), DP, DP]),
         "Kernel": (None, [GP]),
+        "kfs_flux_spectral": (None, [GP, DP, C.POINTER(Xyz), DP]),
+        "kfs_timestep_spectral": (None, [GP, C.c_double]),
         "Compute2ndNorm": (C.c_double, [C.c_size_t, DP]),
         "ComputeAXPY": (None, [C.c_size_t, C.c_double, DP, DP]),
         "ComputeNewAXPY": (None, [C.c_size_t, C.c_double, DP, DP, DP]),
@@ -163,7 +168,8 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "kf_kernel", "kf_set_state", "kf_get_state", "kf_memcpy_d2h", "kf_memcpy_h2d", "kf_device_alloc",
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
-               "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host"]
+               "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
+               "kf_ilu_plan_host"]
 
 
 def options(**kw):
@@ -173,6 +179,8 @@ def options(**kw):
             v = PREC[v] if isinstance(v, str) else v
         if k == "ortho":
             v = ORTHO[v] if isinstance(v, str) else v
+        if k == "krylov":
+            v = KRYLOV[v] if isinstance(v, str) else v
         setattr(o, k, v)
     return o
 
@@ -301,6 +309,17 @@ class Mesh:
                    C.cast(grad_devs[2].ptr, C.POINTER(C.c_double)))
         lib().ComputeFlux(self.ref, q_dev.ptr, C.byref(grad), r_dev.ptr)
 
+    def flux_spectral(self, q_dev, grad_devs, r_dev):
+        """Residual of q that also leaves the time-step sums (the driver's
+        first residual of a pseudo-time step)."""
+        grad = Xyz(C.cast(grad_devs[0].ptr, C.POINTER(C.c_double)), C.cast(grad_devs[1].ptr, C.POINTER(C.c_double)),
+                   C.cast(grad_devs[2].ptr, C.POINTER(C.c_double)))
+        lib().kfs_flux_spectral(self.ref, q_dev.ptr, C.byref(grad), r_dev.ptr)
+
+    def timestep_spectral(self, cfl):
+        """cdt from the sums of the last flux_spectral (same state)."""
+        lib().kfs_timestep_spectral(self.ref, cfl)
+
     def timestep(self, cfl):
         lib().ComputeTimeStep(cfl, self.ref)
 
@@ -336,13 +355,29 @@ class Mesh:
 
     def trsv_plan(self, upper, rows):
         """Host-side schedule of the triangular solves: (task table of shape
-        (ntask, 16 // rows), each task's local depth, each row's local level)."""
-        nt = int(lib().kf_trsv_plan_host(self.ref, int(upper), rows, None, None, None, 0))
+        (ntask, 16 // rows), each task's local depth, each row's local level,
+        per-task row table of shape (ntask, 16, 3))."""
+        nt = int(lib().kf_trsv_plan_host(self.ref, int(upper), rows, None, None, None, None, 0))
         task = np.empty(nt * (16 // rows), dtype=np.uint32)
         tdep = np.empty(nt, dtype=np.uint8)
         lvl = np.empty(self.N, dtype=np.uint8)
-        lib().kf_trsv_plan_host(self.ref, int(upper), rows, task.ctypes.data, tdep.ctypes.data, lvl.ctypes.data, nt)
-        return task.reshape(nt, 16 // rows), tdep, lvl
+        rowtab = np.empty(nt * 48, dtype=np.uint32)
+        lib().kf_trsv_plan_host(self.ref, int(upper), rows, task.ctypes.data, tdep.ctypes.data, lvl.ctypes.data,
+                                rowtab.ctypes.data, nt)
+        return task.reshape(nt, 16 // rows), tdep, lvl, rowtab.reshape(nt, 16, 3)
+
+    def ilu_plan(self, nnzb):
+        """Host-side schedule of the ILU(0) factorisation: (records (N, 8),
+        ldiag, moff (nnzb each), pair blocks, pair positions)."""
+        nm = int(lib().kf_ilu_plan_host(self.ref, None, None, None, None, None, 0))
+        rec = np.empty(8 * self.N, dtype=np.uint32)
+        ldiag = np.empty(nnzb, dtype=np.uint32)
+        moff = np.empty(nnzb, dtype=np.uint32)
+        mjm = np.empty(max(nm, 1), dtype=np.uint32)
+        mlo = np.empty(max(nm, 1), dtype=np.uint16)
+        lib().kf_ilu_plan_host(self.ref, rec.ctypes.data, ldiag.ctypes.data, moff.ctypes.data, mjm.ctypes.data,
+                               mlo.ctypes.data, nm)
+        return rec.reshape(self.N, 8), ldiag, moff, mjm[:nm], mlo[:nm]
 
     def pattern(self):
         nnzb = int(lib().kf_nnzb(self.ref))

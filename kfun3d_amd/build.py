@@ -23,7 +23,7 @@ ARCH = os.environ.get("KF_OFFLOAD_ARCH", "gfx950")
 HOST_SRCS = ["gen.c", "msh.c", "setup.c", "plan.c", "solver.c"]
 HIP_SRCS = ["ctx.hip", "flux.hip", "blas.hip", "fill.hip", "ilu.hip", "sptrsv.hip"]
 
-CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
+CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
           "-Wno-unused-parameter", "-I" + os.path.join(ROOT, "include")]
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
@@ -55,7 +55,7 @@ def build_library(verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
+    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp", "-lm"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -149,12 +149,18 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   d.diag = dupload(p->diag, N);
   for (size_t i = 0; i < N; i++)
     if (p->rowptr[i + 1] - p->rowptr[i] > c->maxrow) c->maxrow = p->rowptr[i + 1] - p->rowptr[i];
-  for (size_t i = 0; i < N; i++) {
-    uint32_t u = 0;
-    const uint32_t nlow = p->diag[i] - p->rowptr[i];
-    for (uint32_t j = p->rowptr[i]; j < p->diag[i]; j++) u += p->rowptr[p->col[j] + 1] - p->diag[p->col[j]] - 1;
-    if (u > c->ilu_umax) c->ilu_umax = u;
-    if (3 * nlow + u > c->ilu_smax) c->ilu_smax = 3 * nlow + u;
+  {
+    kf_ilu_plan ip;
+    kf_plan_ilu(p, &ip);
+    c->ilu_rec = dupload(ip.rec, 8 * N);
+    c->ilu_ldiag = dupload(ip.ldiag, p->nnzb);
+    c->ilu_moff = dupload(ip.moff, p->nnzb);
+    c->ilu_mjm = dupload(ip.mjm, ip.nmatch);
+    c->ilu_mlo = dupload(ip.mlo, ip.nmatch);
+    c->ilu_maxrow = ip.maxrow;
+    c->ilu_maxlow = ip.maxlow;
+    c->ilu_maxmatch = ip.maxmatch;
+    kf_free_ilu_plan(&ip);
   }
   c->nlev_l = p->nlev_l;
   c->nlev_u = p->nlev_u;
@@ -176,14 +182,14 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     c->trsv_S = KF_GROUP / R;
     kf_trsv_plan t;
     kf_plan_trsv(p, 0, R, &t);
-    c->rlvl_l = dupload(t.lvl, N);
-    c->task_l = dupload(t.task, (size_t)t.ntask * t.S);
+    c->trow_l = dupload(t.trow, (size_t)t.ntask * KF_GROUP * 3);
+    c->tlvl_l = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_l = dupload(t.tdep, t.ntask);
     c->ntask_l = t.ntask;
     kf_free_trsv_plan(&t);
     kf_plan_trsv(p, 1, R, &t);
-    c->rlvl_u = dupload(t.lvl, N);
-    c->task_u = dupload(t.task, (size_t)t.ntask * t.S);
+    c->trow_u = dupload(t.trow, (size_t)t.ntask * KF_GROUP * 3);
+    c->tlvl_u = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_u = dupload(t.tdep, t.ntask);
     c->ntask_u = t.ntask;
     kf_free_trsv_plan(&t);
@@ -236,8 +242,8 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->urows, c->lgrp, c->ugrp, c->rdone,
-                  c->rlvl_l, c->rlvl_u, c->task_l, c->task_u, c->tdep_l, c->tdep_u, c->ty64, c->ticket, c->a64, c->a32,
-                  c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b};
+                  c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ticket, c->a64, c->a32,
+                  c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));
   KF_CHECK(hipStreamDestroy(c->stream));
@@ -268,6 +274,7 @@ extern "C" kf_options kf_default_options(void)
   o.verbose = 0;
   o.ilu_exact = 1;
   o.reduce_seq = 0;
+  o.krylov = KF_KRYLOV_GMRES;
   return o;
 }
 
@@ -413,10 +420,30 @@ extern "C" void kf_get_blocks(const GEOMETRY *g, double *blocks_host)
 extern "C" void ComputeFlux(const GEOMETRY *g, const double *q, GRADIENT *grad, double *r)
 {
   kf_ctx *c = kf_ctx_of(g);
-  kf_launch_flux(c, q, grad->x0, grad->x1, grad->x2, r);
+  kf_launch_flux(c, q, grad->x0, grad->x1, grad->x2, r, NULL);
 }
 
 extern "C" void ComputeTimeStep(const double cfl, GEOMETRY *g) { kf_launch_timestep(kf_ctx_of(g), cfl); }
+
+/* Driver-internal pair: the residual of the current state together with its
+ * time-step sums, then the time step from those sums (the state must not
+ * change in between; Kernel() calls them back to back). */
+extern "C" void kfs_flux_spectral(GEOMETRY *g, const double *q, GRADIENT *grad, double *r)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (!c->sr) KF_CHECK(hipMalloc((void **)&c->sr, (size_t)c->d.N * sizeof(double)));
+  kf_launch_flux(c, q, grad->x0, grad->x1, grad->x2, r, c->sr);
+}
+
+extern "C" void kfs_timestep_spectral(GEOMETRY *g, double cfl)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (!c->sr) {
+    fprintf(stderr, "kf: time step from sums requested before a residual with sums\n");
+    abort();
+  }
+  kf_launch_timestep_div(c, c->sr, cfl);
+}
 
 extern "C" void ComputeForces(const GEOMETRY *g, double *forces) { kf_launch_forces(kf_ctx_of(g), forces); }
 

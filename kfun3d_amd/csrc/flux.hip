@@ -88,9 +88,49 @@ __global__ __launch_bounds__(256) void k_grad(kf_dv d, const double *__restrict_
 
 /* ---- Roe-type edge flux: src/flux.c:151-448, one thread per edge --------- */
 
+/* edge term of the local time step (src/timestep.c:58-101), from the states
+ * at the two end points: |u_avg . n ln| + c ln */
+__device__ __forceinline__ double spectral_term(const d4 &qa, const d4 &qb, double xn, double yn, double zn,
+                                                double ln)
+{
+  const double xnorm = xn * ln, ynorm = yn * ln, znorm = zn * ln;
+  const double u = 0.5 * (qa.u + qb.u), vv = 0.5 * (qa.v + qb.v), w = 0.5 * (qa.w + qb.w);
+  const double ubar = xn * u + yn * vv + zn * w;
+  const double c = sqrt(ubar * ubar + B_);
+  double term = u * xnorm;
+  term += vv * ynorm;
+  term += w * znorm;
+  return fabs(term) + c * ln;
+}
+
+/* boundary-node terms of the local time step (src/timestep.c:103-150): solid
+ * nodes, then free nodes, added to acc in that order */
+__device__ __forceinline__ double spectral_boundary(const kf_dv &d, uint32_t v, const d4 &qv, double acc)
+{
+  for (int sec = 0; sec < 2; sec++) {
+    const uint32_t *ptr = sec ? d.fnptr : d.snptr, *ent = sec ? d.fnent : d.snent;
+    const double *nx = sec ? d.fnx : d.snx, *ny = sec ? d.fny : d.sny, *nz = sec ? d.fnz : d.snz;
+    for (uint32_t s = ptr[v]; s < ptr[v + 1]; s++) {
+      const uint32_t i = ent[s];
+      const double xn = nx[i], yn = ny[i], zn = nz[i];
+      const double ln = sqrt(xn * xn + yn * yn + zn * zn);
+      const double ubar = qv.u * xn + qv.v * yn + qv.w * zn;
+      const double ubar_ = ubar / ln;
+      const double c = sqrt(ubar_ * ubar_ + B_);
+      const double Vn = fabs(ubar) + c * ln;
+      acc += Vn;
+    }
+  }
+  return acc;
+}
+
+/* SPECTRAL: also the edge's time-step term into eterm (the driver's first
+ * residual of a pseudo-time step, whose state the time step then uses) */
+template <bool SPECTRAL>
 __global__ __launch_bounds__(256) void k_edge_flux(kf_dv d, const double *__restrict__ q,
                                                    const double *__restrict__ gx, const double *__restrict__ gy,
-                                                   const double *__restrict__ gz, double *__restrict__ eres)
+                                                   const double *__restrict__ gz, double *__restrict__ eres,
+                                                   double *__restrict__ eterm)
 {
   const uint32_t i = kf_xcd_block() * 256u + threadIdx.x;
   if (i >= d.E) return;
@@ -170,6 +210,7 @@ __global__ __launch_bounds__(256) void k_edge_flux(kf_dv d, const double *__rest
   const double res3 = 0.5 * (fluxp3 + fluxm3 - ln * t3);
   const double res4 = 0.5 * (fluxp4 + fluxm4 - ln * t4);
   st4(eres, i, res1, res2, res3, res4);
+  if (SPECTRAL) eterm[i] = spectral_term(q0, q1, xn, yn, zn, ln);
 }
 
 /* ---- slip-wall face pressure terms: src/flux.c:477-537 ------------------- */
@@ -247,12 +288,17 @@ __device__ __forceinline__ void freestream_term(const kf_dv &d, uint32_t i, cons
 /* ---- per-vertex residual: edges (edge order), walls (colour order), free
  * stream, exactly the reference's order of updates to r[v] --------------- */
 
+/* SPECTRAL: also the vertex's time-step sum (edge terms in edge order, then
+ * the boundary-node terms: k_timestep's order) into sr, before the division
+ * by the CFL */
+template <bool SPECTRAL>
 __global__ __launch_bounds__(256) void k_residual(kf_dv d, const double *__restrict__ q,
-                                                  const double *__restrict__ eres, double *__restrict__ r)
+                                                  const double *__restrict__ eres, double *__restrict__ r,
+                                                  const double *__restrict__ eterm, double *__restrict__ sr)
 {
   const uint32_t v = kf_xcd_block() * 256u + threadIdx.x;
   if (v >= d.N) return;
-  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0, acc = 0.0;
   const uint32_t deg = d.deg[v];
   size_t idx = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
   for (uint32_t k = 0; k < deg; k++, idx += 64) {
@@ -260,7 +306,9 @@ __global__ __launch_bounds__(256) void k_residual(kf_dv d, const double *__restr
     const d4 e = ld4(eres, ent >> 1);
     if (ent & 1u) { r0 = r0 - e.p; r1 = r1 - e.u; r2 = r2 - e.v; r3 = r3 - e.w; }
     else { r0 = r0 + e.p; r1 = r1 + e.u; r2 = r2 + e.v; r3 = r3 + e.w; }
+    if (SPECTRAL) acc = acc + eterm[ent >> 1];
   }
+  if (SPECTRAL) sr[v] = spectral_boundary(d, v, ld4(q, v), acc);
   for (uint32_t s = d.sfptr[v]; s < d.sfptr[v + 1]; s++) {
     const uint32_t ent = d.sfent[s];
     double xn, yn, zn, pc;
@@ -279,7 +327,7 @@ __global__ __launch_bounds__(256) void k_residual(kf_dv d, const double *__restr
   st4(r, v, r0, r1, r2, r3);
 }
 
-void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r)
+void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r, double *sr)
 {
   kf_check_aligned(q, "q"); kf_check_aligned(r, "r");
   kf_check_aligned(gx, "grad->x0"); kf_check_aligned(gy, "grad->x1"); kf_check_aligned(gz, "grad->x2");
@@ -290,11 +338,23 @@ void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   s = kf_prof_begin(c, KP_EDGE);
-  hipLaunchKernelGGL(k_edge_flux, dim3(kf_blocks_for(d.E, 256)), dim3(256), 0, c->stream, d, q, gx, gy, gz, d.eres);
+  if (sr) {
+    if (!c->eterm) KF_CHECK(hipMalloc((void **)&c->eterm, (size_t)d.E * sizeof(double)));
+    hipLaunchKernelGGL(k_edge_flux<true>, dim3(kf_blocks_for(d.E, 256)), dim3(256), 0, c->stream, d, q, gx, gy, gz,
+                       d.eres, c->eterm);
+  } else {
+    hipLaunchKernelGGL(k_edge_flux<false>, dim3(kf_blocks_for(d.E, 256)), dim3(256), 0, c->stream, d, q, gx, gy, gz,
+                       d.eres, (double *)NULL);
+  }
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   s = kf_prof_begin(c, KP_RES);
-  hipLaunchKernelGGL(k_residual, dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q, d.eres, r);
+  if (sr)
+    hipLaunchKernelGGL(k_residual<true>, dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q, d.eres, r,
+                       c->eterm, sr);
+  else
+    hipLaunchKernelGGL(k_residual<false>, dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q, d.eres, r,
+                       (const double *)NULL, (double *)NULL);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
 }
@@ -314,38 +374,33 @@ __global__ __launch_bounds__(256) void k_timestep(kf_dv d, const double *__restr
     const uint32_t ent = d.sell_ent[idx], e = ent >> 1;
     const d4 qn = ld4(q, d.sell_nbr[idx]);
     const d4 qa = (ent & 1u) ? qn : qv, qb = (ent & 1u) ? qv : qn;   /* (n0, n1) */
-    const double xn = d.enx[e], yn = d.eny[e], zn = d.enz[e], ln = d.ea[e];
-    const double xnorm = xn * ln, ynorm = yn * ln, znorm = zn * ln;
-    const double u = 0.5 * (qa.u + qb.u), vv = 0.5 * (qa.v + qb.v), w = 0.5 * (qa.w + qb.w);
-    const double ubar = xn * u + yn * vv + zn * w;
-    const double c = sqrt(ubar * ubar + B_);
-    double term = u * xnorm;
-    term += vv * ynorm;
-    term += w * znorm;
-    term = fabs(term) + c * ln;
-    acc = acc + term;
+    acc = acc + spectral_term(qa, qb, d.enx[e], d.eny[e], d.enz[e], d.ea[e]);
   }
-  for (int sec = 0; sec < 2; sec++) {
-    const uint32_t *ptr = sec ? d.fnptr : d.snptr, *ent = sec ? d.fnent : d.snent;
-    const double *nx = sec ? d.fnx : d.snx, *ny = sec ? d.fny : d.sny, *nz = sec ? d.fnz : d.snz;
-    for (uint32_t s = ptr[v]; s < ptr[v + 1]; s++) {
-      const uint32_t i = ent[s];
-      const double xn = nx[i], yn = ny[i], zn = nz[i];
-      const double ln = sqrt(xn * xn + yn * yn + zn * zn);
-      const double ubar = qv.u * xn + qv.v * yn + qv.w * zn;
-      const double ubar_ = ubar / ln;
-      const double c = sqrt(ubar_ * ubar_ + B_);
-      const double Vn = fabs(ubar) + c * ln;
-      acc += Vn;
-    }
-  }
-  cdt[v] = acc / cfl;
+  cdt[v] = spectral_boundary(d, v, qv, acc) / cfl;
+}
+
+/* cdt = sr / cfl from the sums the driver's residual left (same rounding as
+ * k_timestep: the sum, then one division) */
+__global__ __launch_bounds__(256) void k_timestep_div(uint32_t N, const double *__restrict__ sr, double cfl,
+                                                      double *__restrict__ cdt)
+{
+  const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+  if (v < N) cdt[v] = sr[v] / cfl;
 }
 
 void kf_launch_timestep(kf_ctx *c, double cfl)
 {
   const int s = kf_prof_begin(c, KP_TSTEP);
   hipLaunchKernelGGL(k_timestep, dim3(kf_blocks_for(c->d.N, 256)), dim3(256), 0, c->stream, c->d, c->q, cfl, c->cdt);
+  KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
+}
+
+void kf_launch_timestep_div(kf_ctx *c, const double *sr, double cfl)
+{
+  const int s = kf_prof_begin(c, KP_TSTEP);
+  hipLaunchKernelGGL(k_timestep_div, dim3(kf_blocks_for(c->d.N, 256)), dim3(256), 0, c->stream, c->d.N, sr, cfl,
+                     c->cdt);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
 }

@@ -183,51 +183,54 @@ __global__ __launch_bounds__(64) void k_ilu(kf_dv d, const uint32_t *__restrict_
   }
 }
 
-/* Shared-memory layout of k_ilu_staged, sized on the host from the mesh:
- * maxrow blocks per row, umax = most upper entries over the rows m of one
- * row's lower pattern, smax = most staged blocks of one row. */
+/* Shared-memory layout of k_ilu_rows, sized on the host from the plan:
+ * maxrow blocks per row, maxlow lower blocks per row, maxmatch update pairs
+ * per row. */
 struct ilu_layout {
-  uint32_t scol, sa, sdg, soff, sjm, slo, sslot, smjm, stage, bytes;
-  __host__ __device__ ilu_layout(uint32_t maxrow, uint32_t umax, uint32_t smax, uint32_t tsize)
+  uint32_t scol, sa, sa0, sldg, smoff, sjm, slo, stage, bytes;
+  __host__ __device__ ilu_layout(uint32_t maxrow, uint32_t maxlow, uint32_t maxmatch, uint32_t tsize, bool exact)
   {
     uint32_t o = 0;
     auto take = [&o](uint32_t n) { const uint32_t r = o; o += (n + 15u) & ~15u; return r; };
     scol = take(4 * maxrow);
     sa = take(16 * tsize * maxrow);
-    sdg = take(4 * maxrow);
-    soff = take(4 * (maxrow + 1));
-    sjm = take(4 * umax);
-    slo = take(umax);
-    sslot = take(2 * umax);
-    smjm = take(4 * smax);
-    stage = take(16 * tsize * smax);
+    sa0 = exact ? take(16 * tsize * maxrow) : 0;
+    sldg = take(4 * maxlow);
+    smoff = take(4 * (maxlow + 1));
+    sjm = take(4 * maxmatch);
+    slo = take(2 * maxmatch);
+    stage = take(16 * tsize * ((exact ? 3 : 1) * maxlow + maxmatch));
     bytes = o;
   }
 };
 
-/* k_ilu with every block the row reads from other rows staged in LDS by one
- * round of loads after the wait, instead of fetching them row by row of the
- * lower pattern.  Before the wait (read-only pattern data and the row's own
- * blocks only): for each lower block m the upper entries (m, k) of row m and
- * the position of column k in row i (or none); the L_old blocks.  After the
- * acquire: inv(U(m,m)), U(m,m) and the matching U(m,k) blocks, all loads in
- * flight together.  Arithmetic and its order are those of k_ilu. */
+#define ILU_Q 4    /* values per lane per load round before the wait */
+#define ILU_QB 12  /* values per lane per staging round after the wait */
+
+/* k_ilu with the row's schedule precomputed on the host (kf_plan_ilu): a
+ * record per ticket, the diagonal position of every row it reads from and
+ * the list of update pairs (U(m,k) block, position of k in row i).  Before
+ * the wait the wave loads, in one round, its row, those tables and its
+ * L_old blocks; after the acquire it stages inv(U(m,m)), U(m,m) and the
+ * U(m,k) blocks in LDS in one round; the rest runs from LDS.  Arithmetic and
+ * its order are those of k_ilu. */
 template <typename T, bool EXACT>
-__global__ __launch_bounds__(64) void k_ilu_staged(kf_dv d, const uint32_t *__restrict__ rows, uint32_t nrows,
-                                                   uint32_t *ticket, uint32_t base, uint32_t *rdone, uint32_t epoch,
-                                                   T *a, T *lold, T *ud, uint32_t maxrow, uint32_t umax,
-                                                   uint32_t smax)
+__global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__restrict__ rec, uint32_t nrows,
+                                                 uint32_t *ticket, uint32_t base, uint32_t *rdone, uint32_t epoch,
+                                                 T *a, T *lold, T *ud, const uint32_t *__restrict__ ldiag,
+                                                 const uint32_t *__restrict__ moff, const uint32_t *__restrict__ mjm,
+                                                 const uint16_t *__restrict__ mlo, uint32_t maxrow, uint32_t maxlow,
+                                                 uint32_t maxmatch)
 {
   extern __shared__ __align__(16) unsigned char ilu_lds[];
-  const ilu_layout lay(maxrow, umax, smax, sizeof(T));
+  const ilu_layout lay(maxrow, maxlow, maxmatch, sizeof(T), EXACT);
   uint32_t *scol = (uint32_t *)(ilu_lds + lay.scol);
   T *sa = (T *)(ilu_lds + lay.sa);
-  uint32_t *sdg = (uint32_t *)(ilu_lds + lay.sdg);
-  uint32_t *soff = (uint32_t *)(ilu_lds + lay.soff);
+  T *sa0 = (T *)(ilu_lds + lay.sa0);
+  uint32_t *sldg = (uint32_t *)(ilu_lds + lay.sldg);
+  uint32_t *smoff = (uint32_t *)(ilu_lds + lay.smoff);
   uint32_t *sjm = (uint32_t *)(ilu_lds + lay.sjm);
-  uint8_t *slo = (uint8_t *)(ilu_lds + lay.slo);
-  uint16_t *sslot = (uint16_t *)(ilu_lds + lay.sslot);
-  uint32_t *smjm = (uint32_t *)(ilu_lds + lay.smjm);
+  uint16_t *slo = (uint16_t *)(ilu_lds + lay.slo);
   T *stage = (T *)(ilu_lds + lay.stage);
   const uint32_t lane = threadIdx.x;
   const uint32_t e = lane & 15, er = e >> 2, ec = e & 3, grp = lane >> 4;
@@ -235,84 +238,86 @@ __global__ __launch_bounds__(64) void k_ilu_staged(kf_dv d, const uint32_t *__re
   for (;;) {
     const uint32_t t = next_ticket(ticket, base);
     if (t >= nrows) break;
-    const uint32_t i = rows[t];
-    const uint32_t j0 = d.rowptr[i], j1 = d.rowptr[i + 1], jd = d.diag[i];
-    const uint32_t L = j1 - j0, nlow = jd - j0, pd = jd - j0;
+    const uint32_t rv = lane < 6 ? rec[8 * (size_t)t + lane] : 0;
+    const uint32_t i = __shfl(rv, 0, 64), j0 = __shfl(rv, 1, 64), L = __shfl(rv, 2, 64);
+    const uint32_t nlow = __shfl(rv, 3, 64), mbeg = __shfl(rv, 4, 64), mcnt = __shfl(rv, 5, 64);
+    const uint32_t pd = nlow;
     /* staged slots: [0, nlow) inv(U(m,m)); EXACT: [nlow, 2 nlow) U(m,m),
-     * [2 nlow, 3 nlow) L_old; then the matching U(m,k) */
+     * [2 nlow, 3 nlow) L_old; then the U(m,k) blocks of the update pairs */
     const uint32_t mbase = EXACT ? 3 * nlow : nlow;
-    for (uint32_t k = lane; k < L; k += 64) scol[k] = d.col[j0 + k];
-    for (uint32_t k = lane; k < 16 * L; k += 64) sa[k] = a[16 * (size_t)j0 + k];
-    __syncthreads();
-    for (uint32_t p = lane; p < nlow; p += 64) {
-      const uint32_t m = scol[p], dg = d.diag[m];
-      sdg[p] = dg;
-      soff[p + 1] = d.rowptr[m + 1] - dg - 1;
-    }
-    __syncthreads();
-    if (lane == 0) {
-      soff[0] = 0;
-      for (uint32_t p = 0; p < nlow; p++) soff[p + 1] += soff[p];
-    }
-    __syncthreads();
-    const uint32_t U = soff[nlow];
-    uint32_t nmatch = 0;
-    for (uint32_t b0 = 0; b0 < U; b0 += 64) {
-      const uint32_t idx = b0 + lane;
-      bool hit = false;
-      uint32_t lo = 0, jm = 0;
-      if (idx < U) {
-        uint32_t pl = 0, ph = nlow - 1;   /* last p with soff[p] <= idx */
-        while (pl < ph) {
-          const uint32_t mid = (pl + ph + 1) >> 1;
-          if (soff[mid] <= idx) pl = mid;
-          else ph = mid - 1;
+    /* one round of loads: row i, the tables, L_old (rows beyond the
+     * register round take the loops below) */
+    {
+      T av[ILU_Q], lv[ILU_Q];
+      uint32_t cv = 0, gv = 0, mo = 0, jv = 0;
+      uint16_t ov = 0;
+#pragma unroll
+      for (int q = 0; q < ILU_Q; q++) {
+        const uint32_t k = lane + 64 * q;
+        if (k < 16 * L) av[q] = a[16 * (size_t)j0 + k];
+        if (EXACT && k < 16 * nlow) {
+          const uint32_t pp = k >> 4, ee = k & 15;
+          lv[q] = pp == 0 ? a[16 * ((size_t)j0 + nlow - 1) + ee] : lold[16 * ((size_t)j0 + pp) + ee];
         }
-        jm = sdg[pl] + 1 + (idx - soff[pl]);
-        const uint32_t kc = d.col[jm];
-        uint32_t l0 = 0, l1 = L;
-        while (l0 < l1) {
-          const uint32_t mid = (l0 + l1) >> 1;
-          if (scol[mid] < kc) l0 = mid + 1;
-          else l1 = mid;
+      }
+      if (lane < L) cv = d.col[j0 + lane];
+      if (lane < nlow) { gv = ldiag[j0 + lane]; mo = moff[j0 + lane]; }
+      if (lane < mcnt) { jv = mjm[mbeg + lane]; ov = mlo[mbeg + lane]; }
+#pragma unroll
+      for (int q = 0; q < ILU_Q; q++) {
+        const uint32_t k = lane + 64 * q;
+        if (k < 16 * L) {
+          sa[k] = av[q];
+          if (EXACT) sa0[k] = av[q];
         }
-        lo = l0;
-        hit = l0 < L && scol[l0] == kc;
-        sjm[idx] = jm;
+        if (EXACT && k < 16 * nlow) stage[32 * nlow + k] = lv[q];
       }
-      const uint64_t mask = __ballot(hit);
-      const uint32_t slot = mbase + nmatch + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      if (idx < U) {
-        slo[idx] = hit ? (uint8_t)lo : (uint8_t)0xff;
-        sslot[idx] = hit ? (uint16_t)slot : (uint16_t)0xffff;
-        if (hit) smjm[slot] = jm;
+      if (lane < L) scol[lane] = cv;
+      if (lane < nlow) { sldg[lane] = gv; smoff[lane] = mo; }
+      if (lane < mcnt) { sjm[lane] = jv; slo[lane] = ov; }
+      if (lane == 0) smoff[nlow] = mcnt;
+      for (uint32_t k = lane + 64 * ILU_Q; k < 16 * L; k += 64) {
+        const T v = a[16 * (size_t)j0 + k];
+        sa[k] = v;
+        if (EXACT) sa0[k] = v;
       }
-      nmatch += (uint32_t)__popcll(mask);
+      if (EXACT)
+        for (uint32_t k = lane + 64 * ILU_Q; k < 16 * nlow; k += 64)
+          stage[32 * nlow + k] = lold[16 * ((size_t)j0 + (k >> 4)) + (k & 15)];
+      for (uint32_t k = lane + 64; k < L; k += 64) scol[k] = d.col[j0 + k];
+      for (uint32_t k = lane + 64; k < nlow; k += 64) { sldg[k] = ldiag[j0 + k]; smoff[k] = moff[j0 + k]; }
+      for (uint32_t k = lane + 64; k < mcnt; k += 64) { sjm[k] = mjm[mbeg + k]; slo[k] = mlo[mbeg + k]; }
     }
-    const uint32_t nst = mbase + nmatch;
-    if (EXACT)
-      for (uint32_t k = lane; k < 16 * nlow; k += 64) {
-        const uint32_t p = k >> 4, ee = k & 15;
-        stage[16 * (2 * nlow + p) + ee] =
-            p == 0 ? a[16 * ((size_t)j0 + nlow - 1) + ee] : lold[16 * ((size_t)j0 + p) + ee];
-      }
     __syncthreads();
     for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err, aborted, d.spin_sleep);
     aborted = __any(aborted);
     consume_fence();
-    /* one round of loads: slots other than L_old */
+    /* staging after the acquire, ILU_QB values per lane per round */
     {
-      const uint32_t npost = (EXACT ? 2 * nlow : nlow) + nmatch;
-#pragma unroll 4
-      for (uint32_t k = lane; k < 16 * npost; k += 64) {
-        uint32_t sl = k >> 4;
-        const uint32_t ee = k & 15;
-        if (EXACT && sl >= 2 * nlow) sl += nlow;   /* skip the L_old slots */
-        const T *src;
-        if (sl < nlow) src = a + 16 * (size_t)sdg[sl];
-        else if (EXACT && sl < 2 * nlow) src = ud + 16 * (size_t)scol[sl - nlow];
-        else src = a + 16 * (size_t)smjm[sl];
-        stage[16 * sl + ee] = src[ee];
+      const uint32_t npre = EXACT ? 2 * nlow : nlow;
+      const uint32_t tot = 16 * (npre + mcnt);
+      for (uint32_t b0 = 0; b0 < tot; b0 += 64 * ILU_QB) {
+        T v[ILU_QB];
+#pragma unroll
+        for (int q = 0; q < ILU_QB; q++) {
+          const uint32_t k = b0 + lane + 64 * q;
+          if (k < tot) {
+            const uint32_t sl = k >> 4, ee = k & 15;
+            const T *src;
+            if (sl < nlow) src = a + 16 * (size_t)sldg[sl];
+            else if (EXACT && sl < 2 * nlow) src = ud + 16 * (size_t)scol[sl - nlow];
+            else src = a + 16 * (size_t)sjm[sl - npre];
+            v[q] = src[ee];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < ILU_QB; q++) {
+          const uint32_t k = b0 + lane + 64 * q;
+          if (k < tot) {
+            const uint32_t sl = k >> 4;
+            stage[16 * (sl < npre ? sl : sl - npre + mbase) + (k & 15)] = v[q];
+          }
+        }
       }
     }
     __syncthreads();
@@ -335,22 +340,19 @@ __global__ __launch_bounds__(64) void k_ilu_staged(kf_dv d, const uint32_t *__re
       __syncthreads();
       if (lane < 16) sa[16 * p + e] = lv;
       __syncthreads();
-      /* a(i,k) -= L(i,m) U(m,k) for the upper blocks (m,k) of row m in row i */
-      for (uint32_t idx = soff[p] + grp; idx < soff[p + 1]; idx += 4) {
+      /* a(i,k) -= L(i,m) U(m,k) for the update pairs of m (distinct k) */
+      for (uint32_t idx = smoff[p] + grp; idx < smoff[p + 1]; idx += 4) {
         const uint32_t lo = slo[idx];
-        if (lo != 0xffu) {
-          const T s = blk_mul(sa + 16 * p, stage + 16 * (size_t)sslot[idx], er, ec);
-          sa[16 * lo + e] = sa[16 * lo + e] - s;
-        }
+        const T s = blk_mul(sa + 16 * p, stage + 16 * (size_t)(mbase + idx), er, ec);
+        sa[16 * lo + e] = sa[16 * lo + e] - s;
       }
       __syncthreads();
     }
-    (void)nst;
 
     if (EXACT) {
       /* upper blocks: a = (a - I A(i,k)) + I A(i,k), A = the fill's block */
       for (uint32_t p = pd + grp; p < L; p += 4) {
-        const T *a0 = a + 16 * ((size_t)j0 + p);
+        const T *a0 = sa0 + 16 * p;
         T dq = T(0);
 #pragma unroll
         for (int k = 0; k < 4; k++) dq += (er == (uint32_t)k ? T(1) : T(0)) * a0[4 * k + ec];
@@ -404,12 +406,12 @@ static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
 {
   const uint32_t nw = persistent_waves(c->d.N, c->ilu_waves);
   const uint32_t epoch = ++c->epoch;
-  const ilu_layout lay(c->maxrow, c->ilu_umax, c->ilu_smax, sizeof(T));
+  const ilu_layout lay(c->ilu_maxrow, c->ilu_maxlow, c->ilu_maxmatch, sizeof(T), EXACT);
   const int s = kf_prof_begin(c, KP_ILU);
   if (lay.bytes <= 64 * 1024)
-    hipLaunchKernelGGL((k_ilu_staged<T, EXACT>), dim3(nw), dim3(64), lay.bytes, c->stream, c->d, c->lrows, c->d.N,
-                       c->ticket + 2, c->ticket_base[2], c->rdone, epoch, a, lold, ud, c->maxrow, c->ilu_umax,
-                       c->ilu_smax);
+    hipLaunchKernelGGL((k_ilu_rows<T, EXACT>), dim3(nw), dim3(64), lay.bytes, c->stream, c->d, c->ilu_rec, c->d.N,
+                       c->ticket + 2, c->ticket_base[2], c->rdone, epoch, a, lold, ud, c->ilu_ldiag, c->ilu_moff,
+                       c->ilu_mjm, c->ilu_mlo, c->ilu_maxrow, c->ilu_maxlow, c->ilu_maxmatch);
   else
     hipLaunchKernelGGL((k_ilu<T, EXACT>), dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
                        c->ticket_base[2], c->rdone, epoch, a, lold, ud);

@@ -92,8 +92,11 @@ struct kf_ctx {
   kf_options opt;
   size_t nnzb;
   uint32_t maxrow;          /* most blocks in one BSR row */
-  uint32_t ilu_umax;        /* most upper entries of the rows one ILU row reads */
-  uint32_t ilu_smax;        /* most blocks one ILU row stages */
+  /* ILU(0) schedule (kf_plan_ilu): record per ticket, per-position and
+   * per-pair tables; the largest row, lower part and pair list */
+  uint32_t *ilu_rec, *ilu_ldiag, *ilu_moff, *ilu_mjm;
+  uint16_t *ilu_mlo;
+  uint32_t ilu_maxrow, ilu_maxlow, ilu_maxmatch;
   uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch */
   uint32_t nlev_l, nlev_u;
   /* level schedule */
@@ -103,8 +106,8 @@ struct kf_ctx {
   /* triangular solves: contiguous ranges of trsv_R rows, trsv_S = 16 / R
    * ranges of one DAG level per task (one wavefront) */
   uint32_t trsv_R, trsv_S;
-  uint8_t *rlvl_l, *rlvl_u;  /* local level of each row inside its range */
-  uint32_t *task_l, *task_u; /* ranges of each task, in ticket order */
+  uint32_t *trow_l, *trow_u; /* per task (ticket order) and row slot: row, block bounds */
+  uint8_t *tlvl_l, *tlvl_u;  /* per task and row slot: local level inside the range */
   uint8_t *tdep_l, *tdep_u;  /* local depth of each task */
   uint32_t ntask_l, ntask_u;
   double *ty64;             /* forward-sweep output (FP64) */
@@ -126,6 +129,7 @@ struct kf_ctx {
   double *gx, *gy, *gz, *r, *w, *t, *xk, *vecs;
   int nvecs;
   float *f32a, *f32b;       /* single-precision forward / backward outputs */
+  double *eterm, *sr;       /* per-edge time-step terms, per-vertex time-step sums */
 };
 
 static inline kf_ctx *kf_ctx_of(const GEOMETRY *g)
@@ -164,8 +168,10 @@ int kf_prof_begin(kf_ctx *c, int id);
 void kf_prof_end(kf_ctx *c, int slot);
 
 /* launches (defined in the .hip files) */
-void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r);
+/* sr != NULL: also the time-step sums of the state q (see k_residual) */
+void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r, double *sr);
 void kf_launch_timestep(kf_ctx *c, double cfl);
+void kf_launch_timestep_div(kf_ctx *c, const double *sr, double cfl);
 void kf_launch_forces(kf_ctx *c, double *forces_host);
 void kf_launch_fill(kf_ctx *c);
 void kf_launch_ilu(kf_ctx *c);

@@ -182,25 +182,26 @@ __device__ __forceinline__ bool wait_prefetched(const TV *v, const uint32_t cp[K
 /* forward sweep: y = L^-1 b; presets the backward output x of its rows to
  * the backward sentinel */
 template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ task, uint32_t S,
-                                                   uint32_t lg2R, const uint8_t *__restrict__ lvl,
-                                                   const uint8_t *__restrict__ tdep, uint32_t ntask,
-                                                   uint32_t *ticket, uint32_t base,
+__global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ trow,
+                                                   const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
+                                                   uint32_t lg2R, uint32_t ntask, uint32_t *ticket, uint32_t base,
                                                    typename sentinel<TV>::bits sy, typename sentinel<TV>::bits sx,
                                                    const TM *__restrict__ a, const double *__restrict__ b, TV *y,
                                                    TV *xnext)
 {
   __shared__ TV xs[64];
   const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
+  const uint32_t kr = qd & ((1u << lg2R) - 1u), sq = qd - kr;
   bool aborted = false;
+  uint32_t t = first_ticket(ticket, base);
   for (;;) {
-    const uint32_t t = next_ticket(ticket, base);
     if (t >= ntask) break;
-    const uint32_t slot = qd >> lg2R, g = task[(size_t)t * S + slot];
-    const uint32_t lo = g << lg2R, r = lo + (qd & ((1u << lg2R) - 1u)), sq = slot << lg2R;
-    const bool active = g != ~0u && r < d.N;
-    const uint32_t j0 = active ? d.rowptr[r] : 0, jd = active ? d.diag[r] : 0;
-    const uint32_t my = active ? lvl[r] : 0xffu, dep = tdep[t];
+    const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
+    const uint32_t r = rt[0], j0 = rt[1], jd = rt[2];
+    const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
+    const uint32_t tn = draw_ticket(ticket);   /* the next task's ticket, read at the end */
+    const bool active = r != ~0u;
+    const uint32_t lo = r - kr;                 /* first row of this lane's range */
     uint32_t cp[KF_TRSV_PF];
     TV mp[KF_TRSV_PF][4];
 #pragma unroll
@@ -251,34 +252,33 @@ __global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__re
       __syncthreads();
     }
     if (active) st_coherent(y + 4 * (size_t)r + c, xr);
+    t = take_ticket(tn, base);
   }
 }
 
 /* backward sweep: x = U^-1 y (y from the forward sweep); resets y of its rows
  * to the forward sentinel of the next call */
 template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ task, uint32_t S,
-                                                   uint32_t lg2R, const uint8_t *__restrict__ lvl,
-                                                   const uint8_t *__restrict__ tdep, uint32_t ntask,
-                                                   uint32_t *ticket, uint32_t base,
+__global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ trow,
+                                                   const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
+                                                   uint32_t lg2R, uint32_t ntask, uint32_t *ticket, uint32_t base,
                                                    typename sentinel<TV>::bits sx, typename sentinel<TV>::bits synext,
                                                    const TM *__restrict__ a, TV *y, TV *x, double *__restrict__ xout)
 {
   __shared__ TV xs[64];
   const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
-  const uint32_t last = d.N - 1;
+  const uint32_t kr = qd & ((1u << lg2R) - 1u), sq = qd - kr;
   bool aborted = false;
+  uint32_t t = first_ticket(ticket, base);
   for (;;) {
-    const uint32_t t = next_ticket(ticket, base);
     if (t >= ntask) break;
-    const uint32_t slot = qd >> lg2R, g = task[(size_t)t * S + slot], kr = qd & ((1u << lg2R) - 1u);
-    const uint32_t sq = slot << lg2R;
-    const bool valid = g != ~0u;
-    const uint32_t hi = valid ? last - (g << lg2R) : 0;
-    const bool active = valid && kr <= hi;
-    const uint32_t r = active ? hi - kr : 0;
-    const uint32_t jd = active ? d.diag[r] : 0, j0 = jd + 1, j1 = active ? d.rowptr[r + 1] : 0;
-    const uint32_t my = active ? lvl[r] : 0xffu, dep = tdep[t];
+    const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
+    const uint32_t rr = rt[0], jd = rt[1], j0 = jd + 1, j1 = rt[2];
+    const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
+    const uint32_t tn = draw_ticket(ticket);   /* the next task's ticket, read at the end */
+    const bool active = rr != ~0u;
+    const uint32_t r = active ? rr : 0;
+    const uint32_t hi = r + kr;                 /* top row of this lane's range */
     uint32_t cp[KF_TRSV_PF];
     TV mp[KF_TRSV_PF][4], dg[4];
     if (active) ld_row(a, jd, c, dg);
@@ -342,6 +342,7 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
       st_coherent(x + 4 * (size_t)r + c, res);
       if ((void *)xout != (void *)x) xout[4 * (size_t)r + c] = (double)res;
     }
+    t = take_ticket(tn, base);
   }
 }
 
@@ -368,15 +369,15 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
     KF_LAUNCH_CHECK();
   }
   int s = kf_prof_begin(c, KP_TRSV_L);
-  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64), 0, c->stream, c->d, c->task_l, c->trsv_S, lg2R,
-                     c->rlvl_l, c->tdep_l, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
+  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64), 0, c->stream, c->d, c->trow_l, c->tlvl_l, c->tdep_l,
+                     lg2R, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
                      sentinel<TV>::of(2 * k + 1), a, b, y, x);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   c->ticket_base[0] += c->ntask_l + nwl;
   s = kf_prof_begin(c, KP_TRSV_U);
-  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64), 0, c->stream, c->d, c->task_u, c->trsv_S, lg2R,
-                     c->rlvl_u, c->tdep_u, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
+  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64), 0, c->stream, c->d, c->trow_u, c->tlvl_u, c->tdep_u,
+                     lg2R, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
                      sentinel<TV>::of(2 * k + 2), a, y, x, xout);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);

@@ -64,6 +64,28 @@ __device__ __forceinline__ bool wait_row(const uint32_t *rdone, uint32_t row, ui
   return true;
 }
 
+/* Tickets drawn one task ahead: draw_ticket issues the atomic (lane 0) and
+ * returns at once; take_ticket, at the end of the current task, waits for the
+ * value and gives it to every lane.  A wave works through its tickets in the
+ * order drawn, so the oldest unfinished ticket always belongs to the task its
+ * wave is working on and the hand-off stays deadlock-free. */
+__device__ __forceinline__ uint32_t draw_ticket(uint32_t *ticket)
+{
+  uint32_t t = 0;
+  if (threadIdx.x == 0) t = atomicAdd(ticket, 1u);
+  return t;
+}
+
+__device__ __forceinline__ uint32_t take_ticket(uint32_t drawn, uint32_t base)
+{
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)drawn) - base;
+}
+
+__device__ __forceinline__ uint32_t first_ticket(uint32_t *ticket, uint32_t base)
+{
+  return take_ticket(draw_ticket(ticket), base);
+}
+
 /* next work ticket of this wave (lane 0 draws, all lanes receive it) */
 __device__ __forceinline__ uint32_t next_ticket(uint32_t *ticket, uint32_t base)
 {

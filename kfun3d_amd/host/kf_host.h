@@ -47,9 +47,27 @@ typedef struct kf_trsv_plan {
   uint8_t *lvl;        /* [N] local level of each row inside its range */
   uint32_t *task;      /* [ntask * S] ranges of each task, ~0u = none */
   uint8_t *tdep;       /* [ntask] local depth of each task */
+  uint32_t *trow;      /* [ntask * 16 * 3] per task and row slot: row, then
+                          forward: first block, diagonal position;
+                          backward: diagonal position, row end (~0u, 0, 0
+                          for an empty slot) */
+  uint8_t *tlvl;       /* [ntask * 16] local level of the slot's row, 0xff empty */
 } kf_trsv_plan;
 void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, kf_trsv_plan *out);
 void kf_free_trsv_plan(kf_trsv_plan *t);
+
+/* ILU(0) rows in ticket (level) order (plan.c) */
+typedef struct kf_ilu_plan {
+  uint32_t *rec;       /* [8 N] per ticket: i, j0, L, nlow, mbeg, mcnt, 0, 0 */
+  uint32_t *ldiag;     /* [nnzb] lower position j of row i: diag of row col[j] */
+  uint32_t *moff;      /* [nnzb] lower position j: its first match, from mbeg */
+  uint32_t *mjm;       /* [nmatch] block (m, k) of row m's upper part ... */
+  uint16_t *mlo;       /* [nmatch] ... and the position of column k in row i */
+  size_t nmatch;
+  uint32_t maxrow, maxlow, maxmatch;
+} kf_ilu_plan;
+void kf_plan_ilu(const kf_prep *p, kf_ilu_plan *out);
+void kf_free_ilu_plan(kf_ilu_plan *t);
 
 /* .msh reader (src/mesh.c:22-154 without the device part) */
 void kf_read_msh(const char *fpath, GEOMETRY *g);
@@ -97,6 +115,11 @@ void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int
 void kfs_zero(GEOMETRY *g, size_t n, double *x);
 void kfs_cgs_dots(GEOMETRY *g, size_t n, const double *vbase, int nv, const double *w, int slot);
 void kfs_cgs_update(GEOMETRY *g, size_t n, const double *vbase, int nv, int slot, double *w);
+/* residual of q together with its time-step sums, then cdt = sums / cfl
+ * (src/timestep.c folded into the residual pass; q must not change between
+ * the two calls) */
+void kfs_flux_spectral(GEOMETRY *g, const double *q, GRADIENT *grad, double *r);
+void kfs_timestep_spectral(GEOMETRY *g, double cfl);
 /* Kernel() work vectors: grad x/y/z, r, w, t, x, and nbasis Krylov vectors */
 void kfs_work(GEOMETRY *g, int nbasis, double **grad3, double **r, double **w, double **t, double **x, double **vecs);
 

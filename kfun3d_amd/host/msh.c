@@ -64,6 +64,7 @@ static uint32_t *take_u32(cursor *c, size_t n)
 {
   const unsigned char *p = take(c, 4 * n);
   uint32_t *out = (uint32_t *)kf_xmalloc(n * sizeof(uint32_t));
+#pragma omp parallel for schedule(static) if (n > 65536)
   for (size_t i = 0; i < n; i++) out[i] = rd_u32(p + 4 * i);
   return out;
 }
@@ -72,6 +73,7 @@ static double *take_f64(cursor *c, size_t n)
 {
   const unsigned char *p = take(c, 8 * n);
   double *out = (double *)kf_xmalloc(n * sizeof(double));
+#pragma omp parallel for schedule(static) if (n > 65536)
   for (size_t i = 0; i < n; i++) out[i] = rd_f64(p + 8 * i);
   return out;
 }

@@ -104,6 +104,26 @@ void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, kf_trsv_plan *out)
     fprintf(stderr, "kf: triangular-solve plan wrote %zu of %zu tasks\n", k, nt);
     abort();
   }
+  /* per task and row slot q (range slot q / R, row q % R of that range):
+   * the row, its block range and its local level; ~0u / 0 / 0 / 0xff for a
+   * slot without a row */
+  uint32_t *trow = (uint32_t *)kf_xmalloc(nt * KF_TASK_ROWS * 3 * sizeof(uint32_t));
+  uint8_t *tlvl = (uint8_t *)kf_xmalloc(nt * KF_TASK_ROWS + 1);
+  for (size_t tt = 0; tt < nt; tt++)
+    for (uint32_t q = 0; q < KF_TASK_ROWS; q++) {
+      const uint32_t g = task[tt * S + q / R], kr = q % R;
+      const size_t o = tt * KF_TASK_ROWS + q;
+      uint32_t r = ~0u, ja = 0, jb = 0;
+      uint8_t l = 0xff;
+      if (g != ~0u && (size_t)g * R + kr < N) {
+        r = upper ? (uint32_t)(N - 1 - ((size_t)g * R + kr)) : g * R + kr;
+        ja = upper ? p->diag[r] : p->rowptr[r];
+        jb = upper ? p->rowptr[r + 1] : p->diag[r];
+        l = lvl[r];
+      }
+      trow[3 * o] = r; trow[3 * o + 1] = ja; trow[3 * o + 2] = jb;
+      tlvl[o] = l;
+    }
   free(ord);
   free(begin);
   free(lev);
@@ -114,6 +134,8 @@ void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, kf_trsv_plan *out)
   out->lvl = lvl;
   out->task = task;
   out->tdep = tdep;
+  out->trow = trow;
+  out->tlvl = tlvl;
 }
 
 void kf_free_trsv_plan(kf_trsv_plan *t)
@@ -121,12 +143,14 @@ void kf_free_trsv_plan(kf_trsv_plan *t)
   free(t->lvl);
   free(t->task);
   free(t->tdep);
+  free(t->trow);
+  free(t->tlvl);
   memset(t, 0, sizeof(*t));
 }
 
 /* exported for tests (include/kfun3d.h) */
 size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, unsigned *task, unsigned char *tdep,
-                         unsigned char *lvl, size_t cap)
+                         unsigned char *lvl, unsigned *rowtab, size_t cap)
 {
   kf_prep p;
   kf_build_prep(g, &p);
@@ -138,7 +162,123 @@ size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, unsigned *tas
     memcpy(tdep, t.tdep, nt);
     memcpy(lvl, t.lvl, p.N);
   }
+  if (rowtab && cap >= nt) memcpy(rowtab, t.trow, nt * KF_TASK_ROWS * 3 * sizeof(uint32_t));
   kf_free_trsv_plan(&t);
   kf_free_prep(&p);
   return nt;
+}
+
+/* ILU(0) rows in ticket (level) order with everything a row needs to know
+ * about the rows it reads from, so that a wavefront fetches it in one round:
+ * per ticket the record {i, j0, L, nlow, mbeg, mcnt}; per lower position j
+ * of row i (block (i, m), m = col[j]) the diagonal position of row m and the
+ * start of its matches; per match the block (m, k) of row m's upper part and
+ * the position of column k in row i (matrix/ilu.cpp:49-97 walks the same
+ * pairs by merging the two rows). */
+/* position of column k in the sorted row [j0, j1), or ~0u */
+static uint32_t find_in_row(const uint32_t *col, uint32_t j0, uint32_t j1, uint32_t k)
+{
+  uint32_t lo = j0, hi = j1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (col[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < j1 && col[lo] == k) ? lo - j0 : ~0u;
+}
+
+/* update pairs of row i (written when mjm is not NULL); returns their count */
+static uint32_t ilu_row_pairs(const kf_prep *p, uint32_t i, uint32_t *ldiag, uint32_t *moff, uint32_t *mjm,
+                              uint16_t *mlo)
+{
+  const uint32_t j0 = p->rowptr[i], j1 = p->rowptr[i + 1], jd = p->diag[i];
+  uint32_t n = 0;
+  for (uint32_t j = j0; j < jd; j++) {
+    const uint32_t m = p->col[j];
+    if (mjm) { ldiag[j] = p->diag[m]; moff[j] = n; }
+    for (uint32_t jm = p->diag[m] + 1; jm < p->rowptr[m + 1]; jm++) {
+      const uint32_t lo = find_in_row(p->col, j0, j1, p->col[jm]);
+      if (lo == ~0u) continue;
+      if (mjm) { mjm[n] = jm; mlo[n] = (uint16_t)lo; }
+      n++;
+    }
+  }
+  return n;
+}
+
+/* ILU(0) rows in ticket (level) order with everything a row needs to know
+ * about the rows it reads from, so that a wavefront fetches it in one round:
+ * per ticket the record {i, j0, L, nlow, mbeg, mcnt}; per lower position j
+ * of row i (block (i, m), m = col[j]) the diagonal position of row m and the
+ * start of its pairs; per pair the block (m, k) of row m's upper part and
+ * the position of column k in row i (matrix/ilu.cpp:49-97 walks the same
+ * pairs by merging the two rows).  Two passes over the rows (count, then
+ * fill at the prefix offsets), each in parallel. */
+void kf_plan_ilu(const kf_prep *p, kf_ilu_plan *out)
+{
+  const size_t N = p->N;
+  uint32_t *rec = (uint32_t *)kf_xcalloc(8 * N + 8, sizeof(uint32_t));
+  uint32_t *ldiag = (uint32_t *)kf_xcalloc(p->nnzb + 1, sizeof(uint32_t));
+  uint32_t *moff = (uint32_t *)kf_xcalloc(p->nnzb + 1, sizeof(uint32_t));
+  uint32_t maxrow = 0, maxlow = 0, maxmatch = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(max : maxrow, maxlow, maxmatch)
+  for (size_t t = 0; t < N; t++) {
+    const uint32_t i = p->lrows[t];
+    const uint32_t L = p->rowptr[i + 1] - p->rowptr[i], nlow = p->diag[i] - p->rowptr[i];
+    const uint32_t mc = ilu_row_pairs(p, i, NULL, NULL, NULL, NULL);
+    uint32_t *r = rec + 8 * t;
+    r[0] = i; r[1] = p->rowptr[i]; r[2] = L; r[3] = nlow; r[5] = mc;
+    if (L > maxrow) maxrow = L;
+    if (nlow > maxlow) maxlow = nlow;
+    if (mc > maxmatch) maxmatch = mc;
+  }
+  size_t nm = 0;
+  for (size_t t = 0; t < N; t++) {
+    if (nm > 0xffffffffu - rec[8 * t + 5]) { fprintf(stderr, "kf: ILU pair table exceeds 32-bit offsets\n"); abort(); }
+    rec[8 * t + 4] = (uint32_t)nm;
+    nm += rec[8 * t + 5];
+  }
+  uint32_t *mjm = (uint32_t *)kf_xmalloc((nm + 1) * sizeof(uint32_t));
+  uint16_t *mlo = (uint16_t *)kf_xmalloc((nm + 1) * sizeof(uint16_t));
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (size_t t = 0; t < N; t++) {
+    const uint32_t *r = rec + 8 * t;
+    ilu_row_pairs(p, r[0], ldiag, moff, mjm + r[4], mlo + r[4]);
+  }
+  out->rec = rec;
+  out->ldiag = ldiag;
+  out->moff = moff;
+  out->mjm = mjm;
+  out->mlo = mlo;
+  out->nmatch = nm;
+  out->maxrow = maxrow;
+  out->maxlow = maxlow;
+  out->maxmatch = maxmatch;
+}
+
+void kf_free_ilu_plan(kf_ilu_plan *t)
+{
+  free(t->rec); free(t->ldiag); free(t->moff); free(t->mjm); free(t->mlo);
+  memset(t, 0, sizeof(*t));
+}
+
+/* exported for tests (include/kfun3d.h) */
+size_t kf_ilu_plan_host(const GEOMETRY *g, unsigned *rec, unsigned *ldiag, unsigned *moff, unsigned *mjm,
+                        unsigned short *mlo, size_t cap)
+{
+  kf_prep p;
+  kf_build_prep(g, &p);
+  kf_ilu_plan t;
+  kf_plan_ilu(&p, &t);
+  const size_t nm = t.nmatch;
+  if (rec && ldiag && moff && mjm && mlo && cap >= nm) {
+    memcpy(rec, t.rec, 8 * p.N * sizeof(uint32_t));
+    memcpy(ldiag, t.ldiag, p.nnzb * sizeof(uint32_t));
+    memcpy(moff, t.moff, p.nnzb * sizeof(uint32_t));
+    memcpy(mjm, t.mjm, nm * sizeof(uint32_t));
+    memcpy(mlo, t.mlo, nm * sizeof(uint16_t));
+  }
+  kf_free_ilu_plan(&t);
+  kf_free_prep(&p);
+  return nm;
 }

@@ -54,8 +54,13 @@ static void index_sort_rec(const uint32_t *key, uint32_t *l, uint32_t *h, uint32
   }
   uint32_t *m = l + (h - l) / 2;
   uint32_t *mt = tmp + (m - l), *ht = tmp + (h - l);
+  /* the two halves are independent: large ones run as tasks (the merge
+   * tree, and so the result, is that of the sequential recursion) */
+#pragma omp task if (h - l > (1 << 16))
   index_sort_rec(key, l, m, tmp, !in_place);
+#pragma omp task if (h - l > (1 << 16))
   index_sort_rec(key, m, h, mt, !in_place);
+#pragma omp taskwait
   if (in_place) merge_runs(key, tmp, mt, mt, ht, l);
   else merge_runs(key, l, m, m, h, tmp);
 }
@@ -65,6 +70,8 @@ void kf_index_sort(size_t sz, const uint32_t *key, uint32_t *perm)
   for (size_t i = 0; i < sz; i++) perm[i] = (uint32_t)i;
   if (sz == 0) return;
   uint32_t *tmp = (uint32_t *)kf_xmalloc(sz * sizeof(uint32_t));
+#pragma omp parallel
+#pragma omp single
   index_sort_rec(key, perm, perm + sz, tmp, 1);
   free(tmp);
 }
@@ -333,6 +340,7 @@ void kf_build_prep(const GEOMETRY *g, kf_prep *p)
   }
   free(fillp);
   p->diag = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; i++) {
     qsort(p->col + p->rowptr[i], p->rowptr[i + 1] - p->rowptr[i], sizeof(uint32_t), cmp_u32);
     for (uint32_t j = p->rowptr[i] + 1; j < p->rowptr[i + 1]; j++)
@@ -354,6 +362,7 @@ void kf_build_prep(const GEOMETRY *g, kf_prep *p)
     bucket(N, 2 * E, owner, pay, &p->vptr, &p->vent);
     free(owner); free(pay);
     p->vblk = (uint32_t *)kf_xmalloc((2 * E + 1) * sizeof(uint32_t));
+#pragma omp parallel for schedule(static)
     for (size_t v = 0; v < N; v++)
       for (uint32_t s = p->vptr[v]; s < p->vptr[v + 1]; s++) {
         const uint32_t e = p->vent[s] >> 1;

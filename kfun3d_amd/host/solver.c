@@ -8,7 +8,9 @@
  *   iJacVec  src/kernel.c:24-45  matrix-free (J + diag(cdt)) x by forward
  *            difference: w = (h/|x|) x + q, y = ((R(w) - r) + cdt (w - q)) |x|/h
  *   Kernel   src/kernel.c:47-232 pseudo-time loop, left-preconditioned
- *            restarted GMRES with modified Gram-Schmidt (or batched CGS2)
+ *            restarted GMRES with modified Gram-Schmidt (or batched CGS2);
+ *            optionally right-preconditioned flexible GMRES (z_j = M(v_j),
+ *            x += Z y), the form a varying preconditioner needs
  *
  * The only host/device round trips per Arnoldi step are the restart+2
  * Hessenberg scalars (one copy) needed by the Givens recurrence.
@@ -46,16 +48,19 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
   const kf_options o = kf_get_options(g);
   const size_t sz = g->c->sz;
   const int restart = o.gmres_restart, nb = restart + 1;
+  const int flexible = o.krylov == KF_KRYLOV_FGMRES;
   double *gxyz[3], *r, *w, *t, *x, *vecs;
-  kfs_work(g, nb, gxyz, &r, &w, &t, &x, &vecs);
+  kfs_work(g, flexible ? nb + restart : nb, gxyz, &r, &w, &t, &x, &vecs);
   GRADIENT grad = {gxyz[0], gxyz[1], gxyz[2]};
 #define VEC(k) (vecs + (size_t)(k) * sz)
+#define ZVEC(k) (vecs + (size_t)(nb + (k)) * sz)   /* FGMRES: z_k = M(v_k) */
   double *ghh = (double *)calloc((size_t)nb * nb + nb, sizeof(double));
   double cc[64], ss[64], rs[64], hcol[64];
   double fnorm_init = 0.0, cfl = o.cfl_init;
   int its = 0, nrec = 0;
   while (its < o.ts_max) {
-    ComputeFlux(g, g->q->q, &grad, r);
+    /* the residual also leaves the time-step sums of this state */
+    kfs_flux_spectral(g, g->q->q, &grad, r);
     kfs_nrm2(g, sz, r, KFS_SLOT_FNORM);
     const double fnorm = fetch1(g, KFS_SLOT_FNORM);
     if (!isfinite(fnorm)) {
@@ -68,20 +73,25 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
       cfl0 /= fnorm;
       cfl = (cfl0 < o.cfl_max) ? cfl0 : o.cfl_max;
     }
-    ComputeTimeStep(cfl, g);
+    kfs_timestep_spectral(g, cfl);   /* = ComputeTimeStep(cfl, g) on the same state */
     ComputeA(g);
     kfs_nrm2(g, sz, g->q->q, KFS_SLOT_QNORM);
     const double qnorm = sqrt(1.0 + fetch1(g, KFS_SLOT_QNORM));
     kfs_zero(g, sz, x);
     ComputeNumericalILU(g);
-    ComputeSparseTriangularSolve(g, r, VEC(0));
+    if (!flexible) ComputeSparseTriangularSolve(g, r, VEC(0));
+    else kfs_newaxpy(g, sz, 0.0, r, r, VEC(0));   /* v = r (x = 0) */
     double tol = 0.0;
     int total = 0, flag = 0;
     while (total <= o.gmres_max_it) {
       if (total > 0) {
         iJacVec(g, x, r, qnorm, &grad, w, t);
-        kfs_newaxpy(g, sz, -1.0, t, r, w);
-        ComputeSparseTriangularSolve(g, w, VEC(0));
+        if (!flexible) {
+          kfs_newaxpy(g, sz, -1.0, t, r, w);
+          ComputeSparseTriangularSolve(g, w, VEC(0));
+        } else {
+          kfs_newaxpy(g, sz, -1.0, t, r, VEC(0));   /* v = r - A x */
+        }
       }
       kfs_nrm2(g, sz, VEC(0), KFS_SLOT_BETA);
       kfs_scale_recip(g, sz, VEC(0), KFS_SLOT_BETA);
@@ -89,8 +99,13 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
       tol = (total == 0) ? o.gmres_rtol * rs[0] : tol;
       int it = 0;
       while (it < restart) {
-        iJacVec(g, VEC(it), r, qnorm, &grad, w, t);
-        ComputeSparseTriangularSolve(g, t, VEC(it + 1));
+        if (!flexible) {
+          iJacVec(g, VEC(it), r, qnorm, &grad, w, t);
+          ComputeSparseTriangularSolve(g, t, VEC(it + 1));
+        } else {
+          ComputeSparseTriangularSolve(g, VEC(it), ZVEC(it));   /* z = M(v) */
+          iJacVec(g, ZVEC(it), r, qnorm, &grad, w, VEC(it + 1));
+        }
         double *hh = ghh + (size_t)it * (nb + 1);
         if (o.ortho == KF_ORTHO_CGS2) {
           /* two passes of batched classical Gram-Schmidt */
@@ -141,7 +156,7 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
         for (int j = k + 1; j <= it - 1; j++) tt -= ghh[(size_t)j * (nb + 1) + k] * rs[j];
         rs[k] = tt / ghh[(size_t)k * (nb + 1) + k];
       }
-      for (int v = 0; v < it; v++) kfs_axpy(g, sz, rs[v], VEC(v), x);
+      for (int v = 0; v < it; v++) kfs_axpy(g, sz, rs[v], flexible ? ZVEC(v) : VEC(v), x);
       if (flag) break;
     }
     kfs_axpy(g, sz, -1.0, x, g->q->q);
@@ -166,6 +181,7 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
     its++;
   }
 #undef VEC
+#undef ZVEC
   free(ghh);
   return nrec;
 }

@@ -96,6 +96,24 @@ def test_timestep_fill_forces_bitwise(pair):
     assert _bitwise(f, fo), (f, fo)
 
 
+def test_timestep_folded_into_residual(pair):
+    """The driver's residual that also leaves the time-step sums: the same
+    residual, and cdt bit-identical to ComputeTimeStep on that state."""
+    name, m, orc = pair
+    q = perturbed_state(m.N, 3.0, 31)
+    m.set_state(q)
+    m.timestep(37.0)
+    cdt = m.cdt()
+    qd = K.DeviceVec(m.sz, q)
+    g = [K.DeviceVec(m.sz) for _ in range(3)]
+    r = K.DeviceVec(m.sz)
+    m.flux_spectral(qd, g, r)
+    m.timestep_spectral(37.0)
+    assert _bitwise(m.cdt(), cdt), _describe(m.cdt(), cdt)
+    r0, _ = _flux_gpu(m, q)
+    assert _bitwise(r.numpy(), r0)
+
+
 def test_ilu_and_sptrsv(pair):
     """Default FP64 ILU = the reference's one-thread sweep (ilu_Edmond) bit for
     bit, on a first call and on a second call that starts from the L blocks

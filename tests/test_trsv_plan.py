@@ -21,7 +21,7 @@ def _lower_upper(m):
 
 def _check(m, rp, col, upper, R):
     N = m.N
-    task, tdep, lvl = m.trsv_plan(upper, R)
+    task, tdep, lvl, rowtab = m.trsv_plan(upper, R)
     S = 16 // R
     ng = (N + R - 1) // R
     flat = task.ravel()
@@ -58,6 +58,17 @@ def _check(m, rp, col, upper, R):
     tmax = np.zeros(len(task), dtype=np.int64)
     np.maximum.at(tmax, np.nonzero(valid)[0] // S, depth[flat[valid]])
     assert np.array_equal(tdep.astype(np.int64), tmax)
+    # per-task row table: slot q holds row q % R of range task[t, q // R]
+    diag = np.array([rp[r] + np.nonzero(col[rp[r]:rp[r + 1]] == r)[0][0] for r in range(N)])
+    for t in range(len(task)):
+        for q in range(16):
+            g = int(task[t, q // R]); kr = q % R
+            r, ja, jb = (int(v) for v in rowtab[t, q])
+            if g == 0xFFFFFFFF or g * R + kr >= N:
+                assert (r, ja, jb) == (0xFFFFFFFF, 0, 0)
+                continue
+            assert r == (N - 1 - (g * R + kr) if upper else g * R + kr)
+            assert (ja, jb) == ((diag[r], rp[r + 1]) if upper else (rp[r], diag[r]))
     return len(task)
 
 
@@ -79,3 +90,42 @@ def test_plan_lattice(mesh_c1, R):
         # 48x16x16 lattice: the range DAG has (48 / R) + 15 + 15 levels
         assert nt >= (48 // R + 30) * 1
     m.close()
+
+
+@pytest.mark.parametrize("which", ["mesh_a", "mesh_c1"])
+def test_ilu_plan(which, mesh_a, mesh_c1):
+    """The ILU(0) schedule: rows in level order (a row's lower columns come
+    at earlier tickets), and for every lower block (i, m) exactly the pairs
+    (U(m,k), position of k in row i) a merge of rows m and i finds, in
+    ascending order of the block of row m."""
+    path = {"mesh_a": mesh_a, "mesh_c1": mesh_c1}[which]
+    m = K.Mesh(path, device=False)
+    rp, col = _lower_upper(m)
+    N = m.N
+    diag = np.array([rp[r] + np.nonzero(col[rp[r]:rp[r + 1]] == r)[0][0] for r in range(N)])
+    rec, ldiag, moff, mjm, mlo = m.ilu_plan(len(col))
+    m.close()
+    rows = rec[:, 0].astype(np.int64)
+    assert np.array_equal(np.sort(rows), np.arange(N))
+    pos = np.empty(N, dtype=np.int64)
+    pos[rows] = np.arange(N)
+    nxt = 0
+    for t in range(N):
+        i, j0, L, nlow, mbeg, mcnt = (int(v) for v in rec[t, :6])
+        assert (j0, L, nlow) == (rp[i], rp[i + 1] - rp[i], diag[i] - rp[i])
+        assert mbeg == nxt
+        cols_i = col[rp[i]:rp[i + 1]]
+        k = 0
+        for p in range(nlow):
+            mm = col[j0 + p]
+            assert pos[mm] < t                      # row m is factored before row i
+            assert ldiag[j0 + p] == diag[mm]
+            assert moff[j0 + p] == k
+            for jm in range(diag[mm] + 1, rp[mm + 1]):
+                hit = np.nonzero(cols_i == col[jm])[0]
+                if len(hit):
+                    assert mjm[mbeg + k] == jm and mlo[mbeg + k] == hit[0]
+                    k += 1
+        assert k == mcnt
+        nxt = mbeg + mcnt
+    assert nxt == len(mjm)
