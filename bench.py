@@ -52,6 +52,22 @@ def alg_bytes(kind, N, E, S, Nf, nnzb, s):
     }[kind]
 
 
+# kernel classes of kf_profile -> the FP64 kernel names of rocprofv3 traces
+PROFILE_TO_KERNEL = {"trsv_upper": "k_trsv_upper<double, double>", "trsv_lower": "k_trsv_lower<double, double>",
+                     "ilu": "k_ilu_rows<double, true>"}
+
+
+def measured_traffic(dom, prec):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    passes (profiles/traffic_c2_fp64.json: FETCH_SIZE and WRITE_SIZE runs,
+    scaled by the reading of a copy of known size); None when not covered."""
+    path = os.path.join(ROOT, "profiles", "traffic_c2_fp64.json")
+    if prec != "fp64" or dom not in PROFILE_TO_KERNEL or not os.path.exists(path):
+        return None
+    rec = json.load(open(path))["kernels"].get(PROFILE_TO_KERNEL[dom])
+    return None if rec is None else rec["bytes"]
+
+
 PROFILE_TO_KIND = {"grad": "grad", "edge_flux": "edge_flux", "residual": "residual_gather", "timestep": "timestep",
                    "fill": "fill", "ilu": "ilu", "trsv_lower": "trsv_lower", "trsv_upper": "trsv_upper"}
 
@@ -249,8 +265,9 @@ def main():
         "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
         "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": ab,
-                     "avg_launch_ms": ms / n},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, args.prec),
+                     "alg_bytes_per_launch": ab, "avg_launch_ms": ms / n,
+                     "traffic_source": "profiles/traffic_c2_fp64.md (FETCH_SIZE/WRITE_SIZE passes, copy-calibrated)"},
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
     }
