@@ -47,7 +47,7 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   /* launch-shape knobs of the level-scheduled kernels (tuning only) */
   const char *ev;
   c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 512;
-  c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 2048;
+  c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 1024;
   d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
   if (c->trsv_waves < 1) c->trsv_waves = 1;
   if (c->ilu_waves < 1) c->ilu_waves = 1;
@@ -242,7 +242,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->urows, c->lgrp, c->ugrp, c->rdone,
-                  c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ticket, c->a64, c->a32,
+                  c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
                   c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));

@@ -19,6 +19,7 @@
  * matrix/inverse.cpp:195-352.
  */
 #include "sync.h"
+#include "handoff.h"
 
 /* matrix/inverse.cpp:200-326: entry e of the adjugate is
  * s0*(a b c) s1*(...) ... six signed triple products summed left to right */
@@ -209,15 +210,19 @@ struct ilu_layout {
 
 /* k_ilu with the row's schedule precomputed on the host (kf_plan_ilu): a
  * record per ticket, the diagonal position of every row it reads from and
- * the list of update pairs (U(m,k) block, position of k in row i).  Before
- * the wait the wave loads, in one round, its row, those tables and its
- * L_old blocks; after the acquire it stages inv(U(m,m)), U(m,m) and the
- * U(m,k) blocks in LDS in one round; the rest runs from LDS.  Arithmetic and
- * its order are those of k_ilu. */
+ * the list of update pairs (U(m,k) block, position of k in row i).  The wave
+ * loads, in one round, its row, those tables and its L_old blocks, then
+ * stages inv(U(m,m)), U(m,m) and the U(m,k) blocks in LDS from the
+ * published copies (apub, udpub), retrying values that still hold the
+ * call's sentinel (handoff.h): no flags, no fences.  The rest runs from
+ * LDS; the row then stores its factors in place and publishes its diagonal
+ * inverse, upper blocks and U(i,i).  Arithmetic and its order are those of
+ * k_ilu. */
 template <typename T, bool EXACT>
 __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__restrict__ rec, uint32_t nrows,
-                                                 uint32_t *ticket, uint32_t base, uint32_t *rdone, uint32_t epoch,
-                                                 T *a, T *lold, T *ud, const uint32_t *__restrict__ ldiag,
+                                                 uint32_t *ticket, uint32_t base, T *apub, T *udpub,
+                                                 typename sentinel<T>::bits snt, T *a, T *lold,
+                                                 const uint32_t *__restrict__ ldiag,
                                                  const uint32_t *__restrict__ moff, const uint32_t *__restrict__ mjm,
                                                  const uint16_t *__restrict__ mlo, uint32_t maxrow, uint32_t maxlow,
                                                  uint32_t maxmatch)
@@ -289,36 +294,46 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
       for (uint32_t k = lane + 64; k < mcnt; k += 64) { sjm[k] = mjm[mbeg + k]; slo[k] = mlo[mbeg + k]; }
     }
     __syncthreads();
-    for (uint32_t k = lane; k < nlow; k += 64) wait_row(rdone, scol[k], epoch, d.err, aborted, d.spin_sleep);
-    aborted = __any(aborted);
-    consume_fence();
-    /* staging after the acquire, ILU_QB values per lane per round */
+    /* staging from the published copies, ILU_QB values per lane per round;
+     * a value still holding the sentinel is loaded again */
     {
       const uint32_t npre = EXACT ? 2 * nlow : nlow;
       const uint32_t tot = 16 * (npre + mcnt);
       for (uint32_t b0 = 0; b0 < tot; b0 += 64 * ILU_QB) {
-        T v[ILU_QB];
+        uint32_t need = 0;
 #pragma unroll
-        for (int q = 0; q < ILU_QB; q++) {
-          const uint32_t k = b0 + lane + 64 * q;
-          if (k < tot) {
-            const uint32_t sl = k >> 4, ee = k & 15;
-            const T *src;
-            if (sl < nlow) src = a + 16 * (size_t)sldg[sl];
-            else if (EXACT && sl < 2 * nlow) src = ud + 16 * (size_t)scol[sl - nlow];
-            else src = a + 16 * (size_t)sjm[sl - npre];
-            v[q] = src[ee];
+        for (int q = 0; q < ILU_QB; q++)
+          if (b0 + lane + 64 * q < tot) need |= 1u << q;
+        for (uint32_t n = 0; need && !aborted; n++) {
+          if (n > 0) {
+            if (n > KF_SPIN_LIMIT) {
+              atomicOr(d.err, KF_ERR_SPIN_LIMIT);
+              aborted = true;
+              break;
+            }
+            for (uint32_t z = 0; z < d.spin_sleep; z++) __builtin_amdgcn_s_sleep(1);
           }
-        }
+          T v[ILU_QB];
 #pragma unroll
-        for (int q = 0; q < ILU_QB; q++) {
-          const uint32_t k = b0 + lane + 64 * q;
-          if (k < tot) {
-            const uint32_t sl = k >> 4;
-            stage[16 * (sl < npre ? sl : sl - npre + mbase) + (k & 15)] = v[q];
-          }
+          for (int q = 0; q < ILU_QB; q++)
+            if (need & (1u << q)) {
+              const uint32_t k = b0 + lane + 64 * q, sl = k >> 4, ee = k & 15;
+              const T *src;
+              if (sl < nlow) src = apub + 16 * (size_t)sldg[sl];
+              else if (EXACT && sl < 2 * nlow) src = udpub + 16 * (size_t)scol[sl - nlow];
+              else src = apub + 16 * (size_t)sjm[sl - npre];
+              v[q] = ld_coherent(src + ee);
+            }
+#pragma unroll
+          for (int q = 0; q < ILU_QB; q++)
+            if ((need & (1u << q)) && sentinel<T>::raw(v[q]) != snt) {
+              const uint32_t k = b0 + lane + 64 * q, sl = k >> 4;
+              stage[16 * (sl < npre ? sl : sl - npre + mbase) + (k & 15)] = v[q];
+              need &= ~(1u << q);
+            }
         }
       }
+      aborted = __any(aborted);   /* a wave gives up at most once per launch */
     }
     __syncthreads();
 
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
         sa[16 * p + e] = v;
       }
       __syncthreads();
-      if (lane < 16) ud[16 * (size_t)i + e] = sa[16 * pd + e];
+      if (lane < 16) st_coherent(udpub + 16 * (size_t)i + e, canon(sa[16 * pd + e]));
     }
 
     {
@@ -390,9 +405,9 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
     for (uint32_t k = lane; k < 16 * L; k += 64) a[16 * (size_t)j0 + k] = sa[k];
     if (EXACT)
       for (uint32_t k = lane; k < 16 * nlow; k += 64) lold[16 * (size_t)j0 + k] = sa[k];
-    publish_fence();
-    __syncthreads();
-    if (lane == 0) flag_store(rdone + i, epoch);
+    /* published copies of what later rows read: inv(U(i,i)) and U(i,k) */
+    for (uint32_t k = lane; k < 16 * (L - pd); k += 64)
+      st_coherent(apub + 16 * ((size_t)j0 + pd) + k, canon(sa[16 * pd + k]));
   }
 }
 
@@ -405,18 +420,34 @@ template <typename T, bool EXACT>
 static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
 {
   const uint32_t nw = persistent_waves(c->d.N, c->ilu_waves);
-  const uint32_t epoch = ++c->epoch;
   const ilu_layout lay(c->ilu_maxrow, c->ilu_maxlow, c->ilu_maxmatch, sizeof(T), EXACT);
-  const int s = kf_prof_begin(c, KP_ILU);
-  if (lay.bytes <= 64 * 1024)
+  if (lay.bytes <= 64 * 1024) {
+    /* published copies, preset to this call's sentinel */
+    const size_t na = 16 * c->nnzb, nu = 16 * (size_t)c->d.N;
+    if (!c->ilu_apub) KF_CHECK(hipMalloc(&c->ilu_apub, na * sizeof(double)));
+    if (EXACT && !c->ilu_udpub) KF_CHECK(hipMalloc(&c->ilu_udpub, nu * sizeof(double)));
+    const typename sentinel<T>::bits snt = sentinel<T>::of(++c->icall);
+    T *apub = (T *)c->ilu_apub, *udpub = EXACT ? (T *)c->ilu_udpub : NULL;
+    hipLaunchKernelGGL((k_fill_bits<T>), dim3(kf_blocks_for(na, 256)), dim3(256), 0, c->stream, apub, na, snt);
+    KF_LAUNCH_CHECK();
+    if (EXACT) {
+      hipLaunchKernelGGL((k_fill_bits<T>), dim3(kf_blocks_for(nu, 256)), dim3(256), 0, c->stream, udpub, nu, snt);
+      KF_LAUNCH_CHECK();
+    }
+    const int s = kf_prof_begin(c, KP_ILU);
     hipLaunchKernelGGL((k_ilu_rows<T, EXACT>), dim3(nw), dim3(64), lay.bytes, c->stream, c->d, c->ilu_rec, c->d.N,
-                       c->ticket + 2, c->ticket_base[2], c->rdone, epoch, a, lold, ud, c->ilu_ldiag, c->ilu_moff,
+                       c->ticket + 2, c->ticket_base[2], apub, udpub, snt, a, lold, c->ilu_ldiag, c->ilu_moff,
                        c->ilu_mjm, c->ilu_mlo, c->ilu_maxrow, c->ilu_maxlow, c->ilu_maxmatch);
-  else
+    KF_LAUNCH_CHECK();
+    kf_prof_end(c, s);
+  } else {
+    const uint32_t epoch = ++c->epoch;
+    const int s = kf_prof_begin(c, KP_ILU);
     hipLaunchKernelGGL((k_ilu<T, EXACT>), dim3(nw), dim3(64), 0, c->stream, c->d, c->lrows, c->d.N, c->ticket + 2,
                        c->ticket_base[2], c->rdone, epoch, a, lold, ud);
-  KF_LAUNCH_CHECK();
-  kf_prof_end(c, s);
+    KF_LAUNCH_CHECK();
+    kf_prof_end(c, s);
+  }
   c->ticket_base[2] += c->d.N + nw;
 }
 

@@ -97,6 +97,8 @@ struct kf_ctx {
   uint32_t *ilu_rec, *ilu_ldiag, *ilu_moff, *ilu_mjm;
   uint16_t *ilu_mlo;
   uint32_t ilu_maxrow, ilu_maxlow, ilu_maxmatch;
+  void *ilu_apub, *ilu_udpub; /* published diagonal inverses and U blocks, U(i,i) */
+  uint64_t icall;             /* ILU calls so far (sentinel numbering) */
   uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch */
   uint32_t nlev_l, nlev_u;
   /* level schedule */

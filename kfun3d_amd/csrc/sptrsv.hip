@@ -49,6 +49,7 @@
  * KF_ERR_SPIN_LIMIT and stops waiting for the rest of the launch.
  */
 #include "sync.h"
+#include "handoff.h"
 
 static_assert(KF_GROUP == 16, "one quad of lanes per row, 16 rows per task");
 
@@ -57,34 +58,6 @@ static_assert(KF_GROUP == 16, "one quad of lanes per row, 16 rows per task");
 __device__ __forceinline__ double mat_ld(const double *a, size_t i) { return a[i]; }
 __device__ __forceinline__ float mat_ld(const float *a, size_t i) { return a[i]; }
 __device__ __forceinline__ float mat_ld(const uint16_t *a, size_t i) { return __uint_as_float((uint32_t)a[i] << 16); }
-
-/* sentinels: signalling NaNs (quiet bit clear, top payload bit set so the
- * payload is never zero) carrying the low bits of the sweep number k */
-template <typename TV> struct sentinel;
-template <> struct sentinel<double> {
-  typedef unsigned long long bits;
-  static __host__ __device__ bits of(uint64_t k) { return 0x7ff4000000000000ull | (k & 0x0003ffffffffffffull); }
-  static __device__ bits raw(double v) { return (bits)__double_as_longlong(v); }
-  static __device__ double val(bits b) { return __longlong_as_double((long long)b); }
-};
-template <> struct sentinel<float> {
-  typedef unsigned int bits;
-  static __host__ __device__ bits of(uint64_t k) { return 0x7fa00000u | (uint32_t)(k & 0x1fffffu); }
-  static __device__ bits raw(float v) { return __float_as_uint(v); }
-  static __device__ float val(bits b) { return __uint_as_float(b); }
-};
-
-template <typename TV>
-__device__ __forceinline__ TV ld_coherent(const TV *p)
-{
-  return __hip_atomic_load(const_cast<TV *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <typename TV>
-__device__ __forceinline__ void st_coherent(TV *p, TV v)
-{
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 /* s = A(j) row c . x  (dgemv row; summed from zero in column order) */
 template <typename TV>
@@ -344,13 +317,6 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
     }
     t = take_ticket(tn, base);
   }
-}
-
-template <typename TV>
-__global__ void k_fill_bits(TV *v, size_t n, typename sentinel<TV>::bits s)
-{
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = sentinel<TV>::val(s);
 }
 
 /* forward output y (held between calls, preset to the next call's forward
