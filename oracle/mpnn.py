@@ -19,6 +19,11 @@ factor's diagonal block (16 values as stored: the inverse of U(v,v)),
 s = RMS of v over all entries and sd = RMS of all the D blocks.
 
 Parameters are one flat float32 vector in the order of PARAM_SHAPES.
+
+bf16=True rounds both operands of every dense layer (W0, A, B, U, V; the
+small C, Wo products stay exact) to bfloat16 (round to nearest even), as the
+GPU's matrix cores consume them; the sums themselves stay exact, so the GPU
+differs from this mode only by its FP32 summation order.
 """
 import numpy as np
 
@@ -32,6 +37,21 @@ for _l in range(NROUND):
                      ("U%d" % _l, (H, H)), ("V%d" % _l, (H, H)), ("u%d" % _l, (H,))]
 PARAM_SHAPES += [("Wo", (4, H)), ("bo", (4,))]
 NPARAM = sum(int(np.prod(s)) for _, s in PARAM_SHAPES)
+
+
+def bf16(a):
+    """Round to the nearest bfloat16 (ties to even), returned as float64."""
+    a32 = np.asarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _mm(x, w, round_bf16):
+    """x @ w.T with both operands rounded to bfloat16 when asked."""
+    if round_bf16:
+        return bf16(x) @ bf16(w).T
+    return x @ w.T
 
 
 def unpack(theta):
@@ -96,23 +116,23 @@ def features(g, v, D):
     return np.concatenate([v / s, D / sd], axis=1), s, sd
 
 
-def forward(theta, g, v, D, keep=False):
+def forward(theta, g, v, D, keep=False, round_bf16=False):
     """delta (N, 4); with keep, also the activations backward needs."""
     p = unpack(theta)
     x, s, sd = features(g, v, D)
-    z0 = x @ p["W0"].T + p["b0"]
+    z0 = _mm(x, p["W0"], round_bf16) + p["b0"]
     h = np.maximum(z0, 0.0)
     cache = {"x": x, "s": s, "z0": z0, "rounds": []}
     for l in range(NROUND):
-        P = h @ p["A%d" % l].T
-        Q = h @ p["B%d" % l].T
+        P = _mm(h, p["A%d" % l], round_bf16)
+        Q = _mm(h, p["B%d" % l], round_bf16)
         Ce = g.e @ p["C%d" % l].T
         zm = P[g.owner] + Q[g.nbr] + Ce + p["c%d" % l]
         mm = np.maximum(zm, 0.0)
         m = np.zeros((g.N, H))
         np.add.at(m, g.owner, mm)
         m *= g.inv_deg[:, None]
-        zu = h @ p["U%d" % l].T + m @ p["V%d" % l].T + p["u%d" % l]
+        zu = _mm(h, p["U%d" % l], round_bf16) + _mm(m, p["V%d" % l], round_bf16) + p["u%d" % l]
         cache["rounds"].append({"h": h, "zm": zm, "m": m, "zu": zu})
         h = h + np.maximum(zu, 0.0)
     cache["h"] = h
