@@ -237,6 +237,44 @@ int  kf_last_error(const GEOMETRY *g);
 void kf_profile(GEOMETRY *g, int on);
 int  kf_profile_get(GEOMETRY *g, const char *name, double *total_ms, long *count);
 
+/* ---- learned preconditioner correction (no counterpart in the reference) ----
+ * An edge message-passing network (3 rounds, hidden width 32) maps a vector
+ * v (4 per vertex) and the ILU factor's diagonal blocks to a per-vertex
+ * correction delta that flexible GMRES adds to the ILU(0) output:
+ * M(v) = ILU(0)^-1 v + delta(v).  BF16 weights on the matrix cores, FP32
+ * accumulation; parameters are kf_mpnn_nparam() FP32 values.  A network is
+ * bound to one GEOMETRY (its stream and mesh); vectors are device pointers
+ * of length 4N.  Diagonal blocks are held in `slot`s (0 .. nslot-1), each a
+ * snapshot of one factorisation. */
+typedef struct kf_mpnn kf_mpnn;
+size_t   kf_mpnn_nparam(void);
+kf_mpnn *kf_mpnn_create(GEOMETRY *g, const float *theta_host, int nslot);
+void     kf_mpnn_destroy(kf_mpnn *net);
+void     kf_mpnn_set_params(kf_mpnn *net, const float *theta_host);  /* also clears the Adam moments */
+void     kf_mpnn_get_params(const kf_mpnn *net, float *theta_host);
+/* diagonal blocks of the current ILU factor (after ComputeNumericalILU) */
+void     kf_mpnn_take_diag(kf_mpnn *net, int slot);
+/* diagonal blocks from the host, FP64 [N][16] */
+void     kf_mpnn_set_diag(kf_mpnn *net, int slot, const double *D_host);
+/* out = delta(v) (add = 0) or out += delta(v) (add = 1) */
+void     kf_mpnn_forward(kf_mpnn *net, int slot, const double *v, double *out, int add);
+/* |base + delta(v) - target|^2 / |target|^2 and its gradient (FP32 host copy
+ * when grad_host != NULL); synchronises */
+double   kf_mpnn_loss_grad(kf_mpnn *net, int slot, const double *v, const double *base, const double *target,
+                           float *grad_host);
+/* Adam step with the last gradient; gclip > 0 clips its global 2-norm */
+void     kf_mpnn_adam(kf_mpnn *net, float lr, float beta1, float beta2, float eps, float gclip);
+/* loss + gradient + Adam (0.9, 0.999, 1e-8) queued without a host round
+ * trip; the loss is appended to the loss log */
+void     kf_mpnn_train_step(kf_mpnn *net, int slot, const double *v, const double *base, const double *target,
+                            float lr, float gclip);
+int      kf_mpnn_losses(kf_mpnn *net, float *host, int cap);
+void     kf_mpnn_reset_losses(kf_mpnn *net);
+/* activations of the last forward (tests): 0 features (N x 24), 1 first
+ * pre-activation, 2+5l h_l, 3+5l A h, 4+5l B h, 5+5l messages, 6+5l update
+ * pre-activation of round l, 17 the final h (N x 32 each) */
+void     kf_mpnn_activation(kf_mpnn *net, int which, float *host);
+
 #ifdef __cplusplus
 }
 #endif

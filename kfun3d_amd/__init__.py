@@ -143,6 +143,20 @@ def _load():
         "ComputeAXPY": (None, [C.c_size_t, C.c_double, DP, DP]),
         "ComputeNewAXPY": (None, [C.c_size_t, C.c_double, DP, DP, DP]),
         "Normalize": (C.c_double, [C.c_size_t, DP]),
+        "kf_mpnn_nparam": (C.c_size_t, []),
+        "kf_mpnn_create": (C.c_void_p, [GP, C.c_void_p, C.c_int]),
+        "kf_mpnn_destroy": (None, [C.c_void_p]),
+        "kf_mpnn_set_params": (None, [C.c_void_p, C.c_void_p]),
+        "kf_mpnn_get_params": (None, [C.c_void_p, C.c_void_p]),
+        "kf_mpnn_take_diag": (None, [C.c_void_p, C.c_int]),
+        "kf_mpnn_set_diag": (None, [C.c_void_p, C.c_int, C.c_void_p]),
+        "kf_mpnn_forward": (None, [C.c_void_p, C.c_int, DP, DP, C.c_int]),
+        "kf_mpnn_loss_grad": (C.c_double, [C.c_void_p, C.c_int, DP, DP, DP, C.c_void_p]),
+        "kf_mpnn_adam": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "kf_mpnn_train_step": (None, [C.c_void_p, C.c_int, DP, DP, DP, C.c_float, C.c_float]),
+        "kf_mpnn_losses": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+        "kf_mpnn_reset_losses": (None, [C.c_void_p]),
+        "kf_mpnn_activation": (None, [C.c_void_p, C.c_int, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -169,7 +183,10 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
                "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
-               "kf_ilu_plan_host"]
+               "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
+               "kf_mpnn_get_params", "kf_mpnn_take_diag", "kf_mpnn_set_diag", "kf_mpnn_forward",
+               "kf_mpnn_loss_grad", "kf_mpnn_adam", "kf_mpnn_train_step", "kf_mpnn_losses", "kf_mpnn_reset_losses",
+               "kf_mpnn_activation"]
 
 
 def options(**kw):
@@ -410,3 +427,68 @@ class Mesh:
 
     def synchronize(self):
         lib().kf_synchronize(self.ref)
+
+
+class Network:
+    """The learned preconditioner's network bound to a device Mesh
+    (kf_mpnn_*): parameters are kf_mpnn_nparam() float32 values, vectors are
+    DeviceVec of 4N, diagonal blocks live in slots 0 .. nslot-1."""
+
+    def __init__(self, mesh, theta, nslot=1):
+        self.mesh = mesh
+        self.nparam = int(lib().kf_mpnn_nparam())
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        assert th.size == self.nparam
+        self.h = lib().kf_mpnn_create(mesh.ref, th.ctypes.data, int(nslot))
+
+    def close(self):
+        if self.h:
+            lib().kf_mpnn_destroy(self.h)
+            self.h = None
+
+    def params(self):
+        out = np.empty(self.nparam, dtype=np.float32)
+        lib().kf_mpnn_get_params(self.h, out.ctypes.data)
+        return out
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        assert th.size == self.nparam
+        lib().kf_mpnn_set_params(self.h, th.ctypes.data)
+
+    def take_diag(self, slot=0):
+        lib().kf_mpnn_take_diag(self.h, slot)
+
+    def set_diag(self, D, slot=0):
+        d = np.ascontiguousarray(D, dtype=np.float64)
+        assert d.size == 16 * self.mesh.N
+        lib().kf_mpnn_set_diag(self.h, slot, d.ctypes.data)
+
+    def forward(self, v_dev, out_dev, slot=0, add=False):
+        lib().kf_mpnn_forward(self.h, slot, v_dev.ptr, out_dev.ptr, 1 if add else 0)
+
+    def loss_grad(self, v_dev, base_dev, target_dev, slot=0):
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = lib().kf_mpnn_loss_grad(self.h, slot, v_dev.ptr, base_dev.ptr, target_dev.ptr, g.ctypes.data)
+        return loss, g
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, gclip=0.0):
+        lib().kf_mpnn_adam(self.h, lr, beta1, beta2, eps, gclip)
+
+    def train_step(self, v_dev, base_dev, target_dev, lr, slot=0, gclip=0.0):
+        lib().kf_mpnn_train_step(self.h, slot, v_dev.ptr, base_dev.ptr, target_dev.ptr, lr, gclip)
+
+    def losses(self):
+        n = lib().kf_mpnn_losses(self.h, None, 0)
+        out = np.empty(max(min(n, 1 << 16), 1), dtype=np.float32)
+        lib().kf_mpnn_losses(self.h, out.ctypes.data, out.size)
+        return out[:min(n, 1 << 16)]
+
+    def reset_losses(self):
+        lib().kf_mpnn_reset_losses(self.h)
+
+    def activation(self, which):
+        n = self.mesh.N * (24 if which == 0 else 32)
+        out = np.empty(n, dtype=np.float32)
+        lib().kf_mpnn_activation(self.h, which, out.ctypes.data)
+        return out.reshape(self.mesh.N, -1)

@@ -133,30 +133,49 @@ def forward(theta, g, v, D, keep=False, round_bf16=False):
         np.add.at(m, g.owner, mm)
         m *= g.inv_deg[:, None]
         zu = _mm(h, p["U%d" % l], round_bf16) + _mm(m, p["V%d" % l], round_bf16) + p["u%d" % l]
-        cache["rounds"].append({"h": h, "zm": zm, "m": m, "zu": zu})
+        cache["rounds"].append({"h": h, "P": P, "Q": Q, "zm": zm, "m": m, "zu": zu})
         h = h + np.maximum(zu, 0.0)
     cache["h"] = h
     delta = s * (h @ p["Wo"].T + p["bo"])
     return (delta, cache) if keep else delta
 
 
-def backward(theta, g, cache, ddelta):
-    """Gradient of sum(ddelta * delta) with respect to theta (flat, float64)."""
+def _wg(dy, x, round_bf16):
+    """dy.T @ x (a weight gradient), operands rounded to bfloat16 when asked."""
+    if round_bf16:
+        return bf16(dy).T @ bf16(x)
+    return dy.T @ x
+
+
+def _dg(dy, w, round_bf16):
+    """dy @ w (a data gradient through a dense layer), rounded likewise."""
+    if round_bf16:
+        return bf16(dy) @ bf16(w)
+    return dy @ w
+
+
+def backward(theta, g, cache, ddelta, round_bf16=False):
+    """Gradient of sum(ddelta * delta) with respect to theta (flat, float64).
+    round_bf16 rounds the operands of the dense-layer products (the weight
+    gradients and the data gradients through W0, A, B, U, V) to bfloat16;
+    the decoder's data gradient, the message backward and the bias sums stay
+    exact."""
     p = unpack(theta)
+    rb = round_bf16
     gr = {name: np.zeros(shape) for name, shape in PARAM_SHAPES}
     dout = np.asarray(ddelta, dtype=np.float64).reshape(g.N, 4) * cache["s"]
-    gr["Wo"] = dout.T @ cache["h"]
+    gr["Wo"] = _wg(dout, cache["h"], rb)
     gr["bo"] = dout.sum(axis=0)
     dh = dout @ p["Wo"]
     for l in reversed(range(NROUND)):
         rc = cache["rounds"][l]
         h, zm, m, zu = rc["h"], rc["zm"], rc["m"], rc["zu"]
         dzu = dh * (zu > 0)
-        gr["U%d" % l] = dzu.T @ h
-        gr["V%d" % l] = dzu.T @ m
+        gr["U%d" % l] = _wg(dzu, h, rb)
+        gr["V%d" % l] = _wg(dzu, m, rb)
         gr["u%d" % l] = dzu.sum(axis=0)
-        dhn = dh + dzu @ p["U%d" % l]
-        dm = (dzu @ p["V%d" % l]) * g.inv_deg[:, None]
+        dhn = dh + _dg(dzu, p["U%d" % l], rb)
+        dm = _dg(dzu, p["V%d" % l], rb) * g.inv_deg[:, None]
         dzm = dm[g.owner] * (zm > 0)
         gr["C%d" % l] = dzm.T @ g.e
         gr["c%d" % l] = dzm.sum(axis=0)
@@ -164,20 +183,37 @@ def backward(theta, g, cache, ddelta):
         np.add.at(dP, g.owner, dzm)
         dQ = np.zeros((g.N, H))
         np.add.at(dQ, g.nbr, dzm)
-        gr["A%d" % l] = dP.T @ h
-        gr["B%d" % l] = dQ.T @ h
-        dh = dhn + dP @ p["A%d" % l] + dQ @ p["B%d" % l]
+        gr["A%d" % l] = _wg(dP, h, rb)
+        gr["B%d" % l] = _wg(dQ, h, rb)
+        dh = dhn + _dg(dP, p["A%d" % l], rb) + _dg(dQ, p["B%d" % l], rb)
     dz0 = dh * (cache["z0"] > 0)
-    gr["W0"] = dz0.T @ cache["x"]
+    gr["W0"] = _wg(dz0, cache["x"], rb)
     gr["b0"] = dz0.sum(axis=0)
     return pack(gr)
 
 
-def loss_and_grad(theta, g, v, D, base, target):
+def loss_and_grad(theta, g, v, D, base, target, round_bf16=False):
     """Training loss of one pair: |base + delta - target|^2 / |target|^2,
     where base = ILU(0)^-1 v; and its gradient."""
-    delta, cache = forward(theta, g, v, D, keep=True)
+    delta, cache = forward(theta, g, v, D, keep=True, round_bf16=round_bf16)
     res = np.asarray(base).reshape(g.N, 4) + delta - np.asarray(target).reshape(g.N, 4)
     tn = float(np.sum(np.asarray(target) ** 2))
     loss = float(np.sum(res * res)) / tn
-    return loss, backward(theta, g, cache, 2.0 * res / tn)
+    return loss, backward(theta, g, cache, 2.0 * res / tn, round_bf16=round_bf16)
+
+
+def adam(theta, m, v, grad, t, lr, b1=0.9, b2=0.999, eps=1e-8, gclip=0.0):
+    """One Adam step in float32 (t counts from 1), with optional clipping of
+    the gradient's global 2-norm to gclip; returns (theta, m, v)."""
+    f = np.float32
+    b1, b2, lr, eps, gclip = f(b1), f(b2), f(lr), f(eps), f(gclip)
+    g = np.asarray(grad, dtype=f)
+    gn = float(np.sqrt(np.sum(np.asarray(grad, dtype=np.float64) ** 2)))
+    if gclip > 0 and gn > float(gclip):
+        g = g * f(float(gclip) / gn)
+    m = b1 * m + (f(1) - b1) * g
+    v = b2 * v + (f(1) - b2) * (g * g)
+    c1 = f(1.0 / (1.0 - float(b1) ** t))
+    c2 = f(1.0 / (1.0 - float(b2) ** t))
+    theta = theta - lr * (m * c1) / (np.sqrt(v * c2) + eps)
+    return theta.astype(f), m.astype(f), v.astype(f)

@@ -80,3 +80,50 @@ def test_message_sum_matches_edge_loop(graph):
     # ascending edge order within each vertex
     same = g.owner[1:] == g.owner[:-1]
     assert np.all(g.eid[1:][same] > g.eid[:-1][same])
+
+
+def test_bf16_rounding_matches_torch():
+    """The oracle's bfloat16 rounding (nearest, ties to even) is torch's."""
+    import torch
+    x = np.random.default_rng(0).standard_normal(100000).astype(np.float32) * np.float32(3.7)
+    x[:8] = np.array([1 + 2 ** -8, 1 + 3 * 2 ** -8, -(1 + 2 ** -8), 2 ** -130, 3e38, 0.0, 1.00390625, 7.0],
+                     dtype=np.float32)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(P.bf16(x), ref)
+
+
+def test_adam_matches_torch():
+    """The oracle's float32 Adam step against torch.optim.Adam (an
+    independent arrangement of the same update), three steps."""
+    import torch
+    rng = np.random.default_rng(1)
+    th0 = rng.standard_normal(1000).astype(np.float32)
+    grads = [rng.standard_normal(1000).astype(np.float32) * s for s in (1.0, 0.1, 3.0)]
+    p = torch.nn.Parameter(torch.from_numpy(th0.copy()))
+    opt = torch.optim.Adam([p], lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    th, m, v = th0.copy(), np.zeros(1000, np.float32), np.zeros(1000, np.float32)
+    for t, gr in enumerate(grads, start=1):
+        p.grad = torch.from_numpy(gr.copy())
+        opt.step()
+        th, m, v = P.adam(th, m, v, gr, t, 1e-3)
+        np.testing.assert_allclose(th, p.detach().numpy(), rtol=0, atol=2e-7 * np.abs(th0).max())
+
+
+def test_bf16_gradient_close_to_exact(graph):
+    """Rounding the dense operands to bfloat16 moves the gradient by well
+    under 5 percent per parameter block."""
+    g = graph
+    theta = P.init_params(seed=6).astype(np.float64)
+    theta[P.NPARAM - 132:] *= 30.0
+    v, D = _data(g, 7)
+    rng = np.random.default_rng(8)
+    target, base = rng.standard_normal(4 * g.N), rng.standard_normal(4 * g.N) * 0.5
+    l1, g1 = P.loss_and_grad(theta, g, v, D, base, target)
+    l2, g2 = P.loss_and_grad(theta, g, v, D, base, target, round_bf16=True)
+    assert abs(l1 - l2) <= 1e-2 * l1
+    o = 0
+    for name, shape in P.PARAM_SHAPES:
+        n = int(np.prod(shape))
+        a, b = g1[o:o + n], g2[o:o + n]
+        assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(a), name
+        o += n
