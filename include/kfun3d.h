@@ -165,6 +165,11 @@ typedef struct kf_step_record {
   double fnorm;         /* |R(q)| before the update                           */
   int    krylov;        /* GMRES iterations of this step                      */
   double forces[3];     /* lift, drag, moment after the update                */
+  int    learned;       /* 1: the update came from FGMRES with the learned
+                           correction; 0: ILU(0) alone                        */
+  int    krylov_ilu;    /* iterations of the ILU(0)-alone solve of this step's
+                           system, -1 when it did not run                    */
+  int    krylov_net;    /* ... of the solve with the learned correction      */
 } kf_step_record;
 
 kf_options kf_default_options(void);              /* reference profile */
@@ -238,16 +243,20 @@ void kf_profile(GEOMETRY *g, int on);
 int  kf_profile_get(GEOMETRY *g, const char *name, double *total_ms, long *count);
 
 /* ---- learned preconditioner correction (no counterpart in the reference) ----
- * An edge message-passing network (3 rounds, hidden width 32) maps a vector
- * v (4 per vertex) and the ILU factor's diagonal blocks to a per-vertex
- * correction delta that flexible GMRES adds to the ILU(0) output:
- * M(v) = ILU(0)^-1 v + delta(v).  BF16 weights on the matrix cores, FP32
- * accumulation; parameters are kf_mpnn_nparam() FP32 values.  A network is
+ * An edge message-passing network (3 rounds, hidden width 32, BF16 weights on
+ * the matrix cores, FP32 accumulation) maps the ILU factor's diagonal blocks
+ * and the mesh geometry to a sparse linear correction operator G: per
+ * vertex a 4 x 4 block S_v, per edge direction a block T_vu, and
+ *   delta_v = S_v z_v + (1 / deg v) sum_u T_vu z_u,   z = ILU(0)^-1 v.
+ * Flexible GMRES preconditions with M v = z + G z, linear in v.  The
+ * parameters are kf_mpnn_nparam() FP32 values; the last kf_mpnn_nhead() of
+ * them are the operator heads (all zero: G = 0, M = ILU(0)).  A network is
  * bound to one GEOMETRY (its stream and mesh); vectors are device pointers
- * of length 4N.  Diagonal blocks are held in `slot`s (0 .. nslot-1), each a
- * snapshot of one factorisation. */
+ * of length 4N.  Diagonal blocks (and, for training, assembled Jacobians)
+ * are held in `slot`s 0 .. nslot-1, each a snapshot of one step. */
 typedef struct kf_mpnn kf_mpnn;
 size_t   kf_mpnn_nparam(void);
+size_t   kf_mpnn_nhead(void);
 kf_mpnn *kf_mpnn_create(GEOMETRY *g, const float *theta_host, int nslot);
 void     kf_mpnn_destroy(kf_mpnn *net);
 void     kf_mpnn_set_params(kf_mpnn *net, const float *theta_host);  /* also clears the Adam moments */
@@ -256,10 +265,22 @@ void     kf_mpnn_get_params(const kf_mpnn *net, float *theta_host);
 void     kf_mpnn_take_diag(kf_mpnn *net, int slot);
 /* diagonal blocks from the host, FP64 [N][16] */
 void     kf_mpnn_set_diag(kf_mpnn *net, int slot, const double *D_host);
-/* out = delta(v) (add = 0) or out += delta(v) (add = 1) */
-void     kf_mpnn_forward(kf_mpnn *net, int slot, const double *v, double *out, int add);
-/* |base + delta(v) - target|^2 / |target|^2 and its gradient (FP32 host copy
- * when grad_host != NULL); synchronises */
+/* the assembled Jacobian A (after ComputeA, before ComputeNumericalILU
+ * factors it in place) into `slot`, for the A-norm loss */
+void     kf_mpnn_take_matrix(kf_mpnn *net, int slot);
+/* G of `slot` (the network on its diagonal blocks), once per factorisation;
+ * kf_mpnn_forward prepares on demand */
+void     kf_mpnn_prepare(kf_mpnn *net, int slot);
+/* out = G z (add = 0) or out += G z (add = 1) with the operator of `slot` */
+void     kf_mpnn_forward(kf_mpnn *net, int slot, const double *z, double *out, int add);
+/* loss of a pair (v, base = ILU^-1 v, target) with the slot's assembled
+ * Jacobian A: 2 (default) |A (base + G base) - v|^2 / |v|^2, the residual
+ * of the preconditioned system; 1 |A e|^2 / |v|^2 and 0 |e|^2 / |target|^2
+ * with e = base + G base - target.  The tight solution (target) sets the
+ * size of each pair's Krylov work and enters losses 0 and 1 only. */
+void     kf_mpnn_set_loss(kf_mpnn *net, int metric);
+/* the loss of (v, base, target) and its gradient (FP32 host copy when
+ * grad_host != NULL); synchronises */
 double   kf_mpnn_loss_grad(kf_mpnn *net, int slot, const double *v, const double *base, const double *target,
                            float *grad_host);
 /* Adam step with the last gradient; gclip > 0 clips its global 2-norm */
@@ -270,10 +291,61 @@ void     kf_mpnn_train_step(kf_mpnn *net, int slot, const double *v, const doubl
                             float lr, float gclip);
 int      kf_mpnn_losses(kf_mpnn *net, float *host, int cap);
 void     kf_mpnn_reset_losses(kf_mpnn *net);
-/* activations of the last forward (tests): 0 features (N x 24), 1 first
+/* activations of the last network pass (tests): 0 features (N x 16), 1 first
  * pre-activation, 2+5l h_l, 3+5l A h, 4+5l B h, 5+5l messages, 6+5l update
- * pre-activation of round l, 17 the final h (N x 32 each) */
+ * pre-activation of round l, 17 the final h, 18 [S | P], 19 [Q | 0]
+ * (N x 32 each) */
 void     kf_mpnn_activation(kf_mpnn *net, int which, float *host);
+
+/* Learning inside Kernel: with a network attached, kf_kernel runs flexible
+ * GMRES and, during the first collect_steps pseudo-time steps, solves the
+ * Newton residual and rhs_per_step Gaussian right-hand sides tightly with
+ * ILU(0)-FGMRES; each (v / |v|, ILU^-1 v / |v|, x / |v|) is a training pair
+ * with that step's diagonal blocks and assembled Jacobian.  After the last
+ * collecting step it runs train_iters Adam steps over the pairs (held-out
+ * fraction excluded), and from the next step on M v = z + a G z, z =
+ * ILU^-1 v, a = min(1, guard |z| / |G z|).  Judged steps solve the Newton
+ * system with ILU(0) alone and then with the correction, stopped once it
+ * has used as many iterations, and keep the update of the one that needed
+ * fewer; between judged steps a solve with the correction that needs more
+ * than twice the last judged ILU(0) count is redone with ILU(0) alone.
+ * Once ILU(0) alone wins it is used for the rest of the run. */
+typedef struct kf_learn_options {
+  int      collect_steps;  /* pseudo-time steps that collect pairs (10)          */
+  int      rhs_per_step;   /* Gaussian right-hand sides per step (64)            */
+  double   pair_rtol;      /* tight-solve relative tolerance (1e-10)             */
+  int      pair_maxit;     /* iteration cap of one tight solve (300)             */
+  int      train_iters;    /* Adam steps (2000)                                  */
+  double   lr;             /* learning rate (1e-3)                               */
+  double   gclip;          /* clip of the gradient's 2-norm, 0 = none (1)        */
+  double   guard;          /* largest |delta| / |ILU^-1 v| applied (1)           */
+  int      judge_every;    /* judge every this many learned steps (5); the first
+                              learned step is always judged                    */
+  double   holdout;        /* fraction of pairs held out of training (0.1)        */
+  uint64_t seed;           /* right-hand sides and pair order (3)                */
+  int      ablate;         /* 1: every learned step solves with both and keeps
+                              ILU(0)'s update (the run is the plain run, with
+                              both iteration counts recorded); 0               */
+} kf_learn_options;
+
+typedef struct kf_learn_stats {
+  int    npairs, nheld;    /* pairs collected, of which held out                 */
+  long   pair_iters;       /* FGMRES iterations of the tight solves              */
+  double pair_ms, train_ms;/* wall time of collection and of training            */
+  double loss_before;      /* held-out mean loss before training (~ ILU alone)   */
+  double loss_after;       /* ... after training                                 */
+  double loss_ilu;         /* held-out mean loss with delta = 0                  */
+  int    steps_learned;    /* steps whose update used the network                */
+  int    steps_judged, judged_won;
+  int    fallback;         /* 1: ILU(0) alone took over                          */
+} kf_learn_stats;
+
+kf_learn_options kf_default_learn_options(void);
+/* attach (net != NULL) or detach (net == NULL) a network; the network must
+ * have at least 1 + collect_steps diagonal slots and outlive the attachment */
+void kf_attach_mpnn(GEOMETRY *g, kf_mpnn *net, const kf_learn_options *lo);
+void kf_learn_get_stats(const GEOMETRY *g, kf_learn_stats *st);
+
 
 #ifdef __cplusplus
 }

@@ -90,7 +90,22 @@ class Options(C.Structure):
 
 class StepRecord(C.Structure):
     _fields_ = [("step", C.c_int), ("cfl", C.c_double), ("fnorm", C.c_double), ("krylov", C.c_int),
-                ("forces", C.c_double * 3)]
+                ("forces", C.c_double * 3), ("learned", C.c_int), ("krylov_ilu", C.c_int),
+                ("krylov_net", C.c_int)]
+
+
+class LearnOptions(C.Structure):
+    _fields_ = [("collect_steps", C.c_int), ("rhs_per_step", C.c_int), ("pair_rtol", C.c_double),
+                ("pair_maxit", C.c_int), ("train_iters", C.c_int), ("lr", C.c_double), ("gclip", C.c_double),
+                ("guard", C.c_double), ("judge_every", C.c_int), ("holdout", C.c_double), ("seed", C.c_uint64),
+                ("ablate", C.c_int)]
+
+
+class LearnStats(C.Structure):
+    _fields_ = [("npairs", C.c_int), ("nheld", C.c_int), ("pair_iters", C.c_long), ("pair_ms", C.c_double),
+                ("train_ms", C.c_double), ("loss_before", C.c_double), ("loss_after", C.c_double),
+                ("loss_ilu", C.c_double), ("steps_learned", C.c_int), ("steps_judged", C.c_int),
+                ("judged_won", C.c_int), ("fallback", C.c_int)]
 
 
 GP = C.POINTER(Geometry)
@@ -144,6 +159,8 @@ def _load():
         "ComputeNewAXPY": (None, [C.c_size_t, C.c_double, DP, DP, DP]),
         "Normalize": (C.c_double, [C.c_size_t, DP]),
         "kf_mpnn_nparam": (C.c_size_t, []),
+        "kf_mpnn_nhead": (C.c_size_t, []),
+        "kf_mpnn_prepare": (None, [C.c_void_p, C.c_int]),
         "kf_mpnn_create": (C.c_void_p, [GP, C.c_void_p, C.c_int]),
         "kf_mpnn_destroy": (None, [C.c_void_p]),
         "kf_mpnn_set_params": (None, [C.c_void_p, C.c_void_p]),
@@ -151,12 +168,17 @@ def _load():
         "kf_mpnn_take_diag": (None, [C.c_void_p, C.c_int]),
         "kf_mpnn_set_diag": (None, [C.c_void_p, C.c_int, C.c_void_p]),
         "kf_mpnn_forward": (None, [C.c_void_p, C.c_int, DP, DP, C.c_int]),
+        "kf_mpnn_take_matrix": (None, [C.c_void_p, C.c_int]),
+        "kf_mpnn_set_loss": (None, [C.c_void_p, C.c_int]),
         "kf_mpnn_loss_grad": (C.c_double, [C.c_void_p, C.c_int, DP, DP, DP, C.c_void_p]),
         "kf_mpnn_adam": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
         "kf_mpnn_train_step": (None, [C.c_void_p, C.c_int, DP, DP, DP, C.c_float, C.c_float]),
         "kf_mpnn_losses": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
         "kf_mpnn_reset_losses": (None, [C.c_void_p]),
         "kf_mpnn_activation": (None, [C.c_void_p, C.c_int, C.c_void_p]),
+        "kf_default_learn_options": (LearnOptions, []),
+        "kf_attach_mpnn": (None, [GP, C.c_void_p, C.POINTER(LearnOptions)]),
+        "kf_learn_get_stats": (None, [GP, C.POINTER(LearnStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -183,10 +205,11 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
                "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
-               "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
+               "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
                "kf_mpnn_get_params", "kf_mpnn_take_diag", "kf_mpnn_set_diag", "kf_mpnn_forward",
                "kf_mpnn_loss_grad", "kf_mpnn_adam", "kf_mpnn_train_step", "kf_mpnn_losses", "kf_mpnn_reset_losses",
-               "kf_mpnn_activation"]
+               "kf_mpnn_activation", "kf_mpnn_take_matrix", "kf_mpnn_set_loss",
+               "kf_default_learn_options", "kf_attach_mpnn", "kf_learn_get_stats"]
 
 
 def options(**kw):
@@ -407,8 +430,25 @@ class Mesh:
     def kernel(self, max_rec=1000):
         rec = (StepRecord * max_rec)()
         n = lib().kf_kernel(self.ref, rec, max_rec)
-        return [dict(step=r.step, cfl=r.cfl, fnorm=r.fnorm, krylov=r.krylov, forces=list(r.forces))
+        return [dict(step=r.step, cfl=r.cfl, fnorm=r.fnorm, krylov=r.krylov, forces=list(r.forces),
+                     learned=r.learned, krylov_ilu=r.krylov_ilu, krylov_net=r.krylov_net)
                 for r in rec[:min(n, max_rec)]]
+
+    def attach(self, net, **kw):
+        """Attach a Network to Kernel (None detaches); keywords override
+        kf_default_learn_options()."""
+        if net is None:
+            lib().kf_attach_mpnn(self.ref, None, None)
+            return
+        lo = lib().kf_default_learn_options()
+        for k, v in kw.items():
+            setattr(lo, k, v)
+        lib().kf_attach_mpnn(self.ref, net.h, C.byref(lo))
+
+    def learn_stats(self):
+        st = LearnStats()
+        lib().kf_learn_get_stats(self.ref, C.byref(st))
+        return {name: getattr(st, name) for name, _ in LearnStats._fields_}
 
     def last_error(self):
         return lib().kf_last_error(self.ref)
@@ -464,8 +504,21 @@ class Network:
         assert d.size == 16 * self.mesh.N
         lib().kf_mpnn_set_diag(self.h, slot, d.ctypes.data)
 
-    def forward(self, v_dev, out_dev, slot=0, add=False):
-        lib().kf_mpnn_forward(self.h, slot, v_dev.ptr, out_dev.ptr, 1 if add else 0)
+    def prepare(self, slot=0):
+        lib().kf_mpnn_prepare(self.h, slot)
+
+    def forward(self, z_dev, out_dev, slot=0, add=False):
+        """out = G z (or out += G z) with the operator of `slot`"""
+        lib().kf_mpnn_forward(self.h, slot, z_dev.ptr, out_dev.ptr, 1 if add else 0)
+
+    def take_matrix(self, slot):
+        lib().kf_mpnn_take_matrix(self.h, slot)
+
+    def set_loss(self, metric):
+        """"residual" (|A (base + G base) - v|^2 / |v|^2, default), "a"
+        (|A e|^2 / |v|^2) or "euclid" (|e|^2 / |target|^2), e = base + G base
+        - target"""
+        lib().kf_mpnn_set_loss(self.h, {"a": 1, "euclid": 0, "residual": 2}[metric])
 
     def loss_grad(self, v_dev, base_dev, target_dev, slot=0):
         g = np.empty(self.nparam, dtype=np.float32)
@@ -488,7 +541,7 @@ class Network:
         lib().kf_mpnn_reset_losses(self.h)
 
     def activation(self, which):
-        n = self.mesh.N * (24 if which == 0 else 32)
+        n = self.mesh.N * (16 if which == 0 else 32)
         out = np.empty(n, dtype=np.float32)
         lib().kf_mpnn_activation(self.h, which, out.ctypes.data)
         return out.reshape(self.mesh.N, -1)

@@ -372,6 +372,52 @@ void kfs_cgs_update(GEOMETRY *g, size_t n, const double *vbase, int nv, int slot
   KF_LAUNCH_CHECK();
 }
 
+/* standard normal deviates: x[i] from a counter-based generator keyed by
+ * (seed, stream, i) -- splitmix64 of the counter for two uniforms, then the
+ * Box-Muller cosine branch */
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(RB) void k_gaussian(size_t n, uint64_t key, double *__restrict__ x)
+{
+  const size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = splitmix64(key + 2 * (uint64_t)i), b = splitmix64(key + 2 * (uint64_t)i + 1);
+  const double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53; /* (0, 1] */
+  const double u2 = (double)(b >> 11) * 0x1.0p-53;         /* [0, 1) */
+  x[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/* scal[out] = min(1, gamma scal[a] / scal[b]) (1 when scal[b] is 0) */
+__global__ void k_guard(double *scal, int a, int b, double gamma, int out)
+{
+  if (threadIdx.x != 0) return;
+  const double lim = gamma * scal[a], nb = scal[b];
+  scal[out] = (nb > lim) ? lim / nb : 1.0;
+}
+
+void kfs_gaussian(GEOMETRY *g, size_t n, uint64_t seed, uint64_t stream, double *x)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  const uint64_t key = seed * 0xD1B54A32D192ED03ull ^ (stream * 0x8CB92BA72F3D8DD7ull);
+  hipLaunchKernelGGL(k_gaussian, dim3(kf_blocks_for(n, RB)), dim3(RB), 0, c->stream, n, key, x);
+  KF_LAUNCH_CHECK();
+}
+
+void kfs_guard(GEOMETRY *g, int slot_a, int slot_b, double gamma, int slot_out)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  hipLaunchKernelGGL(k_guard, dim3(1), dim3(64), 0, c->stream, c->d.scal, slot_a, slot_b, gamma, slot_out);
+  KF_LAUNCH_CHECK();
+}
+
+void **kfs_learn_state(GEOMETRY *g) { return &kf_ctx_of(g)->learn; }
+
 void kfs_work(GEOMETRY *g, int nbasis, double **grad3, double **r, double **w, double **t, double **x, double **vecs)
 {
   kf_ctx *c = kf_ctx_of(g);

@@ -236,6 +236,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
 {
   if (!g->matrix) return;
   kf_ctx *c = (kf_ctx *)g->matrix;
+  kfs_learn_detach(g);
   KF_CHECK(hipStreamSynchronize(c->stream));
   kf_dv &d = c->d;
   void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,

@@ -132,6 +132,7 @@ struct kf_ctx {
   int nvecs;
   float *f32a, *f32b;       /* single-precision forward / backward outputs */
   double *eterm, *sr;       /* per-edge time-step terms, per-vertex time-step sums */
+  void *learn;              /* learned-preconditioner state of the driver (solver.c) */
 };
 
 static inline kf_ctx *kf_ctx_of(const GEOMETRY *g)

@@ -2,31 +2,36 @@
  * mpnn.hip -- the learned preconditioner's edge message-passing network on
  * gfx950: forward, backward (parameter gradients) and Adam.
  *
- * Per vertex v (u over v's neighbours in ascending edge order, e_vu the
- * edge's unit normal times its area, signed + when v is the edge's first
- * vertex):
- *   x_v   = [v_v / s, D_v / sd]                       20 features
+ * The network turns the ILU(0) factor's diagonal blocks and the mesh
+ * geometry into a sparse linear correction operator G; the preconditioner
+ * is M v = z + G z with z = ILU(0)^-1 v, linear in v.  Context (per vertex
+ * v; u over v's neighbours in ascending edge order, e_vu the edge's unit
+ * normal times its area, + when v is the edge's first vertex):
+ *   x_v   = D_v / sd                                  16 features
  *   h0_v  = relu(W0 x_v + b0)                          32
  *   round l = 0..2:
  *     m_v = mean_u relu(A_l h_v + B_l h_u + C_l e_vu + c_l)
  *     h_v = h_v + relu(U_l h_v + V_l m_v + u_l)
- *   delta_v = s (Wo h_v + bo)                          4
- * v is the vector being preconditioned, D_v the ILU factor's diagonal block
- * as stored (the inverse of U(v,v)), s the RMS of v and sd the RMS of all D
- * blocks, so the network is equivariant to the scale of v.
+ * Operator heads (16 values = one 4 x 4 block, row-major):
+ *   [S_v; P_v] = WSA h_v + bSA,   Q_v = Wb h_v,   T_vu = P_v + Q_u + We e_vu
+ * Correction:
+ *   delta_v = S_v z_v + (1 / deg v) sum_u T_vu z_u
+ * D_v is the diagonal block as stored (the inverse of U(v,v)) and sd the
+ * RMS of all D blocks.  The context runs once per factorisation
+ * (kf_mpnn_prepare); applying G is one gather over the incidence lists.
  *
- * Dense layers -- forward (W0, A_l, B_l, [U_l V_l]), the data gradients of
- * backward (U_l^T, V_l^T, [A_l^T B_l^T]) and every weight gradient -- run on
- * the matrix cores: v_mfma_f32_32x32x8_bf16 with both operands rounded to
- * bfloat16 and FP32 accumulation.  The message sums, the message backward,
- * the decoder and the loss are FP32 vector code; every sum has a fixed order
- * (per-vertex gathers over the sliced-ELL incidence in ascending edge order,
- * fixed vertex partitions, fixed-tree and index-order reductions), so the
- * network, its gradient and its training are deterministic.
+ * Dense layers -- forward (W0, A_l, B_l, [U_l V_l], WSA, Wb), the data
+ * gradients of backward and every weight gradient -- run on the matrix
+ * cores: v_mfma_f32_32x32x8_bf16 with both operands rounded to bfloat16 and
+ * FP32 accumulation.  Messages, the operator application and its backward
+ * and the loss are FP32 vector code; every sum has a fixed order (per-vertex
+ * gathers over the sliced-ELL incidence in ascending edge order, fixed
+ * vertex partitions, fixed-tree and index-order reductions), so the network,
+ * its gradient and its training are deterministic.
  *
- * Parameters are one FP32 vector in the order W0 b0 {A B C c U V u}x3 Wo bo;
- * it is the master copy Adam updates, and the bf16 copies the matrix cores
- * read are repacked after every update.
+ * Parameters are one FP32 vector in the order W0 b0 {A B C c U V u}x3
+ * WSA bSA Wb We; it is the master copy Adam updates, and the bf16 copies
+ * the matrix cores read are repacked after every update.
  */
 #include <math.h>
 #include <string.h>
@@ -34,8 +39,7 @@
 #include "kf_dev.h"
 
 #define MP_H 32
-#define MP_NIN 20
-#define MP_KIN 24 /* input width padded to the MFMA k-step */
+#define MP_NIN 16 /* inputs: one diagonal block (a multiple of the MFMA k-step) */
 #define MP_NR 3
 
 /* offsets into the parameter vector */
@@ -50,24 +54,31 @@
 #define MP_U(l) (MP_CB(l) + MP_H)
 #define MP_V(l) (MP_U(l) + MP_H * MP_H)
 #define MP_UB(l) (MP_V(l) + MP_H * MP_H)
-#define MP_WO (MP_R0 + MP_NR * MP_RSZ)
-#define MP_BO (MP_WO + 4 * MP_H)
-#define MP_NPARAM (MP_BO + 4)
+#define MP_WSA (MP_R0 + MP_NR * MP_RSZ) /* 32 x 32: S (rows 0-15), P (16-31) */
+#define MP_BSA (MP_WSA + 32 * MP_H)
+#define MP_WB (MP_BSA + 32)             /* 16 x 32: Q */
+#define MP_WE (MP_WB + 16 * MP_H)       /* 16 x 3 */
+#define MP_NPARAM (MP_WE + 16 * 3)
+#define MP_HEADS (MP_NPARAM - MP_WSA)
 
-static_assert(MP_NPARAM == 13572, "parameter layout of the network");
+static_assert(MP_NPARAM == 14928, "parameter layout of the network");
 
-/* bf16 copies for the matrix cores, row-major [out][k].  Forward: W0 padded
- * to k = 24, per round A, B (32 x 32) and [U V] (32 x 64).  Backward
- * (transposed): per round U^T, V^T (32 x 32) and [A^T B^T] (32 x 64). */
+/* bf16 copies for the matrix cores, row-major [out][k].  Forward: W0
+ * (32 x 16), per round A, B (32 x 32) and [U V] (32 x 64), then WSA and Wb
+ * padded to 32 rows.  Backward (transposed): per round U^T, V^T (32 x 32)
+ * and [A^T B^T] (32 x 64), then [WSA^T Wb^T] (32 x 64, Wb^T padded). */
 #define MPB_W0 0
-#define MPB_A(l) (MP_H * MP_KIN + (l) * 4 * MP_H * MP_H)
+#define MPB_A(l) (MP_H * MP_NIN + (l) * 4 * MP_H * MP_H)
 #define MPB_B(l) (MPB_A(l) + MP_H * MP_H)
 #define MPB_UV(l) (MPB_B(l) + MP_H * MP_H)
-#define MPB_FWD (MP_H * MP_KIN + MP_NR * 4 * MP_H * MP_H)
+#define MPB_WSA (MP_H * MP_NIN + MP_NR * 4 * MP_H * MP_H)
+#define MPB_WB (MPB_WSA + MP_H * MP_H)
+#define MPB_FWD (MPB_WB + MP_H * MP_H)
 #define MPB_UT(l) (MPB_FWD + (l) * 4 * MP_H * MP_H)
 #define MPB_VT(l) (MPB_UT(l) + MP_H * MP_H)
 #define MPB_ABT(l) (MPB_VT(l) + MP_H * MP_H)
-#define MPB_SIZE (MPB_FWD + MP_NR * 4 * MP_H * MP_H)
+#define MPB_HT (MPB_FWD + MP_NR * 4 * MP_H * MP_H)
+#define MPB_SIZE (MPB_HT + 2 * MP_H * MP_H)
 
 #define MP_PARTS 1024 /* vertex partitions of the gradient reductions */
 #define MP_RB 256
@@ -150,7 +161,8 @@ __global__ __launch_bounds__(64) void k_dense32(uint32_t N, const float *__restr
  *   part[p][offw + o KX + k] = sum_n dY[n][o] X[n][k]      o < OY, k < KX
  *   part[p][offb + o]        = sum_n dY[n][o]              (offb >= 0)
  * dY has row stride DYW, X row stride XW; MASK: dY[n][o] counts only where
- * Mz[n][o] > 0.  The k dimension of the MFMA runs over 8 vertices at a time:
+ * Mz[n][o] > 0 (Mz with dY's layout).  The k dimension of the MFMA runs
+ * over 8 vertices at a time:
  *   A: dY[n0 + 4 (l / 32) + j][l % 32]      (rows: o)
  *   B: X[n0 + 4 (l / 32) + j][l % 32]       (columns: k)
  * The bias sums are taken of the unrounded FP32 values. */
@@ -159,6 +171,7 @@ __global__ __launch_bounds__(64) void k_wgrad(uint32_t N, uint32_t chunk, const 
                                               const float *__restrict__ Mz, const float *__restrict__ X,
                                               float *__restrict__ part, uint32_t offw, int offb)
 {
+  static_assert(OY <= 32 && OY <= DYW && KX <= 32 && KX <= XW, "one 32 x 32 tile");
   const uint32_t lane = threadIdx.x, li = lane & 31u, lk = (lane >> 5) * 4u;
   const uint32_t p = blockIdx.x, n0 = p * chunk, n1 = min(N, n0 + chunk);
   f32x16 acc;
@@ -172,11 +185,11 @@ __global__ __launch_bounds__(64) void k_wgrad(uint32_t N, uint32_t chunk, const 
       const uint32_t n = nb + lk + j;
       float y = 0.0f, x = 0.0f;
       if (n < n1) {
-        if (li < (uint32_t)DYW) {
+        if (li < (uint32_t)OY) {
           y = dY[(size_t)n * DYW + li];
           if (MASK && !(Mz[(size_t)n * DYW + li] > 0.0f)) y = 0.0f;
         }
-        if (li < (uint32_t)XW) x = X[(size_t)n * XW + li];
+        if (li < (uint32_t)KX) x = X[(size_t)n * XW + li];
       }
       bsum += y;
       av[j] = (short)f2bf(y);
@@ -205,7 +218,7 @@ __global__ __launch_bounds__(MP_RB) void k_mp_grad_reduce(const float *__restric
   grad[i] = (float)acc;
 }
 
-/* ---- scales and features ---------------------------------------------------- */
+/* ---- reductions and features ---------------------------------------------------- */
 
 __device__ __forceinline__ double mp_block_sum(double v, double *sh)
 {
@@ -235,26 +248,16 @@ __global__ __launch_bounds__(MP_RB) void k_mp_sumsq(size_t n, const T *__restric
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-/* One lane sums the partials in index order.  what 0: out[0] = s,
- * out[1] = 1/s with s the RMS over `count` values (1 when zero or not
- * finite); what 1: out[0] = 1/RMS; what 2: out[3] = 1/sum (0 when zero). */
-__global__ void k_mp_finish(const double *__restrict__ part, double count, int what, float *__restrict__ out)
+/* One lane sums the KF_NRED partials in index order into *out = 1 / RMS
+ * over `count` values (1 when zero or not finite). */
+__global__ void k_mp_finish(const double *__restrict__ part, double count, float *__restrict__ out)
 {
   if (threadIdx.x != 0) return;
   double a = 0.0;
   for (int i = 0; i < KF_NRED; i++) a += part[i];
-  if (what == 2) {
-    out[3] = a > 0.0 ? (float)(1.0 / a) : 0.0f;
-    return;
-  }
   double s = sqrt(a / count);
   if (!(s > 0.0) || !isfinite(s)) s = 1.0;
-  if (what == 0) {
-    out[0] = (float)s;
-    out[1] = (float)(1.0 / s);
-  } else {
-    out[0] = (float)(1.0 / s);
-  }
+  *out = (float)(1.0 / s);
 }
 
 /* the ILU factor's diagonal blocks, as FP32 [N][16] */
@@ -267,20 +270,12 @@ __global__ __launch_bounds__(MP_RB) void k_mp_take_diag(uint32_t N, const uint32
   D[k] = (float)a[16 * (size_t)diag[k >> 4] + (k & 15)];
 }
 
-__global__ __launch_bounds__(MP_RB) void k_mp_features(uint32_t N, const double *__restrict__ v,
-                                                       const float *__restrict__ D, const float *__restrict__ sc,
+/* x_n = D_n / sd */
+__global__ __launch_bounds__(MP_RB) void k_mp_features(size_t n, const float *__restrict__ D,
                                                        const float *__restrict__ isd_p, float *__restrict__ X)
 {
-  const uint32_t n = blockIdx.x * MP_RB + threadIdx.x;
-  if (n >= N) return;
-  const float is = sc[1], isd = *isd_p;
-  float *x = X + (size_t)n * MP_KIN;
-#pragma unroll
-  for (int c = 0; c < 4; c++) x[c] = (float)v[4 * (size_t)n + c] * is;
-#pragma unroll
-  for (int c = 0; c < 16; c++) x[4 + c] = D[16 * (size_t)n + c] * isd;
-#pragma unroll
-  for (int c = MP_NIN; c < MP_KIN; c++) x[c] = 0.0f;
+  const size_t k = (size_t)blockIdx.x * MP_RB + threadIdx.x;
+  if (k < n) X[k] = D[k] * *isd_p;
 }
 
 /* signed edge vector of every sliced-ELL slot: (nx, ny, nz) area, negated
@@ -379,74 +374,228 @@ __global__ __launch_bounds__(MP_RB) void k_mp_message_bwd(kf_dv d, uint32_t chun
   }
 }
 
-/* ---- decoder and loss ------------------------------------------------------------ */
+/* ---- the correction operator ------------------------------------------------------ */
 
-/* delta_v = s (Wo h_v + bo): stored, or added to out */
-__global__ __launch_bounds__(MP_RB) void k_mp_decode(uint32_t N, const float *__restrict__ Hf,
-                                                     const float *__restrict__ theta, const float *__restrict__ sc,
-                                                     double *__restrict__ out, int add)
+__device__ __forceinline__ float4 ld4(const double *z, uint32_t v)
 {
-  __shared__ float wo[4 * MP_H + 4];
-  for (int k = threadIdx.x; k < 4 * MP_H + 4; k += MP_RB) wo[k] = theta[MP_WO + k];
+  return make_float4((float)z[4 * (size_t)v], (float)z[4 * (size_t)v + 1], (float)z[4 * (size_t)v + 2],
+                     (float)z[4 * (size_t)v + 3]);
+}
+
+/* delta_v = S_v z_v + (1/deg v) [(P_v + b) zs_v + We : Ez_v + sum_u Q_u z_u]
+ * with zs_v = sum_u z_u and Ez_v[j][k] = sum_u z_u[j] e_vu[k] (the edge
+ * blocks T_vu = P_v + b + Q_u + We e_vu factored), one thread per vertex.
+ * OP1 = [S | P + b] (N x 32), OPQ = [Q | 0] (N x 32).  Output: FP64 out
+ * (stored, or added when add) or, when outf is given, FP32 outf. */
+__global__ __launch_bounds__(MP_RB) void k_mp_apply(kf_dv d, const float4 *__restrict__ ev,
+                                                    const float *__restrict__ OP1, const float *__restrict__ OPQ,
+                                                    const float *__restrict__ theta, const double *__restrict__ z,
+                                                    double *__restrict__ out, int add, float *__restrict__ outf)
+{
+  __shared__ float we[48];
+  if (threadIdx.x < 48) we[threadIdx.x] = theta[MP_WE + threadIdx.x];
   __syncthreads();
-  const uint32_t n = blockIdx.x * MP_RB + threadIdx.x;
-  if (n >= N) return;
-  const float *h = Hf + (size_t)n * MP_H;
-  const float s = sc[0];
+  const uint32_t v = blockIdx.x * MP_RB + threadIdx.x;
+  if (v >= d.N) return;
+  const uint32_t deg = d.deg[v];
+  size_t idx = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
+  float qz[4] = {0.0f, 0.0f, 0.0f, 0.0f}, zs[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ez[12];
 #pragma unroll
-  for (int o = 0; o < 4; o++) {
-    float z = wo[4 * MP_H + o];
-    for (int k = 0; k < MP_H; k++) z += wo[o * MP_H + k] * h[k];
-    const double dl = (double)(s * z);
-    out[4 * (size_t)n + o] = add ? out[4 * (size_t)n + o] + dl : dl;
+  for (int i = 0; i < 12; i++) ez[i] = 0.0f;
+  for (uint32_t k = 0; k < deg; k++, idx += 64) {
+    const uint32_t u = d.sell_nbr[idx];
+    const float4 e = ev[idx], zu4 = ld4(z, u);
+    const float zu[4] = {zu4.x, zu4.y, zu4.z, zu4.w};
+    const float4 *q = (const float4 *)(OPQ + (size_t)u * MP_H);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float4 qi = q[i];
+      qz[i] += qi.x * zu[0] + qi.y * zu[1] + qi.z * zu[2] + qi.w * zu[3];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      zs[j] += zu[j];
+      ez[3 * j] += zu[j] * e.x;
+      ez[3 * j + 1] += zu[j] * e.y;
+      ez[3 * j + 2] += zu[j] * e.z;
+    }
+  }
+  const float4 zv4 = ld4(z, v);
+  const float zv[4] = {zv4.x, zv4.y, zv4.z, zv4.w};
+  const float *op = OP1 + (size_t)v * MP_H;
+  const float idg = 1.0f / (float)max(deg, 1u);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    float sv = 0.0f, t = qz[i];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      sv += op[4 * i + j] * zv[j];
+      t += op[16 + 4 * i + j] * zs[j];
+      t += we[3 * (4 * i + j)] * ez[3 * j] + we[3 * (4 * i + j) + 1] * ez[3 * j + 1] +
+           we[3 * (4 * i + j) + 2] * ez[3 * j + 2];
+    }
+    const float dl = sv + t * idg;
+    if (outf) outf[4 * (size_t)v + i] = dl;
+    else out[4 * (size_t)v + i] = add ? out[4 * (size_t)v + i] + (double)dl : (double)dl;
   }
 }
 
-/* Loss |base + delta - target|^2 / |target|^2 and the start of backward:
- *   dout = 2 (base + delta - target) / |target|^2 * s      (N x 4)
- *   dh   = dout Wo                                         (N x 32)
- * and the block's sum of squared residuals (lpart[blockIdx.x]). */
-__global__ __launch_bounds__(MP_RB) void k_mp_loss_bwd(uint32_t N, const float *__restrict__ Hf,
-                                                       const float *__restrict__ theta, const float *__restrict__ sc,
-                                                       const double *__restrict__ base,
-                                                       const double *__restrict__ target, float *__restrict__ dout,
-                                                       float *__restrict__ dh, double *__restrict__ lpart)
+/* Backward of the correction given gv = dL/d delta (N x 4, FP32) and z:
+ *   dOP1_v = [g_v z_v^T | (g_v / deg v) zs_v^T]       (N x 32)
+ *   dOPQ_v = [gbar_v z_v^T | 0],  gbar_v = sum_u g_u / deg u   (N x 32)
+ *   gd_v   = g_v / deg v (N x 4),  Ez_v (N x 12, [j][k] as 3 j + k)
+ * (the gradient of We is then sum_v gd_v Ez_v^T, a weight gradient). */
+__global__ __launch_bounds__(MP_RB) void k_mp_apply_bwd(kf_dv d, const float4 *__restrict__ ev,
+                                                        const double *__restrict__ z, const float *__restrict__ gv,
+                                                        float *__restrict__ dOP1, float *__restrict__ dOPQ,
+                                                        float *__restrict__ gd, float *__restrict__ Ez)
 {
-  __shared__ float wo[4 * MP_H + 4];
+  const uint32_t v = blockIdx.x * MP_RB + threadIdx.x;
+  if (v >= d.N) return;
+  const uint32_t deg = d.deg[v];
+  size_t idx = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
+  float zs[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gb[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ez[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) ez[i] = 0.0f;
+  for (uint32_t k = 0; k < deg; k++, idx += 64) {
+    const uint32_t u = d.sell_nbr[idx];
+    const float4 e = ev[idx], zu4 = ld4(z, u), gu = *(const float4 *)(gv + 4 * (size_t)u);
+    const float iu = 1.0f / (float)max(d.deg[u], 1u);
+    const float zu[4] = {zu4.x, zu4.y, zu4.z, zu4.w};
+    gb[0] += gu.x * iu;
+    gb[1] += gu.y * iu;
+    gb[2] += gu.z * iu;
+    gb[3] += gu.w * iu;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      zs[j] += zu[j];
+      ez[3 * j] += zu[j] * e.x;
+      ez[3 * j + 1] += zu[j] * e.y;
+      ez[3 * j + 2] += zu[j] * e.z;
+    }
+  }
+  const float4 zv4 = ld4(z, v), g4 = *(const float4 *)(gv + 4 * (size_t)v);
+  const float zv[4] = {zv4.x, zv4.y, zv4.z, zv4.w}, g[4] = {g4.x, g4.y, g4.z, g4.w};
+  const float idg = 1.0f / (float)max(deg, 1u);
+  float *o1 = dOP1 + (size_t)v * MP_H, *oq = dOPQ + (size_t)v * MP_H;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float gi = g[i] * idg;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      o1[4 * i + j] = g[i] * zv[j];
+      o1[16 + 4 * i + j] = gi * zs[j];
+      oq[4 * i + j] = gb[i] * zv[j];
+      oq[16 + 4 * i + j] = 0.0f;
+    }
+    gd[4 * (size_t)v + i] = gi;
+  }
+#pragma unroll
+  for (int i = 0; i < 12; i++) Ez[12 * (size_t)v + i] = ez[i];
+}
+
+/* ---- loss ---------------------------------------------------------------------------- */
+
+/* e = base + delta - target (target may be NULL: e = base + delta), FP32
+ * computed in FP64 */
+__global__ __launch_bounds__(MP_RB) void k_mp_err(size_t n, const float *__restrict__ dl,
+                                                  const double *__restrict__ base, const double *__restrict__ target,
+                                                  float *__restrict__ e)
+{
+  const size_t k = (size_t)blockIdx.x * MP_RB + threadIdx.x;
+  if (k < n) e[k] = (float)(base[k] + (double)dl[k] - (target ? target[k] : 0.0));
+}
+
+/* y -= v (FP32 y, FP64 v) */
+__global__ __launch_bounds__(MP_RB) void k_mp_sub(size_t n, const double *__restrict__ v, float *__restrict__ y)
+{
+  const size_t k = (size_t)blockIdx.x * MP_RB + threadIdx.x;
+  if (k < n) y[k] = (float)((double)y[k] - v[k]);
+}
+
+/* y_i = sum over row i's blocks A(i, j) x_j (one thread per block row) */
+__global__ __launch_bounds__(MP_RB) void k_mp_bsr_mv(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                     const uint32_t *__restrict__ col, const float *__restrict__ A,
+                                                     const float *__restrict__ x, float *__restrict__ y)
+{
+  const uint32_t i = blockIdx.x * MP_RB + threadIdx.x;
+  if (i >= N) return;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+    const float4 xv = *(const float4 *)(x + 4 * (size_t)col[k]);
+    const float *b = A + 16 * (size_t)k;
+    a0 += b[0] * xv.x + b[1] * xv.y + b[2] * xv.z + b[3] * xv.w;
+    a1 += b[4] * xv.x + b[5] * xv.y + b[6] * xv.z + b[7] * xv.w;
+    a2 += b[8] * xv.x + b[9] * xv.y + b[10] * xv.z + b[11] * xv.w;
+    a3 += b[12] * xv.x + b[13] * xv.y + b[14] * xv.z + b[15] * xv.w;
+  }
+  *(float4 *)(y + 4 * (size_t)i) = make_float4(a0, a1, a2, a3);
+}
+
+/* g_j = sum over row j's positions k (column i = col[k]) of A(i, j)^T y_i,
+ * A(i, j) at tpos[k] (the pattern is structurally symmetric) */
+__global__ __launch_bounds__(MP_RB) void k_mp_bsr_mvt(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                      const uint32_t *__restrict__ col,
+                                                      const uint32_t *__restrict__ tpos, const float *__restrict__ A,
+                                                      const float *__restrict__ y, float *__restrict__ g)
+{
+  const uint32_t j = blockIdx.x * MP_RB + threadIdx.x;
+  if (j >= N) return;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  for (uint32_t k = rowptr[j]; k < rowptr[j + 1]; k++) {
+    const float4 yv = *(const float4 *)(y + 4 * (size_t)col[k]);
+    const float *b = A + 16 * (size_t)tpos[k];
+    a0 += b[0] * yv.x + b[4] * yv.y + b[8] * yv.z + b[12] * yv.w;
+    a1 += b[1] * yv.x + b[5] * yv.y + b[9] * yv.z + b[13] * yv.w;
+    a2 += b[2] * yv.x + b[6] * yv.y + b[10] * yv.z + b[14] * yv.w;
+    a3 += b[3] * yv.x + b[7] * yv.y + b[11] * yv.z + b[15] * yv.w;
+  }
+  *(float4 *)(g + 4 * (size_t)j) = make_float4(a0, a1, a2, a3);
+}
+
+__global__ __launch_bounds__(MP_RB) void k_mp_sq4(uint32_t N, const float *__restrict__ x, double *__restrict__ lpart)
+{
   __shared__ double sh[MP_RB];
-  for (int k = threadIdx.x; k < 4 * MP_H + 4; k += MP_RB) wo[k] = theta[MP_WO + k];
-  __syncthreads();
   const uint32_t n = blockIdx.x * MP_RB + threadIdx.x;
-  double lsum = 0.0;
+  double acc = 0.0;
   if (n < N) {
-    const float *h = Hf + (size_t)n * MP_H;
-    const float s = sc[0], itn = sc[3];
-    float g[4];
 #pragma unroll
     for (int o = 0; o < 4; o++) {
-      float z = wo[4 * MP_H + o];
-      for (int k = 0; k < MP_H; k++) z += wo[o * MP_H + k] * h[k];
-      const double res = base[4 * (size_t)n + o] + (double)(s * z) - target[4 * (size_t)n + o];
-      lsum += res * res;
-      g[o] = (float)(2.0 * res) * itn * s;
-      dout[4 * (size_t)n + o] = g[o];
+      const double v = (double)x[4 * (size_t)n + o];
+      acc += v * v;
     }
-    for (int k = 0; k < MP_H; k++)
-      dh[(size_t)n * MP_H + k] = g[0] * wo[k] + g[1] * wo[MP_H + k] + g[2] * wo[2 * MP_H + k] + g[3] * wo[3 * MP_H + k];
   }
-  const double s = mp_block_sum(lsum, sh);
+  const double s = mp_block_sum(acc, sh);
   if (threadIdx.x == 0) lpart[blockIdx.x] = s;
 }
 
-/* loss = (block partials summed in order) / (target partials summed in order) */
+/* loss = (block partials summed in order) / (reference partials summed in
+ * order) into out[0]; the gradient scale 2 / that reference sum into sc[3] */
 __global__ void k_mp_loss_finish(const double *__restrict__ lpart, uint32_t nb, const double *__restrict__ tpart,
-                                 float *__restrict__ out)
+                                 float *__restrict__ out, float *__restrict__ sc)
 {
   if (threadIdx.x != 0) return;
   double a = 0.0, t = 0.0;
   for (uint32_t i = 0; i < nb; i++) a += lpart[i];
   for (int i = 0; i < KF_NRED; i++) t += tpart[i];
   out[0] = t > 0.0 ? (float)(a / t) : 0.0f;
+  sc[3] = t > 0.0 ? (float)(2.0 / t) : 0.0f;
+}
+
+/* g = f x, f = sc[3] */
+__global__ __launch_bounds__(MP_RB) void k_mp_scale(size_t n, const float *__restrict__ x,
+                                                    const float *__restrict__ sc, float *__restrict__ g)
+{
+  const size_t k = (size_t)blockIdx.x * MP_RB + threadIdx.x;
+  if (k < n) g[k] = x[k] * sc[3];
+}
+
+/* A (FP64 or FP32 BSR values) as FP32 */
+template <typename TA>
+__global__ __launch_bounds__(MP_RB) void k_mp_to_f32(size_t n, const TA *__restrict__ a, float *__restrict__ out)
+{
+  const size_t k = (size_t)blockIdx.x * MP_RB + threadIdx.x;
+  if (k < n) out[k] = (float)a[k];
 }
 
 /* ---- Adam ---------------------------------------------------------------------- */
@@ -488,27 +637,37 @@ __global__ void k_mp_pack(const float *__restrict__ theta, unsigned short *__res
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= MPB_SIZE) return;
   float w = 0.0f;
-  if (t < MP_H * MP_KIN) {
-    const uint32_t o = t / MP_KIN, k = t % MP_KIN;
-    w = k < MP_NIN ? theta[MP_W0 + o * MP_NIN + k] : 0.0f;
-  } else if (t < MPB_FWD) {
-    const uint32_t r = t - MP_H * MP_KIN, l = r / (4 * MP_H * MP_H), q = r % (4 * MP_H * MP_H);
-    if (q < MP_H * MP_H) w = theta[MP_A(l) + q];
-    else if (q < 2 * MP_H * MP_H) w = theta[MP_B(l) + q - MP_H * MP_H];
+  const uint32_t HH = MP_H * MP_H;
+  if (t < MPB_A(0)) {
+    w = theta[MP_W0 + t]; /* 32 x 16 as stored */
+  } else if (t < MPB_WSA) {
+    const uint32_t r = t - MPB_A(0), l = r / (4 * HH), q = r % (4 * HH);
+    if (q < HH) w = theta[MP_A(l) + q];
+    else if (q < 2 * HH) w = theta[MP_B(l) + q - HH];
     else {
-      const uint32_t qq = q - 2 * MP_H * MP_H, o = qq / (2 * MP_H), k = qq % (2 * MP_H);
+      const uint32_t qq = q - 2 * HH, o = qq / (2 * MP_H), k = qq % (2 * MP_H);
       w = k < MP_H ? theta[MP_U(l) + o * MP_H + k] : theta[MP_V(l) + o * MP_H + (k - MP_H)];
     }
-  } else {
-    const uint32_t r = t - MPB_FWD, l = r / (4 * MP_H * MP_H), q = r % (4 * MP_H * MP_H);
-    if (q < 2 * MP_H * MP_H) {
+  } else if (t < MPB_WB) {
+    w = theta[MP_WSA + (t - MPB_WSA)];
+  } else if (t < MPB_FWD) {
+    const uint32_t q = t - MPB_WB, o = q / MP_H, k = q % MP_H;
+    w = o < 16 ? theta[MP_WB + o * MP_H + k] : 0.0f;
+  } else if (t < MPB_HT) {
+    const uint32_t r = t - MPB_FWD, l = r / (4 * HH), q = r % (4 * HH);
+    if (q < 2 * HH) {
       /* U^T then V^T: row o, column k = U[k][o] (V[k][o]) */
-      const uint32_t which = q / (MP_H * MP_H), qq = q % (MP_H * MP_H), o = qq / MP_H, k = qq % MP_H;
+      const uint32_t which = q / HH, qq = q % HH, o = qq / MP_H, k = qq % MP_H;
       w = theta[(which ? MP_V(l) : MP_U(l)) + k * MP_H + o];
     } else {
-      const uint32_t qq = q - 2 * MP_H * MP_H, o = qq / (2 * MP_H), k = qq % (2 * MP_H);
+      const uint32_t qq = q - 2 * HH, o = qq / (2 * MP_H), k = qq % (2 * MP_H);
       w = k < MP_H ? theta[MP_A(l) + k * MP_H + o] : theta[MP_B(l) + (k - MP_H) * MP_H + o];
     }
+  } else {
+    /* [WSA^T Wb^T]: row o (hidden channel), column k < 32: WSA[k][o], then Wb[k - 32][o] */
+    const uint32_t q = t - MPB_HT, o = q / (2 * MP_H), k = q % (2 * MP_H);
+    if (k < MP_H) w = theta[MP_WSA + k * MP_H + o];
+    else if (k - MP_H < 16) w = theta[MP_WB + (k - MP_H) * MP_H + o];
   }
   wb[t] = f2bf(w);
 }
@@ -523,10 +682,18 @@ struct kf_mpnn {
   unsigned short *wb;
   float4 *ev;
   float *X, *Z0, *H[MP_NR + 1], *P[MP_NR], *Q[MP_NR], *M[MP_NR], *ZU[MP_NR];
-  float *dh, *dhn, *dm, *dP, *dQ, *dout;
-  float *dstore, *isd; /* per slot: FP32 [N][16] diagonal blocks, 1 / their RMS */
+  float *OP1, *OPQ;          /* operator coefficients of the prepared slot */
+  float *dh, *dhn, *dm, *dP, *dQ;
+  float *dOP1, *dOPQ, *gd, *Ez;
+  float *dstore, *isd;       /* per slot: FP32 [N][16] diagonal blocks, 1 / their RMS */
   double *red, *tred, *lpart;
-  int t; /* Adam steps so far */
+  float *dl, *ebuf, *ybuf, *gbuf; /* loss: correction, error, A error, gradient (N x 4) */
+  float **mslot;             /* per slot: the assembled Jacobian (FP32 BSR) or NULL */
+  uint32_t *tpos;            /* BSR position of the transposed block */
+  int t;                     /* Adam steps so far */
+  int metric;                /* loss: 0 |e|^2/|target|^2, 1 |A e|^2/|v|^2,
+                                2 |A (base + G base) - v|^2/|v|^2 */
+  int prepared;              /* slot whose operator OP1/OPQ hold, -1: none */
 };
 
 template <typename T>
@@ -540,7 +707,7 @@ static T *mp_alloc(size_t n)
 static void mp_check_slot(const kf_mpnn *net, int slot)
 {
   if (slot < 0 || (uint32_t)slot >= net->nslot) {
-    fprintf(stderr, "kf: network diagonal slot %d out of range [0, %u)\n", slot, net->nslot);
+    fprintf(stderr, "kf: network slot %d out of range [0, %u)\n", slot, net->nslot);
     abort();
   }
 }
@@ -550,23 +717,22 @@ static void mp_repack(kf_mpnn *net)
   hipLaunchKernelGGL(k_mp_pack, dim3(kf_blocks_for(MPB_SIZE, 256)), dim3(256), 0, net->c->stream, net->theta,
                      net->wb);
   KF_LAUNCH_CHECK();
+  net->prepared = -1;
 }
 
-/* forward of the whole network on v (4N, FP64) with diagonal slot `slot`;
- * every activation stays in the object for backward */
-static void mp_forward(kf_mpnn *net, int slot, const double *v)
+/* the context network on slot `slot`'s diagonal blocks and the operator
+ * coefficients OP1, OPQ; every activation stays in the object for backward */
+static void mp_context(kf_mpnn *net, int slot)
 {
   kf_ctx *c = net->c;
   hipStream_t st = c->stream;
-  const uint32_t N = net->N, nb = kf_blocks_for(N, MP_RB), nw = kf_blocks_for(N, 32);
+  const uint32_t N = net->N, nw = kf_blocks_for(N, 32);
   const float *nf = NULL;
   const uint32_t *nu = NULL;
   float *nz = NULL;
-  hipLaunchKernelGGL(k_mp_sumsq<double>, dim3(KF_NRED), dim3(MP_RB), 0, st, 4 * (size_t)N, v, net->red);
-  hipLaunchKernelGGL(k_mp_finish, dim3(1), dim3(64), 0, st, net->red, 4.0 * N, 0, net->sc);
-  hipLaunchKernelGGL(k_mp_features, dim3(nb), dim3(MP_RB), 0, st, N, v, net->dstore + (size_t)slot * 16 * N,
-                     net->sc, net->isd + slot, net->X);
-  hipLaunchKernelGGL((k_dense32<MP_KIN, 0, 1, false>), dim3(nw), dim3(64), 0, st, N, net->X, nf, nf,
+  hipLaunchKernelGGL(k_mp_features, dim3(kf_blocks_for(16 * (size_t)N, MP_RB)), dim3(MP_RB), 0, st,
+                     16 * (size_t)N, net->dstore + (size_t)slot * 16 * N, net->isd + slot, net->X);
+  hipLaunchKernelGGL((k_dense32<MP_NIN, 0, 1, false>), dim3(nw), dim3(64), 0, st, N, net->X, nf, nf,
                      net->wb + MPB_W0, net->theta + MP_B0, nf, nu, net->Z0, net->H[0]);
   for (int l = 0; l < MP_NR; l++) {
     hipLaunchKernelGGL((k_dense32<MP_H, 0, 0, false>), dim3(nw), dim3(64), 0, st, N, net->H[l], nf, nf,
@@ -578,28 +744,59 @@ static void mp_forward(kf_mpnn *net, int slot, const double *v)
     hipLaunchKernelGGL((k_dense32<MP_H, MP_H, 2, false>), dim3(nw), dim3(64), 0, st, N, net->H[l], net->M[l], nf,
                        net->wb + MPB_UV(l), net->theta + MP_UB(l), net->H[l], nu, net->ZU[l], net->H[l + 1]);
   }
+  hipLaunchKernelGGL((k_dense32<MP_H, 0, 0, false>), dim3(nw), dim3(64), 0, st, N, net->H[MP_NR], nf, nf,
+                     net->wb + MPB_WSA, net->theta + MP_BSA, nf, nu, nz, net->OP1);
+  hipLaunchKernelGGL((k_dense32<MP_H, 0, 0, false>), dim3(nw), dim3(64), 0, st, N, net->H[MP_NR], nf, nf,
+                     net->wb + MPB_WB, nf, nf, nu, nz, net->OPQ);
   KF_LAUNCH_CHECK();
+  net->prepared = slot;
 }
 
-/* loss of (v, base, target) and the parameter gradient into net->grad;
- * the loss lands in *loss_dev */
+/* loss of (v, base, target) and the parameter gradient into net->grad; the
+ * loss lands in *loss_dev */
 static void mp_loss_grad(kf_mpnn *net, int slot, const double *v, const double *base, const double *target,
                          float *loss_dev)
 {
   kf_ctx *c = net->c;
   hipStream_t st = c->stream;
-  const uint32_t N = net->N, nw = kf_blocks_for(N, 32), P = net->nparts, ch = net->chunk;
+  const uint32_t N = net->N, nw = kf_blocks_for(N, 32), P = net->nparts, ch = net->chunk, nb = net->nlblk;
+  const size_t n4 = 4 * (size_t)N;
+  const uint32_t nb4 = kf_blocks_for(n4, MP_RB);
   const float *nf = NULL;
   const uint32_t *nu = NULL;
   float *nz = NULL;
-  mp_forward(net, slot, v);
-  hipLaunchKernelGGL(k_mp_sumsq<double>, dim3(KF_NRED), dim3(MP_RB), 0, st, 4 * (size_t)N, target, net->tred);
-  hipLaunchKernelGGL(k_mp_finish, dim3(1), dim3(64), 0, st, net->tred, 1.0, 2, net->sc);
-  hipLaunchKernelGGL(k_mp_loss_bwd, dim3(net->nlblk), dim3(MP_RB), 0, st, N, net->H[MP_NR], net->theta, net->sc,
-                     base, target, net->dout, net->dh, net->lpart);
-  hipLaunchKernelGGL(k_mp_loss_finish, dim3(1), dim3(64), 0, st, net->lpart, net->nlblk, net->tred, loss_dev);
-  hipLaunchKernelGGL((k_wgrad<4, MP_H, 4, MP_H, false>), dim3(P), dim3(64), 0, st, N, ch, net->dout, nf,
-                     net->H[MP_NR], net->part, (uint32_t)MP_WO, (int)MP_BO);
+  const int metric = net->metric;
+  const float *A = net->mslot[slot];
+  if (metric >= 1 && !A) {
+    fprintf(stderr, "kf: the A-norm loss needs the Jacobian of slot %d (kf_mpnn_take_matrix)\n", slot);
+    abort();
+  }
+  mp_context(net, slot);
+  hipLaunchKernelGGL(k_mp_apply, dim3(nb), dim3(MP_RB), 0, st, c->d, net->ev, net->OP1, net->OPQ, net->theta, base,
+                     (double *)NULL, 0, net->dl);
+  hipLaunchKernelGGL(k_mp_err, dim3(nb4), dim3(MP_RB), 0, st, n4, net->dl, base, metric == 2 ? NULL : target,
+                     net->ebuf);
+  hipLaunchKernelGGL(k_mp_sumsq<double>, dim3(KF_NRED), dim3(MP_RB), 0, st, n4, metric ? v : target, net->tred);
+  if (metric >= 1)
+    hipLaunchKernelGGL(k_mp_bsr_mv, dim3(nb), dim3(MP_RB), 0, st, N, c->d.rowptr, c->d.col, A, net->ebuf, net->ybuf);
+  if (metric == 2) hipLaunchKernelGGL(k_mp_sub, dim3(nb4), dim3(MP_RB), 0, st, n4, v, net->ybuf);
+  hipLaunchKernelGGL(k_mp_sq4, dim3(nb), dim3(MP_RB), 0, st, N, metric ? net->ybuf : net->ebuf, net->lpart);
+  hipLaunchKernelGGL(k_mp_loss_finish, dim3(1), dim3(64), 0, st, net->lpart, nb, net->tred, loss_dev, net->sc);
+  if (metric >= 1)
+    hipLaunchKernelGGL(k_mp_bsr_mvt, dim3(nb), dim3(MP_RB), 0, st, N, c->d.rowptr, c->d.col, net->tpos, A, net->ybuf,
+                       net->ebuf);
+  hipLaunchKernelGGL(k_mp_scale, dim3(nb4), dim3(MP_RB), 0, st, n4, net->ebuf, net->sc, net->gbuf);
+  /* the operator heads */
+  hipLaunchKernelGGL(k_mp_apply_bwd, dim3(nb), dim3(MP_RB), 0, st, c->d, net->ev, base, net->gbuf, net->dOP1,
+                     net->dOPQ, net->gd, net->Ez);
+  hipLaunchKernelGGL((k_wgrad<MP_H, MP_H, 32, MP_H, false>), dim3(P), dim3(64), 0, st, N, ch, net->dOP1, nf,
+                     net->H[MP_NR], net->part, (uint32_t)MP_WSA, (int)MP_BSA);
+  hipLaunchKernelGGL((k_wgrad<MP_H, MP_H, 16, MP_H, false>), dim3(P), dim3(64), 0, st, N, ch, net->dOPQ, nf,
+                     net->H[MP_NR], net->part, (uint32_t)MP_WB, -1);
+  hipLaunchKernelGGL((k_wgrad<4, 12, 4, 12, false>), dim3(P), dim3(64), 0, st, N, ch, net->gd, nf, net->Ez,
+                     net->part, (uint32_t)MP_WE, -1);
+  hipLaunchKernelGGL((k_dense32<MP_H, MP_H, 0, false>), dim3(nw), dim3(64), 0, st, N, net->dOP1, net->dOPQ, nf,
+                     net->wb + MPB_HT, nf, nf, nu, nz, net->dh);
   for (int l = MP_NR - 1; l >= 0; l--) {
     /* dzu = dh [zu > 0]: gradients of U, u and V */
     hipLaunchKernelGGL((k_wgrad<MP_H, MP_H, MP_H, MP_H, true>), dim3(P), dim3(64), 0, st, N, ch, net->dh, net->ZU[l],
@@ -621,7 +818,7 @@ static void mp_loss_grad(kf_mpnn *net, int slot, const double *v, const double *
     hipLaunchKernelGGL((k_dense32<MP_H, MP_H, 3, false>), dim3(nw), dim3(64), 0, st, N, net->dP, net->dQ, nf,
                        net->wb + MPB_ABT(l), nf, net->dhn, nu, nz, net->dh);
   }
-  hipLaunchKernelGGL((k_wgrad<MP_H, MP_KIN, MP_H, MP_NIN, true>), dim3(P), dim3(64), 0, st, N, ch, net->dh, net->Z0,
+  hipLaunchKernelGGL((k_wgrad<MP_H, MP_NIN, MP_H, MP_NIN, true>), dim3(P), dim3(64), 0, st, N, ch, net->dh, net->Z0,
                      net->X, net->part, (uint32_t)MP_W0, (int)MP_B0);
   hipLaunchKernelGGL(k_mp_grad_reduce, dim3(kf_blocks_for(MP_NPARAM, MP_RB)), dim3(MP_RB), 0, st, net->part, P,
                      net->grad);
@@ -629,6 +826,7 @@ static void mp_loss_grad(kf_mpnn *net, int slot, const double *v, const double *
 }
 
 extern "C" size_t kf_mpnn_nparam(void) { return MP_NPARAM; }
+extern "C" size_t kf_mpnn_nhead(void) { return MP_HEADS; }
 
 extern "C" void kf_mpnn_set_params(kf_mpnn *net, const float *theta_host);
 
@@ -636,12 +834,14 @@ extern "C" kf_mpnn *kf_mpnn_create(GEOMETRY *g, const float *theta_host, int nsl
 {
   kf_ctx *c = kf_ctx_of(g);
   if (!theta_host || nslot < 1) {
-    fprintf(stderr, "kf: kf_mpnn_create needs initial parameters and at least one diagonal slot\n");
+    fprintf(stderr, "kf: kf_mpnn_create needs initial parameters and at least one slot\n");
     abort();
   }
   kf_mpnn *net = new kf_mpnn();
   memset((void *)net, 0, sizeof(*net));
   net->c = c;
+  net->metric = 2;
+  net->prepared = -1;
   const uint32_t N = c->d.N;
   net->N = N;
   net->nslot = (uint32_t)nslot;
@@ -666,7 +866,7 @@ extern "C" kf_mpnn *kf_mpnn_create(GEOMETRY *g, const float *theta_host, int nsl
   KF_CHECK(hipMemcpy(&nsl_ent, c->d.sell_ptr + c->d.nslice, sizeof(uint32_t), hipMemcpyDeviceToHost));
   net->ev = mp_alloc<float4>((size_t)nsl_ent * 64u);
   const size_t NH = (size_t)N * MP_H;
-  net->X = mp_alloc<float>((size_t)N * MP_KIN);
+  net->X = mp_alloc<float>((size_t)N * MP_NIN);
   net->Z0 = mp_alloc<float>(NH);
   for (int l = 0; l <= MP_NR; l++) net->H[l] = mp_alloc<float>(NH);
   for (int l = 0; l < MP_NR; l++) {
@@ -675,17 +875,57 @@ extern "C" kf_mpnn *kf_mpnn_create(GEOMETRY *g, const float *theta_host, int nsl
     net->M[l] = mp_alloc<float>(NH);
     net->ZU[l] = mp_alloc<float>(NH);
   }
+  net->OP1 = mp_alloc<float>(NH);
+  net->OPQ = mp_alloc<float>(NH);
   net->dh = mp_alloc<float>(NH);
   net->dhn = mp_alloc<float>(NH);
   net->dm = mp_alloc<float>(NH);
   net->dP = mp_alloc<float>(NH);
   net->dQ = mp_alloc<float>(NH);
-  net->dout = mp_alloc<float>(4 * (size_t)N);
+  net->dOP1 = mp_alloc<float>(NH);
+  net->dOPQ = mp_alloc<float>(NH);
+  net->gd = mp_alloc<float>(4 * (size_t)N);
+  net->Ez = mp_alloc<float>(12 * (size_t)N);
   net->dstore = mp_alloc<float>((size_t)nslot * 16 * N);
   net->isd = mp_alloc<float>(nslot);
   net->red = mp_alloc<double>(KF_NRED);
   net->tred = mp_alloc<double>(KF_NRED);
   net->lpart = mp_alloc<double>(net->nlblk);
+  net->dl = mp_alloc<float>(4 * (size_t)N);
+  net->ebuf = mp_alloc<float>(4 * (size_t)N);
+  net->ybuf = mp_alloc<float>(4 * (size_t)N);
+  net->gbuf = mp_alloc<float>(4 * (size_t)N);
+  net->mslot = (float **)calloc((size_t)nslot, sizeof(float *));
+  if (!net->mslot) abort();
+  {
+    /* transposed-block positions: (i, j) at k  ->  (j, i) */
+    const size_t nnzb = c->nnzb;
+    uint32_t *rp = (uint32_t *)malloc(sizeof(uint32_t) * (N + 1)), *cl = (uint32_t *)malloc(sizeof(uint32_t) * nnzb);
+    uint32_t *tp = (uint32_t *)malloc(sizeof(uint32_t) * (nnzb ? nnzb : 1));
+    if (!rp || !cl || !tp) abort();
+    KF_CHECK(hipMemcpy(rp, c->d.rowptr, sizeof(uint32_t) * (N + 1), hipMemcpyDeviceToHost));
+    KF_CHECK(hipMemcpy(cl, c->d.col, sizeof(uint32_t) * nnzb, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < N; i++)
+      for (uint32_t k = rp[i]; k < rp[i + 1]; k++) {
+        const uint32_t j = cl[k];
+        uint32_t lo = rp[j], hi = rp[j + 1];
+        while (lo < hi) { /* columns of a row are sorted */
+          const uint32_t mid = (lo + hi) / 2;
+          if (cl[mid] < i) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo == rp[j + 1] || cl[lo] != i) {
+          fprintf(stderr, "kf: block pattern is not structurally symmetric\n");
+          abort();
+        }
+        tp[k] = lo;
+      }
+    net->tpos = mp_alloc<uint32_t>(nnzb ? nnzb : 1);
+    KF_CHECK(hipMemcpy(net->tpos, tp, sizeof(uint32_t) * nnzb, hipMemcpyHostToDevice));
+    free(rp);
+    free(cl);
+    free(tp);
+  }
   KF_CHECK(hipMemsetAsync(net->dstore, 0, sizeof(float) * (size_t)nslot * 16 * N, c->stream));
   float *ones = (float *)malloc(sizeof(float) * nslot);
   for (int i = 0; i < nslot; i++) ones[i] = 1.0f;
@@ -719,11 +959,16 @@ extern "C" void kf_mpnn_destroy(kf_mpnn *net)
 {
   if (!net) return;
   KF_CHECK(hipStreamSynchronize(net->c->stream));
-  void *bufs[] = {net->theta, net->am, net->av, net->grad, net->part, net->sc, net->gnorm, net->losses, net->wb,
-                  net->ev, net->X, net->Z0, net->dh, net->dhn, net->dm, net->dP, net->dQ, net->dout,
-                  net->dstore, net->isd, net->red, net->tred, net->lpart};
+  void *bufs[] = {net->theta, net->am,   net->av,   net->grad,  net->part, net->sc,   net->gnorm, net->losses,
+                  net->wb,    net->ev,   net->X,    net->Z0,    net->OP1,  net->OPQ,  net->dh,    net->dhn,
+                  net->dm,    net->dP,   net->dQ,   net->dOP1,  net->dOPQ, net->gd,   net->Ez,    net->dstore,
+                  net->isd,   net->red,  net->tred, net->lpart, net->dl,   net->ebuf, net->ybuf,  net->gbuf,
+                  net->tpos};
   for (void *p : bufs)
     if (p) KF_CHECK(hipFree(p));
+  for (uint32_t k = 0; k < net->nslot; k++)
+    if (net->mslot[k]) KF_CHECK(hipFree(net->mslot[k]));
+  free(net->mslot);
   for (int l = 0; l <= MP_NR; l++) KF_CHECK(hipFree(net->H[l]));
   for (int l = 0; l < MP_NR; l++) {
     KF_CHECK(hipFree(net->P[l]));
@@ -749,8 +994,36 @@ extern "C" void kf_mpnn_take_diag(kf_mpnn *net, int slot)
     hipLaunchKernelGGL(k_mp_take_diag<float>, dim3(nb), dim3(MP_RB), 0, st, N, c->d.diag, (const float *)c->a32, D);
   hipLaunchKernelGGL(k_mp_sumsq<float>, dim3(KF_NRED), dim3(MP_RB), 0, st, 16 * (size_t)N, (const float *)D,
                      net->red);
-  hipLaunchKernelGGL(k_mp_finish, dim3(1), dim3(64), 0, st, net->red, 16.0 * N, 1, net->isd + slot);
+  hipLaunchKernelGGL(k_mp_finish, dim3(1), dim3(64), 0, st, net->red, 16.0 * N, net->isd + slot);
   KF_LAUNCH_CHECK();
+  if (net->prepared == slot) net->prepared = -1;
+}
+
+/* the assembled Jacobian (after ComputeA, before ComputeNumericalILU
+ * factors it in place) into `slot`, as FP32 */
+extern "C" void kf_mpnn_take_matrix(kf_mpnn *net, int slot)
+{
+  mp_check_slot(net, slot);
+  kf_ctx *c = net->c;
+  const size_t n = 16 * c->nnzb;
+  if (!net->mslot[slot]) net->mslot[slot] = mp_alloc<float>(n);
+  if (c->opt.prec == KF_PREC_FP64)
+    hipLaunchKernelGGL(k_mp_to_f32<double>, dim3(kf_blocks_for(n, MP_RB)), dim3(MP_RB), 0, c->stream, n,
+                       (const double *)c->a64, net->mslot[slot]);
+  else
+    KF_CHECK(hipMemcpyAsync(net->mslot[slot], c->a32, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  KF_LAUNCH_CHECK();
+}
+
+/* 0: |e|^2 / |target|^2;  1: |A e|^2 / |v|^2 with the slot's A;
+ * 2 (default): |A (base + G base) - v|^2 / |v|^2 (the target is not read) */
+extern "C" void kf_mpnn_set_loss(kf_mpnn *net, int metric)
+{
+  if (metric < 0 || metric > 2) {
+    fprintf(stderr, "kf: kf_mpnn_set_loss: no loss %d\n", metric);
+    abort();
+  }
+  net->metric = metric;
 }
 
 /* diagonal blocks given by the caller (host FP64 [N][16]) into `slot` */
@@ -764,24 +1037,34 @@ extern "C" void kf_mpnn_set_diag(kf_mpnn *net, int slot, const double *D_host)
   float *D = net->dstore + (size_t)slot * n;
   KF_CHECK(hipMemcpyAsync(D, tmp, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_mp_sumsq<float>, dim3(KF_NRED), dim3(MP_RB), 0, c->stream, n, (const float *)D, net->red);
-  hipLaunchKernelGGL(k_mp_finish, dim3(1), dim3(64), 0, c->stream, net->red, (double)n, 1, net->isd + slot);
+  hipLaunchKernelGGL(k_mp_finish, dim3(1), dim3(64), 0, c->stream, net->red, (double)n, net->isd + slot);
   KF_LAUNCH_CHECK();
   KF_CHECK(hipStreamSynchronize(c->stream));
   free(tmp);
+  if (net->prepared == slot) net->prepared = -1;
 }
 
-/* out = net(v) (add = 0) or out += net(v) (add = 1); device vectors of 4N */
-extern "C" void kf_mpnn_forward(kf_mpnn *net, int slot, const double *v, double *out, int add)
+/* the correction operator of `slot` (the context network on its diagonal
+ * blocks); kf_mpnn_forward prepares on demand */
+extern "C" void kf_mpnn_prepare(kf_mpnn *net, int slot)
 {
   mp_check_slot(net, slot);
-  mp_forward(net, slot, v);
-  hipLaunchKernelGGL(k_mp_decode, dim3(kf_blocks_for(net->N, MP_RB)), dim3(MP_RB), 0, net->c->stream, net->N,
-                     net->H[MP_NR], net->theta, net->sc, out, add);
+  mp_context(net, slot);
+}
+
+/* out = G z (add = 0) or out += G z (add = 1), z = ILU^-1 v; device vectors
+ * of 4N */
+extern "C" void kf_mpnn_forward(kf_mpnn *net, int slot, const double *z, double *out, int add)
+{
+  mp_check_slot(net, slot);
+  if (net->prepared != slot) mp_context(net, slot);
+  hipLaunchKernelGGL(k_mp_apply, dim3(net->nlblk), dim3(MP_RB), 0, net->c->stream, net->c->d, net->ev, net->OP1,
+                     net->OPQ, net->theta, z, out, add, (float *)NULL);
   KF_LAUNCH_CHECK();
 }
 
-/* loss |base + net(v) - target|^2 / |target|^2 and its gradient (host FP32
- * copy when grad_host is given); device vectors of 4N.  Synchronises. */
+/* loss of (v, base, target) and its gradient (host FP32 copy when grad_host
+ * is given); device vectors of 4N.  Synchronises. */
 extern "C" double kf_mpnn_loss_grad(kf_mpnn *net, int slot, const double *v, const double *base, const double *target,
                                     float *grad_host)
 {
@@ -813,7 +1096,7 @@ extern "C" void kf_mpnn_adam(kf_mpnn *net, float lr, float beta1, float beta2, f
   KF_CHECK(hipStreamSynchronize(net->c->stream));
 }
 
-/* forward + backward + Adam (0.9, 0.999, 1e-8) on one pair, queued on the
+/* loss + gradient + Adam (0.9, 0.999, 1e-8) on one pair, queued on the
  * stream with no host round trip; the loss goes to the loss log */
 extern "C" void kf_mpnn_train_step(kf_mpnn *net, int slot, const double *v, const double *base, const double *target,
                                    float lr, float gclip)
@@ -838,9 +1121,9 @@ extern "C" int kf_mpnn_losses(kf_mpnn *net, float *host, int cap)
 
 extern "C" void kf_mpnn_reset_losses(kf_mpnn *net) { net->nloss = 0; }
 
-/* activations of the last forward, for tests: 0 x (N x 24), 1 z0, then per
- * round l: 2+5l h_l, 3+5l P, 4+5l Q, 5+5l m, 6+5l zu; 17 the final h
- * (N x 32 each) */
+/* activations of the last context pass, for tests: 0 x (N x 16), 1 z0, then
+ * per round l: 2+5l h_l, 3+5l P, 4+5l Q, 5+5l m, 6+5l zu; 17 the final h,
+ * 18 OP1, 19 OPQ (N x 32 each) */
 extern "C" void kf_mpnn_activation(kf_mpnn *net, int which, float *host)
 {
   KF_CHECK(hipStreamSynchronize(net->c->stream));
@@ -849,9 +1132,11 @@ extern "C" void kf_mpnn_activation(kf_mpnn *net, int which, float *host)
   size_t n = NH;
   if (which == 0) {
     src = net->X;
-    n = (size_t)net->N * MP_KIN;
+    n = (size_t)net->N * MP_NIN;
   } else if (which == 1) src = net->Z0;
   else if (which == 17) src = net->H[MP_NR];
+  else if (which == 18) src = net->OP1;
+  else if (which == 19) src = net->OPQ;
   else if (which >= 2 && which < 17) {
     const int l = (which - 2) / 5, k = (which - 2) % 5;
     const float *tab[5] = {net->H[l], net->P[l], net->Q[l], net->M[l], net->ZU[l]};

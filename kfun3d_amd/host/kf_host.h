@@ -98,7 +98,10 @@ enum {
   KFS_SLOT_H = 2,        /* Hessenberg column: slots 2 .. 2+restart+1 */
   KFS_SLOT_FNORM = 40,
   KFS_SLOT_QNORM = 41,
-  KFS_SLOT_BETA = 42
+  KFS_SLOT_BETA = 42,
+  KFS_SLOT_GA = 43,      /* learned correction guard: |ILU^-1 v|, |delta|, coefficient */
+  KFS_SLOT_GB = 44,
+  KFS_SLOT_GC = 45
 };
 void kfs_nrm2(GEOMETRY *g, size_t n, const double *x, int slot);
 void kfs_dot(GEOMETRY *g, size_t n, const double *x, const double *y, int slot);
@@ -120,6 +123,13 @@ void kfs_cgs_update(GEOMETRY *g, size_t n, const double *vbase, int nv, int slot
  * the two calls) */
 void kfs_flux_spectral(GEOMETRY *g, const double *q, GRADIENT *grad, double *r);
 void kfs_timestep_spectral(GEOMETRY *g, double cfl);
+/* x[i] ~ N(0, 1) from a counter-based generator keyed by (seed, stream, i) */
+void kfs_gaussian(GEOMETRY *g, size_t n, uint64_t seed, uint64_t stream, double *x);
+/* slot_out = min(1, gamma slot_a / slot_b) */
+void kfs_guard(GEOMETRY *g, int slot_a, int slot_b, double gamma, int slot_out);
+/* the driver's learned-preconditioner state hangs off the device context */
+void **kfs_learn_state(GEOMETRY *g);
+void kfs_learn_detach(GEOMETRY *g);   /* solver.c: frees that state */
 /* Kernel() work vectors: grad x/y/z, r, w, t, x, and nbasis Krylov vectors */
 void kfs_work(GEOMETRY *g, int nbasis, double **grad3, double **r, double **w, double **t, double **x, double **vecs);
 

@@ -14,11 +14,16 @@
  *
  * The only host/device round trips per Arnoldi step are the restart+2
  * Hessenberg scalars (one copy) needed by the Givens recurrence.
+ *
+ * With a learned correction attached (kf_attach_mpnn), flexible GMRES runs
+ * with M(v) = ILU^-1 v + a delta(v); the same Krylov routine solves the
+ * tight training systems of the collecting steps.
  */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "kf_host.h"
 
@@ -43,25 +48,352 @@ static double fetch1(GEOMETRY *g, int slot)
   return v;
 }
 
+static double now_ms(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
+
+/* ---- learned correction state ------------------------------------------------ */
+
+typedef struct {
+  kf_mpnn *net;
+  kf_learn_options lo;
+  kf_learn_stats st;
+  int cap;               /* pairs the store holds */
+  double *pv, *pb, *px;  /* per pair: v / |v|, ILU^-1 of it, tight solution */
+  int *pslot;            /* diagonal slot of each pair */
+  int enabled;           /* network in M(v) for the Newton solves */
+  int since_judge;
+  int last_ilu;          /* iterations of the last judged ILU(0) solve */
+} learn_state;
+
+kf_learn_options kf_default_learn_options(void)
+{
+  kf_learn_options lo;
+  memset(&lo, 0, sizeof(lo));
+  lo.collect_steps = 10;
+  lo.rhs_per_step = 64;
+  lo.pair_rtol = 1e-10;
+  lo.pair_maxit = 300;
+  lo.train_iters = 2000;
+  lo.lr = 1e-3;
+  lo.gclip = 1.0;
+  lo.guard = 1.0;
+  lo.judge_every = 5;
+  lo.holdout = 0.1;
+  lo.seed = 3;
+  return lo;
+}
+
+void kfs_learn_detach(GEOMETRY *g)
+{
+  void **slot = kfs_learn_state(g);
+  learn_state *L = (learn_state *)*slot;
+  if (!L) return;
+  kf_synchronize(g);
+  if (L->pv) kf_device_free(L->pv);
+  if (L->pb) kf_device_free(L->pb);
+  if (L->px) kf_device_free(L->px);
+  free(L->pslot);
+  free(L);
+  *slot = NULL;
+}
+
+void kf_attach_mpnn(GEOMETRY *g, kf_mpnn *net, const kf_learn_options *lo)
+{
+  kfs_learn_detach(g);
+  if (!net) return;
+  learn_state *L = (learn_state *)calloc(1, sizeof(learn_state));
+  if (!L) abort();
+  L->net = net;
+  L->lo = lo ? *lo : kf_default_learn_options();
+  if (L->lo.collect_steps < 0 || L->lo.rhs_per_step < 0 || L->lo.train_iters < 0 || L->lo.holdout < 0.0 ||
+      L->lo.holdout >= 1.0) {
+    fprintf(stderr, "kf: invalid learning options\n");
+    abort();
+  }
+  L->cap = L->lo.collect_steps * (1 + L->lo.rhs_per_step);
+  L->enabled = L->lo.collect_steps == 0; /* a pre-trained network is used at once */
+  *kfs_learn_state(g) = L;
+}
+
+void kf_learn_get_stats(const GEOMETRY *g, kf_learn_stats *st)
+{
+  learn_state *L = (learn_state *)*kfs_learn_state((GEOMETRY *)g);
+  if (L) *st = L->st;
+  else memset(st, 0, sizeof(*st));
+}
+
+/* ---- the Krylov solve ---------------------------------------------------------- */
+
+typedef struct {
+  GEOMETRY *g;
+  kf_options o;
+  size_t sz;
+  int restart, nb, flexible;
+  GRADIENT grad;
+  double *r, *w, *t, *vecs, *ghh;
+  double qnorm;
+  learn_state *L;
+} kspace;
+
+#define VEC(k) (K->vecs + (size_t)(k) * K->sz)
+#define ZVEC(k) (K->vecs + (size_t)(K->nb + (k)) * K->sz) /* FGMRES: z_k = M(v_k) */
+
+/* z = ILU^-1 v on entry; z += a G z, a = min(1, guard |z| / |G z|); w is
+ * scratch */
+static void add_correction(kspace *K, double *z)
+{
+  GEOMETRY *g = K->g;
+  kf_mpnn_forward(K->L->net, 0, z, K->w, 0);
+  kfs_nrm2(g, K->sz, z, KFS_SLOT_GA);
+  kfs_nrm2(g, K->sz, K->w, KFS_SLOT_GB);
+  kfs_guard(g, KFS_SLOT_GA, KFS_SLOT_GB, K->L->lo.guard, KFS_SLOT_GC);
+  kfs_axpy_slot(g, K->sz, KFS_SLOT_GC, K->w, z);
+}
+
+/* Solves (J + diag(cdt)) x = b at the current state from x = 0 (x is
+ * zeroed here), restarting every `restart` steps, until the residual
+ * estimate drops below rtol times the first cycle's |M b| (left GMRES) or
+ * |b| (FGMRES) or more than maxit iterations ran; returns the iterations.
+ * src/kernel.c:102-200 with b = r.  cap > 0 also stops the solve after cap
+ * iterations (mid-cycle); *conv (if given) tells whether the tolerance was
+ * met. */
+static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit, int use_net, int cap, int *conv)
+{
+  GEOMETRY *g = K->g;
+  const kf_options *o = &K->o;
+  const size_t sz = K->sz;
+  const int restart = K->restart, nb = K->nb, flexible = K->flexible;
+  double *ghh = K->ghh, *w = K->w, *t = K->t;
+  double cc[64], ss[64], rs[64], hcol[64];
+  kfs_zero(g, sz, x);
+  if (!flexible) ComputeSparseTriangularSolve(g, b, VEC(0));
+  else kfs_newaxpy(g, sz, 0.0, b, b, VEC(0)); /* v = b (x = 0) */
+  double tol = 0.0;
+  int total = 0, flag = 0;
+  while (total <= maxit) {
+    if (total > 0) {
+      iJacVec(g, x, K->r, K->qnorm, &K->grad, w, t);
+      if (!flexible) {
+        kfs_newaxpy(g, sz, -1.0, t, b, w);
+        ComputeSparseTriangularSolve(g, w, VEC(0));
+      } else {
+        kfs_newaxpy(g, sz, -1.0, t, b, VEC(0)); /* v = b - A x */
+      }
+    }
+    kfs_nrm2(g, sz, VEC(0), KFS_SLOT_BETA);
+    kfs_scale_recip(g, sz, VEC(0), KFS_SLOT_BETA);
+    rs[0] = fetch1(g, KFS_SLOT_BETA);
+    tol = (total == 0) ? rtol * rs[0] : tol;
+    int it = 0;
+    while (it < restart) {
+      if (!flexible) {
+        iJacVec(g, VEC(it), K->r, K->qnorm, &K->grad, w, t);
+        ComputeSparseTriangularSolve(g, t, VEC(it + 1));
+      } else {
+        ComputeSparseTriangularSolve(g, VEC(it), ZVEC(it)); /* z = M(v) */
+        if (use_net) add_correction(K, ZVEC(it));
+        iJacVec(g, ZVEC(it), K->r, K->qnorm, &K->grad, w, VEC(it + 1));
+      }
+      double *hh = ghh + (size_t)it * (nb + 1);
+      if (o->ortho == KF_ORTHO_CGS2) {
+        /* two passes of batched classical Gram-Schmidt */
+        kfs_cgs_dots(g, sz, VEC(0), it + 1, VEC(it + 1), KFS_SLOT_H);
+        kfs_cgs_update(g, sz, VEC(0), it + 1, KFS_SLOT_H, VEC(it + 1));
+        kfs_fetch(g, KFS_SLOT_H, it + 1, hcol);
+        kfs_cgs_dots(g, sz, VEC(0), it + 1, VEC(it + 1), KFS_SLOT_H);
+        kfs_cgs_update(g, sz, VEC(0), it + 1, KFS_SLOT_H, VEC(it + 1));
+        kfs_nrm2(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
+        kfs_scale_recip(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
+        double h2[64];
+        kfs_fetch(g, KFS_SLOT_H, it + 2, h2);
+        for (int v = 0; v <= it; v++) hh[v] = hcol[v] + h2[v];
+        hh[it + 1] = h2[it + 1];
+      } else {
+        /* modified Gram-Schmidt, src/kernel.c:129-141 */
+        kfs_mgs(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H);
+        kfs_scale_recip(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
+        kfs_fetch(g, KFS_SLOT_H, it + 2, hh);
+      }
+      /* Givens, src/kernel.c:143-173 */
+      for (int i = 0; i < it; i++) {
+        const double tt = *hh;
+        *hh = cc[i] * *hh + ss[i] * *(hh + 1);
+        hh++;
+        *hh = cc[i] * *hh - ss[i] * tt;
+      }
+      const double tt = sqrt(*hh * *hh + *(hh + 1) * *(hh + 1));
+      const double ct = *hh / tt, st = *(hh + 1) / tt;
+      cc[it] = ct;
+      ss[it] = st;
+      rs[it + 1] = -(st * rs[it]);
+      rs[it] = ct * rs[it];
+      *hh = ct * *hh + st * *(hh + 1);
+      const double res = fabs(rs[it + 1]);
+      it++;
+      if (res <= tol) {
+        flag = 1;
+        break;
+      }
+      if (cap > 0 && total + it >= cap) break;
+    }
+    total += it;
+    /* back substitution, src/kernel.c:178-188 */
+    rs[it - 1] /= ghh[(size_t)(it - 1) * (nb + 1) + (it - 1)];
+    for (int i = 1; i <= it - 1; i++) {
+      const int k = (it - 1) - i;
+      double tt = rs[k];
+      for (int j = k + 1; j <= it - 1; j++) tt -= ghh[(size_t)j * (nb + 1) + k] * rs[j];
+      rs[k] = tt / ghh[(size_t)k * (nb + 1) + k];
+    }
+    for (int v = 0; v < it; v++) kfs_axpy(g, sz, rs[v], flexible ? ZVEC(v) : VEC(v), x);
+    if (flag || (cap > 0 && total >= cap)) break;
+  }
+  if (conv) *conv = flag;
+  return total;
+}
+
+/* ---- training pairs and training ---------------------------------------------------- */
+
+/* pairs of pseudo-time step `its` (factor and state of this step): the
+ * Newton residual r and rhs_per_step Gaussian vectors, each normalised,
+ * preconditioned by ILU(0) and solved tightly with ILU(0)-FGMRES */
+static void collect_pairs(kspace *K, int its)
+{
+  learn_state *L = K->L;
+  GEOMETRY *g = K->g;
+  const size_t sz = K->sz;
+  const double t0 = now_ms();
+  if (!L->pv) {
+    L->pv = (double *)kf_device_alloc((size_t)L->cap * sz * sizeof(double));
+    L->pb = (double *)kf_device_alloc((size_t)L->cap * sz * sizeof(double));
+    L->px = (double *)kf_device_alloc((size_t)L->cap * sz * sizeof(double));
+    L->pslot = (int *)calloc((size_t)L->cap, sizeof(int));
+    if (!L->pslot) abort();
+  }
+  const int slot = 1 + its;
+  kf_mpnn_take_diag(L->net, slot);
+  for (int k = 0; k <= L->lo.rhs_per_step; k++) {
+    const int p = L->st.npairs;
+    if (p >= L->cap) abort();
+    double *v = L->pv + (size_t)p * sz, *base = L->pb + (size_t)p * sz, *x = L->px + (size_t)p * sz;
+    if (k == 0) kfs_newaxpy(g, sz, 0.0, K->r, K->r, v);
+    else kfs_gaussian(g, sz, L->lo.seed, (uint64_t)its * 100003u + (uint64_t)k, v);
+    kfs_nrm2(g, sz, v, KFS_SLOT_API);
+    kfs_scale_recip(g, sz, v, KFS_SLOT_API);
+    ComputeSparseTriangularSolve(g, v, base);
+    L->st.pair_iters += ksolve(K, v, x, L->lo.pair_rtol, L->lo.pair_maxit, 0, 0, NULL);
+    L->pslot[p] = slot;
+    L->st.npairs++;
+  }
+  kf_synchronize(g);
+  L->st.pair_ms += now_ms() - t0;
+}
+
+static uint64_t mix64(uint64_t *s)
+{
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* mean loss over the held-out pairs (the last nheld collected) */
+static double heldout_loss(learn_state *L, size_t sz)
+{
+  if (L->st.nheld == 0) return 0.0;
+  double s = 0.0;
+  for (int p = L->st.npairs - L->st.nheld; p < L->st.npairs; p++)
+    s += kf_mpnn_loss_grad(L->net, L->pslot[p], L->pv + (size_t)p * sz, L->pb + (size_t)p * sz,
+                           L->px + (size_t)p * sz, NULL);
+  return s / L->st.nheld;
+}
+
+/* train_iters Adam steps over the training pairs in shuffled epochs */
+static void train(kspace *K)
+{
+  learn_state *L = K->L;
+  const size_t sz = K->sz;
+  const double t0 = now_ms();
+  const int n = L->st.npairs;
+  int nheld = (int)floor(L->lo.holdout * n);
+  if (nheld >= n) nheld = n - 1;
+  if (nheld < 0) nheld = 0;
+  L->st.nheld = nheld;
+  const int ntrain = n - nheld;
+  /* the loss of ILU(0) alone: the decoder switched off */
+  const size_t np = kf_mpnn_nparam();
+  float *theta = (float *)malloc(np * sizeof(float)), *th0 = (float *)malloc(np * sizeof(float));
+  if (!theta || !th0) abort();
+  kf_mpnn_get_params(L->net, theta);
+  memcpy(th0, theta, np * sizeof(float));
+  for (size_t i = np - kf_mpnn_nhead(); i < np; i++) th0[i] = 0.0f; /* the operator heads: G = 0 */
+  kf_mpnn_set_params(L->net, th0);
+  L->st.loss_ilu = heldout_loss(L, sz);
+  kf_mpnn_set_params(L->net, theta);
+  L->st.loss_before = heldout_loss(L, sz);
+  free(th0);
+  free(theta);
+  int *perm = (int *)malloc((size_t)(ntrain > 0 ? ntrain : 1) * sizeof(int));
+  if (!perm) abort();
+  uint64_t rs = L->lo.seed ^ 0x5851F42D4C957F2Dull;
+  int pos = ntrain;
+  for (int it = 0; it < L->lo.train_iters && ntrain > 0; it++) {
+    if (pos == ntrain) { /* a new epoch: Fisher-Yates */
+      for (int i = 0; i < ntrain; i++) perm[i] = i;
+      for (int i = ntrain - 1; i > 0; i--) {
+        const int j = (int)(mix64(&rs) % (uint64_t)(i + 1));
+        const int tmp = perm[i];
+        perm[i] = perm[j];
+        perm[j] = tmp;
+      }
+      pos = 0;
+    }
+    const int p = perm[pos++];
+    kf_mpnn_train_step(L->net, L->pslot[p], L->pv + (size_t)p * sz, L->pb + (size_t)p * sz, L->px + (size_t)p * sz,
+                       (float)L->lo.lr, (float)L->lo.gclip);
+  }
+  free(perm);
+  L->st.loss_after = heldout_loss(L, sz);
+  L->st.train_ms += now_ms() - t0;
+}
+
+/* ---- the driver ------------------------------------------------------------------ */
+
 int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
 {
   const kf_options o = kf_get_options(g);
-  const size_t sz = g->c->sz;
-  const int restart = o.gmres_restart, nb = restart + 1;
-  const int flexible = o.krylov == KF_KRYLOV_FGMRES;
-  double *gxyz[3], *r, *w, *t, *x, *vecs;
-  kfs_work(g, flexible ? nb + restart : nb, gxyz, &r, &w, &t, &x, &vecs);
-  GRADIENT grad = {gxyz[0], gxyz[1], gxyz[2]};
-#define VEC(k) (vecs + (size_t)(k) * sz)
-#define ZVEC(k) (vecs + (size_t)(nb + (k)) * sz)   /* FGMRES: z_k = M(v_k) */
-  double *ghh = (double *)calloc((size_t)nb * nb + nb, sizeof(double));
-  double cc[64], ss[64], rs[64], hcol[64];
+  learn_state *L = (learn_state *)*kfs_learn_state(g);
+  kspace KS, *K = &KS;
+  memset(K, 0, sizeof(KS));
+  K->g = g;
+  K->o = o;
+  K->sz = g->c->sz;
+  K->restart = o.gmres_restart;
+  K->nb = o.gmres_restart + 1;
+  K->flexible = o.krylov == KF_KRYLOV_FGMRES || L != NULL;
+  K->L = L;
+  const size_t sz = K->sz;
+  double *gxyz[3], *x;
+  kfs_work(g, K->flexible ? K->nb + K->restart : K->nb, gxyz, &K->r, &K->w, &K->t, &x, &K->vecs);
+  K->grad.x0 = gxyz[0];
+  K->grad.x1 = gxyz[1];
+  K->grad.x2 = gxyz[2];
+  K->ghh = (double *)calloc((size_t)K->nb * K->nb + K->nb, sizeof(double));
+  double *x2 = NULL; /* judged steps: the second solution */
+  if (L) {
+    if (L->lo.judge_every < 1) L->lo.judge_every = 1;
+    x2 = (double *)kf_device_alloc(sz * sizeof(double));
+  }
   double fnorm_init = 0.0, cfl = o.cfl_init;
   int its = 0, nrec = 0;
   while (its < o.ts_max) {
     /* the residual also leaves the time-step sums of this state */
-    kfs_flux_spectral(g, g->q->q, &grad, r);
-    kfs_nrm2(g, sz, r, KFS_SLOT_FNORM);
+    kfs_flux_spectral(g, g->q->q, &K->grad, K->r);
+    kfs_nrm2(g, sz, K->r, KFS_SLOT_FNORM);
     const double fnorm = fetch1(g, KFS_SLOT_FNORM);
     if (!isfinite(fnorm)) {
       fprintf(stderr, "kf: non-finite residual norm at pseudo-step %d\n", its);
@@ -73,91 +405,65 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
       cfl0 /= fnorm;
       cfl = (cfl0 < o.cfl_max) ? cfl0 : o.cfl_max;
     }
-    kfs_timestep_spectral(g, cfl);   /* = ComputeTimeStep(cfl, g) on the same state */
+    kfs_timestep_spectral(g, cfl); /* = ComputeTimeStep(cfl, g) on the same state */
     ComputeA(g);
+    if (L && its < L->lo.collect_steps) kf_mpnn_take_matrix(L->net, 1 + its); /* the loss metric */
     kfs_nrm2(g, sz, g->q->q, KFS_SLOT_QNORM);
-    const double qnorm = sqrt(1.0 + fetch1(g, KFS_SLOT_QNORM));
-    kfs_zero(g, sz, x);
+    K->qnorm = sqrt(1.0 + fetch1(g, KFS_SLOT_QNORM));
     ComputeNumericalILU(g);
-    if (!flexible) ComputeSparseTriangularSolve(g, r, VEC(0));
-    else kfs_newaxpy(g, sz, 0.0, r, r, VEC(0));   /* v = r (x = 0) */
-    double tol = 0.0;
-    int total = 0, flag = 0;
-    while (total <= o.gmres_max_it) {
-      if (total > 0) {
-        iJacVec(g, x, r, qnorm, &grad, w, t);
-        if (!flexible) {
-          kfs_newaxpy(g, sz, -1.0, t, r, w);
-          ComputeSparseTriangularSolve(g, w, VEC(0));
+    int total, learned = 0, k_ilu = -1, k_net = -1;
+    if (L && L->enabled) {
+      kf_mpnn_take_diag(L->net, 0);
+      kf_mpnn_prepare(L->net, 0); /* G of this factorisation */
+      if (L->lo.ablate || L->since_judge % L->lo.judge_every == 0) {
+        /* judged: ILU(0) alone, then with the correction on the same system,
+         * stopped once it has used as many iterations (it has lost then) */
+        k_ilu = ksolve(K, K->r, x2, o.gmres_rtol, o.gmres_max_it, 0, 0, NULL);
+        int conv = 0;
+        k_net = ksolve(K, K->r, x, o.gmres_rtol, o.gmres_max_it, 1, L->lo.ablate ? 0 : k_ilu, &conv);
+        L->st.steps_judged++;
+        L->last_ilu = k_ilu;
+        const int won = conv && k_net < k_ilu;
+        if (won) L->st.judged_won++;
+        if (won && !L->lo.ablate) {
+          learned = 1;
+          total = k_net;
         } else {
-          kfs_newaxpy(g, sz, -1.0, t, r, VEC(0));   /* v = r - A x */
+          kfs_newaxpy(g, sz, 0.0, x2, x2, x); /* x = x2: keep ILU(0)'s update */
+          total = k_ilu;
+          if (!L->lo.ablate) {
+            L->enabled = 0;
+            L->st.fallback = 1;
+          }
+        }
+      } else {
+        /* between judged steps: the correction alone, redone with ILU(0)
+         * alone (which then takes over) if it needs more than twice the
+         * iterations of the last judged ILU(0) solve */
+        int conv = 0;
+        k_net = ksolve(K, K->r, x, o.gmres_rtol, o.gmres_max_it, 1, 2 * L->last_ilu, &conv);
+        if (conv) {
+          total = k_net;
+          learned = 1;
+        } else {
+          total = k_ilu = ksolve(K, K->r, x, o.gmres_rtol, o.gmres_max_it, 0, 0, NULL);
+          L->enabled = 0;
+          L->st.fallback = 1;
         }
       }
-      kfs_nrm2(g, sz, VEC(0), KFS_SLOT_BETA);
-      kfs_scale_recip(g, sz, VEC(0), KFS_SLOT_BETA);
-      rs[0] = fetch1(g, KFS_SLOT_BETA);
-      tol = (total == 0) ? o.gmres_rtol * rs[0] : tol;
-      int it = 0;
-      while (it < restart) {
-        if (!flexible) {
-          iJacVec(g, VEC(it), r, qnorm, &grad, w, t);
-          ComputeSparseTriangularSolve(g, t, VEC(it + 1));
-        } else {
-          ComputeSparseTriangularSolve(g, VEC(it), ZVEC(it));   /* z = M(v) */
-          iJacVec(g, ZVEC(it), r, qnorm, &grad, w, VEC(it + 1));
-        }
-        double *hh = ghh + (size_t)it * (nb + 1);
-        if (o.ortho == KF_ORTHO_CGS2) {
-          /* two passes of batched classical Gram-Schmidt */
-          kfs_cgs_dots(g, sz, VEC(0), it + 1, VEC(it + 1), KFS_SLOT_H);
-          kfs_cgs_update(g, sz, VEC(0), it + 1, KFS_SLOT_H, VEC(it + 1));
-          kfs_fetch(g, KFS_SLOT_H, it + 1, hcol);
-          kfs_cgs_dots(g, sz, VEC(0), it + 1, VEC(it + 1), KFS_SLOT_H);
-          kfs_cgs_update(g, sz, VEC(0), it + 1, KFS_SLOT_H, VEC(it + 1));
-          kfs_nrm2(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
-          kfs_scale_recip(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
-          double h2[64];
-          kfs_fetch(g, KFS_SLOT_H, it + 2, h2);
-          for (int v = 0; v <= it; v++) hh[v] = hcol[v] + h2[v];
-          hh[it + 1] = h2[it + 1];
-        } else {
-          /* modified Gram-Schmidt, src/kernel.c:129-141 */
-          kfs_mgs(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H);
-          kfs_scale_recip(g, sz, VEC(it + 1), KFS_SLOT_H + it + 1);
-          kfs_fetch(g, KFS_SLOT_H, it + 2, hh);
-        }
-        /* Givens, src/kernel.c:143-173 */
-        for (int i = 0; i < it; i++) {
-          const double tt = *hh;
-          *hh = cc[i] * *hh + ss[i] * *(hh + 1);
-          hh++;
-          *hh = cc[i] * *hh - ss[i] * tt;
-        }
-        const double tt = sqrt(*hh * *hh + *(hh + 1) * *(hh + 1));
-        const double ct = *hh / tt, st = *(hh + 1) / tt;
-        cc[it] = ct;
-        ss[it] = st;
-        rs[it + 1] = -(st * rs[it]);
-        rs[it] = ct * rs[it];
-        *hh = ct * *hh + st * *(hh + 1);
-        const double res = fabs(rs[it + 1]);
-        it++;
-        if (res <= tol) {
-          flag = 1;
-          break;
-        }
+      L->since_judge++;
+      L->st.steps_learned += learned;
+    } else {
+      total = ksolve(K, K->r, x, o.gmres_rtol, o.gmres_max_it, 0, 0, NULL);
+      if (L) k_ilu = total;
+    }
+    if (L && its < L->lo.collect_steps) {
+      collect_pairs(K, its);
+      if (its == L->lo.collect_steps - 1) {
+        train(K);
+        L->enabled = 1;
+        L->since_judge = 0;
       }
-      total += it;
-      /* back substitution, src/kernel.c:178-188 */
-      rs[it - 1] /= ghh[(size_t)(it - 1) * (nb + 1) + (it - 1)];
-      for (int i = 1; i <= it - 1; i++) {
-        const int k = (it - 1) - i;
-        double tt = rs[k];
-        for (int j = k + 1; j <= it - 1; j++) tt -= ghh[(size_t)j * (nb + 1) + k] * rs[j];
-        rs[k] = tt / ghh[(size_t)k * (nb + 1) + k];
-      }
-      for (int v = 0; v < it; v++) kfs_axpy(g, sz, rs[v], flexible ? ZVEC(v) : VEC(v), x);
-      if (flag) break;
     }
     kfs_axpy(g, sz, -1.0, x, g->q->q);
     double forces[3];
@@ -174,16 +480,21 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
       rec[nrec].fnorm = fnorm;
       rec[nrec].krylov = total;
       memcpy(rec[nrec].forces, forces, sizeof(forces));
+      rec[nrec].learned = learned;
+      rec[nrec].krylov_ilu = k_ilu;
+      rec[nrec].krylov_net = k_net;
     }
     nrec++;
     /* src/kernel.c:214 rounds the ratio; compared in double (no uint32 wrap) */
     if (round(fnorm_init / fnorm) >= o.fnorm_ratio) break;
     its++;
   }
-#undef VEC
-#undef ZVEC
-  free(ghh);
+  if (x2) kf_device_free(x2);
+  free(K->ghh);
   return nrec;
 }
+
+#undef VEC
+#undef ZVEC
 
 void Kernel(GEOMETRY *g) { kf_kernel(g, NULL, 0); }
