@@ -16,7 +16,13 @@ for a bounded sample -- per-call times of ComputeFlux, ComputeA,
 ComputeNumericalILU, ComputeSparseTriangularSolve and one BLAS-1 pass on the
 same mesh -- and extrapolates the solve with the GPU run's call counts.
 
+mixed_precision repeats the solve with the FP32 and BF16-factor
+preconditioners; learned_preconditioner runs flexible GMRES with ILU(0)
+alone and with the learned correction trained online on the same mesh
+(collection and training timed separately).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--prec fp64|fp32|bf16]
+                    [--no-cpu] [--no-mixed] [--no-learned] [--learn-rhs R] [--learn-iters T]
 """
 import argparse
 import json
@@ -138,6 +144,60 @@ def mixed_precision(K, m64, mesh, hist64):
     return out
 
 
+def learned(K, mesh, hist64, rhs_per_step, train_iters):
+    """The learned correction on the same mesh: flexible GMRES with ILU(0)
+    alone (the method without the network), then with the network trained
+    online -- pairs from the first 10 pseudo-time steps (the Newton
+    residual and rhs_per_step Gaussian right-hand sides, each solved to
+    1e-10 or 300 iterations), Adam with learning rate 1e-3 for train_iters
+    steps -- and used from step 11 on with judged steps and the ILU(0)
+    fallback.  Wall-clock with and without the collection and training."""
+    import numpy as np
+    f64 = np.array(hist64[-1]["forces"])
+    m = K.Mesh(mesh, K.config_options(krylov="fgmres"))
+    q0 = m.state()
+    m.kernel()
+    m.set_state(q0)
+    m.synchronize()
+    t0 = time.perf_counter()
+    hf = m.kernel()
+    m.synchronize()
+    t_ilu = time.perf_counter() - t0
+    m.set_state(q0)
+    net = K.Network(m, K.init_params(seed=0), nslot=11)
+    m.attach(net, collect_steps=10, rhs_per_step=rhs_per_step, pair_rtol=1e-10, pair_maxit=300,
+             train_iters=train_iters, lr=1e-3, judge_every=5, holdout=0.1, seed=3)
+    m.synchronize()
+    t0 = time.perf_counter()
+    hl = m.kernel()
+    m.synchronize()
+    t_l = time.perf_counter() - t0
+    st = m.learn_stats()
+    assert m.last_error() == 0, "device error word set (learned)"
+    m.attach(None)
+    net.close()
+    m.close()
+    fl = np.array(hl[-1]["forces"])
+    after = [x for x in hl if x["step"] >= 10]
+    return {
+        "fgmres_ilu": {"seconds": t_ilu, "pseudo_steps": len(hf), "krylov_total": sum(x["krylov"] for x in hf)},
+        "learned": {"seconds_total": t_l,
+                    "seconds_without_collection_and_training": t_l - (st["pair_ms"] + st["train_ms"]) / 1e3,
+                    "pseudo_steps": len(hl), "krylov_total": sum(x["krylov"] for x in hl),
+                    "krylov_after_training": sum(x["krylov"] for x in after),
+                    "krylov_ilu_same_steps": sum(x["krylov"] for x in hf if x["step"] >= 10),
+                    "residual_drop": hl[0]["fnorm"] / hl[-1]["fnorm"],
+                    "force_rel_err_vs_fp64": float((np.abs(fl - f64) / np.abs(f64)).max()),
+                    "judged_steps": [[x["step"], x["krylov_ilu"], x["krylov_net"]] for x in hl
+                                     if x["krylov_ilu"] >= 0 and x["krylov_net"] >= 0],
+                    "steps_with_network": st["steps_learned"], "fallback": bool(st["fallback"]),
+                    "pairs": st["npairs"], "rhs_per_step": rhs_per_step, "tight_solve_iterations": st["pair_iters"],
+                    "collection_s": st["pair_ms"] / 1e3, "training_s": st["train_ms"] / 1e3,
+                    "adam_steps": train_iters,
+                    "heldout_residual_loss": {"ilu": st["loss_ilu"], "trained": st["loss_after"]}},
+    }
+
+
 COPY_BYTES = 1 << 28
 
 
@@ -191,6 +251,10 @@ def main():
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32/BF16-preconditioner comparison")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
+    ap.add_argument("--learn-rhs", type=int, default=8,
+                    help="Gaussian right-hand sides per collecting step (the full recipe: 64)")
+    ap.add_argument("--learn-iters", type=int, default=2000, help="Adam steps")
     ap.add_argument("--kernel", default="solve", choices=["solve", "trsv", "copy"],
                     help="solve: the benchmark; trsv / copy: short runs for hardware-counter passes (the "
                          "triangular solves on the same mesh; a device-to-device copy of known size)")
@@ -250,6 +314,8 @@ def main():
     host_threads = min(16, len(os.sched_getaffinity(0)))
     cpu = None if args.no_cpu else cpu_baseline(mesh, LATTICE, counts, host_threads)
     mixed = None if (args.no_mixed or args.prec != "fp64") else mixed_precision(K, m, mesh, hist)
+    learn = None if (args.no_learned or args.prec != "fp64") else learned(K, mesh, hist, args.learn_rhs,
+                                                                           args.learn_iters)
 
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -270,6 +336,7 @@ def main():
                      "traffic_source": "profiles/traffic_c2_fp64.md (FETCH_SIZE/WRITE_SIZE passes, copy-calibrated)"},
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "learned_preconditioner": learn,
     }
     if mixed:
         for v in mixed.values():

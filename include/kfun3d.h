@@ -257,6 +257,9 @@ int  kf_profile_get(GEOMETRY *g, const char *name, double *total_ms, long *count
 typedef struct kf_mpnn kf_mpnn;
 size_t   kf_mpnn_nparam(void);
 size_t   kf_mpnn_nhead(void);
+/* initial parameters (host array of kf_mpnn_nparam()): scaled normals for
+ * the network, zero operator heads (G = 0: M = ILU(0)) */
+void     kf_mpnn_init_params(float *theta_host, uint64_t seed);
 kf_mpnn *kf_mpnn_create(GEOMETRY *g, const float *theta_host, int nslot);
 void     kf_mpnn_destroy(kf_mpnn *net);
 void     kf_mpnn_set_params(kf_mpnn *net, const float *theta_host);  /* also clears the Adam moments */

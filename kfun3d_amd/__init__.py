@@ -160,6 +160,7 @@ def _load():
         "Normalize": (C.c_double, [C.c_size_t, DP]),
         "kf_mpnn_nparam": (C.c_size_t, []),
         "kf_mpnn_nhead": (C.c_size_t, []),
+        "kf_mpnn_init_params": (None, [C.c_void_p, C.c_uint64]),
         "kf_mpnn_prepare": (None, [C.c_void_p, C.c_int]),
         "kf_mpnn_create": (C.c_void_p, [GP, C.c_void_p, C.c_int]),
         "kf_mpnn_destroy": (None, [C.c_void_p]),
@@ -205,7 +206,7 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
                "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
-               "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
+               "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_init_params", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
                "kf_mpnn_get_params", "kf_mpnn_take_diag", "kf_mpnn_set_diag", "kf_mpnn_forward",
                "kf_mpnn_loss_grad", "kf_mpnn_adam", "kf_mpnn_train_step", "kf_mpnn_losses", "kf_mpnn_reset_losses",
                "kf_mpnn_activation", "kf_mpnn_take_matrix", "kf_mpnn_set_loss",
@@ -467,6 +468,13 @@ class Mesh:
 
     def synchronize(self):
         lib().kf_synchronize(self.ref)
+
+
+def init_params(seed=0):
+    """kf_mpnn_init_params: the network's initial parameters (float32)"""
+    th = np.empty(int(lib().kf_mpnn_nparam()), dtype=np.float32)
+    lib().kf_mpnn_init_params(th.ctypes.data, seed)
+    return th
 
 
 class Network:

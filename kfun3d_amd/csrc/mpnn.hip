@@ -825,6 +825,42 @@ static void mp_loss_grad(kf_mpnn *net, int slot, const double *v, const double *
   KF_LAUNCH_CHECK();
 }
 
+/* Initial parameters: normals scaled by sqrt(2 / fan-in) for W0, A, B, C,
+ * U and V (U and V halved), zero biases, zero operator heads (G = 0, so
+ * training starts from ILU(0) itself).  Normals from splitmix64 and the
+ * Box-Muller cosine branch. */
+static uint64_t mp_mix(uint64_t *s)
+{
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static double mp_normal(uint64_t *s)
+{
+  const double u1 = ((double)(mp_mix(s) >> 11) + 1.0) * 0x1.0p-53, u2 = (double)(mp_mix(s) >> 11) * 0x1.0p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+extern "C" void kf_mpnn_init_params(float *theta, uint64_t seed)
+{
+  uint64_t st = seed * 0xD1B54A32D192ED03ull + 1;
+  memset(theta, 0, sizeof(float) * MP_NPARAM);
+  auto fill = [&](int off, int n, int fan_in, double scale) {
+    const double sd = sqrt(2.0 / fan_in) * scale;
+    for (int i = 0; i < n; i++) theta[off + i] = (float)(mp_normal(&st) * sd);
+  };
+  fill(MP_W0, MP_H * MP_NIN, MP_NIN, 1.0);
+  for (int l = 0; l < MP_NR; l++) {
+    fill(MP_A(l), MP_H * MP_H, MP_H, 1.0);
+    fill(MP_B(l), MP_H * MP_H, MP_H, 1.0);
+    fill(MP_C(l), MP_H * 3, 3, 1.0);
+    fill(MP_U(l), MP_H * MP_H, MP_H, 0.5);
+    fill(MP_V(l), MP_H * MP_H, MP_H, 0.5);
+  }
+}
+
 extern "C" size_t kf_mpnn_nparam(void) { return MP_NPARAM; }
 extern "C" size_t kf_mpnn_nhead(void) { return MP_HEADS; }
 

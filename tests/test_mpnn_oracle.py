@@ -176,3 +176,19 @@ def test_bf16_gradient_close_to_exact(graph):
         a, b = g1[o:o + n], g2[o:o + n]
         assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(a), name
         o += n
+
+
+def test_library_initial_parameters():
+    """kf_mpnn_init_params: the oracle's layout, normals at sqrt(2 / fan-in)
+    (U and V halved), zero biases and zero operator heads (G = 0)."""
+    th = K.init_params(seed=5).astype(np.float64)
+    assert th.size == P.NPARAM
+    assert np.all(th[-P.HEADS:] == 0.0)
+    p = P.unpack(th)
+    for name, shape in P.PARAM_SHAPES[:-4]:
+        if len(shape) == 1:
+            assert np.all(p[name] == 0.0), name
+            continue
+        want = np.sqrt(2.0 / shape[1]) * (0.5 if name[0] in "UV" else 1.0)
+        assert abs(p[name].std() / want - 1.0) < 0.35, (name, p[name].std(), want)
+    assert not np.array_equal(th, K.init_params(seed=6).astype(np.float64))
