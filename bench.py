@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "wall-clock to 1e-10 residual drop, mixed/learned vs plain FP64 GPU; flux GF/s"
 LATTICE = (144, 48, 48)
 SEED = 1
+CONFIGS = {"c2": ((144, 48, 48), 1), "c3": ((216, 72, 72), 2), "c5": ((288, 96, 96), 4)}
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_GFLOPS = 78600.0
 
@@ -63,12 +64,12 @@ PROFILE_TO_KERNEL = {"trsv_upper": "k_trsv_upper<double, double>", "trsv_lower":
                      "ilu": "k_ilu_rows<double, true>"}
 
 
-def measured_traffic(dom, prec):
+def measured_traffic(dom, prec, config="c2"):
     """HBM bytes per launch of the dominant kernel from the committed counter
     passes (profiles/traffic_c2_fp64.json: FETCH_SIZE and WRITE_SIZE runs,
     scaled by the reading of a copy of known size); None when not covered."""
     path = os.path.join(ROOT, "profiles", "traffic_c2_fp64.json")
-    if prec != "fp64" or dom not in PROFILE_TO_KERNEL or not os.path.exists(path):
+    if config != "c2" or prec != "fp64" or dom not in PROFILE_TO_KERNEL or not os.path.exists(path):
         return None
     rec = json.load(open(path))["kernels"].get(PROFILE_TO_KERNEL[dom])
     return None if rec is None else rec["bytes"]
@@ -125,6 +126,7 @@ def mixed_precision(K, m64, mesh, hist64):
     k64 = sum(x["krylov"] for x in hist64)
     out = {}
     for prec in ("fp32", "bf16"):
+        log("mixed leg: %s" % prec)
         m = K.Mesh(mesh, K.config_options(prec=prec))
         q0 = m.state()
         m.kernel()
@@ -144,17 +146,24 @@ def mixed_precision(K, m64, mesh, hist64):
     return out
 
 
-def learned(K, mesh, hist64, rhs_per_step, train_iters):
+def log(msg):
+    """a progress line on stderr (stdout carries the one JSON line)"""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
+
+
+def learned(K, mesh, hist64, rhs_per_step, train_iters, prec="fp64", verbose=0):
     """The learned correction on the same mesh: flexible GMRES with ILU(0)
     alone (the method without the network), then with the network trained
     online -- pairs from the first 10 pseudo-time steps (the Newton
     residual and rhs_per_step Gaussian right-hand sides, each solved to
     1e-10 or 300 iterations), Adam with learning rate 1e-3 for train_iters
     steps -- and used from step 11 on with judged steps and the ILU(0)
-    fallback.  Wall-clock with and without the collection and training."""
+    fallback.  Wall-clock with and without the collection and training.
+    prec: the preconditioner's precision (the combined form: fp32)."""
     import numpy as np
     f64 = np.array(hist64[-1]["forces"])
-    m = K.Mesh(mesh, K.config_options(krylov="fgmres"))
+    m = K.Mesh(mesh, K.config_options(krylov="fgmres", prec=prec))
+    log("learned leg: ILU(0)-FGMRES warm-up")
     q0 = m.state()
     m.kernel()
     m.set_state(q0)
@@ -163,7 +172,10 @@ def learned(K, mesh, hist64, rhs_per_step, train_iters):
     hf = m.kernel()
     m.synchronize()
     t_ilu = time.perf_counter() - t0
+    log("learned leg: ILU(0)-FGMRES %.3f s; collecting pairs, training, solving" % t_ilu)
     m.set_state(q0)
+    if verbose:
+        m.set_options(K.config_options(krylov="fgmres", prec=prec, verbose=1))
     net = K.Network(m, K.init_params(seed=0), nslot=11)
     m.attach(net, collect_steps=10, rhs_per_step=rhs_per_step, pair_rtol=1e-10, pair_maxit=300,
              train_iters=train_iters, lr=1e-3, judge_every=5, holdout=0.1, seed=3)
@@ -173,6 +185,7 @@ def learned(K, mesh, hist64, rhs_per_step, train_iters):
     m.synchronize()
     t_l = time.perf_counter() - t0
     st = m.learn_stats()
+    log("learned leg: %.3f s" % t_l)
     assert m.last_error() == 0, "device error word set (learned)"
     m.attach(None)
     net.close()
@@ -180,6 +193,7 @@ def learned(K, mesh, hist64, rhs_per_step, train_iters):
     fl = np.array(hl[-1]["forces"])
     after = [x for x in hl if x["step"] >= 10]
     return {
+        "precision": prec,
         "fgmres_ilu": {"seconds": t_ilu, "pseudo_steps": len(hf), "krylov_total": sum(x["krylov"] for x in hf)},
         "learned": {"seconds_total": t_l,
                     "seconds_without_collection_and_training": t_l - (st["pair_ms"] + st["train_ms"]) / 1e3,
@@ -255,6 +269,12 @@ def main():
     ap.add_argument("--learn-rhs", type=int, default=8,
                     help="Gaussian right-hand sides per collecting step (the full recipe: 64)")
     ap.add_argument("--learn-iters", type=int, default=2000, help="Adam steps")
+    ap.add_argument("--learn-prec", default="fp64", choices=["fp64", "fp32", "bf16"],
+                    help="preconditioner precision of the learned leg (fp32: the combined form)")
+    ap.add_argument("--progress", action="store_true",
+                    help="also the solver's per-step lines in the learned leg (on stdout, before the JSON line)")
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS),
+                    help="lattice: c2 144x48x48 seed 1 (the metric's), c3 216x72x72 seed 2, c5 288x96x96 seed 4")
     ap.add_argument("--kernel", default="solve", choices=["solve", "trsv", "copy"],
                     help="solve: the benchmark; trsv / copy: short runs for hardware-counter passes (the "
                          "triangular solves on the same mesh; a device-to-device copy of known size)")
@@ -266,11 +286,15 @@ def main():
 
     if args.kernel == "copy":
         return copy_probe(args.steps)
+    global LATTICE, SEED
+    LATTICE, SEED = CONFIGS[args.config]
     import kfun3d_amd as K
     tmp = tempfile.mkdtemp(prefix="kfbench_")
-    mesh = os.path.join(tmp, "c2.msh")
+    mesh = os.path.join(tmp, "%s.msh" % args.config)
+    log("writing the %dx%dx%d lattice" % LATTICE)
     K.write_lattice(mesh, *LATTICE, SEED)
     opts = K.config_options(prec=args.prec)
+    log("setup")
     m = K.Mesh(mesh, opts)
     q0 = m.state()
     h = m.host_arrays()
@@ -280,10 +304,12 @@ def main():
 
     if args.kernel == "trsv":
         return trsv_probe(K, m, args.steps, args.warmup)
+    log("warm-up solves")
     for _ in range(args.warmup):
         m.set_state(q0)
         m.kernel()
     m.synchronize()
+    log("timed solves")
     times, hist = [], None
     m.profile(True)
     for _ in range(args.steps):
@@ -312,26 +338,28 @@ def main():
     counts = {"flux": prof["edge_flux"][1] // max(args.steps, 1), "trsv": prof["trsv_lower"][1] // max(args.steps, 1),
               "fill": prof["fill"][1] // max(args.steps, 1), "blas_passes": blas_passes(hist, opts.gmres_restart)}
     host_threads = min(16, len(os.sched_getaffinity(0)))
+    log("solve %.3f s" % max(times))
     cpu = None if args.no_cpu else cpu_baseline(mesh, LATTICE, counts, host_threads)
     mixed = None if (args.no_mixed or args.prec != "fp64") else mixed_precision(K, m, mesh, hist)
     learn = None if (args.no_learned or args.prec != "fp64") else learned(K, mesh, hist, args.learn_rhs,
-                                                                           args.learn_iters)
+                                                                           args.learn_iters, args.learn_prec,
+                                                                           1 if args.progress else 0)
 
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64" if args.prec == "fp64" else "f64 residual/Arnoldi, %s preconditioner" % args.prec,
         "data": "synthetic",
-        "config": {"workload": "144x48x48 Kuhn lattice seed 1 (331,776 vertices), second order, %s preconditioner, "
+        "config": {"workload": "%dx%dx%d Kuhn lattice seed %d (%d vertices), second order, %s preconditioner, "
                                "GMRES(30) MGS, CFL ramp from 10, solve from free stream to a 1e-10 residual drop"
-                               % args.prec,
+                               % (LATTICE + (SEED, N, args.prec)),
                    "lattice": list(LATTICE), "seed": SEED, "vertices": N, "edges": E},
         "pseudo_steps": len(hist), "krylov_total": sum(x["krylov"] for x in hist), "residual_drop": drop,
         "flux_gflops": flux_flops / flux_s / 1e9, "flux_fp64_frac": flux_flops / flux_s / 1e9 / FP64_PEAK_GFLOPS,
         "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
         "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, args.prec),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, args.prec, args.config),
                      "alg_bytes_per_launch": ab, "avg_launch_ms": ms / n,
                      "traffic_source": "profiles/traffic_c2_fp64.md (FETCH_SIZE/WRITE_SIZE passes, copy-calibrated)"},
         "cpu_baseline": cpu,

@@ -46,11 +46,15 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   kf_dv &d = c->d;
   /* launch-shape knobs of the level-scheduled kernels (tuning only) */
   const char *ev;
-  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 512;
-  c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 1024;
+  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 1024;
+  c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 2048;
   d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
+  c->trsv_group = (ev = getenv("KF_TRSV_GROUP")) ? (uint32_t)atoi(ev) : 4;
+  c->ilu_group = (ev = getenv("KF_ILU_GROUP")) ? (uint32_t)atoi(ev) : 4;
   if (c->trsv_waves < 1) c->trsv_waves = 1;
   if (c->ilu_waves < 1) c->ilu_waves = 1;
+  if (c->trsv_group < 1 || c->trsv_group > 4) c->trsv_group = 4;
+  if (c->ilu_group < 1 || c->ilu_group > 4) c->ilu_group = 4;
   const size_t N = p->N, E = p->E;
   if (N >= (1u << 31) || E >= (1u << 30) || p->nnzb >= (1u << 31)) {
     fprintf(stderr, "kf: mesh too large for 32-bit indices\n");

@@ -219,7 +219,7 @@ struct ilu_layout {
  * inverse, upper blocks and U(i,i).  Arithmetic and its order are those of
  * k_ilu. */
 template <typename T, bool EXACT>
-__global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__restrict__ rec, uint32_t nrows,
+__global__ __launch_bounds__(256) void k_ilu_rows(kf_dv d, const uint32_t *__restrict__ rec, uint32_t nrows,
                                                  uint32_t *ticket, uint32_t base, T *apub, T *udpub,
                                                  typename sentinel<T>::bits snt, T *a, T *lold,
                                                  const uint32_t *__restrict__ ldiag,
@@ -227,8 +227,11 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
                                                  const uint16_t *__restrict__ mlo, uint32_t maxrow, uint32_t maxlow,
                                                  uint32_t maxmatch)
 {
-  extern __shared__ __align__(16) unsigned char ilu_lds[];
+  extern __shared__ __align__(16) unsigned char ilu_lds_all[];
+  __shared__ uint32_t sbatch;
   const ilu_layout lay(maxrow, maxlow, maxmatch, sizeof(T), EXACT);
+  const uint32_t G = blockDim.x >> 6, wv = threadIdx.x >> 6;
+  unsigned char *ilu_lds = ilu_lds_all + (size_t)wv * lay.bytes;   /* this wave's region */
   uint32_t *scol = (uint32_t *)(ilu_lds + lay.scol);
   T *sa = (T *)(ilu_lds + lay.sa);
   T *sa0 = (T *)(ilu_lds + lay.sa0);
@@ -237,12 +240,20 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
   uint32_t *sjm = (uint32_t *)(ilu_lds + lay.sjm);
   uint16_t *slo = (uint16_t *)(ilu_lds + lay.slo);
   T *stage = (T *)(ilu_lds + lay.stage);
-  const uint32_t lane = threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t e = lane & 15, er = e >> 2, ec = e & 3, grp = lane >> 4;
   bool aborted = false;
+  /* G consecutive tickets per atomic, one row per wavefront; the workgroup
+   * takes the next G when all its rows are done (sync.h) */
+  if (threadIdx.x == 0) sbatch = atomicAdd(ticket, G) - base;
+  __syncthreads();
+  uint32_t bstart = sbatch;
   for (;;) {
-    const uint32_t t = next_ticket(ticket, base);
-    if (t >= nrows) break;
+    if (bstart >= nrows) break;   /* uniform over the workgroup */
+    uint32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(ticket, G);
+    const uint32_t t = bstart + wv;
+    if (t < nrows) {
     const uint32_t rv = lane < 6 ? rec[8 * (size_t)t + lane] : 0;
     const uint32_t i = __shfl(rv, 0, 64), j0 = __shfl(rv, 1, 64), L = __shfl(rv, 2, 64);
     const uint32_t nlow = __shfl(rv, 3, 64), mbeg = __shfl(rv, 4, 64), mcnt = __shfl(rv, 5, 64);
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
       for (uint32_t k = lane + 64; k < nlow; k += 64) { sldg[k] = ldiag[j0 + k]; smoff[k] = moff[j0 + k]; }
       for (uint32_t k = lane + 64; k < mcnt; k += 64) { sjm[k] = mjm[mbeg + k]; slo[k] = mlo[mbeg + k]; }
     }
-    __syncthreads();
+    wave_lds_sync();
     /* staging from the published copies, ILU_QB values per lane per round;
      * a value still holding the sentinel is loaded again */
     {
@@ -335,7 +346,7 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
       }
       aborted = __any(aborted);   /* a wave gives up at most once per launch */
     }
-    __syncthreads();
+    wave_lds_sync();
 
     for (uint32_t p = 0; p < nlow; p++) {
       if (EXACT) {
@@ -347,21 +358,21 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
           v = T(1) * dq + v;
           sa[16 * p + e] = v;
         }
-        __syncthreads();
+        wave_lds_sync();
       }
       /* L(i,m) = a(i,m) inv(U(m,m)) */
       T lv = T(0);
       if (lane < 16) lv = blk_mul(sa + 16 * p, stage + 16 * p, er, ec);
-      __syncthreads();
+      wave_lds_sync();
       if (lane < 16) sa[16 * p + e] = lv;
-      __syncthreads();
+      wave_lds_sync();
       /* a(i,k) -= L(i,m) U(m,k) for the update pairs of m (distinct k) */
       for (uint32_t idx = smoff[p] + grp; idx < smoff[p + 1]; idx += 4) {
         const uint32_t lo = slo[idx];
         const T s = blk_mul(sa + 16 * p, stage + 16 * (size_t)(mbase + idx), er, ec);
         sa[16 * lo + e] = sa[16 * lo + e] - s;
       }
-      __syncthreads();
+      wave_lds_sync();
     }
 
     if (EXACT) {
@@ -376,7 +387,7 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
         v = T(1) * dq + v;
         sa[16 * p + e] = v;
       }
-      __syncthreads();
+      wave_lds_sync();
       if (lane < 16) st_coherent(udpub + 16 * (size_t)i + e, canon(sa[16 * pd + e]));
     }
 
@@ -395,19 +406,24 @@ __global__ __launch_bounds__(64) void k_ilu_rows(kf_dv d, const uint32_t *__rest
       T det = s[0] * d0 + s[1] * d4 + s[2] * d8 + s[3] * d12;
       det = T(1) / det;
       acc *= det;
-      __syncthreads();
+      wave_lds_sync();
       if (lane < 16) {
         if (!isfinite((double)acc)) atomicOr(d.err, KF_ERR_NONFINITE_PIVOT);
         sa[16 * pd + e] = acc;
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     for (uint32_t k = lane; k < 16 * L; k += 64) a[16 * (size_t)j0 + k] = sa[k];
     if (EXACT)
       for (uint32_t k = lane; k < 16 * nlow; k += 64) lold[16 * (size_t)j0 + k] = sa[k];
     /* published copies of what later rows read: inv(U(i,i)) and U(i,k) */
     for (uint32_t k = lane; k < 16 * (L - pd); k += 64)
       st_coherent(apub + 16 * ((size_t)j0 + pd) + k, canon(sa[16 * pd + k]));
+    }
+    __syncthreads();   /* every row of the batch done; sbatch free */
+    if (threadIdx.x == 0) sbatch = nxt - base;
+    __syncthreads();
+    bstart = sbatch;
   }
 }
 
@@ -421,7 +437,11 @@ static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
 {
   const uint32_t nw = persistent_waves(c->d.N, c->ilu_waves);
   const ilu_layout lay(c->ilu_maxrow, c->ilu_maxlow, c->ilu_maxmatch, sizeof(T), EXACT);
+  /* wavefronts per workgroup: as asked, within 4 and 64 KB of LDS */
+  uint32_t G = c->ilu_group;
+  while (G > 1 && G * (size_t)lay.bytes > 64 * 1024) G--;
   if (lay.bytes <= 64 * 1024) {
+    const uint32_t nb = (nw + G - 1) / G;
     /* published copies, preset to this call's sentinel */
     const size_t na = 16 * c->nnzb, nu = 16 * (size_t)c->d.N;
     if (!c->ilu_apub) KF_CHECK(hipMalloc(&c->ilu_apub, na * sizeof(double)));
@@ -435,11 +455,12 @@ static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
       KF_LAUNCH_CHECK();
     }
     const int s = kf_prof_begin(c, KP_ILU);
-    hipLaunchKernelGGL((k_ilu_rows<T, EXACT>), dim3(nw), dim3(64), lay.bytes, c->stream, c->d, c->ilu_rec, c->d.N,
-                       c->ticket + 2, c->ticket_base[2], apub, udpub, snt, a, lold, c->ilu_ldiag, c->ilu_moff,
+    hipLaunchKernelGGL((k_ilu_rows<T, EXACT>), dim3(nb), dim3(64 * G), G * lay.bytes, c->stream, c->d, c->ilu_rec,
+                       c->d.N, c->ticket + 2, c->ticket_base[2], apub, udpub, snt, a, lold, c->ilu_ldiag, c->ilu_moff,
                        c->ilu_mjm, c->ilu_mlo, c->ilu_maxrow, c->ilu_maxlow, c->ilu_maxmatch);
     KF_LAUNCH_CHECK();
     kf_prof_end(c, s);
+    c->ticket_base[2] += kf_ticket_advance_groups(c->d.N, nb, G);
   } else {
     const uint32_t epoch = ++c->epoch;
     const int s = kf_prof_begin(c, KP_ILU);
@@ -447,8 +468,8 @@ static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
                        c->ticket_base[2], c->rdone, epoch, a, lold, ud);
     KF_LAUNCH_CHECK();
     kf_prof_end(c, s);
+    c->ticket_base[2] += c->d.N + nw;
   }
-  c->ticket_base[2] += c->d.N + nw;
 }
 
 __global__ __launch_bounds__(256) void k_to_bf16(size_t n, const float *__restrict__ src, uint16_t *__restrict__ dst)

@@ -100,6 +100,8 @@ struct kf_ctx {
   void *ilu_apub, *ilu_udpub; /* published diagonal inverses and U blocks, U(i,i) */
   uint64_t icall;             /* ILU calls so far (sentinel numbering) */
   uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch */
+  uint32_t trsv_group, ilu_group;   /* wavefronts per workgroup of the triangular
+                                       solves and the ILU: tickets per atomic */
   uint32_t nlev_l, nlev_u;
   /* level schedule */
   uint32_t *lrows, *urows, *lgrp, *ugrp;

@@ -10,8 +10,11 @@
  * 16 / R ranges of one level of that range DAG, owned by one wavefront with
  * a quad of lanes per row (lane c computes component c); ranges of one level
  * are independent.  Tasks are handed out by ticket in level order (host:
- * range_tasks), so a wave only ever waits for ranges whose tickets were
- * issued before its own, to waves that are running: no wait can deadlock.
+ * range_tasks), G consecutive tickets per atomic to a workgroup of G
+ * wavefronts (one task each; the group takes its next G tickets when all
+ * its tasks are done), so a wave only ever waits for ranges whose tickets
+ * were issued before its own, to waves that are running: no wait can
+ * deadlock.
  *
  * Hand-off without flags or fences.  Every result slot is preset to a
  * signalling NaN that names the sweep and the call (the sentinel); a
@@ -52,6 +55,10 @@
 #include "handoff.h"
 
 static_assert(KF_GROUP == 16, "one quad of lanes per row, 16 rows per task");
+
+#define KF_TRSV_GMAX 4 /* wavefronts per workgroup at most */
+
+
 
 #define KF_TRSV_PF 8   /* off-diagonal entries per row held in registers */
 
@@ -155,24 +162,32 @@ __device__ __forceinline__ bool wait_prefetched(const TV *v, const uint32_t cp[K
 /* forward sweep: y = L^-1 b; presets the backward output x of its rows to
  * the backward sentinel */
 template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ trow,
+__global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ trow,
                                                    const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
                                                    uint32_t lg2R, uint32_t ntask, uint32_t *ticket, uint32_t base,
                                                    typename sentinel<TV>::bits sy, typename sentinel<TV>::bits sx,
                                                    const TM *__restrict__ a, const double *__restrict__ b, TV *y,
                                                    TV *xnext)
 {
-  __shared__ TV xs[64];
-  const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
+  __shared__ TV xs_all[KF_TRSV_GMAX][64];
+  __shared__ uint32_t sbatch;
+  const uint32_t G = blockDim.x >> 6, wv = threadIdx.x >> 6;
+  TV *xs = xs_all[wv];
+  const uint32_t lane = threadIdx.x & 63u, qd = lane >> 2, c = lane & 3;
   const uint32_t kr = qd & ((1u << lg2R) - 1u), sq = qd - kr;
   bool aborted = false;
-  uint32_t t = first_ticket(ticket, base);
+  if (threadIdx.x == 0) sbatch = atomicAdd(ticket, G) - base;
+  __syncthreads();
+  uint32_t bstart = sbatch;
   for (;;) {
-    if (t >= ntask) break;
+    if (bstart >= ntask) break;   /* uniform over the workgroup */
+    uint32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(ticket, G);   /* the next batch, read after the tasks */
+    const uint32_t t = bstart + wv;
+    if (t < ntask) {
     const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
     const uint32_t r = rt[0], j0 = rt[1], jd = rt[2];
     const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
-    const uint32_t tn = draw_ticket(ticket);   /* the next task's ticket, read at the end */
     const bool active = r != ~0u;
     const uint32_t lo = r - kr;                 /* first row of this lane's range */
     uint32_t cp[KF_TRSV_PF];
@@ -222,33 +237,45 @@ __global__ __launch_bounds__(64) void k_trsv_lower(kf_dv d, const uint32_t *__re
         xr = __builtin_canonicalize(xr);
         xs[lane] = xr;
       }
-      __syncthreads();
+      wave_lds_sync();
     }
     if (active) st_coherent(y + 4 * (size_t)r + c, xr);
-    t = take_ticket(tn, base);
+    }
+    __syncthreads();   /* every task of the batch done; sbatch free */
+    if (threadIdx.x == 0) sbatch = nxt - base;
+    __syncthreads();
+    bstart = sbatch;
   }
 }
 
 /* backward sweep: x = U^-1 y (y from the forward sweep); resets y of its rows
  * to the forward sentinel of the next call */
 template <typename TM, typename TV>
-__global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ trow,
+__global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ trow,
                                                    const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
                                                    uint32_t lg2R, uint32_t ntask, uint32_t *ticket, uint32_t base,
                                                    typename sentinel<TV>::bits sx, typename sentinel<TV>::bits synext,
                                                    const TM *__restrict__ a, TV *y, TV *x, double *__restrict__ xout)
 {
-  __shared__ TV xs[64];
-  const uint32_t lane = threadIdx.x, qd = lane >> 2, c = lane & 3;
+  __shared__ TV xs_all[KF_TRSV_GMAX][64];
+  __shared__ uint32_t sbatch;
+  const uint32_t G = blockDim.x >> 6, wv = threadIdx.x >> 6;
+  TV *xs = xs_all[wv];
+  const uint32_t lane = threadIdx.x & 63u, qd = lane >> 2, c = lane & 3;
   const uint32_t kr = qd & ((1u << lg2R) - 1u), sq = qd - kr;
   bool aborted = false;
-  uint32_t t = first_ticket(ticket, base);
+  if (threadIdx.x == 0) sbatch = atomicAdd(ticket, G) - base;
+  __syncthreads();
+  uint32_t bstart = sbatch;
   for (;;) {
-    if (t >= ntask) break;
+    if (bstart >= ntask) break;   /* uniform over the workgroup */
+    uint32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(ticket, G);   /* the next batch, read after the tasks */
+    const uint32_t t = bstart + wv;
+    if (t < ntask) {
     const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
     const uint32_t rr = rt[0], jd = rt[1], j0 = jd + 1, j1 = rt[2];
     const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
-    const uint32_t tn = draw_ticket(ticket);   /* the next task's ticket, read at the end */
     const bool active = rr != ~0u;
     const uint32_t r = active ? rr : 0;
     const uint32_t hi = r + kr;                 /* top row of this lane's range */
@@ -309,13 +336,17 @@ __global__ __launch_bounds__(64) void k_trsv_upper(kf_dv d, const uint32_t *__re
         res = __builtin_canonicalize(row_dot(dg, y0, y1, y2, y3));
         xs[lane] = res;
       }
-      __syncthreads();
+      wave_lds_sync();
     }
     if (active) {
       st_coherent(x + 4 * (size_t)r + c, res);
       if ((void *)xout != (void *)x) xout[4 * (size_t)r + c] = (double)res;
     }
-    t = take_ticket(tn, base);
+    }
+    __syncthreads();   /* every task of the batch done; sbatch free */
+    if (threadIdx.x == 0) sbatch = nxt - base;
+    __syncthreads();
+    bstart = sbatch;
   }
 }
 
@@ -325,8 +356,11 @@ template <typename TM, typename TV>
 static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslot, TV *x, double *xout)
 {
   const uint32_t lg2R = (uint32_t)__builtin_ctz(c->trsv_R);
-  const uint32_t nwl = c->ntask_l < c->trsv_waves ? c->ntask_l : c->trsv_waves;
-  const uint32_t nwu = c->ntask_u < c->trsv_waves ? c->ntask_u : c->trsv_waves;
+  const uint32_t G = c->trsv_group;
+  /* persistent workgroups of G wavefronts: one atomic per G tickets */
+  const uint32_t wl = c->ntask_l < c->trsv_waves ? c->ntask_l : c->trsv_waves;
+  const uint32_t wu = c->ntask_u < c->trsv_waves ? c->ntask_u : c->trsv_waves;
+  const uint32_t nwl = (wl + G - 1) / G, nwu = (wu + G - 1) / G;
   const uint64_t k = c->tcall;
   const size_t n = 4 * (size_t)c->d.N;
   if (c->ystate[yslot] != k) {
@@ -335,19 +369,19 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
     KF_LAUNCH_CHECK();
   }
   int s = kf_prof_begin(c, KP_TRSV_L);
-  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64), 0, c->stream, c->d, c->trow_l, c->tlvl_l, c->tdep_l,
-                     lg2R, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
+  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64 * G), 0, c->stream, c->d, c->trow_l, c->tlvl_l,
+                     c->tdep_l, lg2R, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
                      sentinel<TV>::of(2 * k + 1), a, b, y, x);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  c->ticket_base[0] += c->ntask_l + nwl;
+  c->ticket_base[0] += kf_ticket_advance_groups(c->ntask_l, nwl, G);
   s = kf_prof_begin(c, KP_TRSV_U);
-  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64), 0, c->stream, c->d, c->trow_u, c->tlvl_u, c->tdep_u,
-                     lg2R, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
+  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64 * G), 0, c->stream, c->d, c->trow_u, c->tlvl_u,
+                     c->tdep_u, lg2R, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
                      sentinel<TV>::of(2 * k + 2), a, y, x, xout);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  c->ticket_base[1] += c->ntask_u + nwu;
+  c->ticket_base[1] += kf_ticket_advance_groups(c->ntask_u, nwu, G);
   c->tcall = k + 1;
   c->ystate[yslot] = k + 1;
 }
