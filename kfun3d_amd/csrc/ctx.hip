@@ -55,6 +55,12 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   if (c->ilu_waves < 1) c->ilu_waves = 1;
   if (c->trsv_group < 1 || c->trsv_group > 4) c->trsv_group = 4;
   if (c->ilu_group < 1 || c->ilu_group > 4) c->ilu_group = 4;
+  {
+    int dev = 0, lds = 0;
+    KF_CHECK(hipGetDevice(&dev));
+    KF_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    c->lds_max = lds > 0 ? (size_t)lds : 64 * 1024;
+  }
   const size_t N = p->N, E = p->E;
   if (N >= (1u << 31) || E >= (1u << 30) || p->nnzb >= (1u << 31)) {
     fprintf(stderr, "kf: mesh too large for 32-bit indices\n");

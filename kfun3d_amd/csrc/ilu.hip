@@ -439,8 +439,9 @@ static void launch_ilu(kf_ctx *c, T *a, T *lold, T *ud)
   const ilu_layout lay(c->ilu_maxrow, c->ilu_maxlow, c->ilu_maxmatch, sizeof(T), EXACT);
   /* wavefronts per workgroup: as asked, within 4 and 64 KB of LDS */
   uint32_t G = c->ilu_group;
-  while (G > 1 && G * (size_t)lay.bytes > 64 * 1024) G--;
-  if (lay.bytes <= 64 * 1024) {
+  const size_t lds_cap = c->lds_max < 64 * 1024 ? c->lds_max : 64 * 1024;
+  while (G > 1 && G * (size_t)lay.bytes > lds_cap) G--;
+  if (lay.bytes <= lds_cap) {
     const uint32_t nb = (nw + G - 1) / G;
     /* published copies, preset to this call's sentinel */
     const size_t na = 16 * c->nnzb, nu = 16 * (size_t)c->d.N;

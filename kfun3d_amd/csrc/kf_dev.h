@@ -102,6 +102,7 @@ struct kf_ctx {
   uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch */
   uint32_t trsv_group, ilu_group;   /* wavefronts per workgroup of the triangular
                                        solves and the ILU: tickets per atomic */
+  size_t lds_max;                   /* LDS bytes a workgroup may use */
   uint32_t nlev_l, nlev_u;
   /* level schedule */
   uint32_t *lrows, *urows, *lgrp, *ugrp;
