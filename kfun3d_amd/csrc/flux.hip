@@ -441,34 +441,49 @@ __global__ __launch_bounds__(256) void k_force_sum(kf_dv d, const double *__rest
   __shared__ double s[6 * 256];
   const uint32_t nc = d.ncolor;
   const uint32_t j0 = nc ? d.color[0] : 0, j1 = nc ? d.color[nc] : 0;
-  double lift = 0.0, drag = 0.0, momn = 0.0, l = 0.0, dr = 0.0, m = 0.0;
+  /* lane c < 3 carries component c (lift, drag, moment) through the faces
+   * in order: lift (l - t0) + t1, drag (dr + t2) + t3, moment (m + t4) - t5;
+   * a sign of -1 is exact, so (x + (-1) t) == x - t bit for bit */
+  const uint32_t c = threadIdx.x;
+  const double s0 = c == 0 ? -1.0 : 1.0, s1 = c == 2 ? -1.0 : 1.0;
+  double acc = 0.0, part = 0.0;
   uint32_t cl = 0, cend = nc ? d.color[1] : 0;
   for (uint32_t b = j0; b < j1; b += 256) {
     const uint32_t n = j1 - b < 256u ? j1 - b : 256u;
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < 6 * n; k += 256) s[k] = ft[6 * (size_t)b + k];
     __syncthreads();
-    if (threadIdx.x == 0) {
-      for (uint32_t i = 0; i < n; i++) {
+    if (c < 3) {
+      uint32_t i = 0;
+      while (i < n) {
         while (b + i == cend) {   /* colour cl ends before face b + i */
-          lift = lift + l; drag = drag + dr; momn = momn + m;
-          l = 0.0; dr = 0.0; m = 0.0;
+          acc = acc + part;
+          part = 0.0;
           cl++;
           cend = d.color[cl + 1];
         }
-        const double *o = s + 6 * i;
-        l = l - o[0] + o[1];
-        dr = dr + o[2] + o[3];
-        m = m + o[4] - o[5];
+        const uint32_t e = cend - b < n ? cend - b : n;   /* faces of this colour in the chunk */
+        const double *o = s + 2 * c;
+        for (; i + 8 <= e; i += 8) {
+          double x[16];
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            x[2 * q] = o[6 * (i + q)];
+            x[2 * q + 1] = o[6 * (i + q) + 1];
+          }
+#pragma unroll
+          for (int q = 0; q < 8; q++) part = (part + s0 * x[2 * q]) + s1 * x[2 * q + 1];
+        }
+        for (; i < e; i++) part = (part + s0 * o[6 * i]) + s1 * o[6 * i + 1];
       }
     }
   }
-  if (threadIdx.x == 0) {
+  if (c < 3) {
     for (; cl < nc; cl++) {   /* the last colour, then any empty ones */
-      lift = lift + l; drag = drag + dr; momn = momn + m;
-      l = 0.0; dr = 0.0; m = 0.0;
+      acc = acc + part;
+      part = 0.0;
     }
-    out[0] = lift; out[1] = drag; out[2] = momn;
+    out[c] = acc;
   }
 }
 
