@@ -209,15 +209,18 @@ void kf_setup_host(const char *fpath, GEOMETRY *g);
 void kf_bind(GEOMETRY *g);
 
 /* Host-side schedule of the triangular solves (for tests and tuning): ranges
- * of R rows (1, 2, 4, 8 or 16), grouped into tasks of 16 / R ranges of one
- * level of the range DAG.  For the forward (upper = 0) or backward sweep,
- * returns the number of tasks and, when cap covers them, writes the ranges of
- * each task (16 / R per task, ~0u = none), each task's local depth, each
- * row's local level (N entries) and, if rowtab is given, per task and row
- * slot the row and its two block bounds (16 x 3 per task).  Works on a
- * host-only GEOMETRY. */
-size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, unsigned *task, unsigned char *tdep,
-                         unsigned char *lvl, unsigned *rowtab, size_t cap);
+ * of R rows (1, 2, 4, 8 or 16), clustered into tasks of up to 16 / R ranges
+ * that read only from themselves and from tasks of lower level, in ticket
+ * order (cluster = 0: tasks of independent ranges of one level instead).  For the forward (upper = 0) or backward sweep, returns the number
+ * of tasks and, when cap covers them, writes the ranges of each task (16 / R
+ * per task, ~0u = none), each task's local depth, each row's local level
+ * inside its task (N entries), if rowtab is given, per task and row slot q
+ * (row q % R of range q / R) the row and its two block bounds (16 x 3 per
+ * task), and, if eslot is given, per block position of a row's sweep part
+ * the slot of the column's row when it lies in the same task, else 0xff
+ * (nnzb entries).  Works on a host-only GEOMETRY. */
+size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, int cluster, unsigned *task, unsigned char *tdep,
+                         unsigned char *lvl, unsigned *rowtab, unsigned char *eslot, size_t cap);
 
 /* Host-side schedule of the ILU(0) factorisation (for tests): returns the
  * number of update pairs and, when cap covers them, writes per ticket (rows

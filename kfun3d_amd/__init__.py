@@ -141,8 +141,8 @@ def _load():
         "kf_bind": (None, [GP]),
         "kf_profile": (None, [GP, C.c_int]),
         "kf_profile_get": (C.c_int, [GP, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
-        "kf_trsv_plan_host": (C.c_size_t, [GP, C.c_int, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                           C.c_size_t]),
+        "kf_trsv_plan_host": (C.c_size_t, [GP, C.c_int, C.c_uint, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_size_t]),
         "kf_ilu_plan_host": (C.c_size_t, [GP, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
         "ComputeFlux": (None, [GP, DP, C.POINTER(Xyz), DP]),
         "ComputeTimeStep": (None, [C.c_double, GP]),
@@ -394,18 +394,21 @@ class Mesh:
         lib().kf_get_blocks(self.ref, out.ctypes.data)
         return out.reshape(-1, 4, 4)
 
-    def trsv_plan(self, upper, rows):
+    def trsv_plan(self, upper, rows, cluster=True):
         """Host-side schedule of the triangular solves: (task table of shape
-        (ntask, 16 // rows), each task's local depth, each row's local level,
-        per-task row table of shape (ntask, 16, 3))."""
-        nt = int(lib().kf_trsv_plan_host(self.ref, int(upper), rows, None, None, None, None, 0))
+        (ntask, 16 // rows) in ticket order, each task's local depth, each
+        row's local level in its task, per-task row table of shape
+        (ntask, 16, 3), per-block-position slot of the column's row in the
+        same task or 255)."""
+        nt = int(lib().kf_trsv_plan_host(self.ref, int(upper), rows, int(cluster), None, None, None, None, None, 0))
         task = np.empty(nt * (16 // rows), dtype=np.uint32)
         tdep = np.empty(nt, dtype=np.uint8)
         lvl = np.empty(self.N, dtype=np.uint8)
         rowtab = np.empty(nt * 48, dtype=np.uint32)
-        lib().kf_trsv_plan_host(self.ref, int(upper), rows, task.ctypes.data, tdep.ctypes.data, lvl.ctypes.data,
-                                rowtab.ctypes.data, nt)
-        return task.reshape(nt, 16 // rows), tdep, lvl, rowtab.reshape(nt, 16, 3)
+        eslot = np.empty(self.N + 2 * self.E, dtype=np.uint8)   # one diagonal block per vertex, two per edge
+        lib().kf_trsv_plan_host(self.ref, int(upper), rows, int(cluster), task.ctypes.data, tdep.ctypes.data,
+                                lvl.ctypes.data, rowtab.ctypes.data, eslot.ctypes.data, nt)
+        return task.reshape(nt, 16 // rows), tdep, lvl, rowtab.reshape(nt, 16, 3), eslot
 
     def ilu_plan(self, nnzb):
         """Host-side schedule of the ILU(0) factorisation: (records (N, 8),

@@ -46,7 +46,9 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   kf_dv &d = c->d;
   /* launch-shape knobs of the level-scheduled kernels (tuning only) */
   const char *ev;
-  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 1024;
+  /* 2048 triangular-solve waves: every resident slot at two waves per SIMD
+   * (256 CUs x 4 SIMDs) */
+  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 2048;
   c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 2048;
   d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
   c->trsv_group = (ev = getenv("KF_TRSV_GROUP")) ? (uint32_t)atoi(ev) : 4;
@@ -190,18 +192,28 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     if (R != 1 && R != 2 && R != 4 && R != 8 && R != 16) R = 4;
     c->trsv_R = R;
     c->trsv_S = KF_GROUP / R;
+    /* ranges clustered across levels into tasks (1) or tasks of one level (0) */
+    const int cluster = (ev = getenv("KF_TRSV_CLUSTER")) ? atoi(ev) != 0 : 1;
     kf_trsv_plan t;
-    kf_plan_trsv(p, 0, R, &t);
+    kf_plan_trsv(p, 0, R, cluster, &t);
     c->trow_l = dupload(t.trow, (size_t)t.ntask * KF_GROUP * 3);
     c->tlvl_l = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_l = dupload(t.tdep, t.ntask);
     c->ntask_l = t.ntask;
+    /* a block position is read by one sweep only (left of the diagonal by
+     * the forward, right of it by the backward sweep) and holds 0xff in
+     * the other's table: the two tables merge by bitwise and */
+    uint8_t *es = t.eslot;
+    t.eslot = NULL;
     kf_free_trsv_plan(&t);
-    kf_plan_trsv(p, 1, R, &t);
+    kf_plan_trsv(p, 1, R, cluster, &t);
     c->trow_u = dupload(t.trow, (size_t)t.ntask * KF_GROUP * 3);
     c->tlvl_u = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_u = dupload(t.tdep, t.ntask);
     c->ntask_u = t.ntask;
+    for (size_t j = 0; j < p->nnzb; j++) es[j] &= t.eslot[j];
+    c->eslot = dupload(es, p->nnzb);
+    free(es);
     kf_free_trsv_plan(&t);
     c->ystate[0] = c->ystate[1] = ~(uint64_t)0;
   }
@@ -253,7 +265,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->urows, c->lgrp, c->ugrp, c->rdone,
-                  c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
+                  c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
                   c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));

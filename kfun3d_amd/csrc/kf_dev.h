@@ -108,12 +108,14 @@ struct kf_ctx {
   uint32_t *lrows, *urows, *lgrp, *ugrp;
   uint32_t nlgrp, nugrp;
   uint32_t *rdone;          /* per-row completion epoch */
-  /* triangular solves: contiguous ranges of trsv_R rows, trsv_S = 16 / R
-   * ranges of one DAG level per task (one wavefront) */
+  /* triangular solves: contiguous ranges of trsv_R rows, clustered into
+   * tasks of up to trsv_S = 16 / R ranges (one wavefront each) */
   uint32_t trsv_R, trsv_S;
   uint32_t *trow_l, *trow_u; /* per task (ticket order) and row slot: row, block bounds */
-  uint8_t *tlvl_l, *tlvl_u;  /* per task and row slot: local level inside the range */
+  uint8_t *tlvl_l, *tlvl_u;  /* per task and row slot: local level inside the task */
   uint8_t *tdep_l, *tdep_u;  /* local depth of each task */
+  uint8_t *eslot;            /* per block position: slot of the column's row in the
+                                row's task (in the sweep that reads it), 0xff outside */
   uint32_t ntask_l, ntask_u;
   double *ty64;             /* forward-sweep output (FP64) */
   uint64_t tcall;           /* triangular solves so far (sentinel numbering) */

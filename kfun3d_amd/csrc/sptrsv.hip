@@ -3,18 +3,15 @@
  * (matrix/sptrsv.cpp:34-73; the reference schedules rows by level sets,
  * matrix/init.cpp:194-274).
  *
- * Work unit: a contiguous range of R rows (R = 1, 2, 4, 8 or 16) --
- * [R g, R g + R) for the forward sweep, the R rows ending at N - 1 - R g
- * (taken in decreasing order) for the backward sweep.  A row depends only on
- * rows of its own range or of ranges with a smaller index.  A task is up to
- * 16 / R ranges of one level of that range DAG, owned by one wavefront with
- * a quad of lanes per row (lane c computes component c); ranges of one level
- * are independent.  Tasks are handed out by ticket in level order (host:
- * range_tasks), G consecutive tickets per atomic to a workgroup of G
- * wavefronts (one task each; the group takes its next G tickets when all
- * its tasks are done), so a wave only ever waits for ranges whose tickets
- * were issued before its own, to waves that are running: no wait can
- * deadlock.
+ * Work unit: a task of up to 16 rows, owned by one wavefront with a quad of
+ * lanes per row (lane c computes component c).  The host (host/plan.c)
+ * cuts the rows into contiguous ranges of R rows and clusters the ranges
+ * into tasks that read only from themselves and from tasks of lower level;
+ * tasks are handed out by ticket in level order, G consecutive tickets per
+ * atomic to a workgroup of G wavefronts (one task each; the group takes its
+ * next G tickets when all its tasks are done), so a wave only ever waits
+ * for tasks whose tickets were issued before its own, to waves that are
+ * running: no wait can deadlock.
  *
  * Hand-off without flags or fences.  Every result slot is preset to a
  * signalling NaN that names the sweep and the call (the sentinel); a
@@ -29,24 +26,25 @@
  * the backward output of their rows, the backward waves reset the forward
  * output of their rows for the next call.
  *
- *   external dependences (rows of earlier ranges): polled as above;
- *   internal dependences (rows of the same range): resolved inside the wave
- *     through LDS, row by row in the local level order computed on the host
- *     (rlvl; the task's depth tdep).
+ *   external entries (columns in earlier tasks): polled as above, and their
+ *     block products formed as soon as they arrive;
+ *   internal entries (columns in the same task; eslot names the column's
+ *     slot): resolved inside the wave through LDS, row by row in the local
+ *     level order computed on the host (tlvl; the task's depth tdep).
  *
- * Columns and factor blocks of the first KF_TRSV_PF off-diagonal entries of
- * every row are loaded into registers as soon as the wave takes its ticket.
+ * Columns, slots and factor blocks of the first KF_TRSV_PF off-diagonal
+ * entries of every row are loaded into registers as soon as the wave takes
+ * its ticket.
  *
- * Per row the block products are summed in ascending column order exactly
- * as the reference:
+ * Per row the block products are subtracted in ascending column order
+ * exactly as the reference:
  *   forward   x_r = b_r;  x_r -= A(r,j) x_j  for j < r           (dgemv, then axpy -1)
  *   backward  x_r -= A(r,j) x_j  for j > r;  x_r = inv(D_r) x_r
- * Columns of earlier forward ranges are smaller than those of the row's own
- * range, columns of earlier backward ranges larger, so the forward sweep sums
- * its external entries first and the backward sweep last, and the FP64
- * result is bit-identical.  In the single-precision modes the factors are
- * FP32 or BF16 (converted to FP32 on load), the vectors FP32, and the result
- * is widened to FP64 on the last store.
+ * internal and external entries interleaved as the columns come: a product
+ * is the same whenever it is formed, and only the order of the subtractions
+ * matters, so the FP64 result is bit-identical.  In the single-precision
+ * modes the factors are FP32 or BF16 (converted to FP32 on load), the
+ * vectors FP32, and the result is widened to FP64 on the last store.
  *
  * Every wait is bounded (KF_SPIN_LIMIT polls): a lane that gives up records
  * KF_ERR_SPIN_LIMIT and stops waiting for the rest of the launch.
@@ -124,31 +122,28 @@ __device__ __forceinline__ bool wait_vec(const TV *v, uint32_t col, typename sen
   }
 }
 
-/* Loads x of the prefetched entries j0 + k in [jlo, jhi) into xe, retrying
- * those that still hold the sentinel; all loads of one round are in flight
- * together.  Returns false if the lane gave up. */
+/* Per-wave LDS: the task's row results by slot, the external operands of
+ * the prefetched entries by row slot and entry, and a zero block that
+ * entries past the end of a row read. */
+#define KF_TRSV_EXT 64                           /* first external operand */
+#define KF_TRSV_ZERO (64 + KF_GROUP * KF_TRSV_PF * 4)
+#define KF_TRSV_LDS (KF_TRSV_ZERO + 4)
+
+/* Component c of x at the prefetched columns in the mask pend, into xo,
+ * retrying those that still hold the sentinel; all loads of one round are
+ * in flight together.  Returns false if the lane gave up. */
 template <typename TV>
-__device__ __forceinline__ bool wait_prefetched(const TV *v, const uint32_t cp[KF_TRSV_PF], uint32_t j0,
-                                                uint32_t jlo, uint32_t jhi, typename sentinel<TV>::bits s,
-                                                TV xe[KF_TRSV_PF][4], uint32_t *err, uint32_t sleep)
+__device__ __forceinline__ bool wait_own(const TV *v, const uint32_t cp[KF_TRSV_PF], uint32_t pend, uint32_t c,
+                                         typename sentinel<TV>::bits s, TV xo[KF_TRSV_PF], uint32_t *err,
+                                         uint32_t sleep)
 {
-  uint32_t pend = 0;
-#pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++)
-    if (j0 + k >= jlo && j0 + k < jhi) pend |= 1u << k;
   for (uint32_t n = 0; pend; n++) {
 #pragma unroll
     for (int k = 0; k < KF_TRSV_PF; k++)
-      if (pend & (1u << k)) {
-        const TV *p = v + 4 * (size_t)cp[k];
-        xe[k][0] = ld_coherent(p); xe[k][1] = ld_coherent(p + 1);
-        xe[k][2] = ld_coherent(p + 2); xe[k][3] = ld_coherent(p + 3);
-      }
+      if (pend & (1u << k)) xo[k] = ld_coherent(v + 4 * (size_t)cp[k] + c);
 #pragma unroll
     for (int k = 0; k < KF_TRSV_PF; k++)
-      if ((pend & (1u << k)) && sentinel<TV>::raw(xe[k][0]) != s && sentinel<TV>::raw(xe[k][1]) != s &&
-          sentinel<TV>::raw(xe[k][2]) != s && sentinel<TV>::raw(xe[k][3]) != s)
-        pend &= ~(1u << k);
+      if ((pend & (1u << k)) && sentinel<TV>::raw(xo[k]) != s) pend &= ~(1u << k);
     if (!pend) break;
     if (n >= KF_SPIN_LIMIT) {
       atomicOr(err, KF_ERR_SPIN_LIMIT);
@@ -159,23 +154,104 @@ __device__ __forceinline__ bool wait_prefetched(const TV *v, const uint32_t cp[K
   return true;
 }
 
+/* Set-up of a row's sweep part [j0, jend) for one task: blocks of the first
+ * KF_TRSV_PF entries into mp (zero past the end), and per entry the LDS
+ * offset of its operand -- the producing slot for an entry of the same task
+ * (eslot), else the entry's place in the external area, which this lane
+ * fills with component c of the polled value, or the zero block past the
+ * end.  Returns false if the lane gave up waiting.  The caller orders the
+ * external stores before any read with wave_lds_sync(). */
+template <typename TM, typename TV>
+__device__ __forceinline__ bool setup_row(const kf_dv &d, const uint8_t *__restrict__ eslot,
+                                          const TM *__restrict__ a, const TV *v, typename sentinel<TV>::bits s,
+                                          uint32_t j0, uint32_t jend, uint32_t qd, uint32_t c, TV *lds,
+                                          uint32_t off[KF_TRSV_PF], TV mp[KF_TRSV_PF][4], bool aborted)
+{
+  uint32_t cp[KF_TRSV_PF], pend = 0;
+#pragma unroll
+  for (int k = 0; k < KF_TRSV_PF; k++) {
+    cp[k] = 0;
+    off[k] = KF_TRSV_ZERO;
+    mp[k][0] = mp[k][1] = mp[k][2] = mp[k][3] = TV(0);
+    if (j0 + k < jend) {   /* an empty backward slot has j0 = 1, jend = 0 */
+      cp[k] = d.col[j0 + k];
+      const uint32_t e = eslot[j0 + k];
+      ld_row(a, j0 + k, c, mp[k]);
+      if (e == 0xffu) {
+        pend |= 1u << k;
+        off[k] = KF_TRSV_EXT + 4 * (KF_TRSV_PF * qd + k);
+      } else {
+        off[k] = 4 * e;
+      }
+    }
+  }
+  TV xo[KF_TRSV_PF];
+  bool ok = true;
+  if (!aborted) ok = wait_own(v, cp, pend, c, s, xo, d.err, d.spin_sleep);
+#pragma unroll
+  for (int k = 0; k < KF_TRSV_PF; k++)
+    if (pend & (1u << k)) lds[off[k] + c] = xo[k];
+  return ok;
+}
+
+/* x_r minus the products of its entries [j0, jend) in ascending order.  The
+ * operands of the prefetched entries are read from LDS all together (an
+ * entry past the end reads the zero block: its product is +0 and the
+ * subtraction leaves x_r unchanged), the products are formed side by side,
+ * and only the subtractions run in order; entries past the prefetch are
+ * loaded (externals polled) here.  One straight-line body for every row:
+ * a level costs about the same whatever its entries, so nothing is gained
+ * by forming fewer products per level. */
+template <typename TM, typename TV>
+__device__ __forceinline__ TV row_sweep(const kf_dv &d, const uint8_t *__restrict__ eslot, const TM *__restrict__ a,
+                                        const TV *v, typename sentinel<TV>::bits s, uint32_t j0, uint32_t jend,
+                                        uint32_t c, const uint32_t off[KF_TRSV_PF], const TV mp[KF_TRSV_PF][4],
+                                        const TV *lds, TV xr, bool &aborted)
+{
+  TV xin[KF_TRSV_PF][4], pr[KF_TRSV_PF];
+#pragma unroll
+  for (int k = 0; k < KF_TRSV_PF; k++) {
+    const TV *xc = lds + off[k];
+    xin[k][0] = xc[0]; xin[k][1] = xc[1]; xin[k][2] = xc[2]; xin[k][3] = xc[3];
+  }
+  __builtin_amdgcn_sched_barrier(0);   /* every LDS read issued before the first product */
+#pragma unroll
+  for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = row_dot(mp[k], xin[k][0], xin[k][1], xin[k][2], xin[k][3]);
+#pragma unroll
+  for (int k = 0; k < KF_TRSV_PF; k++) xr = xr - pr[k];
+  for (uint32_t j = j0 + KF_TRSV_PF; j < jend; j++) {
+    TV m[4];
+    ld_row(a, j, c, m);
+    const uint32_t e = eslot[j];
+    if (e == 0xffu) {
+      TV xc[4] = {TV(0), TV(0), TV(0), TV(0)};
+      if (!aborted) aborted = !wait_vec(v, d.col[j], s, xc, d.err, d.spin_sleep);
+      xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
+    } else {
+      const TV *xc = lds + 4 * e;
+      xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
+    }
+  }
+  return xr;
+}
+
 /* forward sweep: y = L^-1 b; presets the backward output x of its rows to
  * the backward sentinel */
 template <typename TM, typename TV>
 __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ trow,
                                                    const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
-                                                   uint32_t lg2R, uint32_t ntask, uint32_t *ticket, uint32_t base,
-                                                   typename sentinel<TV>::bits sy, typename sentinel<TV>::bits sx,
-                                                   const TM *__restrict__ a, const double *__restrict__ b, TV *y,
-                                                   TV *xnext)
+                                                   const uint8_t *__restrict__ eslot, uint32_t ntask, uint32_t *ticket,
+                                                   uint32_t base, typename sentinel<TV>::bits sy,
+                                                   typename sentinel<TV>::bits sx, const TM *__restrict__ a,
+                                                   const double *__restrict__ b, TV *y, TV *xnext)
 {
-  __shared__ TV xs_all[KF_TRSV_GMAX][64];
+  __shared__ TV lds_all[KF_TRSV_GMAX][KF_TRSV_LDS];
   __shared__ uint32_t sbatch;
   const uint32_t G = blockDim.x >> 6, wv = threadIdx.x >> 6;
-  TV *xs = xs_all[wv];
+  TV *lds = lds_all[wv];
   const uint32_t lane = threadIdx.x & 63u, qd = lane >> 2, c = lane & 3;
-  const uint32_t kr = qd & ((1u << lg2R) - 1u), sq = qd - kr;
   bool aborted = false;
+  if (lane < 4) lds[KF_TRSV_ZERO + lane] = TV(0);
   if (threadIdx.x == 0) sbatch = atomicAdd(ticket, G) - base;
   __syncthreads();
   uint32_t bstart = sbatch;
@@ -185,61 +261,26 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
     if (threadIdx.x == 0) nxt = atomicAdd(ticket, G);   /* the next batch, read after the tasks */
     const uint32_t t = bstart + wv;
     if (t < ntask) {
-    const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
-    const uint32_t r = rt[0], j0 = rt[1], jd = rt[2];
-    const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
-    const bool active = r != ~0u;
-    const uint32_t lo = r - kr;                 /* first row of this lane's range */
-    uint32_t cp[KF_TRSV_PF];
-    TV mp[KF_TRSV_PF][4];
-#pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++) {
-      cp[k] = j0 + k < jd ? d.col[j0 + k] : 0xffffffffu;
-      if (j0 + k < jd) ld_row(a, j0 + k, c, mp[k]);
-    }
-    TV xr = active ? (TV)b[4 * (size_t)r + c] : TV(0);
-    if (active) xnext[4 * (size_t)r + c] = sentinel<TV>::val(sx);
-    /* first entry inside the range */
-    uint32_t jx = j0;
-#pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++)
-      if (cp[k] < lo) jx = j0 + k + 1;
-    if (jx == j0 + KF_TRSV_PF)
-      while (jx < jd && d.col[jx] < lo) jx++;
-    /* external entries, ascending */
-    TV xe[KF_TRSV_PF][4];
-    if (!aborted) aborted = !wait_prefetched(y, cp, j0, j0, jx, sy, xe, d.err, d.spin_sleep);
-    aborted = __any(aborted);   /* a wave gives up at most once per launch */
-#pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++)
-      if (j0 + k < jx) xr = xr - row_dot(mp[k], xe[k][0], xe[k][1], xe[k][2], xe[k][3]);
-    for (uint32_t j = j0 + KF_TRSV_PF; j < jx; j++) {
-      TV m[4], xc[4];
-      ld_row(a, j, c, m);
-      if (!aborted) aborted = !wait_vec(y, d.col[j], sy, xc, d.err, d.spin_sleep);
-      xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
-    }
-    /* internal entries, level by level through LDS */
-    for (uint32_t l = 0; l < dep; l++) {
-      if (my == l) {
-#pragma unroll
-        for (int k = 0; k < KF_TRSV_PF; k++)
-          if (j0 + k >= jx && j0 + k < jd) {
-            const TV *xc = xs + 4 * (sq + cp[k] - lo);
-            xr = xr - row_dot(mp[k], xc[0], xc[1], xc[2], xc[3]);
-          }
-        for (uint32_t j = jx > j0 + KF_TRSV_PF ? jx : j0 + KF_TRSV_PF; j < jd; j++) {
-          TV m[4];
-          ld_row(a, j, c, m);
-          const TV *xc = xs + 4 * (sq + d.col[j] - lo);
-          xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
-        }
-        xr = __builtin_canonicalize(xr);
-        xs[lane] = xr;
-      }
+      const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
+      const uint32_t r = rt[0], j0 = rt[1], jd = rt[2];
+      const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
+      const bool active = r != ~0u;
+      TV xr = active ? (TV)b[4 * (size_t)r + c] : TV(0);
+      if (active) xnext[4 * (size_t)r + c] = sentinel<TV>::val(sx);
+      uint32_t off[KF_TRSV_PF];
+      TV mp[KF_TRSV_PF][4];
+      if (!setup_row<TM, TV>(d, eslot, a, y, sy, j0, jd, qd, c, lds, off, mp, aborted)) aborted = true;
+      aborted = __any(aborted);   /* a wave gives up at most once per launch */
       wave_lds_sync();
-    }
-    if (active) st_coherent(y + 4 * (size_t)r + c, xr);
+      for (uint32_t l = 0; l < dep; l++) {
+        if (my == l) {   /* the quad's four lanes take this branch together */
+          xr = row_sweep<TM, TV>(d, eslot, a, y, sy, j0, jd, c, off, mp, lds, xr, aborted);
+          xr = __builtin_canonicalize(xr);
+          lds[lane] = xr;
+        }
+        wave_lds_sync();
+      }
+      if (active) st_coherent(y + 4 * (size_t)r + c, xr);
     }
     __syncthreads();   /* every task of the batch done; sbatch free */
     if (threadIdx.x == 0) sbatch = nxt - base;
@@ -253,17 +294,18 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
 template <typename TM, typename TV>
 __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ trow,
                                                    const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
-                                                   uint32_t lg2R, uint32_t ntask, uint32_t *ticket, uint32_t base,
-                                                   typename sentinel<TV>::bits sx, typename sentinel<TV>::bits synext,
-                                                   const TM *__restrict__ a, TV *y, TV *x, double *__restrict__ xout)
+                                                   const uint8_t *__restrict__ eslot, uint32_t ntask, uint32_t *ticket,
+                                                   uint32_t base, typename sentinel<TV>::bits sx,
+                                                   typename sentinel<TV>::bits synext, const TM *__restrict__ a,
+                                                   TV *y, TV *x, double *__restrict__ xout)
 {
-  __shared__ TV xs_all[KF_TRSV_GMAX][64];
+  __shared__ TV lds_all[KF_TRSV_GMAX][KF_TRSV_LDS];
   __shared__ uint32_t sbatch;
   const uint32_t G = blockDim.x >> 6, wv = threadIdx.x >> 6;
-  TV *xs = xs_all[wv];
+  TV *lds = lds_all[wv];
   const uint32_t lane = threadIdx.x & 63u, qd = lane >> 2, c = lane & 3;
-  const uint32_t kr = qd & ((1u << lg2R) - 1u), sq = qd - kr;
   bool aborted = false;
+  if (lane < 4) lds[KF_TRSV_ZERO + lane] = TV(0);
   if (threadIdx.x == 0) sbatch = atomicAdd(ticket, G) - base;
   __syncthreads();
   uint32_t bstart = sbatch;
@@ -273,75 +315,38 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
     if (threadIdx.x == 0) nxt = atomicAdd(ticket, G);   /* the next batch, read after the tasks */
     const uint32_t t = bstart + wv;
     if (t < ntask) {
-    const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
-    const uint32_t rr = rt[0], jd = rt[1], j0 = jd + 1, j1 = rt[2];
-    const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
-    const bool active = rr != ~0u;
-    const uint32_t r = active ? rr : 0;
-    const uint32_t hi = r + kr;                 /* top row of this lane's range */
-    uint32_t cp[KF_TRSV_PF];
-    TV mp[KF_TRSV_PF][4], dg[4];
-    if (active) ld_row(a, jd, c, dg);
-#pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++) {
-      cp[k] = j0 + k < j1 ? d.col[j0 + k] : 0;
-      if (j0 + k < j1) ld_row(a, j0 + k, c, mp[k]);
-    }
-    TV x0 = TV(0);
-    if (active) {
-      x0 = y[4 * (size_t)r + c];
-      y[4 * (size_t)r + c] = sentinel<TV>::val(synext);
-    }
-    /* first entry beyond the range */
-    uint32_t jy = j0;
-#pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++)
-      if (j0 + k < j1 && cp[k] <= hi) jy = j0 + k + 1;
-    if (jy == j0 + KF_TRSV_PF)
-      while (jy < j1 && d.col[jy] <= hi) jy++;
-    /* products of the prefetched external entries, ahead of the serial part */
-    TV xe[KF_TRSV_PF][4], se[KF_TRSV_PF];
-    if (!aborted) aborted = !wait_prefetched(x, cp, j0, jy, j1, sx, xe, d.err, d.spin_sleep);
-    aborted = __any(aborted);
-#pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++)
-      if (j0 + k >= jy && j0 + k < j1) se[k] = row_dot(mp[k], xe[k][0], xe[k][1], xe[k][2], xe[k][3]);
-    TV res = TV(0);
-    for (uint32_t l = 0; l < dep; l++) {
-      if (my == l) {   /* the quad's four lanes take this branch together */
-        TV xr = x0;
-#pragma unroll
-        for (int k = 0; k < KF_TRSV_PF; k++)
-          if (j0 + k < jy) {
-            const TV *xc = xs + 4 * (sq + hi - cp[k]);
-            xr = xr - row_dot(mp[k], xc[0], xc[1], xc[2], xc[3]);
-          }
-        for (uint32_t j = j0 + KF_TRSV_PF; j < jy; j++) {
-          TV m[4];
-          ld_row(a, j, c, m);
-          const TV *xc = xs + 4 * (sq + hi - d.col[j]);
-          xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
-        }
-#pragma unroll
-        for (int k = 0; k < KF_TRSV_PF; k++)
-          if (j0 + k >= jy && j0 + k < j1) xr = xr - se[k];
-        for (uint32_t j = jy > j0 + KF_TRSV_PF ? jy : j0 + KF_TRSV_PF; j < j1; j++) {
-          TV m[4], xc[4];
-          ld_row(a, j, c, m);
-          if (!aborted) aborted = !wait_vec(x, d.col[j], sx, xc, d.err, d.spin_sleep);
-          xr = xr - row_dot(m, xc[0], xc[1], xc[2], xc[3]);
-        }
-        /* x_r = inv(D_r) x_r: the quad exchanges its four components */
-        const TV y0 = quad_get<0>(xr), y1 = quad_get<1>(xr), y2 = quad_get<2>(xr), y3 = quad_get<3>(xr);
-        res = __builtin_canonicalize(row_dot(dg, y0, y1, y2, y3));
-        xs[lane] = res;
+      const uint32_t *rt = trow + 3 * ((size_t)t * KF_GROUP + qd);
+      const uint32_t rr = rt[0], jd = rt[1], j0 = jd + 1, j1 = rt[2];
+      const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
+      const bool active = rr != ~0u;
+      const uint32_t r = active ? rr : 0;
+      TV dg[4];
+      if (active) ld_row(a, jd, c, dg);
+      TV x0 = TV(0);
+      if (active) {
+        x0 = y[4 * (size_t)r + c];
+        y[4 * (size_t)r + c] = sentinel<TV>::val(synext);
       }
+      uint32_t off[KF_TRSV_PF];
+      TV mp[KF_TRSV_PF][4];
+      if (!setup_row<TM, TV>(d, eslot, a, x, sx, j0, j1, qd, c, lds, off, mp, aborted)) aborted = true;
+      aborted = __any(aborted);
       wave_lds_sync();
-    }
-    if (active) {
-      st_coherent(x + 4 * (size_t)r + c, res);
-      if ((void *)xout != (void *)x) xout[4 * (size_t)r + c] = (double)res;
-    }
+      TV res = TV(0);
+      for (uint32_t l = 0; l < dep; l++) {
+        if (my == l) {   /* the quad's four lanes take this branch together */
+          const TV xr = row_sweep<TM, TV>(d, eslot, a, x, sx, j0, j1, c, off, mp, lds, x0, aborted);
+          /* x_r = inv(D_r) x_r: the quad exchanges its four components */
+          const TV y0 = quad_get<0>(xr), y1 = quad_get<1>(xr), y2 = quad_get<2>(xr), y3 = quad_get<3>(xr);
+          res = __builtin_canonicalize(row_dot(dg, y0, y1, y2, y3));
+          lds[lane] = res;
+        }
+        wave_lds_sync();
+      }
+      if (active) {
+        st_coherent(x + 4 * (size_t)r + c, res);
+        if ((void *)xout != (void *)x) xout[4 * (size_t)r + c] = (double)res;
+      }
     }
     __syncthreads();   /* every task of the batch done; sbatch free */
     if (threadIdx.x == 0) sbatch = nxt - base;
@@ -355,7 +360,6 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
 template <typename TM, typename TV>
 static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslot, TV *x, double *xout)
 {
-  const uint32_t lg2R = (uint32_t)__builtin_ctz(c->trsv_R);
   const uint32_t G = c->trsv_group;
   /* persistent workgroups of G wavefronts: one atomic per G tickets */
   const uint32_t wl = c->ntask_l < c->trsv_waves ? c->ntask_l : c->trsv_waves;
@@ -370,14 +374,14 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
   }
   int s = kf_prof_begin(c, KP_TRSV_L);
   hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64 * G), 0, c->stream, c->d, c->trow_l, c->tlvl_l,
-                     c->tdep_l, lg2R, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
+                     c->tdep_l, c->eslot, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
                      sentinel<TV>::of(2 * k + 1), a, b, y, x);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   c->ticket_base[0] += kf_ticket_advance_groups(c->ntask_l, nwl, G);
   s = kf_prof_begin(c, KP_TRSV_U);
   hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64 * G), 0, c->stream, c->d, c->trow_u, c->tlvl_u,
-                     c->tdep_u, lg2R, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
+                     c->tdep_u, c->eslot, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
                      sentinel<TV>::of(2 * k + 2), a, y, x, xout);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);

@@ -42,9 +42,9 @@ typedef struct kf_prep {
 
 /* schedule of the level-scheduled triangular solves (plan.c) */
 typedef struct kf_trsv_plan {
-  uint32_t R, S;       /* rows per range, ranges per task (16 / R) */
+  uint32_t R, S;       /* rows per range, ranges per task (at most 16 / R) */
   uint32_t ntask;
-  uint8_t *lvl;        /* [N] local level of each row inside its range */
+  uint8_t *lvl;        /* [N] local level of each row inside its task */
   uint32_t *task;      /* [ntask * S] ranges of each task, ~0u = none */
   uint8_t *tdep;       /* [ntask] local depth of each task */
   uint32_t *trow;      /* [ntask * 16 * 3] per task and row slot: row, then
@@ -52,8 +52,10 @@ typedef struct kf_trsv_plan {
                           backward: diagonal position, row end (~0u, 0, 0
                           for an empty slot) */
   uint8_t *tlvl;       /* [ntask * 16] local level of the slot's row, 0xff empty */
+  uint8_t *eslot;      /* [nnzb] per position of a row's sweep part: the slot of
+                          the column's row in the row's task, 0xff outside */
 } kf_trsv_plan;
-void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, kf_trsv_plan *out);
+void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, int cluster, kf_trsv_plan *out);
 void kf_free_trsv_plan(kf_trsv_plan *t);
 
 /* ILU(0) rows in ticket (level) order (plan.c) */

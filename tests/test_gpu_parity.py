@@ -145,7 +145,7 @@ def test_ilu_and_sptrsv(pair):
     fresh.close()
 
 
-@pytest.mark.parametrize("rows", [1, 2, 16])
+@pytest.mark.parametrize("rows", [1, 2, 8, 16])
 def test_sptrsv_other_schedules(mesh_a, rows, monkeypatch):
     """The triangular solve is bit-identical to the oracle for every range
     size of its schedule (the default is 4 rows per range)."""

@@ -1,10 +1,11 @@
 """The host-side schedule of the level-scheduled triangular solves
 (kfun3d_amd/host/plan.c), checked on the CPU for every range size: each
 range appears in exactly one task, every slot holds a valid range or the
-empty marker, the ranges of a task never read one another, every range a
-task reads from sits in an earlier task (so the ticket order is a
-topological order and no wait can deadlock), and the local levels and
-depths are those of the in-range dependence chains."""
+empty marker, a row reads only rows of its own task or of tasks at earlier
+tickets (so the ticket order is a topological order and no wait can
+deadlock), the local levels and depths are those of the dependence chains
+inside each task, and the per-position slot table names the slot of every
+same-task column."""
 import numpy as np
 import pytest
 
@@ -19,9 +20,9 @@ def _lower_upper(m):
     return rp.astype(np.int64), col.astype(np.int64)
 
 
-def _check(m, rp, col, upper, R):
+def _check(m, rp, col, upper, R, cluster=True):
     N = m.N
-    task, tdep, lvl, rowtab = m.trsv_plan(upper, R)
+    task, tdep, lvl, rowtab, eslot = m.trsv_plan(upper, R, cluster)
     S = 16 // R
     ng = (N + R - 1) // R
     flat = task.ravel()
@@ -29,47 +30,55 @@ def _check(m, rp, col, upper, R):
     assert np.all(flat[valid] < ng)
     # every range exactly once
     assert np.array_equal(np.sort(flat[valid]), np.arange(ng))
-    # no task is empty
-    assert np.all(valid.reshape(-1, S).any(axis=1))
-    # position of each range in the ticket order
+    # no task is empty, members fill the leading slots
+    v2 = valid.reshape(-1, S)
+    assert np.all(v2[:, 0])
+    assert np.all(v2[:, 1:] <= v2[:, :-1])
+    # ticket and member position of each range
     where = np.empty(ng, dtype=np.int64)
     where[flat[valid]] = np.nonzero(valid)[0] // S
-    # range and in-range position of every row
+    member = np.empty(ng, dtype=np.int64)
+    member[flat[valid]] = np.nonzero(valid)[0] % S
     rows = np.arange(N)
-    if upper:
-        rng = (N - 1 - rows) // R
-    else:
-        rng = rows // R
+    pos = (N - 1 - rows) if upper else rows
+    rng = pos // R
+    rtask = where[rng]
+    slot = member[rng] * R + pos % R
     # dependences: forward row r reads the columns left of its diagonal,
-    # backward row r the columns right of it
-    depth = np.zeros(ng, dtype=np.int64)
+    # backward row r the columns right of it; rows are visited in sweep
+    # order, so a same-task column's level is known before the row's
+    depth = np.zeros(len(task), dtype=np.int64)
     ll = np.zeros(N, dtype=np.int64)
     order = rows[::-1] if upper else rows
     for r in order:
-        cols = col[rp[r]:rp[r + 1]]
-        cols = cols[cols > r] if upper else cols[cols < r]
-        inside = cols[rng[cols] == rng[r]]
-        outside = cols[rng[cols] != rng[r]]
+        js = np.arange(rp[r], rp[r + 1])
+        cols = col[js]
+        keep = (cols > r) if upper else (cols < r)
+        js, cols = js[keep], cols[keep]
+        same = rtask[cols] == rtask[r]
+        inside, outside = cols[same], cols[~same]
         ll[r] = (ll[inside].max() + 1) if len(inside) else 0
-        depth[rng[r]] = max(depth[rng[r]], ll[r] + 1)
+        depth[rtask[r]] = max(depth[rtask[r]], ll[r] + 1)
         if len(outside):
-            assert np.all(where[rng[outside]] < where[rng[r]]), (r, outside)
+            assert np.all(rtask[outside] < rtask[r]), (r, outside)
+        assert np.array_equal(eslot[js[same]].astype(np.int64), slot[inside])
+        assert np.all(eslot[js[~same]] == 0xFF)
+    assert np.all(ll < 255)
     assert np.array_equal(lvl.astype(np.int64), ll)
-    tmax = np.zeros(len(task), dtype=np.int64)
-    np.maximum.at(tmax, np.nonzero(valid)[0] // S, depth[flat[valid]])
-    assert np.array_equal(tdep.astype(np.int64), tmax)
+    assert np.array_equal(tdep.astype(np.int64), depth)
     # per-task row table: slot q holds row q % R of range task[t, q // R]
     diag = np.array([rp[r] + np.nonzero(col[rp[r]:rp[r + 1]] == r)[0][0] for r in range(N)])
     for t in range(len(task)):
         for q in range(16):
-            g = int(task[t, q // R]); kr = q % R
+            g = int(task[t, q // R]) if q // R < S else 0xFFFFFFFF
+            kr = q % R
             r, ja, jb = (int(v) for v in rowtab[t, q])
             if g == 0xFFFFFFFF or g * R + kr >= N:
                 assert (r, ja, jb) == (0xFFFFFFFF, 0, 0)
                 continue
             assert r == (N - 1 - (g * R + kr) if upper else g * R + kr)
             assert (ja, jb) == ((diag[r], rp[r + 1]) if upper else (rp[r], diag[r]))
-    return len(task)
+    return len(task), int(depth.max())
 
 
 @pytest.mark.parametrize("R", [1, 2, 4, 8, 16])
@@ -77,7 +86,32 @@ def test_plan_mesh_a(mesh_a, R):
     m = K.Mesh(mesh_a, device=False)
     rp, col = _lower_upper(m)
     for upper in (0, 1):
-        _check(m, rp, col, upper, R)
+        for cluster in (True, False):
+            _check(m, rp, col, upper, R, cluster)
+    m.close()
+
+
+def test_plan_one_level_tasks(mesh_c1):
+    """cluster = 0: the ranges of a task never read one another, and a
+    task's ranges are consecutive ranges of one level of the range DAG."""
+    m = K.Mesh(mesh_c1, device=False)
+    rp, col = _lower_upper(m)
+    R = 4
+    for upper in (0, 1):
+        _check(m, rp, col, upper, R, cluster=False)
+        task, tdep, lvl, rowtab, eslot = m.trsv_plan(upper, R, False)
+        N = m.N
+        rows = np.arange(N)
+        pos = (N - 1 - rows) if upper else rows
+        rng = pos // R
+        for r in range(N):
+            cs = col[rp[r]:rp[r + 1]]
+            cs = cs[cs > r] if upper else cs[cs < r]
+            js = np.arange(rp[r], rp[r + 1])[np.isin(col[rp[r]:rp[r + 1]], cs)]
+            inside = rng[cs] == rng[r]
+            # only columns of the row's own range are same-task
+            assert np.all((eslot[js] != 0xFF) == inside)
+        assert tdep.max() <= R
     m.close()
 
 
@@ -86,9 +120,11 @@ def test_plan_lattice(mesh_c1, R):
     m = K.Mesh(mesh_c1, device=False)
     rp, col = _lower_upper(m)
     for upper in (0, 1):
-        nt = _check(m, rp, col, upper, R)
-        # 48x16x16 lattice: the range DAG has (48 / R) + 15 + 15 levels
-        assert nt >= (48 // R + 30) * 1
+        nt, dmax = _check(m, rp, col, upper, R)
+        # 48x16x16 lattice: most tasks are full, and clustering ranges
+        # across y and z keeps the chain inside a task short
+        assert nt <= 1.5 * m.N / 16
+        assert dmax <= 2 * R + 4
     m.close()
 
 
