@@ -13,17 +13,28 @@
 
 #define RB 256
 
+/* The fixed tree over the block's RB values: at each halving s, value t
+ * becomes v[t] + v[t + s].  The levels s >= 64 pair threads of different
+ * waves and go through LDS; the levels s <= 32 pair lanes of wave 0 and go
+ * through lane shifts, with the same operands in the same order. */
+static_assert(RB == 256, "block_sum's LDS levels are s = 128, 64");
 __device__ __forceinline__ double block_sum(double v, double *sh)
 {
   const int t = threadIdx.x;
   sh[t] = v;
   __syncthreads();
+  if (t < 128) sh[t] = sh[t] + sh[t + 128];
+  __syncthreads();
+  if (t < 64) {
+    double x = sh[t] + sh[t + 64];
 #pragma unroll
-  for (int s = RB / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] = sh[t] + sh[t + s];
-    __syncthreads();
+    for (int s = 32; s > 0; s >>= 1) x = x + __shfl_down(x, s, 64);
+    if (t == 0) sh[0] = x;
   }
-  return sh[0];
+  __syncthreads();
+  const double r = sh[0];
+  __syncthreads();   /* every thread has read sh[0] before sh is reused */
+  return r;
 }
 
 __global__ __launch_bounds__(RB) void k_dot_part(size_t n, const double *__restrict__ x, const double *__restrict__ y,
@@ -103,16 +114,49 @@ __global__ __launch_bounds__(RB) void k_mgs_step(size_t n, const double *__restr
                                                  double *__restrict__ part_out)
 {
   __shared__ double sh[RB];
+  const bool self = u == w;
+  const size_t stride = (size_t)gridDim.x * RB;
+  size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
+  /* elements in chunks of four: a chunk's operands are loaded ahead of its
+   * updates (u is either w itself or a vector that overlaps neither v nor
+   * w), and the first chunk's loads are in flight while the previous dot's
+   * partials are summed; updates and accumulation keep the element order */
+  double vv[4], ww[4], uu[4];
+  bool chunk = i + 3 * stride < n;
+  if (chunk) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      vv[k] = v[i + k * stride];
+      ww[k] = w[i + k * stride];
+      uu[k] = self ? 0.0 : u[i + k * stride];
+    }
+  }
   double acc = 0.0;
-  for (int i = threadIdx.x; i < KF_NRED; i += RB) acc += part_in[i];
+  for (int j = threadIdx.x; j < KF_NRED; j += RB) acc += part_in[j];
   const double h = block_sum(acc, sh);
   if (blockIdx.x == 0 && threadIdx.x == 0) *hslot = h;
-  __syncthreads();   /* every thread has read sh[0] before sh is reused */
   const double a = -h;
-  const bool self = u == w;
   acc = 0.0;
-  const size_t stride = (size_t)gridDim.x * RB;
-  for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += stride) {
+  while (chunk) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double ax = a * vv[k];
+      const double wn = ax + ww[k];
+      w[i + k * stride] = wn;
+      acc += wn * (self ? wn : uu[k]);
+    }
+    i += 4 * stride;
+    chunk = i + 3 * stride < n;
+    if (chunk) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        vv[k] = v[i + k * stride];
+        ww[k] = w[i + k * stride];
+        uu[k] = self ? 0.0 : u[i + k * stride];
+      }
+    }
+  }
+  for (; i < n; i += stride) {
     const double ax = a * v[i];
     const double wn = ax + w[i];
     w[i] = wn;

@@ -215,8 +215,21 @@ __device__ __forceinline__ TV row_sweep(const kf_dv &d, const uint8_t *__restric
     xin[k][0] = xc[0]; xin[k][1] = xc[1]; xin[k][2] = xc[2]; xin[k][3] = xc[3];
   }
   __builtin_amdgcn_sched_barrier(0);   /* every LDS read issued before the first product */
+  /* the 8 products row_dot(mp[k], xin[k]) stage by stage, so that each
+   * stage's 8 independent operations hide one another's latency; every
+   * product is summed in row_dot's order (0 + m0 x0, + m1 x1, ...) */
 #pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = row_dot(mp[k], xin[k][0], xin[k][1], xin[k][2], xin[k][3]);
+  for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = mp[k][0] * xin[k][0];
+#pragma unroll
+  for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = TV(0) + pr[k];
+#pragma unroll
+  for (int i = 1; i < 4; i++) {
+    TV t[KF_TRSV_PF];
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++) t[k] = mp[k][i] * xin[k][i];
+#pragma unroll
+    for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = pr[k] + t[k];
+  }
 #pragma unroll
   for (int k = 0; k < KF_TRSV_PF; k++) xr = xr - pr[k];
   for (uint32_t j = j0 + KF_TRSV_PF; j < jend; j++) {
