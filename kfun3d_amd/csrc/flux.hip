@@ -300,13 +300,30 @@ __global__ __launch_bounds__(256) void k_residual(kf_dv d, const double *__restr
   if (v >= d.N) return;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0, acc = 0.0;
   const uint32_t deg = d.deg[v];
-  size_t idx = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
-  for (uint32_t k = 0; k < deg; k++, idx += 64) {
-    const uint32_t ent = d.sell_ent[idx];
-    const d4 e = ld4(eres, ent >> 1);
-    if (ent & 1u) { r0 = r0 - e.p; r1 = r1 - e.u; r2 = r2 - e.v; r3 = r3 - e.w; }
-    else { r0 = r0 + e.p; r1 = r1 + e.u; r2 = r2 + e.v; r3 = r3 + e.w; }
-    if (SPECTRAL) acc = acc + eterm[ent >> 1];
+  const size_t idx0 = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
+  /* incidences in chunks of RC: a chunk's edge results are all loaded
+   * before its sums, which run in incidence (edge) order */
+  constexpr uint32_t RC = 8;
+  for (uint32_t k0 = 0; k0 < deg; k0 += RC) {
+    uint32_t ent[RC];
+    d4 e[RC];
+    double et[RC];
+#pragma unroll
+    for (uint32_t i = 0; i < RC; i++)
+      if (k0 + i < deg) ent[i] = d.sell_ent[idx0 + 64u * (k0 + i)];
+#pragma unroll
+    for (uint32_t i = 0; i < RC; i++)
+      if (k0 + i < deg) {
+        e[i] = ld4(eres, ent[i] >> 1);
+        if (SPECTRAL) et[i] = eterm[ent[i] >> 1];
+      }
+#pragma unroll
+    for (uint32_t i = 0; i < RC; i++)
+      if (k0 + i < deg) {
+        if (ent[i] & 1u) { r0 = r0 - e[i].p; r1 = r1 - e[i].u; r2 = r2 - e[i].v; r3 = r3 - e[i].w; }
+        else { r0 = r0 + e[i].p; r1 = r1 + e[i].u; r2 = r2 + e[i].v; r3 = r3 + e[i].w; }
+        if (SPECTRAL) acc = acc + et[i];
+      }
   }
   if (SPECTRAL) sr[v] = spectral_boundary(d, v, ld4(q, v), acc);
   for (uint32_t s = d.sfptr[v]; s < d.sfptr[v + 1]; s++) {
