@@ -24,19 +24,6 @@ static T *dupload(const T *h, size_t n)
 }
 
 /* rows of each level cut into groups of at most KF_GROUP */
-static uint32_t *level_groups(const uint32_t *lptr, uint32_t nlev, uint32_t *ngrp)
-{
-  size_t n = 0;
-  for (uint32_t l = 0; l < nlev; l++) n += (lptr[l + 1] - lptr[l] + KF_GROUP - 1) / KF_GROUP;
-  uint32_t *g = (uint32_t *)kf_xmalloc((n + 1) * sizeof(uint32_t));
-  size_t k = 0;
-  for (uint32_t l = 0; l < nlev; l++)
-    for (uint32_t s = lptr[l]; s < lptr[l + 1]; s += KF_GROUP) g[k++] = s;
-  g[k] = lptr[nlev];
-  *ngrp = (uint32_t)n;
-  return g;
-}
-
 extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o)
 {
   kf_ctx *c = new kf_ctx();
@@ -46,14 +33,13 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   kf_dv &d = c->d;
   /* launch-shape knobs of the level-scheduled kernels (tuning only) */
   const char *ev;
-  /* 2048 triangular-solve waves: every resident slot at two waves per SIMD
-   * (256 CUs x 4 SIMDs) */
-  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 2048;
+  /* triangular-solve waves: sized from the schedule below unless set */
+  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 0;
   c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 2048;
   d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
   c->trsv_group = (ev = getenv("KF_TRSV_GROUP")) ? (uint32_t)atoi(ev) : 4;
   c->ilu_group = (ev = getenv("KF_ILU_GROUP")) ? (uint32_t)atoi(ev) : 4;
-  if (c->trsv_waves < 1) c->trsv_waves = 1;
+
   if (c->ilu_waves < 1) c->ilu_waves = 1;
   if (c->trsv_group < 1 || c->trsv_group > 4) c->trsv_group = 4;
   if (c->ilu_group < 1 || c->ilu_group > 4) c->ilu_group = 4;
@@ -176,13 +162,7 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   }
   c->nlev_l = p->nlev_l;
   c->nlev_u = p->nlev_u;
-  c->lrows = dupload(p->lrows, N);
-  c->urows = dupload(p->urows, N);
-  uint32_t *lg = level_groups(p->lptr, p->nlev_l, &c->nlgrp);
-  uint32_t *ug = level_groups(p->uptr, p->nlev_u, &c->nugrp);
-  c->lgrp = dupload(lg, c->nlgrp + 1);
-  c->ugrp = dupload(ug, c->nugrp + 1);
-  free(lg); free(ug);
+  c->lrows = dupload(p->lrows, N);   /* row order of the ILU kernel without a plan */
   c->rdone = dalloc<uint32_t>(N);
   KF_CHECK(hipMemset(c->rdone, 0, N * sizeof(uint32_t)));
   {
@@ -200,6 +180,16 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     c->tlvl_l = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_l = dupload(t.tdep, t.ntask);
     c->ntask_l = t.ntask;
+    if (c->trsv_waves == 0) {
+      /* about 3.5 levels of tasks in flight (measured best at C2: 1024
+       * waves there, 0.34 / 0.37 ms against 0.35 / 0.39 ms with 2048):
+       * enough waves to start every task of a level as soon as it is ready,
+       * few enough that waves waiting on later levels do not crowd the
+       * memory-side polls; at most every resident slot at two waves per
+       * SIMD (256 CUs x 4 SIMDs), which the wide levels of C5 need */
+      const uint32_t w = t.nlevel ? (uint32_t)(7ull * t.ntask / (2ull * t.nlevel)) : 1;
+      c->trsv_waves = w < 512 ? 512 : w > 2048 ? 2048 : w;
+    }
     /* a block position is read by one sweep only (left of the diagonal by
      * the forward, right of it by the backward sweep) and holds 0xff in
      * the other's table: the two tables merge by bitwise and */
@@ -264,7 +254,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
   void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
-                  d.red, d.scal, d.err, c->lrows, c->urows, c->lgrp, c->ugrp, c->rdone,
+                  d.red, d.scal, d.err, c->lrows, c->rdone,
                   c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
                   c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);

@@ -104,10 +104,9 @@ struct kf_ctx {
                                        solves and the ILU: tickets per atomic */
   size_t lds_max;                   /* LDS bytes a workgroup may use */
   uint32_t nlev_l, nlev_u;
-  /* level schedule */
-  uint32_t *lrows, *urows, *lgrp, *ugrp;
-  uint32_t nlgrp, nugrp;
-  uint32_t *rdone;          /* per-row completion epoch */
+  /* the ILU kernel without a plan: rows in level order, per-row completion epoch */
+  uint32_t *lrows;
+  uint32_t *rdone;
   /* triangular solves: contiguous ranges of trsv_R rows, clustered into
    * tasks of up to trsv_S = 16 / R ranges (one wavefront each) */
   uint32_t trsv_R, trsv_S;

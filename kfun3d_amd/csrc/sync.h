@@ -8,13 +8,15 @@
  * dispatch order of workgroups.
  *
  * One atomic on a single counter serialises at the memory side (about
- * 11 ns each on MI355X), so tickets are handed out in batches of K
- * consecutive values: a wave works through its batch in order and draws the
- * next batch one ahead.  A wave then holds at most 2 K tickets and always
- * works on its oldest unfinished one, so the oldest unfinished ticket of
- * the launch belongs to a running wave whose dependences are all finished:
- * no wait can deadlock.  After a launch of nw waves over ntask tickets the
- * counter has advanced by exactly kf_ticket_advance(ntask, nw, K).
+ * 11 ns each on MI355X), so the planned kernels take G consecutive tickets
+ * per atomic for a workgroup of G wavefronts, one ticket each, and the
+ * group's next G once all G are done.  Every ticket below the oldest
+ * unfinished one is finished and every ticket of a running group is being
+ * worked on, so the oldest unfinished ticket of the launch belongs to a
+ * running wave whose dependences are all finished: no wait can deadlock.
+ * After a launch of nb workgroups over ntask tickets the counter has
+ * advanced by exactly kf_ticket_advance_groups(ntask, nb, G).  The
+ * fallback ILU kernel draws one ticket per wave (next_ticket).
  *
  * The triangular solves and the planned ILU hand results over through
  * sentinel-preset slots (handoff.h).  The fallback ILU kernel (no plan) uses
@@ -71,28 +73,6 @@ __device__ __forceinline__ bool wait_row(const uint32_t *rdone, uint32_t row, ui
   return true;
 }
 
-/* Tickets drawn one task ahead: draw_ticket issues the atomic (lane 0) and
- * returns at once; take_ticket, at the end of the current task, waits for the
- * value and gives it to every lane.  A wave works through its tickets in the
- * order drawn, so the oldest unfinished ticket always belongs to the task its
- * wave is working on and the hand-off stays deadlock-free. */
-__device__ __forceinline__ uint32_t draw_ticket(uint32_t *ticket)
-{
-  uint32_t t = 0;
-  if (threadIdx.x == 0) t = atomicAdd(ticket, 1u);
-  return t;
-}
-
-__device__ __forceinline__ uint32_t take_ticket(uint32_t drawn, uint32_t base)
-{
-  return (uint32_t)__builtin_amdgcn_readfirstlane((int)drawn) - base;
-}
-
-__device__ __forceinline__ uint32_t first_ticket(uint32_t *ticket, uint32_t base)
-{
-  return take_ticket(draw_ticket(ticket), base);
-}
-
 /* LDS written by some lanes of this wave is read by others after this point
  * (the wave's LDS operations execute in order; the fences keep the compiler
  * from moving accesses across) */
@@ -101,42 +81,6 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-/* batches of K consecutive tickets (K a launch constant) */
-__device__ __forceinline__ uint32_t draw_batch(uint32_t *ticket, uint32_t K)
-{
-  uint32_t t = 0;
-  if (threadIdx.x == 0) t = atomicAdd(ticket, K);
-  return t;
-}
-
-struct ticket_batch {
-  uint32_t t;    /* the current ticket */
-  uint32_t left; /* tickets left in the current batch, t included */
-  uint32_t nb;   /* the next batch, drawn (value read when needed) */
-};
-
-__device__ __forceinline__ ticket_batch first_batch(uint32_t *ticket, uint32_t base, uint32_t K)
-{
-  ticket_batch b;
-  b.t = take_ticket(draw_batch(ticket, K), base);
-  b.left = K;
-  b.nb = draw_batch(ticket, K);
-  return b;
-}
-
-/* after finishing ticket b.t: the next one of the batch, or the first of the
- * next batch (and draw the one after) */
-__device__ __forceinline__ void next_in_batch(ticket_batch &b, uint32_t *ticket, uint32_t base, uint32_t K)
-{
-  if (--b.left) {
-    b.t++;
-    return;
-  }
-  b.t = take_ticket(b.nb, base);
-  b.left = K;
-  b.nb = draw_batch(ticket, K);
 }
 
 /* next work ticket of this wave (lane 0 draws, all lanes receive it) */
@@ -154,15 +98,6 @@ __device__ __forceinline__ uint32_t next_ticket(uint32_t *ticket, uint32_t base)
 static inline uint32_t kf_ticket_advance_groups(uint32_t ntask, uint32_t nb, uint32_t G)
 {
   return G * ((ntask + G - 1) / G + nb);
-}
-
-/* counter advance of a launch of nw waves over ntask tickets in batches of
- * K: every batch that starts below ntask, plus two held at exit by every wave
- * except the one that exits inside the batch crossing ntask */
-static inline uint32_t kf_ticket_advance(uint32_t ntask, uint32_t nw, uint32_t K)
-{
-  const uint32_t batches = (ntask + K - 1) / K;
-  return K * (batches + 2 * nw - (ntask % K != 0 ? 1u : 0u));
 }
 
 #endif

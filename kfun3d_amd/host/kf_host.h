@@ -44,6 +44,7 @@ typedef struct kf_prep {
 typedef struct kf_trsv_plan {
   uint32_t R, S;       /* rows per range, ranges per task (at most 16 / R) */
   uint32_t ntask;
+  uint32_t nlevel;     /* levels of the task DAG (tickets are in level order) */
   uint8_t *lvl;        /* [N] local level of each row inside its task */
   uint32_t *task;      /* [ntask * S] ranges of each task, ~0u = none */
   uint8_t *tdep;       /* [ntask] local depth of each task */

@@ -292,6 +292,7 @@ void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, int cluster, kf_trsv_
   out->R = R;
   out->S = S;
   out->ntask = nt;
+  out->nlevel = nt ? tmax + 1 : 0;
   out->lvl = lvl;
   out->task = task;
   out->tdep = tdep;
