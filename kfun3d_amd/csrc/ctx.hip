@@ -34,7 +34,7 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   /* launch-shape knobs of the level-scheduled kernels (tuning only) */
   const char *ev;
   /* triangular-solve waves: sized from the schedule below unless set */
-  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 0;
+  c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 0;   /* 0: sized per launch */
   c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 2048;
   d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
   c->trsv_group = (ev = getenv("KF_TRSV_GROUP")) ? (uint32_t)atoi(ev) : 4;
@@ -180,15 +180,14 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     c->tlvl_l = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_l = dupload(t.tdep, t.ntask);
     c->ntask_l = t.ntask;
-    if (c->trsv_waves == 0) {
+    {
       /* about 3.5 levels of tasks in flight (measured best at C2: 1024
        * waves there, 0.34 / 0.37 ms against 0.35 / 0.39 ms with 2048):
        * enough waves to start every task of a level as soon as it is ready,
        * few enough that waves waiting on later levels do not crowd the
-       * memory-side polls; at most every resident slot at two waves per
-       * SIMD (256 CUs x 4 SIMDs), which the wide levels of C5 need */
+       * memory-side polls.  The launch caps it by precision. */
       const uint32_t w = t.nlevel ? (uint32_t)(7ull * t.ntask / (2ull * t.nlevel)) : 1;
-      c->trsv_waves = w < 512 ? 512 : w > 2048 ? 2048 : w;
+      c->trsv_auto = w < 512 ? 512 : w;
     }
     /* a block position is read by one sweep only (left of the diagonal by
      * the forward, right of it by the backward sweep) and holds 0xff in

@@ -99,7 +99,8 @@ struct kf_ctx {
   uint32_t ilu_maxrow, ilu_maxlow, ilu_maxmatch;
   void *ilu_apub, *ilu_udpub; /* published diagonal inverses and U blocks, U(i,i) */
   uint64_t icall;             /* ILU calls so far (sentinel numbering) */
-  uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch */
+  uint32_t trsv_waves, ilu_waves;   /* persistent wavefronts per launch (trsv: 0 = sized) */
+  uint32_t trsv_auto;               /* triangular solves: waves for ~3.5 task levels in flight */
   uint32_t trsv_group, ilu_group;   /* wavefronts per workgroup of the triangular
                                        solves and the ILU: tickets per atomic */
   size_t lds_max;                   /* LDS bytes a workgroup may use */

@@ -375,8 +375,15 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
 {
   const uint32_t G = c->trsv_group;
   /* persistent workgroups of G wavefronts: one atomic per G tickets */
-  const uint32_t wl = c->ntask_l < c->trsv_waves ? c->ntask_l : c->trsv_waves;
-  const uint32_t wu = c->ntask_u < c->trsv_waves ? c->ntask_u : c->trsv_waves;
+  /* waves: the context's setting, else the schedule's size capped by the
+   * resident slots of this precision's kernels -- the FP64 kernels fit two
+   * waves per SIMD (2048 on 256 CUs x 4 SIMDs), the FP32 ones four, of which
+   * 3072 measured best on the 2.65M-vertex lattice (0.86 / 0.88 ms against
+   * 0.95 / 0.95 ms with 2048 and 0.88 / 0.91 ms with 4096) */
+  const uint32_t cap = sizeof(TV) == 8 ? 2048u : 3072u;
+  const uint32_t W = c->trsv_waves ? c->trsv_waves : (c->trsv_auto < cap ? c->trsv_auto : cap);
+  const uint32_t wl = c->ntask_l < W ? c->ntask_l : W;
+  const uint32_t wu = c->ntask_u < W ? c->ntask_u : W;
   const uint32_t nwl = (wl + G - 1) / G, nwu = (wu + G - 1) / G;
   const uint64_t k = c->tcall;
   const size_t n = 4 * (size_t)c->d.N;
