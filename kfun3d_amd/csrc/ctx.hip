@@ -35,12 +35,11 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   const char *ev;
   /* triangular-solve waves: sized from the schedule below unless set */
   c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 0;   /* 0: sized per launch */
-  c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 2048;
+  c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 0;   /* 0: by precision */
   d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
   c->trsv_group = (ev = getenv("KF_TRSV_GROUP")) ? (uint32_t)atoi(ev) : 4;
   c->ilu_group = (ev = getenv("KF_ILU_GROUP")) ? (uint32_t)atoi(ev) : 4;
 
-  if (c->ilu_waves < 1) c->ilu_waves = 1;
   if (c->trsv_group < 1 || c->trsv_group > 4) c->trsv_group = 4;
   if (c->ilu_group < 1 || c->ilu_group > 4) c->ilu_group = 4;
   {
