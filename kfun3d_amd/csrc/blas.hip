@@ -3,8 +3,10 @@
  * combination of src/kernel.c:24-45 on gfx950.
  *
  * Reductions are deterministic: a fixed grid of KF_NRED blocks walks the
- * vector with a fixed stride, each block reduces through LDS in a fixed
- * tree, and one block combines the KF_NRED partials in a fixed tree.  The
+ * vector in element pairs (2p, 2p + 1) with a fixed stride of pairs, each
+ * thread summing its pairs' products in order (so a lane moves 16 bytes per
+ * load), each block reduces through LDS in a fixed tree, and one block
+ * combines the KF_NRED partials in a fixed tree.  The
  * sum order differs from the reference's one-thread loop (it is a tree), the
  * result is the same on every run.  Scalars stay on the device (d.scal) so
  * that dependent vector updates need no host round trip.
@@ -37,15 +39,50 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
   return r;
 }
 
+/* elements i, i + 1 of x: one 16-byte load when the vector is 16-byte
+ * aligned (VEC), two 8-byte loads otherwise; the same values either way */
+template <bool VEC>
+__device__ __forceinline__ void ld_pair(const double *x, size_t i, double &a, double &b)
+{
+  if (VEC) {
+    const double2 t = *reinterpret_cast<const double2 *>(x + i);
+    a = t.x;
+    b = t.y;
+  } else {
+    a = x[i];
+    b = x[i + 1];
+  }
+}
+
+static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+template <bool VEC>
 __global__ __launch_bounds__(RB) void k_dot_part(size_t n, const double *__restrict__ x, const double *__restrict__ y,
                                                  double *__restrict__ part)
 {
   __shared__ double sh[RB];
   double acc = 0.0;
-  const size_t stride = (size_t)gridDim.x * RB;
-  for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += stride) acc += x[i] * y[i];
+  const size_t stride = (size_t)gridDim.x * RB, np = n / 2;
+  size_t p = (size_t)blockIdx.x * RB + threadIdx.x;
+  for (; p < np; p += stride) {
+    double x0, x1, y0, y1;
+    ld_pair<VEC>(x, 2 * p, x0, x1);
+    ld_pair<VEC>(y, 2 * p, y0, y1);
+    acc += x0 * y0;
+    acc += x1 * y1;
+  }
+  if (p == np && (n & 1)) acc += x[n - 1] * y[n - 1];   /* the half pair of an odd length */
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+static void launch_dot_part(kf_ctx *c, size_t n, const double *x, const double *y, double *part)
+{
+  if (aligned16(x) && aligned16(y))
+    hipLaunchKernelGGL(k_dot_part<true>, dim3(KF_NRED), dim3(RB), 0, c->stream, n, x, y, part);
+  else
+    hipLaunchKernelGGL(k_dot_part<false>, dim3(KF_NRED), dim3(RB), 0, c->stream, n, x, y, part);
+  KF_LAUNCH_CHECK();
 }
 
 __global__ __launch_bounds__(RB) void k_red_final(const double *__restrict__ part, double *__restrict__ out, int take_sqrt)
@@ -92,8 +129,7 @@ static void dot_any(kf_ctx *c, size_t n, const double *x, const double *y, int s
     KF_LAUNCH_CHECK();
     return;
   }
-  hipLaunchKernelGGL(k_dot_part, dim3(KF_NRED), dim3(RB), 0, c->stream, n, x, y, c->d.red);
-  KF_LAUNCH_CHECK();
+  launch_dot_part(c, n, x, y, c->d.red);
   hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, c->d.red, c->d.scal + slot, take_sqrt);
   KF_LAUNCH_CHECK();
 }
@@ -109,26 +145,28 @@ void kf_nrm2_dev(kf_ctx *c, size_t n, const double *x, int slot) { dot_any(c, n,
  * step (u = the next basis vector, or w itself for the closing norm).  The
  * grid and the element order of each thread are k_dot_part's, so the
  * partials are bit-identical to an unfused dot after an unfused update. */
+template <bool VEC>
 __global__ __launch_bounds__(RB) void k_mgs_step(size_t n, const double *__restrict__ part_in, double *hslot,
                                                  const double *v, double *w, const double *u,
                                                  double *__restrict__ part_out)
 {
   __shared__ double sh[RB];
   const bool self = u == w;
-  const size_t stride = (size_t)gridDim.x * RB;
-  size_t i = (size_t)blockIdx.x * RB + threadIdx.x;
-  /* elements in chunks of four: a chunk's operands are loaded ahead of its
+  const size_t stride = (size_t)gridDim.x * RB, np = n / 2;
+  size_t p = (size_t)blockIdx.x * RB + threadIdx.x;
+  /* pairs in chunks of two: a chunk's operands are loaded ahead of its
    * updates (u is either w itself or a vector that overlaps neither v nor
    * w), and the first chunk's loads are in flight while the previous dot's
    * partials are summed; updates and accumulation keep the element order */
   double vv[4], ww[4], uu[4];
-  bool chunk = i + 3 * stride < n;
+  bool chunk = p + stride < np;
   if (chunk) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      vv[k] = v[i + k * stride];
-      ww[k] = w[i + k * stride];
-      uu[k] = self ? 0.0 : u[i + k * stride];
+    for (int k = 0; k < 2; k++) {
+      const size_t i = 2 * (p + k * stride);
+      ld_pair<VEC>(v, i, vv[2 * k], vv[2 * k + 1]);
+      ld_pair<VEC>(w, i, ww[2 * k], ww[2 * k + 1]);
+      if (!self) ld_pair<VEC>(u, i, uu[2 * k], uu[2 * k + 1]);
     }
   }
   double acc = 0.0;
@@ -139,28 +177,48 @@ __global__ __launch_bounds__(RB) void k_mgs_step(size_t n, const double *__restr
   acc = 0.0;
   while (chunk) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const double ax = a * vv[k];
-      const double wn = ax + ww[k];
-      w[i + k * stride] = wn;
-      acc += wn * (self ? wn : uu[k]);
+    for (int k = 0; k < 2; k++) {
+      const size_t i = 2 * (p + k * stride);
+      const double w0 = a * vv[2 * k] + ww[2 * k];   /* contraction off: the product, then the sum */
+      const double w1 = a * vv[2 * k + 1] + ww[2 * k + 1];
+      if (VEC) {
+        *reinterpret_cast<double2 *>(w + i) = make_double2(w0, w1);
+      } else {
+        w[i] = w0;
+        w[i + 1] = w1;
+      }
+      acc += w0 * (self ? w0 : uu[2 * k]);
+      acc += w1 * (self ? w1 : uu[2 * k + 1]);
     }
-    i += 4 * stride;
-    chunk = i + 3 * stride < n;
+    p += 2 * stride;
+    chunk = p + stride < np;
     if (chunk) {
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        vv[k] = v[i + k * stride];
-        ww[k] = w[i + k * stride];
-        uu[k] = self ? 0.0 : u[i + k * stride];
+      for (int k = 0; k < 2; k++) {
+        const size_t i = 2 * (p + k * stride);
+        ld_pair<VEC>(v, i, vv[2 * k], vv[2 * k + 1]);
+        ld_pair<VEC>(w, i, ww[2 * k], ww[2 * k + 1]);
+        if (!self) ld_pair<VEC>(u, i, uu[2 * k], uu[2 * k + 1]);
       }
     }
   }
-  for (; i < n; i += stride) {
-    const double ax = a * v[i];
-    const double wn = ax + w[i];
-    w[i] = wn;
-    acc += wn * (self ? wn : u[i]);
+  for (; p < np; p += stride) {
+    const size_t i = 2 * p;
+    double v0, v1, x0, x1;
+    ld_pair<VEC>(v, i, v0, v1);
+    ld_pair<VEC>(w, i, x0, x1);
+    const double w0 = a * v0 + x0, w1 = a * v1 + x1;
+    w[i] = w0;
+    w[i + 1] = w1;
+    double u0 = w0, u1 = w1;
+    if (!self) ld_pair<VEC>(u, i, u0, u1);
+    acc += w0 * u0;
+    acc += w1 * u1;
+  }
+  if (p == np && (n & 1)) {   /* the half pair of an odd length */
+    const double wn = a * v[n - 1] + w[n - 1];
+    w[n - 1] = wn;
+    acc += wn * (self ? wn : u[n - 1]);
   }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part_out[blockIdx.x] = s;
@@ -180,12 +238,16 @@ void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     return;
   }
   double *part[2] = {c->d.red, c->d.red + KF_NRED};
-  hipLaunchKernelGGL(k_dot_part, dim3(KF_NRED), dim3(RB), 0, c->stream, n, w, vbase, part[0]);
-  KF_LAUNCH_CHECK();
+  launch_dot_part(c, n, w, vbase, part[0]);
+  const bool vec = aligned16(vbase) && aligned16(w) && (stride & 1) == 0;
   for (int k = 0; k < nv; k++) {
     const double *u = k + 1 < nv ? vbase + (size_t)(k + 1) * stride : w;
-    hipLaunchKernelGGL(k_mgs_step, dim3(KF_NRED), dim3(RB), 0, c->stream, n, part[k & 1], c->d.scal + slot + k,
-                       vbase + (size_t)k * stride, w, u, part[(k + 1) & 1]);
+    if (vec)
+      hipLaunchKernelGGL(k_mgs_step<true>, dim3(KF_NRED), dim3(RB), 0, c->stream, n, part[k & 1],
+                         c->d.scal + slot + k, vbase + (size_t)k * stride, w, u, part[(k + 1) & 1]);
+    else
+      hipLaunchKernelGGL(k_mgs_step<false>, dim3(KF_NRED), dim3(RB), 0, c->stream, n, part[k & 1],
+                         c->d.scal + slot + k, vbase + (size_t)k * stride, w, u, part[(k + 1) & 1]);
     KF_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, part[nv & 1], c->d.scal + slot + nv, 1);

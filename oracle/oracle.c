@@ -902,7 +902,8 @@ double or_norm(size_t n, const double *v)
 }
 
 /* the HIP path's reduction order: 1024 blocks x 256 threads, each thread a
- * strided sum, an LDS tree per block, then a 256-thread strided sum of the
+ * strided sum over element pairs (2p, 2p + 1; the half pair of an odd
+ * length last), an LDS tree per block, then a 256-thread strided sum of the
  * 1024 partials and one more tree (a check of the device, not the reference) */
 static double tree256(double *sh)
 {
@@ -919,7 +920,10 @@ static double dot_blocked(size_t n, const double *x, const double *y)
   for (int b = 0; b < 1024; b++) {
     for (int t = 0; t < 256; t++) {
       double acc = 0.0;
-      for (size_t i = (size_t)b * 256 + t; i < n; i += stride) acc += x[i] * y[i];
+      for (size_t p = (size_t)b * 256 + t; 2 * p < n; p += stride) {
+        acc += x[2 * p] * y[2 * p];
+        if (2 * p + 1 < n) acc += x[2 * p + 1] * y[2 * p + 1];
+      }
       sh[t] = acc;
     }
     part[b] = tree256(sh);
