@@ -59,6 +59,8 @@ def lib():
         L.or_ijacvec.argtypes = [vp, dp, dp, C.c_double, dp, dp, dp, dp, dp]
         L.or_norm.restype = C.c_double
         L.or_norm.argtypes = [C.c_size_t, dp]
+        L.or_dot_tree.restype = C.c_double
+        L.or_dot_tree.argtypes = [C.c_size_t, dp, dp]
         L.or_kernel.restype = C.c_int
         L.or_kernel.argtypes = [vp, C.POINTER(OrStep), C.c_int]
         L.or_inv4.argtypes = [dp, dp]
@@ -170,6 +172,13 @@ class Oracle:
         n = lib().or_kernel(self.c, rec, max_rec)
         return [dict(step=s.step, cfl=s.cfl, fnorm=s.fnorm, krylov=s.krylov, forces=list(s.forces))
                 for s in rec[:min(n, max_rec)]]
+
+
+def dot_tree(x, y):
+    """x . y in the HIP path's fixed reduction tree"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().or_dot_tree(x.size, _p(x), _p(y))
 
 
 def norm(v):

@@ -936,6 +936,8 @@ static double dot_blocked(size_t n, const double *x, const double *y)
   return tree256(sh);
 }
 
+double or_dot_tree(size_t n, const double *x, const double *y) { return dot_blocked(n, x, y); }
+
 static double ctx_dot(const or_ctx *c, size_t n, const double *x, const double *y)
 {
   if (c->o.reduce_order) return dot_blocked(n, x, y);

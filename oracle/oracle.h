@@ -78,6 +78,9 @@ void or_forces(or_ctx *c, double *forces);
 void or_ijacvec(or_ctx *c, const double *x, const double *r, double qnorm, double *gx, double *gy, double *gz,
                 double *w, double *y);
 double or_norm(size_t n, const double *v);
+/* x . y summed in the HIP path's fixed tree (element pairs per thread,
+ * 1024 x 256 blocks, two trees) */
+double or_dot_tree(size_t n, const double *x, const double *y);
 int or_kernel(or_ctx *c, or_step *rec, int max_rec);
 void or_inv4(const double *src, double *dst);
 

@@ -265,3 +265,26 @@ def test_kernel_tree_reductions_match_oracle(which, mesh_a, mesh_c1):
     hg = m.kernel()
     m.close()
     _same_history(hg, ho)
+
+
+def test_norm_tree_any_length_and_alignment(mesh_a):
+    """Compute2ndNorm in the fixed tree (element pairs per thread, 16-byte
+    loads when the vector is 16-byte aligned) against the oracle's statement
+    of the same tree, bit for bit: odd and even lengths, and pointers 8 bytes
+    off a 16-byte boundary (the two-load path)."""
+    m = K.Mesh(mesh_a, K.options(reduce_seq=0))
+    try:
+        rng = np.random.default_rng(21)
+        for n in (1, 2, 7, 1000, 4 * m.N - 1, 4 * m.N, 600001):
+            x = rng.standard_normal(n + 1)
+            dv = K.DeviceVec(n + 1, x)
+            got_a = K.lib().Compute2ndNorm(n, dv.ptr)
+            got_u = K.lib().Compute2ndNorm(n, dv.ptr + 8)
+            want_a = np.sqrt(O.dot_tree(x[:n], x[:n]))
+            want_u = np.sqrt(O.dot_tree(x[1:], x[1:]))
+            assert m.last_error() == 0
+            assert _bitwise(np.array([got_a]), np.array([want_a])), (n, got_a, want_a)
+            assert _bitwise(np.array([got_u]), np.array([want_u])), (n, got_u, want_u)
+            dv.free()
+    finally:
+        m.close()
