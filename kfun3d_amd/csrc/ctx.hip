@@ -173,8 +173,8 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     c->trsv_S = KF_GROUP / R;
     /* ranges clustered across levels into tasks (1) or tasks of one level (0) */
     const int cluster = (ev = getenv("KF_TRSV_CLUSTER")) ? atoi(ev) != 0 : 1;
-    kf_trsv_plan t;
-    kf_plan_trsv(p, 0, R, cluster, &t);
+    kf_trsv_plan t, tu;
+    kf_plan_trsv_both(p, R, cluster, &t, &tu);
     c->trow_l = dupload(t.trow, (size_t)t.ntask * KF_GROUP * 3);
     c->tlvl_l = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_l = dupload(t.tdep, t.ntask);
@@ -194,7 +194,7 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     uint8_t *es = t.eslot;
     t.eslot = NULL;
     kf_free_trsv_plan(&t);
-    kf_plan_trsv(p, 1, R, cluster, &t);
+    t = tu;
     c->trow_u = dupload(t.trow, (size_t)t.ntask * KF_GROUP * 3);
     c->tlvl_u = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_u = dupload(t.tdep, t.ntask);

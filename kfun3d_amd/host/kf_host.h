@@ -57,6 +57,7 @@ typedef struct kf_trsv_plan {
                           the column's row in the row's task, 0xff outside */
 } kf_trsv_plan;
 void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, int cluster, kf_trsv_plan *out);
+void kf_plan_trsv_both(const kf_prep *p, uint32_t R, int cluster, kf_trsv_plan *lower, kf_trsv_plan *upper);
 void kf_free_trsv_plan(kf_trsv_plan *t);
 
 /* ILU(0) rows in ticket (level) order (plan.c) */

@@ -145,6 +145,7 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
   e->xyzn->x1 = (double *)kf_xmalloc((E + 1) * sizeof(double));
   e->xyzn->x2 = (double *)kf_xmalloc((E + 1) * sizeof(double));
   e->xyzn->x3 = (double *)kf_xmalloc((E + 1) * sizeof(double));
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < E; i++) {
     e->eptr->n0[i] = b0[perm[i]] - 1;
     e->eptr->n1[i] = b0[perm[i] + E] - 1;
@@ -165,6 +166,7 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
   n->xyz->x0 = (double *)kf_xmalloc((N + 1) * sizeof(double));
   n->xyz->x1 = (double *)kf_xmalloc((N + 1) * sizeof(double));
   n->xyz->x2 = (double *)kf_xmalloc((N + 1) * sizeof(double));
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; i++) {
     n->xyz->x0[i] = nb[i];
     n->xyz->x1[i] = nb[i + N];

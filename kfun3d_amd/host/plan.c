@@ -301,6 +301,18 @@ void kf_plan_trsv(const kf_prep *p, int upper, uint32_t R, int cluster, kf_trsv_
   out->eslot = eslot;
 }
 
+/* the forward and the backward schedule, built side by side */
+void kf_plan_trsv_both(const kf_prep *p, uint32_t R, int cluster, kf_trsv_plan *lower, kf_trsv_plan *upper)
+{
+#pragma omp parallel sections num_threads(2)
+  {
+#pragma omp section
+    kf_plan_trsv(p, 0, R, cluster, lower);
+#pragma omp section
+    kf_plan_trsv(p, 1, R, cluster, upper);
+  }
+}
+
 void kf_free_trsv_plan(kf_trsv_plan *t)
 {
   free(t->lvl);
