@@ -144,8 +144,12 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   d.rowptr = dupload(p->rowptr, N + 1);
   d.col = dupload(p->col, p->nnzb);
   d.diag = dupload(p->diag, N);
-  for (size_t i = 0; i < N; i++)
+  c->trsv_pf = 7;
+  for (size_t i = 0; i < N; i++) {
     if (p->rowptr[i + 1] - p->rowptr[i] > c->maxrow) c->maxrow = p->rowptr[i + 1] - p->rowptr[i];
+    if (p->diag[i] - p->rowptr[i] > 7 || p->rowptr[i + 1] - p->diag[i] - 1 > 7) c->trsv_pf = 8;
+  }
+  if ((ev = getenv("KF_TRSV_HELD")) && atoi(ev) == 8) c->trsv_pf = 8;   /* 8 held entries on any mesh */
   {
     kf_ilu_plan ip;
     kf_plan_ilu(p, &ip);

@@ -111,6 +111,8 @@ struct kf_ctx {
   /* triangular solves: contiguous ranges of trsv_R rows, clustered into
    * tasks of up to trsv_S = 16 / R ranges (one wavefront each) */
   uint32_t trsv_R, trsv_S;
+  uint32_t trsv_pf;           /* off-diagonal entries per row held in registers (7 or 8):
+                                7 when no row has more below or above the diagonal */
   uint32_t *trow_l, *trow_u; /* per task (ticket order) and row slot: row, block bounds */
   uint8_t *tlvl_l, *tlvl_u;  /* per task and row slot: local level inside the task */
   uint8_t *tdep_l, *tdep_u;  /* local depth of each task */

@@ -32,9 +32,11 @@
  *     slot): resolved inside the wave through LDS, row by row in the local
  *     level order computed on the host (tlvl; the task's depth tdep).
  *
- * Columns, slots and factor blocks of the first KF_TRSV_PF off-diagonal
- * entries of every row are loaded into registers as soon as the wave takes
- * its ticket.
+ * Columns, slots and factor blocks of the first PF off-diagonal entries of
+ * every row are loaded into registers as soon as the wave takes its ticket;
+ * PF is 7 when no row has more than 7 entries on either side of the
+ * diagonal (the Kuhn lattices: 14 neighbours, 7 below and 7 above), else
+ * KF_TRSV_PF, so that no level forms a product past the end of every row.
  *
  * Per row the block products are subtracted in ascending column order
  * exactly as the reference:
@@ -58,7 +60,7 @@ static_assert(KF_GROUP == 16, "one quad of lanes per row, 16 rows per task");
 
 
 
-#define KF_TRSV_PF 8   /* off-diagonal entries per row held in registers */
+#define KF_TRSV_PF 8   /* off-diagonal entries per row held in registers, at most */
 
 __device__ __forceinline__ double mat_ld(const double *a, size_t i) { return a[i]; }
 __device__ __forceinline__ float mat_ld(const float *a, size_t i) { return a[i]; }
@@ -132,17 +134,17 @@ __device__ __forceinline__ bool wait_vec(const TV *v, uint32_t col, typename sen
 /* Component c of x at the prefetched columns in the mask pend, into xo,
  * retrying those that still hold the sentinel; all loads of one round are
  * in flight together.  Returns false if the lane gave up. */
-template <typename TV>
-__device__ __forceinline__ bool wait_own(const TV *v, const uint32_t cp[KF_TRSV_PF], uint32_t pend, uint32_t c,
-                                         typename sentinel<TV>::bits s, TV xo[KF_TRSV_PF], uint32_t *err,
+template <int PF, typename TV>
+__device__ __forceinline__ bool wait_own(const TV *v, const uint32_t cp[PF], uint32_t pend, uint32_t c,
+                                         typename sentinel<TV>::bits s, TV xo[PF], uint32_t *err,
                                          uint32_t sleep)
 {
   for (uint32_t n = 0; pend; n++) {
 #pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++)
+    for (int k = 0; k < PF; k++)
       if (pend & (1u << k)) xo[k] = ld_coherent(v + 4 * (size_t)cp[k] + c);
 #pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++)
+    for (int k = 0; k < PF; k++)
       if ((pend & (1u << k)) && sentinel<TV>::raw(xo[k]) != s) pend &= ~(1u << k);
     if (!pend) break;
     if (n >= KF_SPIN_LIMIT) {
@@ -155,21 +157,21 @@ __device__ __forceinline__ bool wait_own(const TV *v, const uint32_t cp[KF_TRSV_
 }
 
 /* Set-up of a row's sweep part [j0, jend) for one task: blocks of the first
- * KF_TRSV_PF entries into mp (zero past the end), and per entry the LDS
+ * PF entries into mp (zero past the end), and per entry the LDS
  * offset of its operand -- the producing slot for an entry of the same task
  * (eslot), else the entry's place in the external area, which this lane
  * fills with component c of the polled value, or the zero block past the
  * end.  Returns false if the lane gave up waiting.  The caller orders the
  * external stores before any read with wave_lds_sync(). */
-template <typename TM, typename TV>
+template <int PF, typename TM, typename TV>
 __device__ __forceinline__ bool setup_row(const kf_dv &d, const uint8_t *__restrict__ eslot,
                                           const TM *__restrict__ a, const TV *v, typename sentinel<TV>::bits s,
                                           uint32_t j0, uint32_t jend, uint32_t qd, uint32_t c, TV *lds,
-                                          uint32_t off[KF_TRSV_PF], TV mp[KF_TRSV_PF][4], bool aborted)
+                                          uint32_t off[PF], TV mp[PF][4], bool aborted)
 {
-  uint32_t cp[KF_TRSV_PF], pend = 0;
+  uint32_t cp[PF], pend = 0;
 #pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++) {
+  for (int k = 0; k < PF; k++) {
     cp[k] = 0;
     off[k] = KF_TRSV_ZERO;
     mp[k][0] = mp[k][1] = mp[k][2] = mp[k][3] = TV(0);
@@ -185,11 +187,11 @@ __device__ __forceinline__ bool setup_row(const kf_dv &d, const uint8_t *__restr
       }
     }
   }
-  TV xo[KF_TRSV_PF];
+  TV xo[PF];
   bool ok = true;
-  if (!aborted) ok = wait_own(v, cp, pend, c, s, xo, d.err, d.spin_sleep);
+  if (!aborted) ok = wait_own<PF, TV>(v, cp, pend, c, s, xo, d.err, d.spin_sleep);
 #pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++)
+  for (int k = 0; k < PF; k++)
     if (pend & (1u << k)) lds[off[k] + c] = xo[k];
   return ok;
 }
@@ -202,15 +204,15 @@ __device__ __forceinline__ bool setup_row(const kf_dv &d, const uint8_t *__restr
  * loaded (externals polled) here.  One straight-line body for every row:
  * a level costs about the same whatever its entries, so nothing is gained
  * by forming fewer products per level. */
-template <typename TM, typename TV>
+template <int PF, typename TM, typename TV>
 __device__ __forceinline__ TV row_sweep(const kf_dv &d, const uint8_t *__restrict__ eslot, const TM *__restrict__ a,
                                         const TV *v, typename sentinel<TV>::bits s, uint32_t j0, uint32_t jend,
-                                        uint32_t c, const uint32_t off[KF_TRSV_PF], const TV mp[KF_TRSV_PF][4],
+                                        uint32_t c, const uint32_t off[PF], const TV mp[PF][4],
                                         const TV *lds, TV xr, bool &aborted)
 {
-  TV xin[KF_TRSV_PF][4], pr[KF_TRSV_PF];
+  TV xin[PF][4], pr[PF];
 #pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++) {
+  for (int k = 0; k < PF; k++) {
     const TV *xc = lds + off[k];
     xin[k][0] = xc[0]; xin[k][1] = xc[1]; xin[k][2] = xc[2]; xin[k][3] = xc[3];
   }
@@ -219,20 +221,20 @@ __device__ __forceinline__ TV row_sweep(const kf_dv &d, const uint8_t *__restric
    * stage's 8 independent operations hide one another's latency; every
    * product is summed in row_dot's order (0 + m0 x0, + m1 x1, ...) */
 #pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = mp[k][0] * xin[k][0];
+  for (int k = 0; k < PF; k++) pr[k] = mp[k][0] * xin[k][0];
 #pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = TV(0) + pr[k];
+  for (int k = 0; k < PF; k++) pr[k] = TV(0) + pr[k];
 #pragma unroll
   for (int i = 1; i < 4; i++) {
-    TV t[KF_TRSV_PF];
+    TV t[PF];
 #pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++) t[k] = mp[k][i] * xin[k][i];
+    for (int k = 0; k < PF; k++) t[k] = mp[k][i] * xin[k][i];
 #pragma unroll
-    for (int k = 0; k < KF_TRSV_PF; k++) pr[k] = pr[k] + t[k];
+    for (int k = 0; k < PF; k++) pr[k] = pr[k] + t[k];
   }
 #pragma unroll
-  for (int k = 0; k < KF_TRSV_PF; k++) xr = xr - pr[k];
-  for (uint32_t j = j0 + KF_TRSV_PF; j < jend; j++) {
+  for (int k = 0; k < PF; k++) xr = xr - pr[k];
+  for (uint32_t j = j0 + PF; j < jend; j++) {
     TV m[4];
     ld_row(a, j, c, m);
     const uint32_t e = eslot[j];
@@ -250,7 +252,7 @@ __device__ __forceinline__ TV row_sweep(const kf_dv &d, const uint8_t *__restric
 
 /* forward sweep: y = L^-1 b; presets the backward output x of its rows to
  * the backward sentinel */
-template <typename TM, typename TV>
+template <int PF, typename TM, typename TV>
 __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ trow,
                                                    const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
                                                    const uint8_t *__restrict__ eslot, uint32_t ntask, uint32_t *ticket,
@@ -280,14 +282,14 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
       const bool active = r != ~0u;
       TV xr = active ? (TV)b[4 * (size_t)r + c] : TV(0);
       if (active) xnext[4 * (size_t)r + c] = sentinel<TV>::val(sx);
-      uint32_t off[KF_TRSV_PF];
-      TV mp[KF_TRSV_PF][4];
-      if (!setup_row<TM, TV>(d, eslot, a, y, sy, j0, jd, qd, c, lds, off, mp, aborted)) aborted = true;
+      uint32_t off[PF];
+      TV mp[PF][4];
+      if (!setup_row<PF, TM, TV>(d, eslot, a, y, sy, j0, jd, qd, c, lds, off, mp, aborted)) aborted = true;
       aborted = __any(aborted);   /* a wave gives up at most once per launch */
       wave_lds_sync();
       for (uint32_t l = 0; l < dep; l++) {
         if (my == l) {   /* the quad's four lanes take this branch together */
-          xr = row_sweep<TM, TV>(d, eslot, a, y, sy, j0, jd, c, off, mp, lds, xr, aborted);
+          xr = row_sweep<PF, TM, TV>(d, eslot, a, y, sy, j0, jd, c, off, mp, lds, xr, aborted);
           xr = __builtin_canonicalize(xr);
           lds[lane] = xr;
         }
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
 
 /* backward sweep: x = U^-1 y (y from the forward sweep); resets y of its rows
  * to the forward sentinel of the next call */
-template <typename TM, typename TV>
+template <int PF, typename TM, typename TV>
 __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const uint32_t *__restrict__ trow,
                                                    const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
                                                    const uint8_t *__restrict__ eslot, uint32_t ntask, uint32_t *ticket,
@@ -340,15 +342,15 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
         x0 = y[4 * (size_t)r + c];
         y[4 * (size_t)r + c] = sentinel<TV>::val(synext);
       }
-      uint32_t off[KF_TRSV_PF];
-      TV mp[KF_TRSV_PF][4];
-      if (!setup_row<TM, TV>(d, eslot, a, x, sx, j0, j1, qd, c, lds, off, mp, aborted)) aborted = true;
+      uint32_t off[PF];
+      TV mp[PF][4];
+      if (!setup_row<PF, TM, TV>(d, eslot, a, x, sx, j0, j1, qd, c, lds, off, mp, aborted)) aborted = true;
       aborted = __any(aborted);
       wave_lds_sync();
       TV res = TV(0);
       for (uint32_t l = 0; l < dep; l++) {
         if (my == l) {   /* the quad's four lanes take this branch together */
-          const TV xr = row_sweep<TM, TV>(d, eslot, a, x, sx, j0, j1, c, off, mp, lds, x0, aborted);
+          const TV xr = row_sweep<PF, TM, TV>(d, eslot, a, x, sx, j0, j1, c, off, mp, lds, x0, aborted);
           /* x_r = inv(D_r) x_r: the quad exchanges its four components */
           const TV y0 = quad_get<0>(xr), y1 = quad_get<1>(xr), y2 = quad_get<2>(xr), y3 = quad_get<3>(xr);
           res = __builtin_canonicalize(row_dot(dg, y0, y1, y2, y3));
@@ -370,17 +372,19 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
 
 /* forward output y (held between calls, preset to the next call's forward
  * sentinel by the previous backward sweep) and backward output x */
-template <typename TM, typename TV>
+template <int PF, typename TM, typename TV>
 static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslot, TV *x, double *xout)
 {
   const uint32_t G = c->trsv_group;
   /* persistent workgroups of G wavefronts: one atomic per G tickets */
   /* waves: the context's setting, else the schedule's size capped by the
-   * resident slots of this precision's kernels -- the FP64 kernels fit two
-   * waves per SIMD (2048 on 256 CUs x 4 SIMDs), the FP32 ones four, of which
-   * 3072 measured best on the 2.65M-vertex lattice (0.86 / 0.88 ms against
-   * 0.95 / 0.95 ms with 2048 and 0.88 / 0.91 ms with 4096) */
-  const uint32_t cap = sizeof(TV) == 8 ? 2048u : 3072u;
+   * resident slots of these kernels -- the FP64 ones holding 8 entries per
+   * row fit two waves per SIMD (188 VGPRs: 2048 on 256 CUs x 4 SIMDs), those
+   * holding 7 three (166 VGPRs: 3072; 0.97 / 1.06 ms on the 2.65M-vertex
+   * lattice against 1.02 / 1.10 ms with 2048), the FP32 ones four, of which
+   * 3072 measured best on that lattice (0.86 / 0.88 ms against 0.95 / 0.95 ms
+   * with 2048 and 0.88 / 0.91 ms with 4096) */
+  const uint32_t cap = (sizeof(TV) == 8 && PF == KF_TRSV_PF) ? 2048u : 3072u;
   const uint32_t W = c->trsv_waves ? c->trsv_waves : (c->trsv_auto < cap ? c->trsv_auto : cap);
   const uint32_t wl = c->ntask_l < W ? c->ntask_l : W;
   const uint32_t wu = c->ntask_u < W ? c->ntask_u : W;
@@ -393,14 +397,14 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
     KF_LAUNCH_CHECK();
   }
   int s = kf_prof_begin(c, KP_TRSV_L);
-  hipLaunchKernelGGL((k_trsv_lower<TM, TV>), dim3(nwl), dim3(64 * G), 0, c->stream, c->d, c->trow_l, c->tlvl_l,
+  hipLaunchKernelGGL((k_trsv_lower<PF, TM, TV>), dim3(nwl), dim3(64 * G), 0, c->stream, c->d, c->trow_l, c->tlvl_l,
                      c->tdep_l, c->eslot, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
                      sentinel<TV>::of(2 * k + 1), a, b, y, x);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   c->ticket_base[0] += kf_ticket_advance_groups(c->ntask_l, nwl, G);
   s = kf_prof_begin(c, KP_TRSV_U);
-  hipLaunchKernelGGL((k_trsv_upper<TM, TV>), dim3(nwu), dim3(64 * G), 0, c->stream, c->d, c->trow_u, c->tlvl_u,
+  hipLaunchKernelGGL((k_trsv_upper<PF, TM, TV>), dim3(nwu), dim3(64 * G), 0, c->stream, c->d, c->trow_u, c->tlvl_u,
                      c->tdep_u, c->eslot, c->ntask_u, c->ticket + 1, c->ticket_base[1], sentinel<TV>::of(2 * k + 1),
                      sentinel<TV>::of(2 * k + 2), a, y, x, xout);
   KF_LAUNCH_CHECK();
@@ -416,11 +420,18 @@ void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x)
   const size_t n = 4 * (size_t)c->d.N;
   if (c->opt.prec == KF_PREC_FP64) {
     if (!c->ty64) KF_CHECK(hipMalloc((void **)&c->ty64, n * sizeof(double)));
-    launch_trsv<double, double>(c, c->a64, b, c->ty64, 0, x, x);
+    if (c->trsv_pf == 7) launch_trsv<7, double, double>(c, c->a64, b, c->ty64, 0, x, x);
+    else launch_trsv<KF_TRSV_PF, double, double>(c, c->a64, b, c->ty64, 0, x, x);
     return;
   }
   if (!c->f32a) KF_CHECK(hipMalloc((void **)&c->f32a, n * sizeof(float)));
   if (!c->f32b) KF_CHECK(hipMalloc((void **)&c->f32b, n * sizeof(float)));
-  if (c->opt.prec == KF_PREC_FP32) launch_trsv<float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x);
-  else launch_trsv<uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x);
+  const bool p7 = c->trsv_pf == 7;
+  if (c->opt.prec == KF_PREC_FP32) {
+    if (p7) launch_trsv<7, float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x);
+    else launch_trsv<KF_TRSV_PF, float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x);
+  } else {
+    if (p7) launch_trsv<7, uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x);
+    else launch_trsv<KF_TRSV_PF, uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x);
+  }
 }

@@ -173,6 +173,44 @@ def test_sptrsv_other_schedules(mesh_a, rows, monkeypatch):
         m.close()
 
 
+@pytest.mark.parametrize("rows", [4, 16])
+def test_sptrsv_held_entries(mesh_c1, rows, monkeypatch):
+    """On a lattice (at most 7 entries on either side of the diagonal) the
+    sweeps hold 7 entries per row in registers; with 8 forced they give the
+    same bits, and both match the oracle."""
+    monkeypatch.setenv("KF_TRSV_ROWS", str(rows))
+    q = perturbed_state(12288, 3.0, 21)
+    b = np.random.default_rng(22).standard_normal(4 * 12288)
+    out = []
+    for held in (None, "8"):
+        if held:
+            monkeypatch.setenv("KF_TRSV_HELD", held)
+        m = K.Mesh(mesh_c1, K.options())
+        try:
+            assert m.N == 12288
+            m.set_state(q)
+            m.timestep(50.0)
+            m.fill()
+            m.ilu()
+            bd, xd = K.DeviceVec(m.sz, b), K.DeviceVec(m.sz)
+            m.sptrsv(bd, xd)
+            m.sptrsv(bd, xd)
+            assert m.last_error() == 0
+            out.append(xd.numpy())
+            if held is None:
+                orc = O.Oracle(m)
+                orc.set_state(q)
+                orc.timestep(50.0)
+                orc.fill()
+                orc.ilu()
+                xe = orc.sptrsv(b)
+                orc.close()
+        finally:
+            m.close()
+    assert _bitwise(out[0], xe), _describe(out[0], xe)
+    assert _bitwise(out[1], out[0]), _describe(out[1], out[0])
+
+
 def test_ilu_textbook(pair):
     """ilu_exact=0: textbook ILU(0), bit for bit against the oracle's textbook
     sweep, and within 1e-10 of the reference's sweep."""
