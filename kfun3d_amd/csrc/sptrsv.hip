@@ -57,9 +57,6 @@
 static_assert(KF_GROUP == 16, "one quad of lanes per row, 16 rows per task");
 
 #define KF_TRSV_GMAX 4 /* wavefronts per workgroup at most */
-
-
-
 #define KF_TRSV_PF 8   /* off-diagonal entries per row held in registers, at most */
 
 __device__ __forceinline__ double mat_ld(const double *a, size_t i) { return a[i]; }
@@ -169,6 +166,7 @@ __device__ __forceinline__ bool setup_row(const kf_dv &d, const uint8_t *__restr
                                           uint32_t j0, uint32_t jend, uint32_t qd, uint32_t c, TV *lds,
                                           uint32_t off[PF], TV mp[PF][4], bool aborted)
 {
+  static_assert(PF <= KF_TRSV_PF, "the external area holds KF_TRSV_PF operands per row");
   uint32_t cp[PF], pend = 0;
 #pragma unroll
   for (int k = 0; k < PF; k++) {
