@@ -12,6 +12,8 @@
  * that dependent vector updates need no host round trip.
  */
 #include "kf_dev.h"
+#include "sync.h"
+#include "handoff.h"
 
 #define RB 256
 
@@ -224,6 +226,256 @@ __global__ __launch_bounds__(RB) void k_mgs_step(size_t n, const double *__restr
   if (threadIdx.x == 0) part_out[blockIdx.x] = s;
 }
 
+/* A partial sum published by another workgroup of the launch: polled until
+ * it no longer holds the call's sentinel (bounded; a thread that gives up
+ * records KF_ERR_SPIN_LIMIT and stops waiting for the rest of the launch). */
+__device__ __forceinline__ double poll_part(const double *p, unsigned long long s, uint32_t *err, bool &aborted,
+                                            uint32_t sleep)
+{
+  double v = ld_coherent(p);
+  for (uint32_t k = 0; sentinel<double>::raw(v) == s && !aborted; k++) {
+    if (k >= KF_SPIN_LIMIT) {
+      atomicOr(err, KF_ERR_SPIN_LIMIT);
+      aborted = true;
+      break;
+    }
+    for (uint32_t z = 0; z < sleep; z++) __builtin_amdgcn_s_sleep(1);
+    v = ld_coherent(p);
+  }
+  return v;
+}
+
+/* The whole modified Gram-Schmidt of w against nv basis vectors in one
+ * cooperative launch, w held in registers throughout (src/kernel.c:129-141
+ * and the closing norm, src/kernel.c:138-141).
+ *
+ * The grid is k_dot_part's KF_NRED x RB threads folded two to one: thread t
+ * of workgroup b plays thread t of blocks 2b and 2b + 1 (the virtual sets
+ * 0 and 1), owns exactly the element pairs those threads walk (pairs
+ * g, g + S, g + 2S, ... with S = KF_NRED RB, and the half pair of an odd
+ * length), and keeps them in registers, at most MP pairs per set.  Step k:
+ *   the partials of <w, V_k> (of <w, w> after the last vector), summed per
+ *   set in the pair order of k_dot_part and reduced in block_sum's tree,
+ *   published to row k of the partial table (sentinel-preset per call);
+ *   h_k from that row in k_red_final's tree, formed by every workgroup;
+ *   w = (-h_k) V_k + w on the registers.
+ * So every partial, every h_k and the norm are bit-identical to the
+ * k_dot_part / k_mgs_step / k_red_final sequence.  Step k reads V_k twice
+ * (dot, then update) unless HOLDV keeps the set's V_k pairs in registers;
+ * w is read once and written once. */
+template <int MP, bool HOLDV>
+__global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const double *__restrict__ vbase, size_t vstride,
+                                                   double *__restrict__ w, double *__restrict__ hslot, double *part,
+                                                   unsigned long long sent, uint32_t *err, uint32_t sleep)
+{
+  __shared__ double sh[RB];
+  /* np = q S + r: every virtual thread owns pairs m < q, those with g < r
+   * also pair q, and the thread g == r the half pair of an odd length.
+   * Every lane walks MP >= q + (r > 0) pair slots without branching: a slot
+   * the lane does not own loads pair 0 in its place and adds +0 for it,
+   * which leaves the sum's bits unchanged (a sum that starts at +0 is never
+   * -0), and is never stored. */
+  const size_t S = (size_t)KF_NRED * RB, np = n / 2;
+  const int q = (int)(np / S);
+  const uint32_t r = (uint32_t)(np % S), PB = KF_NRED * RB * 16u;   /* bytes between a thread's pairs */
+  uint32_t voff[2];
+  bool ext[2], tail[2];
+  double wr[2][MP][2], wt[2] = {0.0, 0.0};
+#pragma unroll
+  for (int v = 0; v < 2; v++) {
+    const uint32_t g = (2 * blockIdx.x + v) * RB + threadIdx.x;
+    ext[v] = g < r;
+    tail[v] = (n & 1) && g == r;
+    voff[v] = 16u * g;
+  }
+  /* whether set v owns slot m, and the slot's byte offset: a uniform base
+   * plus one 32-bit per-lane offset */
+#define KF_ON(m, v) ((m) < q || ((m) == q && ext[v]))
+#define KF_PAIR(x, m, v)                                                                                   \
+  (*reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(x) +                                  \
+                                      (KF_ON(m, v) ? voff[v] + (uint32_t)(m) * PB : 0u)))
+#pragma unroll
+  for (int v = 0; v < 2; v++) {
+#pragma unroll
+    for (int m = 0; m < MP; m++) {
+      const double2 t = KF_PAIR(w, m, v);
+      wr[v][m][0] = t.x;
+      wr[v][m][1] = t.y;
+    }
+    if (tail[v]) wt[v] = w[n - 1];
+  }
+  bool aborted = false;
+  for (int k = 0; k <= nv; k++) {
+    /* the offsets pass through an empty asm each step, so the pair
+     * addresses are formed where they are used instead of being kept in
+     * registers across the steps */
+    asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));
+    const bool self = k == nv;
+    const double *u = vbase + (size_t)k * vstride;
+    double vr[2][HOLDV ? MP : 1][2];
+    double vt[2] = {0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      double acc = 0.0;
+      if (self) {
+#pragma unroll
+        for (int m = 0; m < MP; m++) {
+          const double p0 = wr[v][m][0] * wr[v][m][0], p1 = wr[v][m][1] * wr[v][m][1];
+          acc += KF_ON(m, v) ? p0 : 0.0;
+          acc += KF_ON(m, v) ? p1 : 0.0;
+        }
+        if (tail[v]) acc += wt[v] * wt[v];
+      } else {
+        /* chunks of four pairs: a chunk's loads are in flight together */
+#pragma unroll
+        for (int m0 = 0; m0 < MP; m0 += 4) {
+          double uu[4][2];
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (m0 + j < MP) {
+              const double2 t = KF_PAIR(u, m0 + j, v);
+              uu[j][0] = t.x;
+              uu[j][1] = t.y;
+            }
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int m = m0 + j;
+            if (m < MP) {
+              const double p0 = wr[v][m][0] * uu[j][0], p1 = wr[v][m][1] * uu[j][1];
+              acc += KF_ON(m, v) ? p0 : 0.0;
+              acc += KF_ON(m, v) ? p1 : 0.0;
+              if (HOLDV) {
+                vr[v][HOLDV ? m : 0][0] = uu[j][0];
+                vr[v][HOLDV ? m : 0][1] = uu[j][1];
+              }
+            }
+          }
+          if (!HOLDV) __builtin_amdgcn_sched_barrier(0);   /* one chunk's loads live at a time */
+        }
+        if (tail[v]) {
+          vt[v] = u[n - 1];
+          acc += wt[v] * vt[v];
+        }
+      }
+      const double s = block_sum(acc, sh);
+      if (threadIdx.x == 0) st_coherent(part + (size_t)k * KF_NRED + 2 * blockIdx.x + v, canon(s));
+    }
+    /* h_k (or |w|^2) from row k, in k_red_final's tree */
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < KF_NRED; j += RB)
+      acc += poll_part(part + (size_t)k * KF_NRED + j, sent, err, aborted, sleep);
+    const double h = block_sum(acc, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) hslot[k] = self ? sqrt(h) : h;
+    if (self) continue;   /* the closing norm: the loop ends */
+    const double a = -h;
+    asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));   /* addresses re-formed for the update */
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      if (HOLDV) {
+#pragma unroll
+        for (int m = 0; m < MP; m++) {
+          wr[v][m][0] = a * vr[v][HOLDV ? m : 0][0] + wr[v][m][0];   /* contraction off */
+          wr[v][m][1] = a * vr[v][HOLDV ? m : 0][1] + wr[v][m][1];
+        }
+      } else {
+#pragma unroll
+        for (int m0 = 0; m0 < MP; m0 += 4) {
+          double uu[4][2];
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (m0 + j < MP) {
+              const double2 t = KF_PAIR(u, m0 + j, v);
+              uu[j][0] = t.x;
+              uu[j][1] = t.y;
+            }
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int m = m0 + j;
+            if (m < MP) {
+              wr[v][m][0] = a * uu[j][0] + wr[v][m][0];
+              wr[v][m][1] = a * uu[j][1] + wr[v][m][1];
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      if (tail[v]) wt[v] = a * vt[v] + wt[v];
+    }
+  }
+  asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));
+#pragma unroll
+  for (int v = 0; v < 2; v++) {
+#pragma unroll
+    for (int m = 0; m < MP; m++)
+      if (KF_ON(m, v))
+        *reinterpret_cast<double2 *>(reinterpret_cast<char *>(w) + voff[v] + (uint32_t)m * PB) =
+            make_double2(wr[v][m][0], wr[v][m][1]);
+    if (tail[v]) w[n - 1] = wt[v];
+  }
+#undef KF_PAIR
+#undef KF_ON
+}
+
+/* the resident variants: pairs per virtual thread at most, and whether the
+ * basis vector's pairs stay in registers between the dot and the update */
+#define KF_MGS_VARIANTS 8
+static const int mgs_mp[KF_MGS_VARIANTS] = {1, 2, 3, 4, 8, 12, 16, 21};
+
+static const void *mgs_kernel(int i)
+{
+  switch (i) {
+  case 0: return (const void *)k_mgs_res<1, true>;
+  case 1: return (const void *)k_mgs_res<2, true>;
+  case 2: return (const void *)k_mgs_res<3, true>;
+  case 3: return (const void *)k_mgs_res<4, true>;
+  case 4: return (const void *)k_mgs_res<8, false>;
+  case 5: return (const void *)k_mgs_res<12, false>;
+  case 6: return (const void *)k_mgs_res<16, false>;
+  default: return (const void *)k_mgs_res<21, false>;
+  }
+}
+
+/* One cooperative launch of k_mgs_res when w and the basis are 16-byte
+ * aligned, the pairs fit a variant and its KF_NRED / 2 workgroups are
+ * co-resident on this device; false (nothing launched) otherwise. */
+static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
+{
+  if (!c->mgs_res || nv + 1 > KF_MGS_MAXV || n < 2) return false;   /* a lane without pairs reads pair 0 */
+  if (!(aligned16(vbase) && aligned16(w) && (stride & 1) == 0)) return false;
+  const size_t S = (size_t)KF_NRED * RB, np = n / 2, need = (np + S - 1) / S;   /* pairs per thread, at most */
+  /* a cooperative launch costs about 30 us more than a plain one (measured
+   * on MI355X): at 3 pairs per thread (the 144x48x48 lattice) the per-step
+   * kernels are as fast over a GMRES(30) cycle (0.776 against 0.784 s per
+   * solve), at 21 (the 288x96x96 lattice) the resident launch saves 6%
+   * (6.54 -> 6.15 s).  mgs_res = 2 takes it at every size. */
+  if (c->mgs_res == 1 && need < 4) return false;
+  int vi = 0;
+  while (vi < KF_MGS_VARIANTS && (size_t)mgs_mp[vi] < need) vi++;
+  if (vi == KF_MGS_VARIANTS) return false;
+  const void *fn = mgs_kernel(vi);
+  if (c->mgs_ok[vi] < 0) {
+    int nb = 0, dev = 0, ncu = 0, coop = 0;
+    KF_CHECK(hipGetDevice(&dev));
+    KF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    KF_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, RB, 0));
+    c->mgs_ok[vi] = (coop && (size_t)nb * (size_t)ncu >= KF_NRED / 2) ? 1 : 0;
+  }
+  if (!c->mgs_ok[vi]) return false;
+  const unsigned long long sent = sentinel<double>::of(c->mcall);
+  const size_t rows = (size_t)(nv + 1) * KF_NRED;
+  hipLaunchKernelGGL(k_fill_bits<double>, dim3(kf_blocks_for(rows, RB)), dim3(RB), 0, c->stream, c->mgs_part, rows,
+                     sent);
+  KF_LAUNCH_CHECK();
+  double *hslot = c->d.scal + slot;
+  uint32_t *err = c->d.err;
+  uint32_t sleep = c->d.spin_sleep;
+  void *args[] = {&n, &nv, &vbase, &stride, &w, &hslot, &c->mgs_part, (void *)&sent, &err, &sleep};
+  KF_CHECK(hipLaunchCooperativeKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
+  c->mcall++;
+  return true;
+}
+
 /* w orthogonalised against nv basis vectors vbase + k stride by modified
  * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv].  Index-order
  * sums (reduce_seq) keep the unfused dot / update sequence. */
@@ -237,6 +489,7 @@ void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     dot_any(c, n, w, w, slot + nv, 1);
     return;
   }
+  if (mgs_resident(c, n, vbase, stride, nv, w, slot)) return;
   double *part[2] = {c->d.red, c->d.red + KF_NRED};
   launch_dot_part(c, n, w, vbase, part[0]);
   const bool vec = aligned16(vbase) && aligned16(w) && (stride & 1) == 0;
@@ -437,6 +690,8 @@ extern "C" {
 void kfs_nrm2(GEOMETRY *g, size_t n, const double *x, int slot) { kf_nrm2_dev(kf_ctx_of(g), n, x, slot); }
 void kfs_dot(GEOMETRY *g, size_t n, const double *x, const double *y, int slot) { kf_dot_dev(kf_ctx_of(g), n, x, y, slot); }
 void kfs_fetch(GEOMETRY *g, int slot, int count, double *out) { kf_scal_fetch(kf_ctx_of(g), slot, count, out); }
+
+unsigned long long kfs_mgs_launches(GEOMETRY *g) { return kf_ctx_of(g)->mcall; }
 void kfs_axpy(GEOMETRY *g, size_t n, double a, const double *x, double *y) { kf_axpy(kf_ctx_of(g), n, a, x, y); }
 void kfs_newaxpy(GEOMETRY *g, size_t n, double a, const double *x, const double *y, double *w)
 {

@@ -42,6 +42,7 @@
 
 #define KF_NRED 1024          /* partial sums of a deterministic reduction */
 #define KF_NSCAL 256          /* device scalar slots */
+#define KF_MGS_MAXV 64        /* basis vectors + 1 of one resident MGS launch, at most */
 #define KF_GROUP 16           /* rows per triangular-solve group */
 
 /* device error word bits */
@@ -121,6 +122,13 @@ struct kf_ctx {
   uint32_t ntask_l, ntask_u;
   double *ty64;             /* forward-sweep output (FP64) */
   uint64_t tcall;           /* triangular solves so far (sentinel numbering) */
+  /* modified Gram-Schmidt with w resident in registers (one cooperative
+   * launch per orthogonalisation): per-step partial sums handed over
+   * through sentinel slots, KF_MGS_MAXV rows of KF_NRED */
+  double *mgs_part;
+  uint64_t mcall;            /* resident MGS launches so far (sentinel numbering) */
+  int mgs_res;               /* 0: never, 1: from 4 pairs per thread, 2: wherever it fits */
+  int8_t mgs_ok[8];          /* per register-size variant: -1 unknown, 0 no, 1 co-resident */
   uint64_t ystate[2];       /* call whose forward sentinel y holds (FP64, FP32) */
   uint32_t *ticket;         /* monotonically increasing work tickets */
   uint32_t ticket_base[4];

@@ -117,6 +117,8 @@ void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y);
 /* modified Gram-Schmidt of w against nv vectors vbase + k stride: h_k in
  * slots slot..slot+nv-1, |w| in slot+nv (src/kernel.c:129-141) */
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot);
+/* resident (one-launch) orthogonalisations so far on this context */
+unsigned long long kfs_mgs_launches(GEOMETRY *g);
 void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w);
 void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int slot, double *y);
 void kfs_zero(GEOMETRY *g, size_t n, double *x);

@@ -1,0 +1,145 @@
+"""Modified Gram-Schmidt in one cooperative launch with w held in registers,
+against the per-step kernels (src/kernel.c:129-141 and the closing norm):
+every h_k, |w| and the orthogonalised w bit for bit, for vector lengths that
+give every register-size variant, odd lengths (the half pair) and lengths
+shorter than one pass of the reduction grid."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import kfun3d_amd as K
+
+pytestmark = pytest.mark.gpu
+
+S = 1024 * 256   # element pairs one pass of the reduction grid walks
+SLOT = 100       # device scalar slots used here (h_0 .. h_nv)
+
+
+def _fns():
+    L = K.lib()
+    mgs = L.kfs_mgs
+    mgs.restype = None
+    mgs.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int]
+    fetch = L.kfs_fetch
+    fetch.restype = None
+    fetch.argtypes = [K.GP, C.c_int, C.c_int, C.c_void_p]
+    launches = L.kfs_mgs_launches
+    launches.restype = C.c_ulonglong
+    launches.argtypes = [K.GP]
+    return mgs, fetch, launches
+
+
+@pytest.fixture(scope="module")
+def contexts(mesh_a):
+    """Two contexts on mesh A: per-step kernels, and the resident launch
+    taken at every size (by default only from 4 pairs per thread)."""
+    import os
+    old = os.environ.get("KF_MGS_RESIDENT")
+    out = {}
+    try:
+        for name, val in (("step", "0"), ("resident", "2")):   # 2: at every size
+            os.environ["KF_MGS_RESIDENT"] = val
+            out[name] = K.Mesh(mesh_a, K.options())
+    finally:
+        if old is None:
+            os.environ.pop("KF_MGS_RESIDENT", None)
+        else:
+            os.environ["KF_MGS_RESIDENT"] = old
+    yield out
+    for m in out.values():
+        m.close()
+
+
+CASES = [
+    (11200, 5),                      # mesh A's 4N: one pair per thread at most, most threads idle
+    (1327104, 30),                   # the 144x48x48 lattice's 4N, a full GMRES(30) cycle's last step
+    (2 * (2 * S + 777) + 1, 3),      # odd length: the half pair
+    (2 * 4 * S, 7),                  # exactly four pairs per thread
+    (2 * (7 * S + 100), 4),          # eight pair slots
+    (2 * (11 * S + 4) + 1, 2),       # twelve, odd
+    (10616832, 6),                   # the 288x96x96 lattice's 4N: 21 pair slots
+]
+
+
+@pytest.mark.parametrize("n,nv", CASES)
+def test_resident_mgs_bitwise(contexts, n, nv):
+    mgs, fetch, launches = _fns()
+    rng = np.random.default_rng(n % 1000 + nv)
+    stride = n + (n & 1) + 2   # even, so every basis vector stays 16-byte aligned
+    Q = np.linalg.qr(rng.standard_normal((n, nv)))[0]   # an orthonormal basis, as GMRES holds
+    V = np.zeros((nv, stride))
+    V[:, :n] = Q.T
+    V = V.ravel()
+    w0 = rng.standard_normal(n)
+    vb = K.DeviceVec(nv * stride, V)
+    res = {}
+    for name in ("step", "resident"):
+        m = contexts[name]
+        before = launches(m.ref)
+        w = K.DeviceVec(n, w0)
+        mgs(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT)
+        h = np.empty(nv + 1)
+        fetch(m.ref, SLOT, nv + 1, h.ctypes.data)
+        assert m.last_error() == 0
+        res[name] = (h, w.numpy(), launches(m.ref) - before)
+        w.free()
+    vb.free()
+    (hs, ws, ls), (hr, wr, lr) = res["step"], res["resident"]
+    assert ls == 0 and lr == 1, "the resident launch did not run (%d, %d)" % (ls, lr)
+    assert np.array_equal(hs.view(np.uint64), hr.view(np.uint64)), (hs, hr)
+    assert np.array_equal(ws.view(np.uint64), wr.view(np.uint64))
+    # and the result is an orthogonalisation: w is orthogonal to the basis to
+    # rounding, and |w| is the stored norm
+    Vm = V.reshape(nv, stride)[:, :n]
+    assert np.max(np.abs(Vm @ wr)) < 1e-12 * np.linalg.norm(w0)
+    assert abs(np.linalg.norm(wr) - hr[nv]) <= 1e-12 * hr[nv]
+
+
+def test_resident_mgs_repeated(contexts):
+    """Consecutive launches on one context use fresh sentinels: the same
+    input gives the same bits every time."""
+    mgs, fetch, launches = _fns()
+    n, nv = 1327104, 4
+    rng = np.random.default_rng(5)
+    V = rng.standard_normal(nv * n)
+    w0 = rng.standard_normal(n)
+    vb = K.DeviceVec(nv * n, V)
+    m = contexts["resident"]
+    outs = []
+    for _ in range(3):
+        w = K.DeviceVec(n, w0)
+        mgs(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT)
+        h = np.empty(nv + 1)
+        fetch(m.ref, SLOT, nv + 1, h.ctypes.data)
+        outs.append((h, w.numpy()))
+        w.free()
+    vb.free()
+    assert m.last_error() == 0
+    for h, w in outs[1:]:
+        assert np.array_equal(h.view(np.uint64), outs[0][0].view(np.uint64))
+        assert np.array_equal(w.view(np.uint64), outs[0][1].view(np.uint64))
+
+
+def test_resident_mgs_default_sizes(mesh_a, monkeypatch):
+    """By default the resident launch is taken from 4 pairs per thread on:
+    not for the 144x48x48 lattice's vectors, yes for the 288x96x96 one's."""
+    monkeypatch.delenv("KF_MGS_RESIDENT", raising=False)
+    mgs, fetch, launches = _fns()
+    m = K.Mesh(mesh_a, K.options())
+    try:
+        for n, taken in ((1327104, 0), (10616832, 1)):
+            nv = 2
+            rng = np.random.default_rng(7)
+            vb = K.DeviceVec(nv * n, rng.standard_normal(nv * n))
+            w = K.DeviceVec(n, rng.standard_normal(n))
+            before = launches(m.ref)
+            mgs(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT)
+            h = np.empty(nv + 1)
+            fetch(m.ref, SLOT, nv + 1, h.ctypes.data)
+            assert launches(m.ref) - before == taken, n
+            w.free()
+            vb.free()
+        assert m.last_error() == 0
+    finally:
+        m.close()
