@@ -260,15 +260,20 @@ __device__ __forceinline__ double poll_part(const double *p, unsigned long long 
  *   h_k from that row in k_red_final's tree, formed by every workgroup;
  *   w = (-h_k) V_k + w on the registers.
  * So every partial, every h_k and the norm are bit-identical to the
- * k_dot_part / k_mgs_step / k_red_final sequence.  Step k reads V_k twice
- * (dot, then update) unless HOLDV keeps the set's V_k pairs in registers;
- * w is read once and written once. */
+ * k_dot_part / k_mgs_step / k_red_final sequence.  The update reuses the
+ * V_k values the dot loaded: from registers (HOLDV), else from LDS for the
+ * first NL slots and from memory again for the rest; w is read once and
+ * written once. */
 template <int MP, bool HOLDV>
 __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const double *__restrict__ vbase, size_t vstride,
                                                    double *__restrict__ w, double *__restrict__ hslot, double *part,
                                                    unsigned long long sent, uint32_t *err, uint32_t sleep)
 {
+  /* without HOLDV the first NL slots of V_k are kept in LDS between the dot
+   * and the update (72 KB for NL = 9: two workgroups still fit a CU) */
+  constexpr int NL = HOLDV ? 0 : (MP < 9 ? MP : 9);
   __shared__ double sh[RB];
+  __shared__ double2 vl[2][NL > 0 ? NL : 1][RB];
   /* np = q S + r: every virtual thread owns pairs m < q, those with g < r
    * also pair q, and the thread g == r the half pair of an odd length.
    * Every lane walks MP >= q + (r > 0) pair slots without branching: a slot
@@ -347,6 +352,8 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
               if (HOLDV) {
                 vr[v][HOLDV ? m : 0][0] = uu[j][0];
                 vr[v][HOLDV ? m : 0][1] = uu[j][1];
+              } else if (m < NL) {
+                vl[v][m < NL ? m : 0][threadIdx.x] = make_double2(uu[j][0], uu[j][1]);
               }
             }
           }
@@ -384,7 +391,8 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
 #pragma unroll
           for (int j = 0; j < 4; j++)
             if (m0 + j < MP) {
-              const double2 t = KF_PAIR(u, m0 + j, v);
+              /* the same values the dot read: from LDS, else again from memory */
+              const double2 t = m0 + j < NL ? vl[v][m0 + j < NL ? m0 + j : 0][threadIdx.x] : KF_PAIR(u, m0 + j, v);
               uu[j][0] = t.x;
               uu[j][1] = t.y;
             }
