@@ -267,7 +267,7 @@ __device__ __forceinline__ double poll_part(const double *p, unsigned long long 
 template <int MP, bool HOLDV>
 __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const double *__restrict__ vbase, size_t vstride,
                                                    double *__restrict__ w, double *__restrict__ hslot, double *part,
-                                                   unsigned long long sent, uint32_t *err, uint32_t sleep)
+                                                   unsigned long long sent, uint32_t *err, uint32_t sleep, int unit)
 {
   /* without HOLDV the first NL slots of V_k are kept in LDS between the dot
    * and the update (72 KB for NL = 9: two workgroups still fit a CU) */
@@ -310,6 +310,7 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
     if (tail[v]) wt[v] = w[n - 1];
   }
   bool aborted = false;
+  double nrm2 = 0.0;
   for (int k = 0; k <= nv; k++) {
     /* the offsets pass through an empty asm each step, so the pair
      * addresses are formed where they are used instead of being kept in
@@ -373,7 +374,10 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
       acc += poll_part(part + (size_t)k * KF_NRED + j, sent, err, aborted, sleep);
     const double h = block_sum(acc, sh);
     if (blockIdx.x == 0 && threadIdx.x == 0) hslot[k] = self ? sqrt(h) : h;
-    if (self) continue;   /* the closing norm: the loop ends */
+    if (self) {   /* the closing norm: the loop ends */
+      nrm2 = h;
+      continue;
+    }
     const double a = -h;
     asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));   /* addresses re-formed for the update */
 #pragma unroll
@@ -408,6 +412,19 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
         }
       }
       if (tail[v]) wt[v] = a * vt[v] + wt[v];
+    }
+  }
+  /* unit: w scaled to unit length as k_scale_recip does, x *= 1.0 / |w| */
+  if (unit) {
+    const double f = 1.0 / sqrt(nrm2);
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+#pragma unroll
+      for (int m = 0; m < MP; m++) {
+        wr[v][m][0] *= f;
+        wr[v][m][1] *= f;
+      }
+      wt[v] *= f;
     }
   }
   asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));
@@ -446,7 +463,8 @@ static const void *mgs_kernel(int i)
 /* One cooperative launch of k_mgs_res when w and the basis are 16-byte
  * aligned, the pairs fit a variant and its KF_NRED / 2 workgroups are
  * co-resident on this device; false (nothing launched) otherwise. */
-static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
+static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
+                         int unit)
 {
   if (!c->mgs_res || nv + 1 > KF_MGS_MAXV || n < 2) return false;   /* a lane without pairs reads pair 0 */
   if (!(aligned16(vbase) && aligned16(w) && (stride & 1) == 0)) return false;
@@ -478,16 +496,17 @@ static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride
   double *hslot = c->d.scal + slot;
   uint32_t *err = c->d.err;
   uint32_t sleep = c->d.spin_sleep;
-  void *args[] = {&n, &nv, &vbase, &stride, &w, &hslot, &c->mgs_part, (void *)&sent, &err, &sleep};
+  void *args[] = {&n, &nv, &vbase, &stride, &w, &hslot, &c->mgs_part, (void *)&sent, &err, &sleep, &unit};
   KF_CHECK(hipLaunchCooperativeKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
   c->mcall++;
   return true;
 }
 
 /* w orthogonalised against nv basis vectors vbase + k stride by modified
- * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv].  Index-order
- * sums (reduce_seq) keep the unfused dot / update sequence. */
-void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
+ * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv]; unit: w then
+ * scaled to unit length (x *= 1.0 / |w|, kf_scale_recip).  Index-order sums
+ * (reduce_seq) keep the unfused dot / update sequence. */
+void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit)
 {
   if (c->opt.reduce_seq) {
     for (int k = 0; k < nv; k++) {
@@ -495,9 +514,10 @@ void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
       kf_axpy_slot(c, n, slot + k, vbase + (size_t)k * stride, w);
     }
     dot_any(c, n, w, w, slot + nv, 1);
+    if (unit) kf_scale_recip(c, n, w, slot + nv);
     return;
   }
-  if (mgs_resident(c, n, vbase, stride, nv, w, slot)) return;
+  if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit)) return;
   double *part[2] = {c->d.red, c->d.red + KF_NRED};
   launch_dot_part(c, n, w, vbase, part[0]);
   const bool vec = aligned16(vbase) && aligned16(w) && (stride & 1) == 0;
@@ -513,6 +533,7 @@ void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
   }
   hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, part[nv & 1], c->d.scal + slot + nv, 1);
   KF_LAUNCH_CHECK();
+  if (unit) kf_scale_recip(c, n, w, slot + nv);
 }
 
 void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out)
@@ -709,7 +730,12 @@ void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot) { kf_scale_reci
 void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y) { kf_axpy_slot(kf_ctx_of(g), n, slot, x, y); }
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
 {
-  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot);
+  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 0);
+}
+
+void kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
+{
+  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 1);
 }
 void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w) { kf_fd_perturb(kf_ctx_of(g), x, h, slot, w); }
 void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int slot, double *y)

@@ -143,3 +143,39 @@ def test_resident_mgs_default_sizes(mesh_a, monkeypatch):
         assert m.last_error() == 0
     finally:
         m.close()
+
+
+@pytest.mark.parametrize("n,nv", [(1327104, 3), (2 * (2 * S + 777) + 1, 2), (10616832, 4)])
+def test_resident_mgs_unit_bitwise(contexts, n, nv):
+    """The GMRES form (kfs_mgs_unit: then w *= 1.0 / |w|): scaled inside the
+    resident launch, by k_scale_recip after the per-step kernels; the same
+    bits either way, and |w| = 1 to rounding."""
+    L = K.lib()
+    unit = L.kfs_mgs_unit
+    unit.restype = None
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int]
+    _, fetch, launches = _fns()
+    rng = np.random.default_rng(n % 997 + nv)
+    stride = n + (n & 1) + 2
+    Q = np.linalg.qr(rng.standard_normal((n, nv)))[0]
+    V = np.zeros((nv, stride))
+    V[:, :n] = Q.T
+    vb = K.DeviceVec(nv * stride, V.ravel())
+    w0 = rng.standard_normal(n)
+    res = {}
+    for name in ("step", "resident"):
+        m = contexts[name]
+        before = launches(m.ref)
+        w = K.DeviceVec(n, w0)
+        unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT)
+        h = np.empty(nv + 1)
+        fetch(m.ref, SLOT, nv + 1, h.ctypes.data)
+        assert m.last_error() == 0
+        res[name] = (h, w.numpy(), launches(m.ref) - before)
+        w.free()
+    vb.free()
+    (hs, ws, ls), (hr, wr, lr) = res["step"], res["resident"]
+    assert ls == 0 and lr == 1
+    assert np.array_equal(hs.view(np.uint64), hr.view(np.uint64))
+    assert np.array_equal(ws.view(np.uint64), wr.view(np.uint64))
+    assert abs(np.linalg.norm(wr) - 1.0) < 1e-12
