@@ -179,3 +179,29 @@ def test_resident_mgs_unit_bitwise(contexts, n, nv):
     assert np.array_equal(hs.view(np.uint64), hr.view(np.uint64))
     assert np.array_equal(ws.view(np.uint64), wr.view(np.uint64))
     assert abs(np.linalg.norm(wr) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(48, 16, 16, 0), (144, 48, 48, 1)])
+def test_solve_history_with_resident_mgs(mesh_dir, dims, monkeypatch):
+    """Whole solves (the 48x16x16 and 144x48x48 lattices, to a 1e-10 drop)
+    with the resident launch taken for every orthogonalisation have the
+    history of the per-step kernels bit for bit: every step's CFL, residual
+    norm, Krylov count and forces."""
+    from conftest import lattice
+    path = lattice(mesh_dir, *dims)
+    hist = {}
+    for val in ("0", "2"):
+        monkeypatch.setenv("KF_MGS_RESIDENT", val)
+        m = K.Mesh(path, K.config_options())
+        try:
+            hist[val] = m.kernel()
+            assert m.last_error() == 0
+        finally:
+            m.close()
+    a, b = hist["0"], hist["2"]
+    assert len(a) == len(b) and len(a) > 5
+    for ra, rb in zip(a, b):
+        assert ra["krylov"] == rb["krylov"], (ra, rb)
+        fa = np.array([ra["cfl"], ra["fnorm"]] + ra["forces"])
+        fb = np.array([rb["cfl"], rb["fnorm"]] + rb["forces"])
+        assert np.array_equal(fa.view(np.uint64), fb.view(np.uint64)), (ra, rb)
