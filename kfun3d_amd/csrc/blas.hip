@@ -267,7 +267,8 @@ __device__ __forceinline__ double poll_part(const double *p, unsigned long long 
 template <int MP, bool HOLDV>
 __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const double *__restrict__ vbase, size_t vstride,
                                                    double *__restrict__ w, double *__restrict__ hslot, double *part,
-                                                   unsigned long long sent, uint32_t *err, uint32_t sleep, int unit)
+                                                   unsigned long long sent, uint32_t *err, uint32_t sleep, int unit,
+                                                   uint32_t *sync, int force_abort)
 {
   /* without HOLDV the first NL slots of V_k are kept in LDS between the dot
    * and the update (72 KB for NL = 9: two workgroups still fit a CU) */
@@ -280,6 +281,38 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
    * the lane does not own loads pair 0 in its place and adds +0 for it,
    * which leaves the sum's bits unchanged (a sum that starts at +0 is never
    * -0), and is never stored. */
+  /* Co-residency, decided once per launch: sync[0] counts arrivals, sync[1]
+   * goes from 0 to 1 (go: the last workgroup arrived) or to 2 (abort: a
+   * workgroup waited KF_MGS_ARRIVE polls first), by compare-and-swap so that
+   * every workgroup follows the same decision.  On abort nothing is written
+   * but the flag hslot[nv + 1] = 1 (the host then repeats the call with the
+   * per-step kernels); on go block 0 leaves hslot[nv + 1] = 0. */
+  __shared__ uint32_t decision;
+  if (threadIdx.x == 0) {
+    uint32_t st = 0;
+    if (force_abort) {
+      atomicCAS(sync + 1, 0u, 2u);
+      st = 2;
+    } else if (atomicAdd(sync, 1u) + 1 == gridDim.x) {
+      const uint32_t old = atomicCAS(sync + 1, 0u, 1u);
+      st = old == 0 ? 1u : old;
+    } else {
+      st = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (uint32_t k = 0; st == 0; k++) {
+        if (k >= KF_MGS_ARRIVE) {
+          const uint32_t old = atomicCAS(sync + 1, 0u, 2u);
+          st = old == 0 ? 2u : old;
+          break;
+        }
+        for (uint32_t z = 0; z < sleep; z++) __builtin_amdgcn_s_sleep(1);
+        st = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (st != 1) hslot[nv + 1] = 1.0;
+    decision = st;
+  }
+  __syncthreads();
+  if (decision != 1) return;   /* uniform over the workgroup */
   const size_t S = (size_t)KF_NRED * RB, np = n / 2;
   const int q = (int)(np / S);
   const uint32_t r = (uint32_t)(np % S), PB = KF_NRED * RB * 16u;   /* bytes between a thread's pairs */
@@ -437,8 +470,18 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
             make_double2(wr[v][m][0], wr[v][m][1]);
     if (tail[v]) w[n - 1] = wt[v];
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) hslot[nv + 1] = 0.0;
 #undef KF_PAIR
 #undef KF_ON
+}
+
+/* the partial table of a resident launch preset to the call's sentinel, and
+ * its arrival counter and decision cleared */
+__global__ void k_mgs_preset(double *part, size_t rows, unsigned long long sent, uint32_t *sync)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rows) part[i] = sentinel<double>::val(sent);
+  if (i < 2) sync[i] = 0;
 }
 
 /* the resident variants: pairs per virtual thread at most, and whether the
@@ -460,64 +503,51 @@ static const void *mgs_kernel(int i)
   }
 }
 
-/* One cooperative launch of k_mgs_res when w and the basis are 16-byte
- * aligned, the pairs fit a variant and its KF_NRED / 2 workgroups are
- * co-resident on this device; false (nothing launched) otherwise. */
+/* One launch of k_mgs_res when w and the basis are 16-byte aligned, the
+ * pairs fit a variant and the occupancy of its KF_NRED / 2 workgroups allows
+ * them all on this device at once; false (nothing launched) otherwise.  The
+ * launch is an ordinary one -- co-residency is checked by the kernel itself,
+ * which writes the abort flag to scal[slot + nv + 1] -- because a
+ * cooperative launch cost about 30 us more per call on MI355X. */
 static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
                          int unit)
 {
-  if (!c->mgs_res || nv + 1 > KF_MGS_MAXV || n < 2) return false;   /* a lane without pairs reads pair 0 */
+  if (!c->mgs_res || nv + 2 > KF_MGS_MAXV || n < 2) return false;   /* a lane without pairs reads pair 0 */
+  if (slot + nv + 1 >= KF_NSCAL) return false;
   if (!(aligned16(vbase) && aligned16(w) && (stride & 1) == 0)) return false;
   const size_t S = (size_t)KF_NRED * RB, np = n / 2, need = (np + S - 1) / S;   /* pairs per thread, at most */
-  /* a cooperative launch costs about 30 us more than a plain one (measured
-   * on MI355X): at 3 pairs per thread (the 144x48x48 lattice) the per-step
-   * kernels are as fast over a GMRES(30) cycle (0.776 against 0.784 s per
-   * solve), at 21 (the 288x96x96 lattice) the resident launch saves 6%
-   * (6.54 -> 6.15 s).  mgs_res = 2 takes it at every size. */
-  if (c->mgs_res == 1 && need < 4) return false;
   int vi = 0;
   while (vi < KF_MGS_VARIANTS && (size_t)mgs_mp[vi] < need) vi++;
   if (vi == KF_MGS_VARIANTS) return false;
   const void *fn = mgs_kernel(vi);
   if (c->mgs_ok[vi] < 0) {
-    int nb = 0, dev = 0, ncu = 0, coop = 0;
+    int nb = 0, dev = 0, ncu = 0;
     KF_CHECK(hipGetDevice(&dev));
     KF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    KF_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
     KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, RB, 0));
-    c->mgs_ok[vi] = (coop && (size_t)nb * (size_t)ncu >= KF_NRED / 2) ? 1 : 0;
+    c->mgs_ok[vi] = ((size_t)nb * (size_t)ncu >= KF_NRED / 2) ? 1 : 0;
   }
   if (!c->mgs_ok[vi]) return false;
   const unsigned long long sent = sentinel<double>::of(c->mcall);
   const size_t rows = (size_t)(nv + 1) * KF_NRED;
-  hipLaunchKernelGGL(k_fill_bits<double>, dim3(kf_blocks_for(rows, RB)), dim3(RB), 0, c->stream, c->mgs_part, rows,
-                     sent);
+  hipLaunchKernelGGL(k_mgs_preset, dim3(kf_blocks_for(rows, RB)), dim3(RB), 0, c->stream, c->mgs_part, rows, sent,
+                     c->mgs_sync);
   KF_LAUNCH_CHECK();
   double *hslot = c->d.scal + slot;
   uint32_t *err = c->d.err;
   uint32_t sleep = c->d.spin_sleep;
-  void *args[] = {&n, &nv, &vbase, &stride, &w, &hslot, &c->mgs_part, (void *)&sent, &err, &sleep, &unit};
-  KF_CHECK(hipLaunchCooperativeKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
+  const int force_abort = c->mgs_res == 3;
+  void *args[] = {&n,     &nv,  &vbase, &stride, &w,           &hslot, &c->mgs_part, (void *)&sent,
+                  &err,   &sleep, &unit, &c->mgs_sync, (void *)&force_abort};
+  KF_CHECK(hipLaunchKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
   c->mcall++;
   return true;
 }
 
-/* w orthogonalised against nv basis vectors vbase + k stride by modified
- * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv]; unit: w then
- * scaled to unit length (x *= 1.0 / |w|, kf_scale_recip).  Index-order sums
- * (reduce_seq) keep the unfused dot / update sequence. */
-void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit)
+/* the per-step modified Gram-Schmidt (k_dot_part, k_mgs_step per basis
+ * vector, k_red_final), then the unit scaling */
+static void mgs_steps(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit)
 {
-  if (c->opt.reduce_seq) {
-    for (int k = 0; k < nv; k++) {
-      dot_any(c, n, w, vbase + (size_t)k * stride, slot + k, 0);
-      kf_axpy_slot(c, n, slot + k, vbase + (size_t)k * stride, w);
-    }
-    dot_any(c, n, w, w, slot + nv, 1);
-    if (unit) kf_scale_recip(c, n, w, slot + nv);
-    return;
-  }
-  if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit)) return;
   double *part[2] = {c->d.red, c->d.red + KF_NRED};
   launch_dot_part(c, n, w, vbase, part[0]);
   const bool vec = aligned16(vbase) && aligned16(w) && (stride & 1) == 0;
@@ -534,6 +564,41 @@ void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
   hipLaunchKernelGGL(k_red_final, dim3(1), dim3(RB), 0, c->stream, part[nv & 1], c->d.scal + slot + nv, 1);
   KF_LAUNCH_CHECK();
   if (unit) kf_scale_recip(c, n, w, slot + nv);
+}
+
+/* w orthogonalised against nv basis vectors vbase + k stride by modified
+ * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv]; unit: w then
+ * scaled to unit length (x *= 1.0 / |w|, kf_scale_recip); out (if not NULL):
+ * the nv + 1 values fetched to the host.  A resident launch is followed by
+ * that fetch (one synchronisation, which the caller's fetch would cost
+ * anyway); if it aborted, w is untouched and the call is repeated with the
+ * per-step kernels, which the context then keeps.  Index-order sums
+ * (reduce_seq) keep the unfused dot / update sequence. */
+void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
+            double *out)
+{
+  if (c->opt.reduce_seq) {
+    for (int k = 0; k < nv; k++) {
+      dot_any(c, n, w, vbase + (size_t)k * stride, slot + k, 0);
+      kf_axpy_slot(c, n, slot + k, vbase + (size_t)k * stride, w);
+    }
+    dot_any(c, n, w, w, slot + nv, 1);
+    if (unit) kf_scale_recip(c, n, w, slot + nv);
+  } else if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit)) {
+    double tmp[KF_MGS_MAXV + 1];
+    kf_scal_fetch(c, slot, nv + 2, tmp);
+    if (tmp[nv + 1] == 0.0) {
+      if (out)
+        for (int i = 0; i <= nv; i++) out[i] = tmp[i];
+      return;
+    }
+    c->mgs_res = 0;   /* not co-resident here: the per-step kernels from now on */
+    c->mgs_aborts++;
+    mgs_steps(c, n, vbase, stride, nv, w, slot, unit);
+  } else {
+    mgs_steps(c, n, vbase, stride, nv, w, slot, unit);
+  }
+  if (out) kf_scal_fetch(c, slot, nv + 1, out);
 }
 
 void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out)
@@ -730,13 +795,16 @@ void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot) { kf_scale_reci
 void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y) { kf_axpy_slot(kf_ctx_of(g), n, slot, x, y); }
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
 {
-  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 0);
+  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 0, NULL);
 }
 
-void kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
+void kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
+                  double *out)
 {
-  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 1);
+  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 1, out);
 }
+
+unsigned long long kfs_mgs_aborts(GEOMETRY *g) { return kf_ctx_of(g)->mgs_aborts; }
 void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w) { kf_fd_perturb(kf_ctx_of(g), x, h, slot, w); }
 void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int slot, double *y)
 {

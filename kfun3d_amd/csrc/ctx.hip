@@ -221,8 +221,10 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   d.eres = dalloc<double>(4 * E + 8);
   d.red = dalloc<double>(2 * KF_NRED);   /* two sets of partial sums (fused MGS) */
   c->mgs_part = dalloc<double>((size_t)KF_MGS_MAXV * KF_NRED);
+  c->mgs_sync = dalloc<uint32_t>(2);
   c->mgs_res = (ev = getenv("KF_MGS_RESIDENT")) ? atoi(ev) : 1;
-  if (c->mgs_res < 0 || c->mgs_res > 2) c->mgs_res = 1;
+  if (c->mgs_res == 2) c->mgs_res = 1;
+  if (c->mgs_res < 0 || c->mgs_res > 3) c->mgs_res = 1;
   for (int i = 0; i < 8; i++) c->mgs_ok[i] = -1;
   d.scal = dalloc<double>(KF_NSCAL);
   KF_CHECK(hipMemset(d.scal, 0, KF_NSCAL * sizeof(double)));
@@ -262,7 +264,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->rdone,
                   c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
-                  c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr, c->mgs_part};
+                  c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));
   KF_CHECK(hipStreamDestroy(c->stream));

@@ -43,6 +43,7 @@
 #define KF_NRED 1024          /* partial sums of a deterministic reduction */
 #define KF_NSCAL 256          /* device scalar slots */
 #define KF_MGS_MAXV 64        /* basis vectors + 1 of one resident MGS launch, at most */
+#define KF_MGS_ARRIVE 4096    /* polls a resident MGS workgroup waits for the others to arrive */
 #define KF_GROUP 16           /* rows per triangular-solve group */
 
 /* device error word bits */
@@ -127,7 +128,9 @@ struct kf_ctx {
    * through sentinel slots, KF_MGS_MAXV rows of KF_NRED */
   double *mgs_part;
   uint64_t mcall;            /* resident MGS launches so far (sentinel numbering) */
-  int mgs_res;               /* 0: never, 1: from 4 pairs per thread, 2: wherever it fits */
+  uint32_t *mgs_sync;        /* arrival counter and go/abort decision of the launch */
+  uint64_t mgs_aborts;       /* resident launches that found their workgroups not co-resident */
+  int mgs_res;               /* 0: never, 1: wherever it fits, 3: always abort (tests the fallback) */
   int8_t mgs_ok[8];          /* per register-size variant: -1 unknown, 0 no, 1 co-resident */
   uint64_t ystate[2];       /* call whose forward sentinel y holds (FP64, FP32) */
   uint32_t *ticket;         /* monotonically increasing work tickets */
@@ -204,7 +207,8 @@ void kf_axpy(kf_ctx *c, size_t n, double a, const double *x, double *y);
 void kf_newaxpy(kf_ctx *c, size_t n, double a, const double *x, const double *y, double *w);
 void kf_scale_recip(kf_ctx *c, size_t n, double *x, int slot);
 void kf_axpy_slot(kf_ctx *c, size_t n, int slot, const double *x, double *y);
-void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit);
+void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
+            double *out);
 void kf_fd_perturb(kf_ctx *c, const double *x, double h, int slot_xnorm, double *w);
 void kf_fd_combine(kf_ctx *c, const double *r, const double *w, double h, int slot_xnorm, double *y);
 void kf_fill_zero(kf_ctx *c, size_t n, double *x);

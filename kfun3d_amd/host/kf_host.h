@@ -117,8 +117,12 @@ void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y);
 /* modified Gram-Schmidt of w against nv vectors vbase + k stride: h_k in
  * slots slot..slot+nv-1, |w| in slot+nv (src/kernel.c:129-141) */
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot);
-/* kfs_mgs, then w scaled to unit length (x *= 1.0 / |w|, as kfs_scale_recip) */
-void kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot);
+/* kfs_mgs, then w scaled to unit length (x *= 1.0 / |w|, as kfs_scale_recip);
+ * out (if not NULL) receives h_0 .. h_{nv-1}, |w| */
+void kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
+                  double *out);
+/* resident launches that aborted (workgroups not co-resident) on this context */
+unsigned long long kfs_mgs_aborts(GEOMETRY *g);
 /* resident (one-launch) orthogonalisations so far on this context */
 unsigned long long kfs_mgs_launches(GEOMETRY *g);
 void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w);

@@ -214,8 +214,7 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
         hh[it + 1] = h2[it + 1];
       } else {
         /* modified Gram-Schmidt, src/kernel.c:129-141 */
-        kfs_mgs_unit(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H);   /* then v = w / |w| */
-        kfs_fetch(g, KFS_SLOT_H, it + 2, hh);
+        kfs_mgs_unit(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H, hh);   /* then v = w / |w| */
       }
       /* Givens, src/kernel.c:143-173 */
       for (int i = 0; i < it; i++) {

@@ -32,8 +32,7 @@ def _fns():
 
 @pytest.fixture(scope="module")
 def contexts(mesh_a):
-    """Two contexts on mesh A: per-step kernels, and the resident launch
-    taken at every size (by default only from 4 pairs per thread)."""
+    """Two contexts on mesh A: per-step kernels, and the resident launch."""
     import os
     old = os.environ.get("KF_MGS_RESIDENT")
     out = {}
@@ -122,13 +121,13 @@ def test_resident_mgs_repeated(contexts):
 
 
 def test_resident_mgs_default_sizes(mesh_a, monkeypatch):
-    """By default the resident launch is taken from 4 pairs per thread on:
-    not for the 144x48x48 lattice's vectors, yes for the 288x96x96 one's."""
+    """By default the resident launch is taken wherever it fits: for the
+    144x48x48 and the 288x96x96 lattices' vectors alike."""
     monkeypatch.delenv("KF_MGS_RESIDENT", raising=False)
     mgs, fetch, launches = _fns()
     m = K.Mesh(mesh_a, K.options())
     try:
-        for n, taken in ((1327104, 0), (10616832, 1)):
+        for n, taken in ((1327104, 1), (10616832, 1)):
             nv = 2
             rng = np.random.default_rng(7)
             vb = K.DeviceVec(nv * n, rng.standard_normal(nv * n))
@@ -153,7 +152,7 @@ def test_resident_mgs_unit_bitwise(contexts, n, nv):
     L = K.lib()
     unit = L.kfs_mgs_unit
     unit.restype = None
-    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int]
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
     _, fetch, launches = _fns()
     rng = np.random.default_rng(n % 997 + nv)
     stride = n + (n & 1) + 2
@@ -167,9 +166,8 @@ def test_resident_mgs_unit_bitwise(contexts, n, nv):
         m = contexts[name]
         before = launches(m.ref)
         w = K.DeviceVec(n, w0)
-        unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT)
         h = np.empty(nv + 1)
-        fetch(m.ref, SLOT, nv + 1, h.ctypes.data)
+        unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data)   # the values come back with the call
         assert m.last_error() == 0
         res[name] = (h, w.numpy(), launches(m.ref) - before)
         w.free()
@@ -205,3 +203,45 @@ def test_solve_history_with_resident_mgs(mesh_dir, dims, monkeypatch):
         fa = np.array([ra["cfl"], ra["fnorm"]] + ra["forces"])
         fb = np.array([rb["cfl"], rb["fnorm"]] + rb["forces"])
         assert np.array_equal(fa.view(np.uint64), fb.view(np.uint64)), (ra, rb)
+
+
+def test_resident_mgs_abort_falls_back(mesh_a, monkeypatch):
+    """A resident launch whose workgroups are not all co-resident aborts
+    before writing anything (forced here from the host, KF_MGS_RESIDENT=3);
+    the call is then repeated with the per-step kernels -- the same bits as
+    a context that never tried -- and the context keeps those kernels."""
+    L = K.lib()
+    unit = L.kfs_mgs_unit
+    unit.restype = None
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    aborts = L.kfs_mgs_aborts
+    aborts.restype = C.c_ulonglong
+    aborts.argtypes = [K.GP]
+    _, fetch, launches = _fns()
+    n, nv = 1327104, 5
+    rng = np.random.default_rng(11)
+    Q = np.linalg.qr(rng.standard_normal((n, nv)))[0]
+    vb = K.DeviceVec(nv * n, np.ascontiguousarray(Q.T).ravel())
+    w0 = rng.standard_normal(n)
+    res = {}
+    for val in ("0", "3"):
+        monkeypatch.setenv("KF_MGS_RESIDENT", val)
+        m = K.Mesh(mesh_a, K.options())
+        try:
+            out = []
+            for _ in range(2):   # the second call goes straight to the per-step kernels
+                w = K.DeviceVec(n, w0)
+                h = np.empty(nv + 1)
+                unit(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT, h.ctypes.data)
+                out.append((h, w.numpy()))
+                w.free()
+            assert m.last_error() == 0
+            res[val] = (out, launches(m.ref), aborts(m.ref))
+        finally:
+            m.close()
+    vb.free()
+    (o0, l0, a0), (o3, l3, a3) = res["0"], res["3"]
+    assert (l0, a0) == (0, 0) and (l3, a3) == (1, 1), (l0, a0, l3, a3)
+    for (h0, w0_), (h3, w3) in zip(o0, o3):
+        assert np.array_equal(h0.view(np.uint64), h3.view(np.uint64))
+        assert np.array_equal(w0_.view(np.uint64), w3.view(np.uint64))
