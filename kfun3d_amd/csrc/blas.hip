@@ -268,7 +268,8 @@ template <int MP, bool HOLDV>
 __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const double *__restrict__ vbase, size_t vstride,
                                                    double *__restrict__ w, double *__restrict__ hslot, double *part,
                                                    unsigned long long sent, uint32_t *err, uint32_t sleep, int unit,
-                                                   uint32_t *sync, int force_abort)
+                                                   uint32_t *sync, int force_abort, double *part_next,
+                                                   unsigned long long sent_next, uint32_t *sync_next)
 {
   /* without HOLDV the first NL slots of V_k are kept in LDS between the dot
    * and the update (72 KB for NL = 9: two workgroups still fit a CU) */
@@ -287,6 +288,14 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
    * every workgroup follows the same decision.  On abort nothing is written
    * but the flag hslot[nv + 1] = 1 (the host then repeats the call with the
    * per-step kernels); on go block 0 leaves hslot[nv + 1] = 0. */
+  /* The tables alternate between calls: the other one (last used by the
+   * previous launch, which has finished) is made ready for the next call --
+   * every workgroup presets its own two columns of it, block 0 its arrival
+   * words -- so no separate preset launch is needed. */
+  if (threadIdx.x < 2 * KF_MGS_MAXV)
+    part_next[(size_t)(threadIdx.x >> 1) * KF_NRED + 2 * blockIdx.x + (threadIdx.x & 1)] =
+        sentinel<double>::val(sent_next);
+  if (blockIdx.x == 0 && threadIdx.x < 2) sync_next[threadIdx.x] = 0;
   __shared__ uint32_t decision;
   if (threadIdx.x == 0) {
     uint32_t st = 0;
@@ -475,8 +484,8 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
 #undef KF_ON
 }
 
-/* the partial table of a resident launch preset to the call's sentinel, and
- * its arrival counter and decision cleared */
+/* a partial table preset to a call's sentinel, and its arrival counter and
+ * decision cleared (set-up of the first call's table) */
 __global__ void k_mgs_preset(double *part, size_t rows, unsigned long long sent, uint32_t *sync)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,20 +537,31 @@ static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride
     c->mgs_ok[vi] = ((size_t)nb * (size_t)ncu >= KF_NRED / 2) ? 1 : 0;
   }
   if (!c->mgs_ok[vi]) return false;
-  const unsigned long long sent = sentinel<double>::of(c->mcall);
-  const size_t rows = (size_t)(nv + 1) * KF_NRED;
-  hipLaunchKernelGGL(k_mgs_preset, dim3(kf_blocks_for(rows, RB)), dim3(RB), 0, c->stream, c->mgs_part, rows, sent,
-                     c->mgs_sync);
-  KF_LAUNCH_CHECK();
+  /* this call's table and arrival words (preset by the previous launch, or
+   * at set-up for the first), and the other parity's, for the next call */
+  const uint64_t k = c->mcall;
+  const unsigned long long sent = sentinel<double>::of(k), sent_next = sentinel<double>::of(k + 1);
+  double *part = c->mgs_part + (k & 1) * (size_t)KF_MGS_MAXV * KF_NRED;
+  double *part_next = c->mgs_part + ((k + 1) & 1) * (size_t)KF_MGS_MAXV * KF_NRED;
+  uint32_t *sync = c->mgs_sync + 2 * (k & 1), *sync_next = c->mgs_sync + 2 * ((k + 1) & 1);
   double *hslot = c->d.scal + slot;
   uint32_t *err = c->d.err;
   uint32_t sleep = c->d.spin_sleep;
   const int force_abort = c->mgs_res == 3;
-  void *args[] = {&n,     &nv,  &vbase, &stride, &w,           &hslot, &c->mgs_part, (void *)&sent,
-                  &err,   &sleep, &unit, &c->mgs_sync, (void *)&force_abort};
+  void *args[] = {&n,   &nv,    &vbase, &stride, &w,           &hslot,     &part,      (void *)&sent,
+                  &err, &sleep, &unit,  &sync,   (void *)&force_abort, &part_next, (void *)&sent_next, &sync_next};
   KF_CHECK(hipLaunchKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
   c->mcall++;
   return true;
+}
+
+/* the first resident launch's table and arrival words (call 0, parity 0) */
+void kf_mgs_setup(kf_ctx *c)
+{
+  const size_t rows = (size_t)KF_MGS_MAXV * KF_NRED;
+  hipLaunchKernelGGL(k_mgs_preset, dim3(kf_blocks_for(rows, RB)), dim3(RB), 0, c->stream, c->mgs_part, rows,
+                     sentinel<double>::of(0), c->mgs_sync);
+  KF_LAUNCH_CHECK();
 }
 
 /* the per-step modified Gram-Schmidt (k_dot_part, k_mgs_step per basis

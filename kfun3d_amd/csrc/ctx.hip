@@ -220,12 +220,13 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   KF_CHECK(hipMemset(c->cdt, 0, N * sizeof(double)));
   d.eres = dalloc<double>(4 * E + 8);
   d.red = dalloc<double>(2 * KF_NRED);   /* two sets of partial sums (fused MGS) */
-  c->mgs_part = dalloc<double>((size_t)KF_MGS_MAXV * KF_NRED);
-  c->mgs_sync = dalloc<uint32_t>(2);
+  c->mgs_part = dalloc<double>(2 * (size_t)KF_MGS_MAXV * KF_NRED);
+  c->mgs_sync = dalloc<uint32_t>(4);
   c->mgs_res = (ev = getenv("KF_MGS_RESIDENT")) ? atoi(ev) : 1;
   if (c->mgs_res == 2) c->mgs_res = 1;
   if (c->mgs_res < 0 || c->mgs_res > 3) c->mgs_res = 1;
   for (int i = 0; i < 8; i++) c->mgs_ok[i] = -1;
+  kf_mgs_setup(c);   /* the first resident launch's table */
   d.scal = dalloc<double>(KF_NSCAL);
   KF_CHECK(hipMemset(d.scal, 0, KF_NSCAL * sizeof(double)));
   d.err = dalloc<uint32_t>(1);

@@ -123,12 +123,12 @@ struct kf_ctx {
   uint32_t ntask_l, ntask_u;
   double *ty64;             /* forward-sweep output (FP64) */
   uint64_t tcall;           /* triangular solves so far (sentinel numbering) */
-  /* modified Gram-Schmidt with w resident in registers (one cooperative
-   * launch per orthogonalisation): per-step partial sums handed over
-   * through sentinel slots, KF_MGS_MAXV rows of KF_NRED */
+  /* modified Gram-Schmidt with w resident in registers (one launch per
+   * orthogonalisation): per-step partial sums handed over through sentinel
+   * slots, two tables (by call parity) of KF_MGS_MAXV rows of KF_NRED */
   double *mgs_part;
   uint64_t mcall;            /* resident MGS launches so far (sentinel numbering) */
-  uint32_t *mgs_sync;        /* arrival counter and go/abort decision of the launch */
+  uint32_t *mgs_sync;        /* per parity: arrival counter and go/abort decision of the launch */
   uint64_t mgs_aborts;       /* resident launches that found their workgroups not co-resident */
   int mgs_res;               /* 0: never, 1: wherever it fits, 3: always abort (tests the fallback) */
   int8_t mgs_ok[8];          /* per register-size variant: -1 unknown, 0 no, 1 co-resident */
@@ -209,6 +209,7 @@ void kf_scale_recip(kf_ctx *c, size_t n, double *x, int slot);
 void kf_axpy_slot(kf_ctx *c, size_t n, int slot, const double *x, double *y);
 void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
             double *out);
+void kf_mgs_setup(kf_ctx *c);
 void kf_fd_perturb(kf_ctx *c, const double *x, double h, int slot_xnorm, double *w);
 void kf_fd_combine(kf_ctx *c, const double *r, const double *w, double h, int slot_xnorm, double *y);
 void kf_fill_zero(kf_ctx *c, size_t n, double *x);
