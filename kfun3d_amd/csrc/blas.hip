@@ -269,7 +269,8 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
                                                    double *__restrict__ w, double *__restrict__ hslot, double *part,
                                                    unsigned long long sent, uint32_t *err, uint32_t sleep, int unit,
                                                    uint32_t *sync, int force_abort, double *part_next,
-                                                   unsigned long long sent_next, uint32_t *sync_next)
+                                                   unsigned long long sent_next, uint32_t *sync_next,
+                                                   double *xnorm_out)
 {
   /* without HOLDV the first NL slots of V_k are kept in LDS between the dot
    * and the update (72 KB for NL = 9: two workgroups still fit a CU) */
@@ -469,6 +470,29 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
       wt[v] *= f;
     }
   }
+  /* xnorm_out: |w| of the stored (scaled) w as kf_nrm2_dev forms it --
+   * k_dot_part's partials, then k_red_final's tree and square root -- in one
+   * more hand-off round on the table's row nv + 1 */
+  if (xnorm_out) {
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int m = 0; m < MP; m++) {
+        const double p0 = wr[v][m][0] * wr[v][m][0], p1 = wr[v][m][1] * wr[v][m][1];
+        acc += KF_ON(m, v) ? p0 : 0.0;
+        acc += KF_ON(m, v) ? p1 : 0.0;
+      }
+      if (tail[v]) acc += wt[v] * wt[v];
+      const double s = block_sum(acc, sh);
+      if (threadIdx.x == 0) st_coherent(part + (size_t)(nv + 1) * KF_NRED + 2 * blockIdx.x + v, canon(s));
+    }
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < KF_NRED; j += RB)
+      acc += poll_part(part + (size_t)(nv + 1) * KF_NRED + j, sent, err, aborted, sleep);
+    const double h = block_sum(acc, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *xnorm_out = sqrt(h);
+  }
   asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));
 #pragma unroll
   for (int v = 0; v < 2; v++) {
@@ -519,10 +543,10 @@ static const void *mgs_kernel(int i)
  * which writes the abort flag to scal[slot + nv + 1] -- because a
  * cooperative launch cost about 30 us more per call on MI355X. */
 static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
-                         int unit)
+                         int unit, int xslot)
 {
   if (!c->mgs_res || nv + 2 > KF_MGS_MAXV || n < 2) return false;   /* a lane without pairs reads pair 0 */
-  if (slot + nv + 1 >= KF_NSCAL) return false;
+  if (slot + nv + 1 >= KF_NSCAL || xslot >= KF_NSCAL) return false;
   if (!(aligned16(vbase) && aligned16(w) && (stride & 1) == 0)) return false;
   const size_t S = (size_t)KF_NRED * RB, np = n / 2, need = (np + S - 1) / S;   /* pairs per thread, at most */
   int vi = 0;
@@ -548,8 +572,10 @@ static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride
   uint32_t *err = c->d.err;
   uint32_t sleep = c->d.spin_sleep;
   const int force_abort = c->mgs_res == 3;
+  double *xnorm_out = xslot >= 0 ? c->d.scal + xslot : NULL;
   void *args[] = {&n,   &nv,    &vbase, &stride, &w,           &hslot,     &part,      (void *)&sent,
-                  &err, &sleep, &unit,  &sync,   (void *)&force_abort, &part_next, (void *)&sent_next, &sync_next};
+                  &err, &sleep, &unit,  &sync,   (void *)&force_abort, &part_next, (void *)&sent_next, &sync_next,
+                  &xnorm_out};
   KF_CHECK(hipLaunchKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
   c->mcall++;
   return true;
@@ -592,10 +618,12 @@ static void mgs_steps(kf_ctx *c, size_t n, const double *vbase, size_t stride, i
  * the nv + 1 values fetched to the host.  A resident launch is followed by
  * that fetch (one synchronisation, which the caller's fetch would cost
  * anyway); if it aborted, w is untouched and the call is repeated with the
- * per-step kernels, which the context then keeps.  Index-order sums
- * (reduce_seq) keep the unfused dot / update sequence. */
-void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
-            double *out)
+ * per-step kernels, which the context then keeps.  xslot >= 0 (with unit):
+ * the resident launch also leaves |w| of the scaled w in scal[xslot], as
+ * kf_nrm2_dev would; returns whether it did.  Index-order sums (reduce_seq)
+ * keep the unfused dot / update sequence. */
+bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
+            double *out, int xslot)
 {
   if (c->opt.reduce_seq) {
     for (int k = 0; k < nv; k++) {
@@ -604,13 +632,13 @@ void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     }
     dot_any(c, n, w, w, slot + nv, 1);
     if (unit) kf_scale_recip(c, n, w, slot + nv);
-  } else if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit)) {
+  } else if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit, unit ? xslot : -1)) {
     double tmp[KF_MGS_MAXV + 1];
     kf_scal_fetch(c, slot, nv + 2, tmp);
     if (tmp[nv + 1] == 0.0) {
       if (out)
         for (int i = 0; i <= nv; i++) out[i] = tmp[i];
-      return;
+      return unit && xslot >= 0;
     }
     c->mgs_res = 0;   /* not co-resident here: the per-step kernels from now on */
     c->mgs_aborts++;
@@ -619,6 +647,7 @@ void kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     mgs_steps(c, n, vbase, stride, nv, w, slot, unit);
   }
   if (out) kf_scal_fetch(c, slot, nv + 1, out);
+  return false;
 }
 
 void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out)
@@ -815,13 +844,13 @@ void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot) { kf_scale_reci
 void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y) { kf_axpy_slot(kf_ctx_of(g), n, slot, x, y); }
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
 {
-  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 0, NULL);
+  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 0, NULL, -1);
 }
 
-void kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
-                  double *out)
+int kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
+                 double *out, int xslot)
 {
-  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 1, out);
+  return kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 1, out, xslot) ? 1 : 0;
 }
 
 unsigned long long kfs_mgs_aborts(GEOMETRY *g) { return kf_ctx_of(g)->mgs_aborts; }

@@ -118,9 +118,11 @@ void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y);
  * slots slot..slot+nv-1, |w| in slot+nv (src/kernel.c:129-141) */
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot);
 /* kfs_mgs, then w scaled to unit length (x *= 1.0 / |w|, as kfs_scale_recip);
- * out (if not NULL) receives h_0 .. h_{nv-1}, |w| */
-void kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
-                  double *out);
+ * out (if not NULL) receives h_0 .. h_{nv-1}, |w|.  xslot >= 0: the norm of
+ * the scaled w may be left in scal[xslot] as kfs_nrm2 would; returns 1 if
+ * it was. */
+int kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
+                 double *out, int xslot);
 /* resident launches that aborted (workgroups not co-resident) on this context */
 unsigned long long kfs_mgs_aborts(GEOMETRY *g);
 /* resident (one-launch) orthogonalisations so far on this context */

@@ -29,16 +29,23 @@
 
 #define FD_EPS 1.490116119384766e-08 /* src/kernel.c:12 */
 
-void iJacVec(const GEOMETRY *gc, const double *x, const double *r, const double norm, GRADIENT *grad, double *w,
-             double *y)
+/* iJacVec with |x| already in scal[KFS_SLOT_XNORM] when have_xnorm (left
+ * there by the orthogonalisation that produced x) */
+static void ijacvec(GEOMETRY *g, const double *x, const double *r, const double norm, GRADIENT *grad, double *w,
+                    double *y, int have_xnorm)
 {
-  GEOMETRY *g = (GEOMETRY *)gc;
   const size_t sz = g->c->sz;
-  kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
+  if (!have_xnorm) kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
   const double h = FD_EPS * norm;
   kfs_fd_perturb(g, x, h, KFS_SLOT_XNORM, w);
   ComputeFlux(g, w, grad, y);
   kfs_fd_combine(g, r, w, h, KFS_SLOT_XNORM, y);
+}
+
+void iJacVec(const GEOMETRY *gc, const double *x, const double *r, const double norm, GRADIENT *grad, double *w,
+             double *y)
+{
+  ijacvec((GEOMETRY *)gc, x, r, norm, grad, w, y, 0);
 }
 
 static double fetch1(GEOMETRY *g, int slot)
@@ -188,10 +195,10 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
     kfs_scale_recip(g, sz, VEC(0), KFS_SLOT_BETA);
     rs[0] = fetch1(g, KFS_SLOT_BETA);
     tol = (total == 0) ? rtol * rs[0] : tol;
-    int it = 0;
+    int it = 0, have_xnorm = 0;   /* |VEC(it)| left by the previous orthogonalisation */
     while (it < restart) {
       if (!flexible) {
-        iJacVec(g, VEC(it), K->r, K->qnorm, &K->grad, w, t);
+        ijacvec(g, VEC(it), K->r, K->qnorm, &K->grad, w, t, have_xnorm);
         ComputeSparseTriangularSolve(g, t, VEC(it + 1));
       } else {
         ComputeSparseTriangularSolve(g, VEC(it), ZVEC(it)); /* z = M(v) */
@@ -214,7 +221,9 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
         hh[it + 1] = h2[it + 1];
       } else {
         /* modified Gram-Schmidt, src/kernel.c:129-141 */
-        kfs_mgs_unit(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H, hh);   /* then v = w / |w| */
+        /* then v = w / |w|, and |v| for the next iJacVec */
+        have_xnorm = kfs_mgs_unit(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H, hh,
+                                  flexible ? -1 : KFS_SLOT_XNORM);
       }
       /* Givens, src/kernel.c:143-173 */
       for (int i = 0; i < it; i++) {

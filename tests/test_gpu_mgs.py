@@ -151,8 +151,8 @@ def test_resident_mgs_unit_bitwise(contexts, n, nv):
     bits either way, and |w| = 1 to rounding."""
     L = K.lib()
     unit = L.kfs_mgs_unit
-    unit.restype = None
-    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    unit.restype = C.c_int
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     _, fetch, launches = _fns()
     rng = np.random.default_rng(n % 997 + nv)
     stride = n + (n & 1) + 2
@@ -167,7 +167,7 @@ def test_resident_mgs_unit_bitwise(contexts, n, nv):
         before = launches(m.ref)
         w = K.DeviceVec(n, w0)
         h = np.empty(nv + 1)
-        unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data)   # the values come back with the call
+        unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data, -1)   # the values come back with the call
         assert m.last_error() == 0
         res[name] = (h, w.numpy(), launches(m.ref) - before)
         w.free()
@@ -212,8 +212,8 @@ def test_resident_mgs_abort_falls_back(mesh_a, monkeypatch):
     a context that never tried -- and the context keeps those kernels."""
     L = K.lib()
     unit = L.kfs_mgs_unit
-    unit.restype = None
-    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    unit.restype = C.c_int
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     aborts = L.kfs_mgs_aborts
     aborts.restype = C.c_ulonglong
     aborts.argtypes = [K.GP]
@@ -232,7 +232,7 @@ def test_resident_mgs_abort_falls_back(mesh_a, monkeypatch):
             for _ in range(2):   # the second call goes straight to the per-step kernels
                 w = K.DeviceVec(n, w0)
                 h = np.empty(nv + 1)
-                unit(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT, h.ctypes.data)
+                unit(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT, h.ctypes.data, -1)
                 out.append((h, w.numpy()))
                 w.free()
             assert m.last_error() == 0
@@ -245,3 +245,39 @@ def test_resident_mgs_abort_falls_back(mesh_a, monkeypatch):
     for (h0, w0_), (h3, w3) in zip(o0, o3):
         assert np.array_equal(h0.view(np.uint64), h3.view(np.uint64))
         assert np.array_equal(w0_.view(np.uint64), w3.view(np.uint64))
+
+
+@pytest.mark.parametrize("n", [1327104, 2 * (2 * S + 777) + 1])
+def test_resident_mgs_leaves_next_norm(contexts, n):
+    """With xslot, the resident launch also leaves |v| of the stored unit
+    vector v in scal[xslot] -- the bits kfs_nrm2 forms for v afterwards --
+    so that the next iJacVec need not compute it."""
+    L = K.lib()
+    unit = L.kfs_mgs_unit
+    unit.restype = C.c_int
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    nrm2 = L.kfs_nrm2
+    nrm2.restype = None
+    nrm2.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_int]
+    _, fetch, _ = _fns()
+    nv, XS = 3, SLOT + 60
+    rng = np.random.default_rng(n % 991)
+    stride = n + (n & 1) + 2
+    Q = np.linalg.qr(rng.standard_normal((n, nv)))[0]
+    V = np.zeros((nv, stride))
+    V[:, :n] = Q.T
+    vb = K.DeviceVec(nv * stride, V.ravel())
+    m = contexts["resident"]
+    w = K.DeviceVec(n, rng.standard_normal(n))
+    h = np.empty(nv + 1)
+    assert unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data, XS) == 1
+    left = np.empty(1)
+    fetch(m.ref, XS, 1, left.ctypes.data)
+    nrm2(m.ref, n, w.ptr, XS)
+    again = np.empty(1)
+    fetch(m.ref, XS, 1, again.ctypes.data)
+    assert m.last_error() == 0
+    assert left.view(np.uint64)[0] == again.view(np.uint64)[0], (left, again)
+    assert abs(again[0] - 1.0) < 1e-14
+    w.free()
+    vb.free()
