@@ -270,7 +270,7 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
                                                    unsigned long long sent, uint32_t *err, uint32_t sleep, int unit,
                                                    uint32_t *sync, int force_abort, double *part_next,
                                                    unsigned long long sent_next, uint32_t *sync_next,
-                                                   double *xnorm_out)
+                                                   double *xnorm_out, double *hhost)
 {
   /* without HOLDV the first NL slots of V_k are kept in LDS between the dot
    * and the update (72 KB for NL = 9: two workgroups still fit a CU) */
@@ -318,7 +318,10 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
         st = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if (st != 1) hslot[nv + 1] = 1.0;
+    if (st != 1) {
+      hslot[nv + 1] = 1.0;
+      hhost[nv + 1] = 1.0;
+    }
     decision = st;
   }
   __syncthreads();
@@ -416,7 +419,7 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
     for (int j = threadIdx.x; j < KF_NRED; j += RB)
       acc += poll_part(part + (size_t)k * KF_NRED + j, sent, err, aborted, sleep);
     const double h = block_sum(acc, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) hslot[k] = self ? sqrt(h) : h;
+    if (blockIdx.x == 0 && threadIdx.x == 0) hslot[k] = hhost[k] = self ? sqrt(h) : h;
     if (self) {   /* the closing norm: the loop ends */
       nrm2 = h;
       continue;
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
             make_double2(wr[v][m][0], wr[v][m][1]);
     if (tail[v]) w[n - 1] = wt[v];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) hslot[nv + 1] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) hslot[nv + 1] = hhost[nv + 1] = 0.0;
 #undef KF_PAIR
 #undef KF_ON
 }
@@ -573,9 +576,10 @@ static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride
   uint32_t sleep = c->d.spin_sleep;
   const int force_abort = c->mgs_res == 3;
   double *xnorm_out = xslot >= 0 ? c->d.scal + xslot : NULL;
+  double *hhost = c->hscal + slot;   /* the pinned host mirror: read after the stream synchronises */
   void *args[] = {&n,   &nv,    &vbase, &stride, &w,           &hslot,     &part,      (void *)&sent,
                   &err, &sleep, &unit,  &sync,   (void *)&force_abort, &part_next, (void *)&sent_next, &sync_next,
-                  &xnorm_out};
+                  &xnorm_out, &hhost};
   KF_CHECK(hipLaunchKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
   c->mcall++;
   return true;
@@ -615,10 +619,11 @@ static void mgs_steps(kf_ctx *c, size_t n, const double *vbase, size_t stride, i
 /* w orthogonalised against nv basis vectors vbase + k stride by modified
  * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv]; unit: w then
  * scaled to unit length (x *= 1.0 / |w|, kf_scale_recip); out (if not NULL):
- * the nv + 1 values fetched to the host.  A resident launch is followed by
- * that fetch (one synchronisation, which the caller's fetch would cost
- * anyway); if it aborted, w is untouched and the call is repeated with the
- * per-step kernels, which the context then keeps.  xslot >= 0 (with unit):
+ * the nv + 1 values fetched to the host.  A resident launch writes them to
+ * the pinned host mirror as well and is followed by one synchronisation
+ * (which the caller's fetch would cost anyway); if it aborted, w is
+ * untouched and the call is repeated with the per-step kernels, which the
+ * context then keeps.  xslot >= 0 (with unit):
  * the resident launch also leaves |w| of the scaled w in scal[xslot], as
  * kf_nrm2_dev would; returns whether it did.  Index-order sums (reduce_seq)
  * keep the unfused dot / update sequence. */
@@ -633,8 +638,9 @@ bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     dot_any(c, n, w, w, slot + nv, 1);
     if (unit) kf_scale_recip(c, n, w, slot + nv);
   } else if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit, unit ? xslot : -1)) {
-    double tmp[KF_MGS_MAXV + 1];
-    kf_scal_fetch(c, slot, nv + 2, tmp);
+    /* h_k, |w| and the flag were also written to the pinned host mirror */
+    KF_CHECK(hipStreamSynchronize(c->stream));
+    const double *tmp = c->hscal + slot;
     if (tmp[nv + 1] == 0.0) {
       if (out)
         for (int i = 0; i <= nv; i++) out[i] = tmp[i];
