@@ -419,7 +419,15 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
     for (int j = threadIdx.x; j < KF_NRED; j += RB)
       acc += poll_part(part + (size_t)k * KF_NRED + j, sent, err, aborted, sleep);
     const double h = block_sum(acc, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) hslot[k] = hhost[k] = self ? sqrt(h) : h;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      hslot[k] = hhost[k] = self ? sqrt(h) : h;
+      if (self) {
+        /* every value the host needs is written: the ready flag, behind a
+         * system-scope fence (the host polls it while the launch finishes) */
+        __threadfence_system();
+        hslot[nv + 1] = hhost[nv + 1] = 0.0;
+      }
+    }
     if (self) {   /* the closing norm: the loop ends */
       nrm2 = h;
       continue;
@@ -506,7 +514,6 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
             make_double2(wr[v][m][0], wr[v][m][1]);
     if (tail[v]) w[n - 1] = wt[v];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) hslot[nv + 1] = hhost[nv + 1] = 0.0;
 #undef KF_PAIR
 #undef KF_ON
 }
@@ -576,7 +583,8 @@ static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride
   uint32_t sleep = c->d.spin_sleep;
   const int force_abort = c->mgs_res == 3;
   double *xnorm_out = xslot >= 0 ? c->d.scal + xslot : NULL;
-  double *hhost = c->hscal + slot;   /* the pinned host mirror: read after the stream synchronises */
+  double *hhost = c->hscal + slot;   /* the pinned host mirror, polled by the host */
+  c->hscal[slot + nv + 1] = -1.0;    /* not ready: the launch writes 0 (go, values written) or 1 (abort) */
   void *args[] = {&n,   &nv,    &vbase, &stride, &w,           &hslot,     &part,      (void *)&sent,
                   &err, &sleep, &unit,  &sync,   (void *)&force_abort, &part_next, (void *)&sent_next, &sync_next,
                   &xnorm_out, &hhost};
@@ -638,14 +646,29 @@ bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     dot_any(c, n, w, w, slot + nv, 1);
     if (unit) kf_scale_recip(c, n, w, slot + nv);
   } else if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit, unit ? xslot : -1)) {
-    /* h_k, |w| and the flag were also written to the pinned host mirror */
-    KF_CHECK(hipStreamSynchronize(c->stream));
+    /* h_k, |w| and the flag are written to the pinned host mirror as soon
+     * as they are known: poll the flag (bounded, then a stream
+     * synchronisation) instead of waiting for the whole launch, whose tail
+     * overlaps the caller's work; later launches follow it in the stream */
+    const volatile double *flag = c->hscal + slot + nv + 1;
+    double fv = *flag;
+    for (uint64_t k = 0; fv == -1.0; k++) {
+      if (k >= (1ull << 22)) {
+        KF_CHECK(hipStreamSynchronize(c->stream));
+        fv = *flag;
+        break;
+      }
+      __builtin_ia32_pause();
+      fv = *flag;
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
     const double *tmp = c->hscal + slot;
-    if (tmp[nv + 1] == 0.0) {
+    if (fv == 0.0) {
       if (out)
         for (int i = 0; i <= nv; i++) out[i] = tmp[i];
       return unit && xslot >= 0;
     }
+    KF_CHECK(hipStreamSynchronize(c->stream));   /* the aborted launch has ended */
     c->mgs_res = 0;   /* not co-resident here: the per-step kernels from now on */
     c->mgs_aborts++;
     mgs_steps(c, n, vbase, stride, nv, w, slot, unit);
