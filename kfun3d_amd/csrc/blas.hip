@@ -270,7 +270,8 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
                                                    unsigned long long sent, uint32_t *err, uint32_t sleep, int unit,
                                                    uint32_t *sync, int force_abort, double *part_next,
                                                    unsigned long long sent_next, uint32_t *sync_next,
-                                                   double *xnorm_out, double *hhost)
+                                                   double *xnorm_out, double *hhost, double fd_h,
+                                                   const double *__restrict__ qstate, double *__restrict__ wpert)
 {
   /* without HOLDV the first NL slots of V_k are kept in LDS between the dot
    * and the update (72 KB for NL = 9: two workgroups still fit a CU) */
@@ -502,7 +503,42 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
     for (int j = threadIdx.x; j < KF_NRED; j += RB)
       acc += poll_part(part + (size_t)(nv + 1) * KF_NRED + j, sent, err, aborted, sleep);
     const double h = block_sum(acc, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *xnorm_out = sqrt(h);
+    const double xn = sqrt(h);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *xnorm_out = xn;
+    /* wpert: the next iJacVec's perturbed state, as k_fd_perturb forms it:
+     * a = h_fd / |x|, wpert = a x + q (the product, then the sum) */
+    if (wpert) {
+      const double a = fd_h / xn;
+      asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+#pragma unroll
+        for (int m0 = 0; m0 < MP; m0 += 4) {
+          double qq[4][2];
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (m0 + j < MP) {
+              const double2 t = KF_PAIR(qstate, m0 + j, v);
+              qq[j][0] = t.x;
+              qq[j][1] = t.y;
+            }
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int m = m0 + j;
+            if (m < MP && KF_ON(m, v)) {
+              const double ax0 = a * wr[v][m][0], ax1 = a * wr[v][m][1];
+              *reinterpret_cast<double2 *>(reinterpret_cast<char *>(wpert) + voff[v] + (uint32_t)m * PB) =
+                  make_double2(ax0 + qq[j][0], ax1 + qq[j][1]);
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        if (tail[v]) {
+          const double ax = a * wt[v];
+          wpert[n - 1] = ax + qstate[n - 1];
+        }
+      }
+    }
   }
   asm volatile("" : "+v"(voff[0]), "+v"(voff[1]));
 #pragma unroll
@@ -553,7 +589,7 @@ static const void *mgs_kernel(int i)
  * which writes the abort flag to scal[slot + nv + 1] -- because a
  * cooperative launch cost about 30 us more per call on MI355X. */
 static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
-                         int unit, int xslot)
+                         int unit, int xslot, double fd_h, double *wpert)
 {
   if (!c->mgs_res || nv + 2 > KF_MGS_MAXV || n < 2) return false;   /* a lane without pairs reads pair 0 */
   if (slot + nv + 1 >= KF_NSCAL || xslot >= KF_NSCAL) return false;
@@ -585,9 +621,11 @@ static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride
   double *xnorm_out = xslot >= 0 ? c->d.scal + xslot : NULL;
   double *hhost = c->hscal + slot;   /* the pinned host mirror, polled by the host */
   c->hscal[slot + nv + 1] = -1.0;    /* not ready: the launch writes 0 (go, values written) or 1 (abort) */
+  const double *qstate = c->q;
+  if (!xnorm_out || n != 4 * (size_t)c->d.N || !aligned16(wpert)) wpert = NULL;   /* needs the norm round */
   void *args[] = {&n,   &nv,    &vbase, &stride, &w,           &hslot,     &part,      (void *)&sent,
                   &err, &sleep, &unit,  &sync,   (void *)&force_abort, &part_next, (void *)&sent_next, &sync_next,
-                  &xnorm_out, &hhost};
+                  &xnorm_out, &hhost, &fd_h, &qstate, &wpert};
   KF_CHECK(hipLaunchKernel(fn, dim3(KF_NRED / 2), dim3(RB), args, 0, c->stream));
   c->mcall++;
   return true;
@@ -633,10 +671,13 @@ static void mgs_steps(kf_ctx *c, size_t n, const double *vbase, size_t stride, i
  * untouched and the call is repeated with the per-step kernels, which the
  * context then keeps.  xslot >= 0 (with unit):
  * the resident launch also leaves |w| of the scaled w in scal[xslot], as
- * kf_nrm2_dev would; returns whether it did.  Index-order sums (reduce_seq)
- * keep the unfused dot / update sequence. */
-bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
-            double *out, int xslot)
+ * kf_nrm2_dev would; wpert (with xslot, w of the state's length): the launch
+ * also writes the next iJacVec's perturbed state (fd_h / |w|) w + q as
+ * kf_fd_perturb would.  Returns 2 if both were left, 1 if the norm only, 0
+ * otherwise.  Index-order sums (reduce_seq) keep the unfused dot / update
+ * sequence. */
+int kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
+           double *out, int xslot, double fd_h, double *wpert)
 {
   if (c->opt.reduce_seq) {
     for (int k = 0; k < nv; k++) {
@@ -645,7 +686,7 @@ bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     }
     dot_any(c, n, w, w, slot + nv, 1);
     if (unit) kf_scale_recip(c, n, w, slot + nv);
-  } else if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit, unit ? xslot : -1)) {
+  } else if (mgs_resident(c, n, vbase, stride, nv, w, slot, unit, unit ? xslot : -1, fd_h, wpert)) {
     /* h_k, |w| and the flag are written to the pinned host mirror as soon
      * as they are known: poll the flag (bounded, then a stream
      * synchronisation) instead of waiting for the whole launch, whose tail
@@ -666,7 +707,8 @@ bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     if (fv == 0.0) {
       if (out)
         for (int i = 0; i <= nv; i++) out[i] = tmp[i];
-      return unit && xslot >= 0;
+      if (!(unit && xslot >= 0)) return 0;
+      return (wpert && n == 4 * (size_t)c->d.N && aligned16(wpert)) ? 2 : 1;
     }
     KF_CHECK(hipStreamSynchronize(c->stream));   /* the aborted launch has ended */
     c->mgs_res = 0;   /* not co-resident here: the per-step kernels from now on */
@@ -676,7 +718,7 @@ bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, dou
     mgs_steps(c, n, vbase, stride, nv, w, slot, unit);
   }
   if (out) kf_scal_fetch(c, slot, nv + 1, out);
-  return false;
+  return 0;
 }
 
 void kf_scal_fetch(kf_ctx *c, int slot, int count, double *out)
@@ -873,13 +915,13 @@ void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot) { kf_scale_reci
 void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y) { kf_axpy_slot(kf_ctx_of(g), n, slot, x, y); }
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot)
 {
-  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 0, NULL, -1);
+  kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 0, NULL, -1, 0.0, NULL);
 }
 
 int kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
-                 double *out, int xslot)
+                 double *out, int xslot, double fd_h, double *wpert)
 {
-  return kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 1, out, xslot) ? 1 : 0;
+  return kf_mgs(kf_ctx_of(g), n, vbase, stride, nv, w, slot, 1, out, xslot, fd_h, wpert);
 }
 
 unsigned long long kfs_mgs_aborts(GEOMETRY *g) { return kf_ctx_of(g)->mgs_aborts; }

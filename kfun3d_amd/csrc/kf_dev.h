@@ -207,8 +207,8 @@ void kf_axpy(kf_ctx *c, size_t n, double a, const double *x, double *y);
 void kf_newaxpy(kf_ctx *c, size_t n, double a, const double *x, const double *y, double *w);
 void kf_scale_recip(kf_ctx *c, size_t n, double *x, int slot);
 void kf_axpy_slot(kf_ctx *c, size_t n, int slot, const double *x, double *y);
-bool kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
-            double *out, int xslot);
+int kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot, int unit,
+           double *out, int xslot, double fd_h, double *wpert);
 void kf_mgs_setup(kf_ctx *c);
 void kf_fd_perturb(kf_ctx *c, const double *x, double h, int slot_xnorm, double *w);
 void kf_fd_combine(kf_ctx *c, const double *r, const double *w, double h, int slot_xnorm, double *y);

@@ -119,10 +119,11 @@ void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y);
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot);
 /* kfs_mgs, then w scaled to unit length (x *= 1.0 / |w|, as kfs_scale_recip);
  * out (if not NULL) receives h_0 .. h_{nv-1}, |w|.  xslot >= 0: the norm of
- * the scaled w may be left in scal[xslot] as kfs_nrm2 would; returns 1 if
- * it was. */
+ * the scaled w may be left in scal[xslot] as kfs_nrm2 would, and with wpert
+ * also (fd_h / |w|) w + q there, as kfs_fd_perturb would; returns 2 if both
+ * were, 1 if the norm only, 0 otherwise. */
 int kfs_mgs_unit(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot,
-                 double *out, int xslot);
+                 double *out, int xslot, double fd_h, double *wpert);
 /* resident launches that aborted (workgroups not co-resident) on this context */
 unsigned long long kfs_mgs_aborts(GEOMETRY *g);
 /* resident (one-launch) orthogonalisations so far on this context */

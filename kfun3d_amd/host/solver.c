@@ -29,15 +29,16 @@
 
 #define FD_EPS 1.490116119384766e-08 /* src/kernel.c:12 */
 
-/* iJacVec with |x| already in scal[KFS_SLOT_XNORM] when have_xnorm (left
- * there by the orthogonalisation that produced x) */
+/* iJacVec with |x| already in scal[KFS_SLOT_XNORM] (have >= 1) and the
+ * perturbed state already in w (have == 2), left there by the
+ * orthogonalisation that produced x */
 static void ijacvec(GEOMETRY *g, const double *x, const double *r, const double norm, GRADIENT *grad, double *w,
-                    double *y, int have_xnorm)
+                    double *y, int have)
 {
   const size_t sz = g->c->sz;
-  if (!have_xnorm) kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
+  if (have < 1) kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
   const double h = FD_EPS * norm;
-  kfs_fd_perturb(g, x, h, KFS_SLOT_XNORM, w);
+  if (have < 2) kfs_fd_perturb(g, x, h, KFS_SLOT_XNORM, w);
   ComputeFlux(g, w, grad, y);
   kfs_fd_combine(g, r, w, h, KFS_SLOT_XNORM, y);
 }
@@ -221,9 +222,10 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
         hh[it + 1] = h2[it + 1];
       } else {
         /* modified Gram-Schmidt, src/kernel.c:129-141 */
-        /* then v = w / |w|, and |v| for the next iJacVec */
+        /* then v = w / |w|, and |v| and the perturbed state for the next
+         * iJacVec */
         have_xnorm = kfs_mgs_unit(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H, hh,
-                                  flexible ? -1 : KFS_SLOT_XNORM);
+                                  flexible ? -1 : KFS_SLOT_XNORM, FD_EPS * K->qnorm, flexible ? NULL : w);
       }
       /* Givens, src/kernel.c:143-173 */
       for (int i = 0; i < it; i++) {

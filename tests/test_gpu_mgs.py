@@ -152,7 +152,8 @@ def test_resident_mgs_unit_bitwise(contexts, n, nv):
     L = K.lib()
     unit = L.kfs_mgs_unit
     unit.restype = C.c_int
-    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                     C.c_double, C.c_void_p]
     _, fetch, launches = _fns()
     rng = np.random.default_rng(n % 997 + nv)
     stride = n + (n & 1) + 2
@@ -167,7 +168,7 @@ def test_resident_mgs_unit_bitwise(contexts, n, nv):
         before = launches(m.ref)
         w = K.DeviceVec(n, w0)
         h = np.empty(nv + 1)
-        unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data, -1)   # the values come back with the call
+        unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data, -1, 0.0, None)   # the values come back with the call
         assert m.last_error() == 0
         res[name] = (h, w.numpy(), launches(m.ref) - before)
         w.free()
@@ -213,7 +214,8 @@ def test_resident_mgs_abort_falls_back(mesh_a, monkeypatch):
     L = K.lib()
     unit = L.kfs_mgs_unit
     unit.restype = C.c_int
-    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                     C.c_double, C.c_void_p]
     aborts = L.kfs_mgs_aborts
     aborts.restype = C.c_ulonglong
     aborts.argtypes = [K.GP]
@@ -232,7 +234,7 @@ def test_resident_mgs_abort_falls_back(mesh_a, monkeypatch):
             for _ in range(2):   # the second call goes straight to the per-step kernels
                 w = K.DeviceVec(n, w0)
                 h = np.empty(nv + 1)
-                unit(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT, h.ctypes.data, -1)
+                unit(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT, h.ctypes.data, -1, 0.0, None)
                 out.append((h, w.numpy()))
                 w.free()
             assert m.last_error() == 0
@@ -255,7 +257,8 @@ def test_resident_mgs_leaves_next_norm(contexts, n):
     L = K.lib()
     unit = L.kfs_mgs_unit
     unit.restype = C.c_int
-    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                     C.c_double, C.c_void_p]
     nrm2 = L.kfs_nrm2
     nrm2.restype = None
     nrm2.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_int]
@@ -270,7 +273,7 @@ def test_resident_mgs_leaves_next_norm(contexts, n):
     m = contexts["resident"]
     w = K.DeviceVec(n, rng.standard_normal(n))
     h = np.empty(nv + 1)
-    assert unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data, XS) == 1
+    assert unit(m.ref, n, vb.ptr, stride, nv, w.ptr, SLOT, h.ctypes.data, XS, 0.0, None) == 1
     left = np.empty(1)
     fetch(m.ref, XS, 1, left.ctypes.data)
     nrm2(m.ref, n, w.ptr, XS)
@@ -281,3 +284,34 @@ def test_resident_mgs_leaves_next_norm(contexts, n):
     assert abs(again[0] - 1.0) < 1e-14
     w.free()
     vb.free()
+
+
+def test_resident_mgs_leaves_perturbed_state(contexts):
+    """With xslot and wpert (vectors of the state's length), the launch also
+    writes the next iJacVec's perturbed state (h / |v|) v + q -- the bits
+    kfs_fd_perturb forms from v, |v| and the context's state q."""
+    L = K.lib()
+    unit = L.kfs_mgs_unit
+    unit.restype = C.c_int
+    unit.argtypes = [K.GP, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                     C.c_double, C.c_void_p]
+    perturb = L.kfs_fd_perturb
+    perturb.restype = None
+    perturb.argtypes = [K.GP, C.c_void_p, C.c_double, C.c_int, C.c_void_p]
+    m = contexts["resident"]
+    n, nv, XS = m.sz, 4, SLOT + 60
+    fd_h = 1.490116119384766e-08 * 3.7
+    rng = np.random.default_rng(29)
+    Q = np.linalg.qr(rng.standard_normal((n, nv)))[0]
+    vb = K.DeviceVec(nv * n, np.ascontiguousarray(Q.T).ravel())
+    w = K.DeviceVec(n, rng.standard_normal(n))
+    wp = K.DeviceVec(n)
+    h = np.empty(nv + 1)
+    assert unit(m.ref, n, vb.ptr, n, nv, w.ptr, SLOT, h.ctypes.data, XS, fd_h, wp.ptr) == 2
+    ref = K.DeviceVec(n)
+    perturb(m.ref, w.ptr, fd_h, XS, ref.ptr)
+    a, b = wp.numpy(), ref.numpy()
+    assert m.last_error() == 0
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    for x in (vb, w, wp, ref):
+        x.free()
