@@ -150,6 +150,9 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     if (p->diag[i] - p->rowptr[i] > 7 || p->rowptr[i + 1] - p->diag[i] - 1 > 7) c->trsv_pf = 8;
   }
   if ((ev = getenv("KF_TRSV_HELD")) && atoi(ev) == 8) c->trsv_pf = 8;   /* 8 held entries on any mesh */
+  /* GMRES's combination formed by the forward sweep: -1 where the sweeps are
+   * throughput-bound, 0 never, 1 always */
+  c->trsv_fold = (ev = getenv("KF_TRSV_FOLD")) ? (atoi(ev) != 0) : -1;
   {
     kf_ilu_plan ip;
     kf_plan_ilu(p, &ip);
@@ -479,7 +482,33 @@ extern "C" void ComputeSparseTriangularSolve(const GEOMETRY *g, const double *b,
     fprintf(stderr, "kf: ComputeSparseTriangularSolve: b and x must not alias\n");
     abort();
   }
-  kf_launch_sptrsv(kf_ctx_of(g), b, x);
+  kf_launch_sptrsv(kf_ctx_of(g), b, x, NULL);
+}
+
+/* GMRES's inner step: x = U^-1 L^-1 y with y = ((R(w) - r) + cdt (w - q))
+ * |x| / h, b = R(w) -- iJacVec's combination (src/kernel.c:33-44), then
+ * ComputeSparseTriangularSolve.  Where the sweeps are throughput-bound the
+ * forward sweep forms y as it loads b (one pass less); b may be overwritten
+ * with y.  |x| in scal[slot_xnorm] */
+extern "C" void kfs_sptrsv_fd(GEOMETRY *g, const double *b, double *x, const double *r, const double *w, double h,
+                              int slot_xnorm)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (b == x) {
+    fprintf(stderr, "kf: kfs_sptrsv_fd: b and x must not alias\n");
+    abort();
+  }
+  kf_check_aligned(r, "r"); kf_check_aligned(w, "w");
+  if (c->trsv_fold == 0 || (c->trsv_fold < 0 && !kf_trsv_capped(c))) {
+    /* latency-bound sweeps: the four extra loads at each task's start
+     * would lengthen the chain (0.748 against 0.754 s per solve on the
+     * 0.33M-vertex lattice), so the combination runs on its own first */
+    kf_fd_combine(c, r, w, h, slot_xnorm, (double *)b);
+    kf_launch_sptrsv(c, b, x, NULL);
+    return;
+  }
+  const kf_fdb fd = {r, w, c->q, c->cdt, c->d.scal + slot_xnorm, h};
+  kf_launch_sptrsv(c, b, x, &fd);
 }
 
 /* inc/utils.h:18-28.  The reference's BLAS-1 take no GEOMETRY; these run on

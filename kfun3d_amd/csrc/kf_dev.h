@@ -113,6 +113,7 @@ struct kf_ctx {
   /* triangular solves: contiguous ranges of trsv_R rows, clustered into
    * tasks of up to trsv_S = 16 / R ranges (one wavefront each) */
   uint32_t trsv_R, trsv_S;
+  int trsv_fold;              /* GMRES's combination in the forward sweep: -1 by regime, 0 no, 1 yes */
   uint32_t trsv_pf;           /* off-diagonal entries per row held in registers (7 or 8):
                                 7 when no row has more below or above the diagonal */
   uint32_t *trow_l, *trow_u; /* per task (ticket order) and row slot: row, block bounds */
@@ -196,7 +197,15 @@ void kf_launch_timestep_div(kf_ctx *c, const double *sr, double cfl);
 void kf_launch_forces(kf_ctx *c, double *forces_host);
 void kf_launch_fill(kf_ctx *c);
 void kf_launch_ilu(kf_ctx *c);
-void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x);
+/* the finite-difference combination applied by the forward sweep as it
+ * loads its right-hand side b = R(w): ((R(w) - r) + cdt (w - q0)) |x| / h,
+ * |x| = *xnorm (src/kernel.c:33-44) */
+struct kf_fdb {
+  const double *r, *w, *q0, *cdt, *xnorm;
+  double h;
+};
+void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x, const kf_fdb *fd);
+bool kf_trsv_capped(const kf_ctx *c);
 
 /* BLAS-1 on the context stream; reductions are deterministic (fixed
  * partition, fixed tree) and leave their result in d.scal[slot] */

@@ -250,13 +250,13 @@ __device__ __forceinline__ TV row_sweep(const kf_dv &d, const uint8_t *__restric
 
 /* forward sweep: y = L^-1 b; presets the backward output x of its rows to
  * the backward sentinel */
-template <int PF, typename TM, typename TV>
+template <int PF, typename TM, typename TV, bool FD>
 __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const uint32_t *__restrict__ trow,
                                                    const uint8_t *__restrict__ tlvl, const uint8_t *__restrict__ tdep,
                                                    const uint8_t *__restrict__ eslot, uint32_t ntask, uint32_t *ticket,
                                                    uint32_t base, typename sentinel<TV>::bits sy,
                                                    typename sentinel<TV>::bits sx, const TM *__restrict__ a,
-                                                   const double *__restrict__ b, TV *y, TV *xnext)
+                                                   const double *__restrict__ b, TV *y, TV *xnext, kf_fdb fd)
 {
   __shared__ TV lds_all[KF_TRSV_GMAX][KF_TRSV_LDS];
   __shared__ uint32_t sbatch;
@@ -278,7 +278,18 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
       const uint32_t r = rt[0], j0 = rt[1], jd = rt[2];
       const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
       const bool active = r != ~0u;
-      TV xr = active ? (TV)b[4 * (size_t)r + c] : TV(0);
+      TV xr = TV(0);
+      if (active) {
+        const size_t o = 4 * (size_t)r + c;
+        if (FD) {
+          /* b is R(w): the finite-difference product first, as
+           * src/kernel.c:33-44 forms it, ((R(w) - r) + cdt (w - q)) |x| / h */
+          const double am = *fd.xnorm / fd.h, cd = fd.cdt[r];
+          xr = (TV)((b[o] - fd.r[o] + cd * (fd.w[o] - fd.q0[o])) * am);
+        } else {
+          xr = (TV)b[o];
+        }
+      }
       if (active) xnext[4 * (size_t)r + c] = sentinel<TV>::val(sx);
       uint32_t off[PF];
       TV mp[PF][4];
@@ -371,7 +382,8 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
 /* forward output y (held between calls, preset to the next call's forward
  * sentinel by the previous backward sweep) and backward output x */
 template <int PF, typename TM, typename TV>
-static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslot, TV *x, double *xout)
+static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslot, TV *x, double *xout,
+                        const kf_fdb *fd)
 {
   const uint32_t G = c->trsv_group;
   /* persistent workgroups of G wavefronts: one atomic per G tickets */
@@ -395,9 +407,15 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
     KF_LAUNCH_CHECK();
   }
   int s = kf_prof_begin(c, KP_TRSV_L);
-  hipLaunchKernelGGL((k_trsv_lower<PF, TM, TV>), dim3(nwl), dim3(64 * G), 0, c->stream, c->d, c->trow_l, c->tlvl_l,
-                     c->tdep_l, c->eslot, c->ntask_l, c->ticket + 0, c->ticket_base[0], sentinel<TV>::of(2 * k),
-                     sentinel<TV>::of(2 * k + 1), a, b, y, x);
+  const kf_fdb none = {NULL, NULL, NULL, NULL, NULL, 1.0};
+  if (fd)
+    hipLaunchKernelGGL((k_trsv_lower<PF, TM, TV, true>), dim3(nwl), dim3(64 * G), 0, c->stream, c->d, c->trow_l,
+                       c->tlvl_l, c->tdep_l, c->eslot, c->ntask_l, c->ticket + 0, c->ticket_base[0],
+                       sentinel<TV>::of(2 * k), sentinel<TV>::of(2 * k + 1), a, b, y, x, *fd);
+  else
+    hipLaunchKernelGGL((k_trsv_lower<PF, TM, TV, false>), dim3(nwl), dim3(64 * G), 0, c->stream, c->d, c->trow_l,
+                       c->tlvl_l, c->tdep_l, c->eslot, c->ntask_l, c->ticket + 0, c->ticket_base[0],
+                       sentinel<TV>::of(2 * k), sentinel<TV>::of(2 * k + 1), a, b, y, x, none);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   c->ticket_base[0] += kf_ticket_advance_groups(c->ntask_l, nwl, G);
@@ -412,24 +430,34 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
   c->ystate[yslot] = k + 1;
 }
 
-void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x)
+/* whether the sweeps run at their resident-wave cap (the schedule offers
+ * more tasks per level than fit: throughput-bound, as on the 2.65M-vertex
+ * lattice) rather than below it (latency-bound along the task chain, as on
+ * the 0.33M-vertex one) */
+bool kf_trsv_capped(const kf_ctx *c)
+{
+  const uint32_t cap = (c->opt.prec == KF_PREC_FP64 && c->trsv_pf == KF_TRSV_PF) ? 2048u : 3072u;
+  return c->trsv_waves ? c->trsv_waves >= cap : c->trsv_auto >= cap;
+}
+
+void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x, const kf_fdb *fd)
 {
   kf_check_aligned(b, "b"); kf_check_aligned(x, "x");
   const size_t n = 4 * (size_t)c->d.N;
   if (c->opt.prec == KF_PREC_FP64) {
     if (!c->ty64) KF_CHECK(hipMalloc((void **)&c->ty64, n * sizeof(double)));
-    if (c->trsv_pf == 7) launch_trsv<7, double, double>(c, c->a64, b, c->ty64, 0, x, x);
-    else launch_trsv<KF_TRSV_PF, double, double>(c, c->a64, b, c->ty64, 0, x, x);
+    if (c->trsv_pf == 7) launch_trsv<7, double, double>(c, c->a64, b, c->ty64, 0, x, x, fd);
+    else launch_trsv<KF_TRSV_PF, double, double>(c, c->a64, b, c->ty64, 0, x, x, fd);
     return;
   }
   if (!c->f32a) KF_CHECK(hipMalloc((void **)&c->f32a, n * sizeof(float)));
   if (!c->f32b) KF_CHECK(hipMalloc((void **)&c->f32b, n * sizeof(float)));
   const bool p7 = c->trsv_pf == 7;
   if (c->opt.prec == KF_PREC_FP32) {
-    if (p7) launch_trsv<7, float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x);
-    else launch_trsv<KF_TRSV_PF, float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x);
+    if (p7) launch_trsv<7, float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x, fd);
+    else launch_trsv<KF_TRSV_PF, float, float>(c, c->a32, b, c->f32a, 1, c->f32b, x, fd);
   } else {
-    if (p7) launch_trsv<7, uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x);
-    else launch_trsv<KF_TRSV_PF, uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x);
+    if (p7) launch_trsv<7, uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x, fd);
+    else launch_trsv<KF_TRSV_PF, uint16_t, float>(c, c->abf, b, c->f32a, 1, c->f32b, x, fd);
   }
 }

@@ -116,6 +116,11 @@ void kfs_scale_recip(GEOMETRY *g, size_t n, double *x, int slot);
 void kfs_axpy_slot(GEOMETRY *g, size_t n, int slot, const double *x, double *y);
 /* modified Gram-Schmidt of w against nv vectors vbase + k stride: h_k in
  * slots slot..slot+nv-1, |w| in slot+nv (src/kernel.c:129-141) */
+/* x = U^-1 L^-1 y, y = ((b - r) + cdt (w - q)) |x| / h formed by the
+ * forward sweep from b = R(w) (iJacVec's combination folded into the solve);
+ * |x| in scal[slot_xnorm] */
+void kfs_sptrsv_fd(GEOMETRY *g, const double *b, double *x, const double *r, const double *w, double h,
+                   int slot_xnorm);
 void kfs_mgs(GEOMETRY *g, size_t n, const double *vbase, size_t stride, int nv, double *w, int slot);
 /* kfs_mgs, then w scaled to unit length (x *= 1.0 / |w|, as kfs_scale_recip);
  * out (if not NULL) receives h_0 .. h_{nv-1}, |w|.  xslot >= 0: the norm of

@@ -43,6 +43,21 @@ static void ijacvec(GEOMETRY *g, const double *x, const double *r, const double 
   kfs_fd_combine(g, r, w, h, KFS_SLOT_XNORM, y);
 }
 
+/* GMRES's inner step M^-1 (J x): iJacVec up to the residual R(w) in y, then
+ * the combination and the triangular solves (kfs_sptrsv_fd: where the sweeps
+ * are throughput-bound the forward sweep forms the combination as it loads
+ * y, with the operations of kfs_fd_combine); y is scratch afterwards */
+static void ijacvec_solve(GEOMETRY *g, const double *x, const double *r, const double norm, GRADIENT *grad,
+                          double *w, double *y, int have, double *out)
+{
+  const size_t sz = g->c->sz;
+  if (have < 1) kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
+  const double h = FD_EPS * norm;
+  if (have < 2) kfs_fd_perturb(g, x, h, KFS_SLOT_XNORM, w);
+  ComputeFlux(g, w, grad, y);
+  kfs_sptrsv_fd(g, y, out, r, w, h, KFS_SLOT_XNORM);
+}
+
 void iJacVec(const GEOMETRY *gc, const double *x, const double *r, const double norm, GRADIENT *grad, double *w,
              double *y)
 {
@@ -199,8 +214,7 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
     int it = 0, have_xnorm = 0;   /* |VEC(it)| left by the previous orthogonalisation */
     while (it < restart) {
       if (!flexible) {
-        ijacvec(g, VEC(it), K->r, K->qnorm, &K->grad, w, t, have_xnorm);
-        ComputeSparseTriangularSolve(g, t, VEC(it + 1));
+        ijacvec_solve(g, VEC(it), K->r, K->qnorm, &K->grad, w, t, have_xnorm, VEC(it + 1));
       } else {
         ComputeSparseTriangularSolve(g, VEC(it), ZVEC(it)); /* z = M(v) */
         if (use_net) add_correction(K, ZVEC(it));

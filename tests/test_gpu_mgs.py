@@ -315,3 +315,28 @@ def test_resident_mgs_leaves_perturbed_state(contexts):
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
     for x in (vb, w, wp, ref):
         x.free()
+
+
+@pytest.mark.parametrize("which", ["mesh_a", "c1"])
+def test_solve_history_with_folded_combination(which, mesh_a, mesh_c1, monkeypatch):
+    """GMRES's finite-difference combination formed by the forward sweep as
+    it loads its right-hand side (KF_TRSV_FOLD=1; by default only where the
+    sweeps are throughput-bound) gives the history of the separate
+    combination kernel bit for bit."""
+    path = {"mesh_a": mesh_a, "c1": mesh_c1}[which]
+    hist = {}
+    for val in ("0", "1"):
+        monkeypatch.setenv("KF_TRSV_FOLD", val)
+        m = K.Mesh(path, K.options())   # the reference profile: 15 steps
+        try:
+            hist[val] = m.kernel()
+            assert m.last_error() == 0
+        finally:
+            m.close()
+    a, b = hist["0"], hist["1"]
+    assert len(a) == len(b) and len(a) > 5
+    for ra, rb in zip(a, b):
+        assert ra["krylov"] == rb["krylov"], (ra, rb)
+        fa = np.array([ra["cfl"], ra["fnorm"]] + ra["forces"])
+        fb = np.array([rb["cfl"], rb["fnorm"]] + rb["forces"])
+        assert np.array_equal(fa.view(np.uint64), fb.view(np.uint64)), (ra, rb)
