@@ -692,9 +692,13 @@ int kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, doub
      * synchronisation) instead of waiting for the whole launch, whose tail
      * overlaps the caller's work; later launches follow it in the stream */
     const volatile double *flag = c->hscal + slot + nv + 1;
+    if (c->mgs_nopoll) KF_CHECK(hipStreamSynchronize(c->stream));
     double fv = *flag;
     for (uint64_t k = 0; fv == -1.0; k++) {
       if (k >= (1ull << 22)) {
+        /* the flag did not show while the launch ran: synchronise, here and
+         * from now on */
+        c->mgs_nopoll = 1;
         KF_CHECK(hipStreamSynchronize(c->stream));
         fv = *flag;
         break;
