@@ -143,7 +143,7 @@ typedef struct kf_options {
   int    ts_max;        /* TS_MAX   (reference 15)                            */
   double fnorm_ratio;   /* stop when fnorm_init/fnorm >= this (reference 1e7,
                            evaluated in double, not through uint32)          */
-  int    gmres_restart; /* GMRES_RESTART (30, at most 30)                     */
+  int    gmres_restart; /* GMRES_RESTART (reference 30; 1 to 31 accepted)    */
   double gmres_rtol;    /* GMRES_RELATIVE_TOLERANCE (0.01)                    */
   int    gmres_max_it;  /* GMRES_MAX_IT (10000)                               */
   int    first_order;   /* 1: gradients forced to zero (first-order flux)     */

@@ -60,6 +60,22 @@ def build_library(verbose=False):
     return LIB
 
 
+C_DRIVER_SRC = os.path.join(ROOT, "tests", "c", "kf_main.c")
+C_DRIVER = os.path.join(ROOT, "tests", "c", "kf_main")
+
+
+def build_c_driver():
+    """tests/c/kf_main: a C program built only against include/kfun3d.h and
+    linked to the library the way the reference's driver would be (run by
+    the GPU tests; the rpath is relative, so the binary works wherever the
+    tree is copied)."""
+    tmp = C_DRIVER + ".tmp"
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), C_DRIVER_SRC,
+          "-L" + HERE, "-lkfun3d", "-Wl,-rpath,$ORIGIN/../../kfun3d_amd", "-o", tmp])
+    os.replace(tmp, C_DRIVER)
+    return C_DRIVER
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"])
     if os.path.isdir("/root/reference/src"):

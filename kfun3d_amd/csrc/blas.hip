@@ -226,6 +226,15 @@ __global__ __launch_bounds__(RB) void k_mgs_step(size_t n, const double *__restr
   if (threadIdx.x == 0) part_out[blockIdx.x] = s;
 }
 
+/* The resident launch's ready flag in the pinned (coherent) host mirror: a
+ * system-scope release store, so every earlier store of the thread (the
+ * Hessenberg values) is visible to the host before the flag is. */
+__device__ __forceinline__ void publish_flag(double *p, double v)
+{
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v),
+                     __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 /* A partial sum published by another workgroup of the launch: polled until
  * it no longer holds the call's sentinel (bounded; a thread that gives up
  * records KF_ERR_SPIN_LIMIT and stops waiting for the rest of the launch). */
@@ -246,7 +255,8 @@ __device__ __forceinline__ double poll_part(const double *p, unsigned long long 
 }
 
 /* The whole modified Gram-Schmidt of w against nv basis vectors in one
- * cooperative launch, w held in registers throughout (src/kernel.c:129-141
+ * launch whose workgroups are all resident at once (checked by the kernel
+ * itself, see below), w held in registers throughout (src/kernel.c:129-141
  * and the closing norm, src/kernel.c:138-141).
  *
  * The grid is k_dot_part's KF_NRED x RB threads folded two to one: thread t
@@ -321,7 +331,7 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
     }
     if (st != 1) {
       hslot[nv + 1] = 1.0;
-      hhost[nv + 1] = 1.0;
+      publish_flag(hhost + nv + 1, 1.0);
     }
     decision = st;
   }
@@ -420,13 +430,18 @@ __global__ __launch_bounds__(RB, 2) void k_mgs_res(size_t n, int nv, const doubl
     for (int j = threadIdx.x; j < KF_NRED; j += RB)
       acc += poll_part(part + (size_t)k * KF_NRED + j, sent, err, aborted, sleep);
     const double h = block_sum(acc, sh);
+    /* whether any thread of this workgroup gave up a wait (its h_k hold a
+     * placeholder then); uniform: self is the same for every thread */
+    const int gaveup = self ? __syncthreads_or(aborted ? 1 : 0) : 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       hslot[k] = hhost[k] = self ? sqrt(h) : h;
       if (self) {
-        /* every value the host needs is written: the ready flag, behind a
-         * system-scope fence (the host polls it while the launch finishes) */
-        __threadfence_system();
-        hslot[nv + 1] = hhost[nv + 1] = 0.0;
+        /* every value the host needs is written: the ready flag (0), or 2
+         * when a wait expired and the values are not valid, by a
+         * system-scope release store (the host polls it while the launch
+         * finishes) */
+        hslot[nv + 1] = gaveup ? 2.0 : 0.0;
+        publish_flag(hhost + nv + 1, gaveup ? 2.0 : 0.0);
       }
     }
     if (self) {   /* the closing norm: the loop ends */
@@ -620,7 +635,8 @@ static bool mgs_resident(kf_ctx *c, size_t n, const double *vbase, size_t stride
   const int force_abort = c->mgs_res == 3;
   double *xnorm_out = xslot >= 0 ? c->d.scal + xslot : NULL;
   double *hhost = c->hscal + slot;   /* the pinned host mirror, polled by the host */
-  c->hscal[slot + nv + 1] = -1.0;    /* not ready: the launch writes 0 (go, values written) or 1 (abort) */
+  c->hscal[slot + nv + 1] = -1.0;    /* not ready: the launch writes 0 (go, values written), 1 (abort:
+                                        nothing written) or 2 (a wait expired: the values are not valid) */
   const double *qstate = c->q;
   if (!xnorm_out || n != 4 * (size_t)c->d.N || !aligned16(wpert)) wpert = NULL;   /* needs the norm round */
   void *args[] = {&n,   &nv,    &vbase, &stride, &w,           &hslot,     &part,      (void *)&sent,
@@ -666,10 +682,11 @@ static void mgs_steps(kf_ctx *c, size_t n, const double *vbase, size_t stride, i
  * Gram-Schmidt: h_k in scal[slot + k], |w| in scal[slot + nv]; unit: w then
  * scaled to unit length (x *= 1.0 / |w|, kf_scale_recip); out (if not NULL):
  * the nv + 1 values fetched to the host.  A resident launch writes them to
- * the pinned host mirror as well and is followed by one synchronisation
- * (which the caller's fetch would cost anyway); if it aborted, w is
- * untouched and the call is repeated with the per-step kernels, which the
- * context then keeps.  xslot >= 0 (with unit):
+ * the pinned host mirror as well, then a ready flag there, which the host
+ * polls (the launch's tail overlaps the caller's work); if it aborted
+ * (workgroups not co-resident), w is untouched and the call is repeated
+ * with the per-step kernels, which the context then keeps; if one of its
+ * waits expired, out receives NaN.  xslot >= 0 (with unit):
  * the resident launch also leaves |w| of the scaled w in scal[xslot], as
  * kf_nrm2_dev would; wpert (with xslot, w of the state's length): the launch
  * also writes the next iJacVec's perturbed state (fd_h / |w|) w + q as
@@ -694,20 +711,52 @@ int kf_mgs(kf_ctx *c, size_t n, const double *vbase, size_t stride, int nv, doub
     const volatile double *flag = c->hscal + slot + nv + 1;
     if (c->mgs_nopoll) KF_CHECK(hipStreamSynchronize(c->stream));
     double fv = *flag;
-    for (uint64_t k = 0; fv == -1.0; k++) {
-      if (k >= (1ull << 22)) {
-        /* the flag did not show while the launch ran: synchronise, here and
-         * from now on */
-        c->mgs_nopoll = 1;
-        KF_CHECK(hipStreamSynchronize(c->stream));
-        fv = *flag;
-        break;
+    /* The wait ends when the flag shows, or when the launch has ended
+     * without it: after a first stretch of polls an event is recorded (the
+     * launch is the last work queued on the stream, so the event completes
+     * when it ends) and queried now and then.  Work queued ahead of the
+     * launch (a factorisation, training) only lengthens the wait; it never
+     * expires it. */
+    bool marked = false;
+    for (uint64_t k = 1; fv == -1.0; k++) {
+      if ((k & 0xffffu) == 0) {
+        if (!marked) {
+          if (!c->mgs_ev) KF_CHECK(hipEventCreateWithFlags(&c->mgs_ev, hipEventDisableTiming));
+          KF_CHECK(hipEventRecord(c->mgs_ev, c->stream));
+          marked = true;
+        } else {
+          const hipError_t q = hipEventQuery(c->mgs_ev);
+          if (q != hipErrorNotReady) {
+            KF_CHECK(q);
+            fv = *flag;
+            if (fv == -1.0) {
+              /* the launch ended and its flag never showed in the host
+               * mirror: synchronise, here and from now on (the values are
+               * fetched by copy) */
+              c->mgs_nopoll = 1;
+              KF_CHECK(hipMemcpyAsync(c->hscal + slot, c->d.scal + slot, (nv + 2) * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream));
+              KF_CHECK(hipStreamSynchronize(c->stream));
+              fv = *flag;
+            }
+            break;
+          }
+        }
       }
       __builtin_ia32_pause();
       fv = *flag;
     }
     __atomic_thread_fence(__ATOMIC_ACQUIRE);
     const double *tmp = c->hscal + slot;
+    if (fv == 2.0) {
+      /* a wait inside the launch expired (KF_ERR_SPIN_LIMIT is set): the
+       * values are placeholders, which the caller sees as non-finite */
+      KF_CHECK(hipStreamSynchronize(c->stream));
+      fprintf(stderr, "kf: resident Gram-Schmidt: a wait for another workgroup's partial sum expired\n");
+      if (out)
+        for (int i = 0; i <= nv; i++) out[i] = NAN;
+      return 0;
+    }
     if (fv == 0.0) {
       if (out)
         for (int i = 0; i <= nv; i++) out[i] = tmp[i];

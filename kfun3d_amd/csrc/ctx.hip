@@ -234,7 +234,12 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   KF_CHECK(hipMemset(d.scal, 0, KF_NSCAL * sizeof(double)));
   d.err = dalloc<uint32_t>(1);
   KF_CHECK(hipMemset(d.err, 0, sizeof(uint32_t)));
-  KF_CHECK(hipHostMalloc((void **)&c->hscal, KF_NSCAL * sizeof(double), hipHostMallocDefault));
+  /* coherent: the resident orthogonalisation's stores reach the host while
+   * the launch still runs (its ready flag is polled) */
+  KF_CHECK(hipHostMalloc((void **)&c->hscal, KF_NSCAL * sizeof(double), hipHostMallocCoherent));
+  /* KF_MGS_NOPOLL=1: wait for the resident launch by synchronising the
+   * stream instead of polling its flag (the values are the same) */
+  c->mgs_nopoll = (ev = getenv("KF_MGS_NOPOLL")) ? atoi(ev) != 0 : 0;
 
   /* free-stream state, src/guess.c:8-20 */
   double *qh = (double *)kf_xmalloc(4 * N * sizeof(double));
@@ -271,6 +276,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));
+  if (c->mgs_ev) KF_CHECK(hipEventDestroy(c->mgs_ev));
   KF_CHECK(hipStreamDestroy(c->stream));
   kf_unbind_ctx(c);
   delete c;
@@ -496,6 +502,11 @@ extern "C" void kfs_sptrsv_fd(GEOMETRY *g, const double *b, double *x, const dou
   kf_ctx *c = kf_ctx_of(g);
   if (b == x) {
     fprintf(stderr, "kf: kfs_sptrsv_fd: b and x must not alias\n");
+    abort();
+  }
+  if (x == r || x == w) {
+    /* the forward sweep presets x while other waves still read r and w */
+    fprintf(stderr, "kf: kfs_sptrsv_fd: x must not alias r or w\n");
     abort();
   }
   kf_check_aligned(r, "r"); kf_check_aligned(w, "w");

@@ -132,6 +132,7 @@ struct kf_ctx {
   uint32_t *mgs_sync;        /* per parity: arrival counter and go/abort decision of the launch */
   uint64_t mgs_aborts;       /* resident launches that found their workgroups not co-resident */
   int mgs_nopoll;            /* 1: wait for the resident launch by synchronising, not by polling its flag */
+  hipEvent_t mgs_ev;         /* marks the end of a resident launch whose flag is slow to show */
   int mgs_res;               /* 0: never, 1: wherever it fits, 3: always abort (tests the fallback) */
   int8_t mgs_ok[8];          /* per register-size variant: -1 unknown, 0 no, 1 co-resident */
   uint64_t ystate[2];       /* call whose forward sentinel y holds (FP64, FP32) */

@@ -78,22 +78,37 @@ static double *take_f64(cursor *c, size_t n)
   return out;
 }
 
-/* one boundary section: src/boundaries.c:105-141 */
-static void read_section(cursor *c, size_t nfaces, size_t nnodes, size_t ngroups, uint32_t **faces,
-                         struct bntbl *bn)
+/* one boundary section: src/boundaries.c:105-141.  Every index is checked
+ * against its range here (face corners against the section's node list,
+ * node ids against the N vertices): the reference trusts the file, but an
+ * out-of-range index would reach host arrays and device kernels alike. */
+static void read_section(cursor *c, const char *what, size_t N, size_t nfaces, size_t nnodes, size_t ngroups,
+                         uint32_t **faces, struct bntbl *bn)
 {
   take(c, 2 * ngroups * sizeof(uint32_t));
   uint32_t *fb = take_u32(c, 4 * nfaces);
   *faces = (uint32_t *)kf_xmalloc((3 * nfaces + 1) * sizeof(uint32_t));
   for (size_t i = 0; i < nfaces; i++) {
-    (*faces)[3 * i + 0] = fb[i] - 1;
-    (*faces)[3 * i + 1] = fb[i + nfaces] - 1;
-    (*faces)[3 * i + 2] = fb[i + 2 * nfaces] - 1;
+    for (int k = 0; k < 3; k++) {
+      const uint32_t v = fb[i + (size_t)k * nfaces];   /* 1-based in the file */
+      if (v < 1 || v > nnodes) {
+        fprintf(stderr, "kf: %s: %s face %zu corner %d is node-list entry %u, outside 1..%zu\n", c->path, what, i,
+                k, v, nnodes);
+        abort();
+      }
+      (*faces)[3 * i + k] = v - 1;
+    }
   }
   free(fb);
   bn->sz = nnodes;
   bn->nptr = take_u32(c, nnodes);
-  for (size_t i = 0; i < nnodes; i++) bn->nptr[i]--;
+  for (size_t i = 0; i < nnodes; i++) {
+    if (bn->nptr[i] < 1 || bn->nptr[i] > N) {
+      fprintf(stderr, "kf: %s: %s node %zu is vertex %u, outside 1..%zu\n", c->path, what, i, bn->nptr[i], N);
+      abort();
+    }
+    bn->nptr[i]--;
+  }
   double *nb = take_f64(c, 3 * nnodes);
   bn->xyz = (struct xyz *)kf_xmalloc(sizeof(struct xyz));
   bn->xyz->x0 = (double *)kf_xmalloc((nnodes + 1) * sizeof(double));
@@ -130,8 +145,13 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
   for (int i = 0; i < 13; i++) p[i] = rd_u32(h + 4 * i);
 
   /* edges: src/edges.c:22-87 */
-  const size_t E = p[2];
+  const size_t E = p[2], N = p[1];
   uint32_t *b0 = take_u32(&c, 2 * E);
+  for (size_t i = 0; i < 2 * E; i++)
+    if (b0[i] < 1 || b0[i] > N) {
+      fprintf(stderr, "kf: %s: edge %zu end %zu is vertex %u, outside 1..%zu\n", fpath, i % E, i / E, b0[i], N);
+      abort();
+    }
   double *b1 = take_f64(&c, 4 * E);
   uint32_t *perm = (uint32_t *)kf_xmalloc((E + 1) * sizeof(uint32_t));
   kf_index_sort(E, b0, perm);
@@ -158,7 +178,6 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
   g->e = e;
 
   /* nodes: src/nodes.c:19-56 */
-  const size_t N = p[1];
   double *nb = take_f64(&c, 4 * N);
   struct ntbl *n = (struct ntbl *)kf_xcalloc(1, sizeof(struct ntbl));
   n->sz = N;
@@ -181,8 +200,8 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
   b->f = (struct bntbl *)kf_xcalloc(1, sizeof(struct bntbl));
   b->fc = (struct bftbl *)kf_xcalloc(1, sizeof(struct bftbl));
   uint32_t *sfaces = NULL, *ffaces = NULL;
-  read_section(&c, p[6], p[9], p[3], &sfaces, b->s);
-  read_section(&c, p[8], p[11], p[5], &ffaces, b->f);
+  read_section(&c, "solid", N, p[6], p[9], p[3], &sfaces, b->s);
+  read_section(&c, "free-stream", N, p[8], p[11], p[5], &ffaces, b->f);
   free(ffaces);
   const size_t S = p[6];
   b->fc->sz = S;

@@ -160,6 +160,7 @@ typedef struct {
   double *r, *w, *t, *vecs, *ghh;
   double qnorm;
   learn_state *L;
+  int failed;          /* a Hessenberg value came back non-finite: the solve stopped */
 } kspace;
 
 #define VEC(k) (K->vecs + (size_t)(k) * K->sz)
@@ -257,6 +258,12 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
       *hh = ct * *hh + st * *(hh + 1);
       const double res = fabs(rs[it + 1]);
       it++;
+      if (!isfinite(res)) {
+        /* a non-finite orthogonalisation (a device wait that expired, or a
+         * singular preconditioner): no update is formed from it */
+        K->failed = 1;
+        break;
+      }
       if (res <= tol) {
         flag = 1;
         break;
@@ -264,6 +271,7 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
       if (cap > 0 && total + it >= cap) break;
     }
     total += it;
+    if (K->failed) break;
     /* back substitution, src/kernel.c:178-188 */
     rs[it - 1] /= ghh[(size_t)(it - 1) * (nb + 1) + (it - 1)];
     for (int i = 1; i <= it - 1; i++) {
@@ -418,6 +426,12 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
     kfs_flux_spectral(g, g->q->q, &K->grad, K->r);
     kfs_nrm2(g, sz, K->r, KFS_SLOT_FNORM);
     const double fnorm = fetch1(g, KFS_SLOT_FNORM);
+    const int derr = kf_last_error(g);
+    if (derr) {
+      fprintf(stderr, "kf: device error word %d at pseudo-step %d (%s%s); the solve stops\n", derr, its,
+              (derr & 1) ? "non-finite pivot " : "", (derr & 2) ? "a device wait expired" : "");
+      break;
+    }
     if (!isfinite(fnorm)) {
       fprintf(stderr, "kf: non-finite residual norm at pseudo-step %d\n", its);
       break;
@@ -487,6 +501,11 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
         L->enabled = 1;
         L->since_judge = 0;
       }
+    }
+    if (K->failed) {
+      fprintf(stderr, "kf: non-finite Krylov process at pseudo-step %d (device error word %d); the solve stops "
+              "before the update\n", its, kf_last_error(g));
+      break;
     }
     kfs_axpy(g, sz, -1.0, x, g->q->q);
     double forces[3];

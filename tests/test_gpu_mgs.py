@@ -1,5 +1,7 @@
-"""Modified Gram-Schmidt in one cooperative launch with w held in registers,
-against the per-step kernels (src/kernel.c:129-141 and the closing norm):
+"""Modified Gram-Schmidt in one launch with w held in registers (an ordinary
+launch whose workgroups check their co-residency themselves and publish a
+ready flag to a pinned host mirror, which the host polls), against the
+per-step kernels (src/kernel.c:129-141 and the closing norm):
 every h_k, |w| and the orthogonalised w bit for bit, for vector lengths that
 give every register-size variant, odd lengths (the half pair) and lengths
 shorter than one pass of the reduction grid."""
@@ -317,26 +319,50 @@ def test_resident_mgs_leaves_perturbed_state(contexts):
         x.free()
 
 
-@pytest.mark.parametrize("which", ["mesh_a", "c1"])
-def test_solve_history_with_folded_combination(which, mesh_a, mesh_c1, monkeypatch):
-    """GMRES's finite-difference combination formed by the forward sweep as
-    it loads its right-hand side (KF_TRSV_FOLD=1; by default only where the
-    sweeps are throughput-bound) gives the history of the separate
-    combination kernel bit for bit."""
-    path = {"mesh_a": mesh_a, "c1": mesh_c1}[which]
-    hist = {}
-    for val in ("0", "1"):
-        monkeypatch.setenv("KF_TRSV_FOLD", val)
-        m = K.Mesh(path, K.options())   # the reference profile: 15 steps
-        try:
-            hist[val] = m.kernel()
-            assert m.last_error() == 0
-        finally:
-            m.close()
-    a, b = hist["0"], hist["1"]
+def _same_history(a, b):
     assert len(a) == len(b) and len(a) > 5
     for ra, rb in zip(a, b):
         assert ra["krylov"] == rb["krylov"], (ra, rb)
         fa = np.array([ra["cfl"], ra["fnorm"]] + ra["forces"])
         fb = np.array([rb["cfl"], rb["fnorm"]] + rb["forces"])
         assert np.array_equal(fa.view(np.uint64), fb.view(np.uint64)), (ra, rb)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("which", ["mesh_a", "c1"])
+def test_solve_history_with_folded_combination(which, prec, mesh_a, mesh_c1, monkeypatch):
+    """GMRES's finite-difference combination formed by the forward sweep as
+    it loads its right-hand side (KF_TRSV_FOLD=1; by default only where the
+    sweeps are throughput-bound, which includes the single-precision
+    preconditioners on the largest lattice) gives the history of the
+    separate combination kernel bit for bit, in every precision."""
+    path = {"mesh_a": mesh_a, "c1": mesh_c1}[which]
+    hist = {}
+    for val in ("0", "1"):
+        monkeypatch.setenv("KF_TRSV_FOLD", val)
+        m = K.Mesh(path, K.options(prec=prec))   # the reference profile: 15 steps
+        try:
+            hist[val] = m.kernel()
+            assert m.last_error() == 0
+        finally:
+            m.close()
+    _same_history(hist["0"], hist["1"])
+
+
+@pytest.mark.parametrize("which", ["mesh_a", "c1"])
+def test_solve_history_without_flag_polling(which, mesh_a, mesh_c1, monkeypatch):
+    """The host waits for the resident orthogonalisation by polling its
+    ready flag in the pinned host mirror; KF_MGS_NOPOLL=1 forces the
+    fallback (synchronise the stream), which must give the same history bit
+    for bit."""
+    path = {"mesh_a": mesh_a, "c1": mesh_c1}[which]
+    hist = {}
+    for val in ("0", "1"):
+        monkeypatch.setenv("KF_MGS_NOPOLL", val)
+        m = K.Mesh(path, K.options())
+        try:
+            hist[val] = m.kernel()
+            assert m.last_error() == 0
+        finally:
+            m.close()
+    _same_history(hist["0"], hist["1"])
