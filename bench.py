@@ -1,30 +1,45 @@
 """Benchmark of the MI355X pseudo-transient Newton-Krylov path.
 
-Workload (the configuration the metric is quoted on, one GPU): the 144x48x48
-Kuhn lattice, seed 1 (331,776 vertices, 2,258,399 edges), second-order
-residual, plain FP64 GPU path: block ILU(0), GMRES(30) with modified
-Gram-Schmidt, CFL ramp from 10, stop at a 1e-10 drop of the nonlinear
-residual.  One "step" is one complete solve from the free stream; ``value``
-is its wall-clock in seconds (mesh already resident in HBM).
+Workload (the headline configuration, one GPU): the 288x96x96 Kuhn lattice,
+seed 4 (2,654,208 vertices, 18,322,367 edges), second-order residual,
+GMRES(30) with modified Gram-Schmidt, CFL ramp from 10, stop at a 1e-10 drop
+of the nonlinear residual.  One "step" is one complete solve from the free
+stream; ``value`` is its wall-clock in seconds (mesh already resident in
+HBM) for the fastest mixed-precision path: the Jacobian fill, ILU(0)
+factors and triangular solves in FP32 (or with the factors stored in BF16),
+the residual, Arnoldi, least squares and update in FP64.  The plain FP64
+path is timed beside it on the same mesh and context (``paths``,
+``speedup_vs_fp64``), with the force error of each path against it.
 
-The roofline object is for the kernel class that takes the most time in
-the timed solves (HIP events on the context stream, kf_profile); its
-algorithmic bytes per launch are the compulsory traffic of that kernel
-(formulas in ALG_BYTES).  cpu_baseline runs the reference's own CPU path
-(oracle/_ref/kref, built from the reference's sources) on the host cores
-for a bounded sample -- per-call times of ComputeFlux, ComputeA,
-ComputeNumericalILU, ComputeSparseTriangularSolve and one BLAS-1 pass on the
-same mesh -- and extrapolates the solve with the GPU run's call counts.
+roofline: the kernel class that takes the most time in the timed solves
+(HIP events on the context stream, kf_profile); achieved = its algorithmic
+bytes per launch (ALG_BYTES) / its average launch time; traffic = the HBM
+bytes per launch from the committed FETCH_SIZE / WRITE_SIZE counter passes
+(profiles/traffic_<config>.json), used only when the profile was taken on
+the same kernel (its exact name) built from the same sources (hash of the
+kernel source files), else null.
 
-mixed_precision repeats the solve with the FP32 and BF16-factor
-preconditioners; learned_preconditioner runs flexible GMRES with ILU(0)
-alone and with the learned correction trained online on the same mesh
-(collection and training timed separately).
+cpu_baseline: the reference's own Kernel() (oracle/_ref/kref, built from the
+reference's sources) in its reference profile on the 48x16x16 lattice of
+configuration 1, timed on all physical host cores and on one thread, next to
+the GPU's reference-profile solve of the same mesh.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--prec fp64|fp32|bf16]
-                    [--no-cpu] [--no-mixed] [--no-learned] [--learn-rhs R] [--learn-iters T]
+Also reported: the HBM footprint (hipMemGetInfo deltas), host setup time,
+the cost of exact reference-order reductions (reduce_seq) on the 144x48x48
+lattice, and learned_preconditioner -- configuration 4's recipe on the
+144x48x48 lattice (seed 1): flexible GMRES with ILU(0) alone against ILU(0)
+plus the learned correction trained online on pairs from the first 10
+pseudo-time steps (the Newton residual and 64 seed-3 Gaussian right-hand
+sides each, solved to 1e-10), 2000 Adam steps at learning rate 1e-3.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c3|c2]
+                    [--no-cpu] [--no-mixed] [--no-learned] [--no-seq]
+                    [--learn-rhs R] [--learn-iters T] [--learn-config c2|c5]
+                    [--kernel solve|step|copy]
 """
 import argparse
+import ctypes as C
+import hashlib
 import json
 import os
 import subprocess
@@ -36,15 +51,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "wall-clock to 1e-10 residual drop, mixed/learned vs plain FP64 GPU; flux GF/s"
-LATTICE = (144, 48, 48)
-SEED = 1
-CONFIGS = {"c2": ((144, 48, 48), 1), "c3": ((216, 72, 72), 2), "c5": ((288, 96, 96), 4)}
+CONFIGS = {"c1": ((48, 16, 16), 0), "c2": ((144, 48, 48), 1), "c3": ((216, 72, 72), 2), "c5": ((288, 96, 96), 4)}
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_GFLOPS = 78600.0
+# the kernel sources a traffic profile is valid for (hashed into the profile)
+KERNEL_SOURCES = ["kfun3d_amd/csrc/" + f for f in ("flux.hip", "fill.hip", "ilu.hip", "sptrsv.hip", "blas.hip",
+                                                   "handoff.h", "sync.h", "kf_dev.h")]
 
 
 def alg_bytes(kind, N, E, S, Nf, nnzb, s):
-    """Compulsory HBM bytes of one launch (each array moved once)."""
+    """Compulsory HBM bytes of one launch (each array moved once); s = bytes
+    per stored matrix value."""
     return {
         # idx 8E, normal+area 32E, LSQ terms 48E; xyz 24N, q 32N, r 32N, grad written + read 2*96N
         "residual": 88 * E + 280 * N,
@@ -59,91 +76,63 @@ def alg_bytes(kind, N, E, S, Nf, nnzb, s):
     }[kind]
 
 
-# kernel classes of kf_profile -> the FP64 kernel names of rocprofv3 traces
-PROFILE_TO_KERNEL = {"trsv_upper": "k_trsv_upper<double, double>", "trsv_lower": "k_trsv_lower<double, double>",
-                     "ilu": "k_ilu_rows<double, true>"}
-
-
-def measured_traffic(dom, prec, config="c2"):
-    """HBM bytes per launch of the dominant kernel from the committed counter
-    passes (profiles/traffic_c2_fp64.json: FETCH_SIZE and WRITE_SIZE runs,
-    scaled by the reading of a copy of known size); None when not covered."""
-    path = os.path.join(ROOT, "profiles", "traffic_c2_fp64.json")
-    if config != "c2" or prec != "fp64" or dom not in PROFILE_TO_KERNEL or not os.path.exists(path):
-        return None
-    rec = json.load(open(path))["kernels"].get(PROFILE_TO_KERNEL[dom])
-    return None if rec is None else rec["bytes"]
-
-
 PROFILE_TO_KIND = {"grad": "grad", "edge_flux": "edge_flux", "residual": "residual_gather", "timestep": "timestep",
                    "fill": "fill", "ilu": "ilu", "trsv_lower": "trsv_lower", "trsv_upper": "trsv_upper"}
+MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper")
+TYPE = {"fp64": ("double", "double"), "fp32": ("float", "float"), "bf16": ("unsigned short", "float")}
 
 
-def cpu_baseline(mesh_path, sizes, counts, host_threads):
-    kref = os.path.join(ROOT, "oracle", "_ref", "kref")
-    if not os.path.exists(kref):
-        return None
-    env = dict(os.environ, OMP_NUM_THREADS=str(host_threads), OMP_PROC_BIND="close")
-    reps = 100   # a few seconds of CPU work on 16 threads
-    t0 = time.time()
-    out = subprocess.run([kref, "time", mesh_path, str(host_threads), str(reps)], stdout=subprocess.PIPE, text=True,
-                         env=env, timeout=900, check=True).stdout
-    wall = time.time() - t0
-    t = json.loads(out.strip().splitlines()[-1])
-    est = (counts["flux"] * t["flux_s"] + counts["trsv"] * t["sptrsv_s"] + counts["fill"] * (t["fill_s"] + t["ilu_s"]) +
-           counts["blas_passes"] * t["blas_pass_s"])
-    return {"value": est, "unit": "s", "cores": host_threads, "kind": "reference",
-            "sample": ("reference src/*.c + matrix/*.cpp (gcc -O3 -fopenmp, no FMA) on the same %dx%dx%d mesh, %d "
-                       "threads, contiguous owner partition: per-call ComputeFlux %.4f s, ComputeSparseTriangularSolve "
-                       "%.4f s, ComputeA %.4f s, ComputeNumericalILU %.4f s, one BLAS-1 pass %.5f s (%d reps, %.1f s "
-                       "wall); value = those times x the GPU solve's call counts (%d residuals, %d solves, %d "
-                       "fills, %d BLAS passes)" % (sizes + (host_threads, t["flux_s"], t["sptrsv_s"], t["fill_s"],
-                                                           t["ilu_s"], t["blas_pass_s"], reps, wall, counts["flux"],
-                                                           counts["trsv"], counts["fill"], counts["blas_passes"])))}
+def kernel_name(dom, prec, held, fold):
+    """the rocprofv3 name of the kernel behind a kf_profile class (for the
+    forward sweep, the instantiation GMRES's inner steps launch)"""
+    tm, tv = TYPE[prec]
+    if dom == "trsv_upper":
+        return "k_trsv_upper<%d, %s, %s>" % (held, tm, tv)
+    if dom == "trsv_lower":
+        return "k_trsv_lower<%d, %s, %s, %s>" % (held, tm, tv, "true" if fold else "false")
+    if dom == "ilu":
+        return "k_ilu_rows<%s, %s>" % (("double", "true") if prec == "fp64" else ("float", "false"))
+    if dom == "fill":
+        return "k_fill<%s>" % ("double" if prec == "fp64" else "float")
+    return {"grad": "k_grad", "edge_flux": "k_edge_flux<false>", "residual": "k_residual<false>",
+            "timestep": "k_timestep"}[dom]
 
 
-def blas_passes(hist, restart):
-    """4N-vector passes of the driver's BLAS-1 per solve (MGS: a dot and an
-    axpy per basis vector, plus norms, scalings, the FD combine and update)."""
-    n = 0
-    for h in hist:
-        k = h["krylov"]
-        cycles = max(1, (k + restart - 1) // restart)
-        for j in range(k):
-            jj = j % restart
-            n += 2 * (jj + 1) + 2 + 4          # MGS dots + axpys, norm + scale, iJacVec norm/perturb/combine
-        n += cycles * 3 + k + 4                # restart residual, x += V y, update, |r|, |q|
-    return n
+def source_hash():
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
-def mixed_precision(K, m64, mesh, hist64):
-    """The same solve with the FP32 and the BF16-factor preconditioner
-    (residual, Arnoldi, least squares and update stay FP64): wall-clock
-    (one warm-up solve, one timed), Krylov count and force error against the
-    plain FP64 path just timed."""
+def measured_traffic(config, name):
+    """HBM bytes per launch of one kernel from the committed counter passes
+    (profiles/traffic_<config>.json), or None when the profile does not cover
+    this exact kernel built from the current sources."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
+    if not os.path.exists(path):
+        return None, "no profiles/traffic_%s.json" % config
+    prof = json.load(open(path))
+    if prof.get("source_hash") != source_hash():
+        return None, "profiles/traffic_%s.json was taken on other kernel sources" % config
+    rec = prof["kernels"].get(name)
+    if rec is None:
+        return None, "profiles/traffic_%s.json has no %s" % (config, name)
+    return rec["bytes"], "profiles/traffic_%s.md (FETCH_SIZE/WRITE_SIZE passes, copy-calibrated)" % config
+
+
+def held_entries(m):
+    """entries per row the triangular solves hold in registers: 7 when no row
+    has more than 7 blocks on either side of its diagonal, else 8"""
     import numpy as np
-    f64 = np.array(hist64[-1]["forces"])
-    k64 = sum(x["krylov"] for x in hist64)
-    out = {}
-    for prec in ("fp32", "bf16"):
-        log("mixed leg: %s" % prec)
-        m = K.Mesh(mesh, K.config_options(prec=prec))
-        q0 = m.state()
-        m.kernel()
-        m.set_state(q0)
-        m.synchronize()
-        t0 = time.perf_counter()
-        h = m.kernel()
-        m.synchronize()
-        t = time.perf_counter() - t0
-        assert m.last_error() == 0, "device error word set (%s)" % prec
-        f = np.array(h[-1]["forces"])
-        out[prec] = {"seconds": t, "speedup_vs_fp64": None, "pseudo_steps": len(h),
-                     "krylov_total": sum(x["krylov"] for x in h), "krylov_fp64": k64,
-                     "residual_drop": h[0]["fnorm"] / h[-1]["fnorm"],
-                     "force_rel_err_vs_fp64": (np.abs(f - f64) / np.abs(f64)).max()}
-        m.close()
-    return out
+    rp = np.empty(m.N + 1, dtype=np.uint32)
+    dg = np.empty(m.N, dtype=np.uint32)
+    import kfun3d_amd as K
+    K.lib().kf_pattern(m.ref, rp.ctypes.data, None, dg.ctypes.data)
+    below = (dg - rp[:-1]).max()
+    above = (rp[1:] - dg - 1).max()
+    return 7 if max(below, above) <= 7 else 8
 
 
 def log(msg):
@@ -151,28 +140,161 @@ def log(msg):
     print("bench: %s" % msg, file=sys.stderr, flush=True)
 
 
-def learned(K, mesh, hist64, rhs_per_step, train_iters, prec="fp64", verbose=0):
-    """The learned correction on the same mesh: flexible GMRES with ILU(0)
-    alone (the method without the network), then with the network trained
-    online -- pairs from the first 10 pseudo-time steps (the Newton
-    residual and rhs_per_step Gaussian right-hand sides, each solved to
-    1e-10 or 300 iterations), Adam with learning rate 1e-3 for train_iters
-    steps -- and used from step 11 on with judged steps and the ILU(0)
-    fallback.  Wall-clock with and without the collection and training.
-    prec: the preconditioner's precision (the combined form: fp32)."""
+def _hip():
+    try:
+        return C.CDLL("libamdhip64.so")
+    except OSError:
+        return C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+
+
+def mem_used():
+    """device memory in use (bytes), from hipMemGetInfo"""
+    hip = _hip()
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return total.value - free.value, total.value
+
+
+def lattice_mesh(K, tmp, config):
+    dims, seed = CONFIGS[config]
+    path = os.path.join(tmp, "%s.msh" % config)
+    if not os.path.exists(path):
+        log("writing the %dx%dx%d lattice (seed %d)" % (dims + (seed,)))
+        K.write_lattice(path, *dims, seed)
+    return path
+
+
+def timed_solve(m, q0):
+    m.set_state(q0)
+    m.synchronize()
+    t0 = time.perf_counter()
+    h = m.kernel()
+    m.synchronize()
+    return time.perf_counter() - t0, h
+
+
+def summary(h, t, f_ref=None):
     import numpy as np
-    f64 = np.array(hist64[-1]["forces"])
+    out = {"seconds": t, "pseudo_steps": len(h), "krylov_total": sum(x["krylov"] for x in h),
+           "residual_drop": h[0]["fnorm"] / h[-1]["fnorm"], "forces": h[-1]["forces"]}
+    if f_ref is not None:
+        f, f0 = np.array(h[-1]["forces"]), np.array(f_ref)
+        out["force_rel_err_vs_fp64"] = float((np.abs(f - f0) / np.abs(f0)).max())
+    return out
+
+
+def roofline(prof, prec, sizes, config, held, fold):
+    """the dominant kernel class of a profile and its roofline object"""
+    N, E, S, Nf = sizes
+    nnzb = N + 2 * E
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, n = prof[dom]
+    kind = PROFILE_TO_KIND[dom]
+    s = {"fp64": 8, "fp32": 4, "bf16": 2}[prec] if kind in MATRIX_KINDS else 8
+    ab = alg_bytes(kind, N, E, S, Nf, nnzb, s)
+    achieved = ab / (ms / n / 1e3) / 1e9
+    name = kernel_name(dom, prec, held, fold)
+    traffic, src = measured_traffic(config, name)
+    return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": ab,
+            "avg_launch_ms": ms / n, "launches": n, "traffic_source": src}
+
+
+def physical_cores():
+    """(physical cores this process may run on, the host's physical cores by
+    lscpu, CPUs in the affinity mask): the first counts distinct (package,
+    core) pairs of the CPUs in the affinity mask (sysfs topology)"""
+    cpus = sorted(os.sched_getaffinity(0))
+    host = None
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=30).stdout
+        kv = {l.split(":", 1)[0].strip(): l.split(":", 1)[1].strip() for l in out.splitlines() if ":" in l}
+        host = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except Exception:
+        pass
+    try:
+        cores = set()
+        for c in cpus:
+            base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        return len(cores), host, len(cpus)
+    except OSError:
+        return len(cpus), host, len(cpus)
+
+
+def cpu_baseline(K, tmp):
+    """The reference's own Kernel() (reference profile: alpha 3, CFL 50 to
+    1e5, at most 15 pseudo-time steps, stop at a 1e7 drop) on the 48x16x16
+    lattice, on all physical cores and on one thread, beside the GPU's
+    solve of the same mesh in the same profile."""
+    kref = os.path.join(ROOT, "oracle", "_ref", "kref")
+    if not os.path.exists(kref):
+        return None
+    mesh = lattice_mesh(K, tmp, "c1")
+    cores, phys, aff = physical_cores()
+    res = {}
+    for T in (cores, 1):
+        out = os.path.join(tmp, "kref_%d" % T)
+        env = dict(os.environ, OMP_NUM_THREADS=str(T), OMP_PROC_BIND="close")
+        log("cpu baseline: reference Kernel() on the 48x16x16 lattice, %d thread(s)" % T)
+        subprocess.run([kref, "solve", mesh, out, str(T)], stdout=subprocess.DEVNULL, env=env, timeout=900,
+                       check=True)
+        lines = open(os.path.join(out, "history.txt")).read().split("\n")
+        sec = [float(l.split()[1]) for l in lines if l.startswith("seconds")][0]
+        forces = [[float(x) for x in l.split()[1:]] for l in lines if l.startswith("forces")]
+        res[T] = (sec, len(forces), forces[-1])
+    m = K.Mesh(mesh, K.options())
+    q0 = m.state()
+    m.kernel()
+    t_gpu, h = timed_solve(m, q0)
+    m.close()
+    sec, steps, f = res[cores]
+    return {"value": sec, "unit": "s", "cores": cores, "kind": "reference",
+            "sample": ("the reference's Kernel() (src/kernel.c:47-232; src/*.c and matrix/*.cpp built with gcc -O3 "
+                       "-fopenmp, no FMA, contiguous owner partition) in its reference profile (alpha 3, CFL 50 to "
+                       "1e5, 15 pseudo-time steps at most, stop at a 1e7 drop) on the 48x16x16 lattice of "
+                       "configuration 1 (12,288 vertices, seed 0), second order: one whole solve, %d pseudo-time "
+                       "steps, on %d threads (physical cores %s, affinity %d CPUs)"
+                       % (steps, cores, phys, aff)),
+            "one_thread_s": res[1][0], "gpu_same_solve_s": t_gpu, "gpu_pseudo_steps": len(h),
+            "gpu_krylov_total": sum(x["krylov"] for x in h), "cpu_forces": f, "gpu_forces": h[-1]["forces"]}
+
+
+def exact_parity_cost(K, mesh_c2):
+    """the configuration-profile solve of the 144x48x48 lattice with the
+    default tree reductions and with index-order reductions (reduce_seq: the
+    reference's one-thread sums, the mode whose history equals the
+    reference's bit for bit)"""
+    out = {}
+    for name, seq in (("tree", 0), ("reduce_seq", 1)):
+        m = K.Mesh(mesh_c2, K.config_options(reduce_seq=seq))
+        q0 = m.state()
+        m.kernel()
+        t, h = timed_solve(m, q0)
+        m.close()
+        out[name] = {"seconds": t, "krylov_total": sum(x["krylov"] for x in h), "pseudo_steps": len(h)}
+        log("exact-parity cost: %s %.3f s" % (name, t))
+    out["ratio"] = out["reduce_seq"]["seconds"] / out["tree"]["seconds"]
+    out["workload"] = "144x48x48 lattice seed 1, FP64, configuration profile, one solve to a 1e-10 drop"
+    return out
+
+
+def learned(K, mesh, rhs_per_step, train_iters, prec="fp64", verbose=0):
+    """The learned correction on one mesh: flexible GMRES with ILU(0) alone
+    (the method without the network), then with the network trained online
+    -- pairs from the first 10 pseudo-time steps (the Newton residual and
+    rhs_per_step Gaussian right-hand sides, each solved to 1e-10 or 300
+    iterations), Adam with learning rate 1e-3 for train_iters steps -- and
+    used from step 11 on with judged steps and the ILU(0) fallback.
+    Wall-clock with and without the collection and training."""
+    import numpy as np
     m = K.Mesh(mesh, K.config_options(krylov="fgmres", prec=prec))
     log("learned leg: ILU(0)-FGMRES warm-up")
     q0 = m.state()
     m.kernel()
-    m.set_state(q0)
-    m.synchronize()
-    t0 = time.perf_counter()
-    hf = m.kernel()
-    m.synchronize()
-    t_ilu = time.perf_counter() - t0
-    log("learned leg: ILU(0)-FGMRES %.3f s; collecting pairs, training, solving" % t_ilu)
+    t_ilu, hf = timed_solve(m, q0)
+    log("learned leg: ILU(0)-FGMRES %.3f s; collecting pairs (%d per step), training (%d Adam steps), solving"
+        % (t_ilu, rhs_per_step + 1, train_iters))
     m.set_state(q0)
     if verbose:
         m.set_options(K.config_options(krylov="fgmres", prec=prec, verbose=1))
@@ -190,7 +312,7 @@ def learned(K, mesh, hist64, rhs_per_step, train_iters, prec="fp64", verbose=0):
     m.attach(None)
     net.close()
     m.close()
-    fl = np.array(hl[-1]["forces"])
+    ff, fl = np.array(hf[-1]["forces"]), np.array(hl[-1]["forces"])
     after = [x for x in hl if x["step"] >= 10]
     return {
         "precision": prec,
@@ -201,14 +323,15 @@ def learned(K, mesh, hist64, rhs_per_step, train_iters, prec="fp64", verbose=0):
                     "krylov_after_training": sum(x["krylov"] for x in after),
                     "krylov_ilu_same_steps": sum(x["krylov"] for x in hf if x["step"] >= 10),
                     "residual_drop": hl[0]["fnorm"] / hl[-1]["fnorm"],
-                    "force_rel_err_vs_fp64": float((np.abs(fl - f64) / np.abs(f64)).max()),
+                    "force_rel_err_vs_ilu_fgmres": float((np.abs(fl - ff) / np.abs(ff)).max()),
                     "judged_steps": [[x["step"], x["krylov_ilu"], x["krylov_net"]] for x in hl
                                      if x["krylov_ilu"] >= 0 and x["krylov_net"] >= 0],
                     "steps_with_network": st["steps_learned"], "fallback": bool(st["fallback"]),
                     "pairs": st["npairs"], "rhs_per_step": rhs_per_step, "tight_solve_iterations": st["pair_iters"],
                     "collection_s": st["pair_ms"] / 1e3, "training_s": st["train_ms"] / 1e3,
                     "adam_steps": train_iters,
-                    "heldout_residual_loss": {"ilu": st["loss_ilu"], "trained": st["loss_after"]}},
+                    "heldout_residual_loss": {"ilu": st["loss_ilu"], "before": st["loss_before"],
+                                              "trained": st["loss_after"]}},
     }
 
 
@@ -218,11 +341,7 @@ COPY_BYTES = 1 << 28
 def copy_probe(reps):
     """reps device-to-device copies of COPY_BYTES (the runtime's copy kernel):
     the reference point that scales FETCH_SIZE / WRITE_SIZE to bytes."""
-    import ctypes as C
-    try:
-        hip = C.CDLL("libamdhip64.so")
-    except OSError:
-        hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    hip = _hip()
     src, dst = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(src), C.c_size_t(COPY_BYTES)) == 0
     assert hip.hipMalloc(C.byref(dst), C.c_size_t(COPY_BYTES)) == 0
@@ -235,142 +354,167 @@ def copy_probe(reps):
     print(json.dumps({"probe": "copy", "bytes_per_copy": COPY_BYTES, "copies": max(reps, 1)}))
 
 
-def trsv_probe(K, m, reps, warmup):
-    """The C2 Jacobian, its ILU(0) and reps triangular solves (x = U^-1 L^-1 b),
-    for counter passes on the dominant kernels."""
-    import numpy as np
-    m.timestep(10.0)
-    m.fill()
-    m.ilu()
-    b = K.DeviceVec(m.sz, np.random.default_rng(0).standard_normal(m.sz))
-    x = K.DeviceVec(m.sz)
+def step_probe(K, mesh, prec, reps, warmup):
+    """reps first pseudo-time steps (configuration profile, ts_max 1: residual,
+    time step, fill, ILU(0), one GMRES solve, update, forces) from the free
+    stream, for hardware-counter passes over every kernel of the path"""
+    m = K.Mesh(mesh, K.config_options(prec=prec, ts_max=1))
+    q0 = m.state()
     for _ in range(warmup):
-        m.sptrsv(b, x)
-    m.profile(True)
-    for _ in range(max(reps, 1)):
-        m.sptrsv(b, x)
-    m.synchronize()
-    prof = {k: m.profile_get(k) for k in ("trsv_lower", "trsv_upper")}
+        timed_solve(m, q0)
+    ts = [timed_solve(m, q0)[0] for _ in range(max(reps, 1))]
     assert m.last_error() == 0
-    print(json.dumps({"probe": "trsv", "solves": max(reps, 1),
-                      "kernel_ms": {k: v[0] / v[1] for k, v in prof.items() if v[1]}}))
+    print(json.dumps({"probe": "step", "prec": prec, "steps": len(ts), "seconds": ts, "source_hash": source_hash()}))
     m.close()
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--config", default="c5", choices=["c2", "c3", "c5"],
+                    help="lattice: c5 288x96x96 seed 4 (the headline), c3 216x72x72 seed 2, c2 144x48x48 seed 1")
+    ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16"],
+                    help="the path `value` times: auto = the faster of FP32 and BF16 in the warm-up solves")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32/BF16-preconditioner comparison")
+    ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
-    ap.add_argument("--learn-rhs", type=int, default=8,
-                    help="Gaussian right-hand sides per collecting step (the full recipe: 64)")
+    ap.add_argument("--no-seq", action="store_true", help="skip the exact-parity (reduce_seq) cost")
+    ap.add_argument("--learn-rhs", type=int, default=64, help="Gaussian right-hand sides per collecting step")
     ap.add_argument("--learn-iters", type=int, default=2000, help="Adam steps")
+    ap.add_argument("--learn-config", default="c2", choices=["c2", "c5"],
+                    help="mesh of the learned leg: c2 (configuration 4) or c5 (the combined form)")
     ap.add_argument("--learn-prec", default="fp64", choices=["fp64", "fp32", "bf16"],
                     help="preconditioner precision of the learned leg (fp32: the combined form)")
     ap.add_argument("--progress", action="store_true",
                     help="also the solver's per-step lines in the learned leg (on stdout, before the JSON line)")
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS),
-                    help="lattice: c2 144x48x48 seed 1 (the metric's), c3 216x72x72 seed 2, c5 288x96x96 seed 4")
-    ap.add_argument("--kernel", default="solve", choices=["solve", "trsv", "copy"],
-                    help="solve: the benchmark; trsv / copy: short runs for hardware-counter passes (the "
-                         "triangular solves on the same mesh; a device-to-device copy of known size)")
+    ap.add_argument("--kernel", default="solve", choices=["solve", "step", "copy"],
+                    help="solve: the benchmark; step / copy: short runs for hardware-counter passes (first "
+                         "pseudo-time steps on the same mesh; a device-to-device copy of known size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path's multi-GPU sharding is a design, "
               "not an implementation); use --gpus 1", file=sys.stderr)
         sys.exit(2)
-
     if args.kernel == "copy":
         return copy_probe(args.steps)
-    global LATTICE, SEED
-    LATTICE, SEED = CONFIGS[args.config]
+    import numpy as np
     import kfun3d_amd as K
     tmp = tempfile.mkdtemp(prefix="kfbench_")
-    mesh = os.path.join(tmp, "%s.msh" % args.config)
-    log("writing the %dx%dx%d lattice" % LATTICE)
-    K.write_lattice(mesh, *LATTICE, SEED)
-    opts = K.config_options(prec=args.prec)
+    mesh = lattice_mesh(K, tmp, args.config)
+    if args.kernel == "step":
+        return step_probe(K, mesh, "fp64" if args.prec == "auto" else args.prec, args.steps, args.warmup)
+    dims, seed = CONFIGS[args.config]
+
+    used0, total = mem_used()
     log("setup")
-    m = K.Mesh(mesh, opts)
+    t0 = time.perf_counter()
+    m = K.Mesh(mesh, K.config_options(prec="fp64"))
+    m.synchronize()
+    setup_s = time.perf_counter() - t0
     q0 = m.state()
     h = m.host_arrays()
     N, E, S, Nf = m.N, m.E, len(h["fc_n0"]), len(h["f_nptr"])
-    nnzb = N + 2 * E
-    s_bytes = {"fp64": 8, "fp32": 4, "bf16": 2}[args.prec]
+    del h
+    held = held_entries(m)
+    log("setup %.2f s; FP64 warm-up solve" % setup_s)
+    t64w, h64 = timed_solve(m, q0)
+    used64, _ = mem_used()
+    f64 = h64[-1]["forces"]
+    log("FP64 %.3f s" % t64w)
 
-    if args.kernel == "trsv":
-        return trsv_probe(K, m, args.steps, args.warmup)
-    log("warm-up solves")
+    precs = ["fp32", "bf16"] if not args.no_mixed else []
+    warm = {"fp64": t64w}
+    for p in precs:
+        m.set_options(K.config_options(prec=p))
+        warm[p], _ = timed_solve(m, q0)
+        log("%s warm-up %.3f s" % (p, warm[p]))
+    used_all, _ = mem_used()
+    vprec = args.prec if args.prec != "auto" else (min(precs, key=lambda p: warm[p]) if precs else "fp64")
+
+    # the value path: W more warm-up solves, then K timed, profiled
+    m.set_options(K.config_options(prec=vprec))
     for _ in range(args.warmup):
-        m.set_state(q0)
-        m.kernel()
-    m.synchronize()
-    log("timed solves")
+        timed_solve(m, q0)
+    log("timed solves (%s)" % vprec)
     times, hist = [], None
     m.profile(True)
-    for _ in range(args.steps):
-        m.set_state(q0)
-        m.synchronize()
-        t0 = time.perf_counter()
-        hist = m.kernel()
-        m.synchronize()
-        times.append(time.perf_counter() - t0)
+    for i in range(args.steps):
+        t, hist = timed_solve(m, q0)
+        times.append(t)
+        log("solve %d/%d %.3f s" % (i + 1, args.steps, t))
     prof = {k: m.profile_get(k) for k in K.Mesh.PROFILED}
     m.profile(False)
     assert m.last_error() == 0, "device error word set"
-    drop = hist[0]["fnorm"] / hist[-1]["fnorm"]
-    value = max(times) if len(times) == 1 else sorted(times)[len(times) // 2]
+    value = sum(times) / len(times)
+    fold = {vprec: K.lib().kf_trsv_fold(m.ref)}
 
-    # dominant kernel and its roofline
-    dom = max(prof, key=lambda k: prof[k][0])
-    ms, n = prof[dom]
-    kind = PROFILE_TO_KIND[dom]
-    s_mat = 8 if kind not in ("fill", "ilu", "trsv_lower", "trsv_upper") else s_bytes
-    ab = alg_bytes(kind, N, E, S, Nf, nnzb, s_mat)
-    avg_s = ms / n / 1e3
-    achieved = ab / avg_s / 1e9
+    # the other paths beside it (FP64 profiled too)
+    paths = {vprec: summary(hist, value, f64 if vprec != "fp64" else None)}
+    paths[vprec]["timed_solves"] = len(times)
+    prof64 = None
+    for p in ["fp64"] + [x for x in precs if x != vprec]:
+        if p == vprec:
+            continue
+        m.set_options(K.config_options(prec=p))
+        reps = min(args.steps, 3) if p == "fp64" else 1
+        if p == "fp64":
+            m.profile(True)
+            fold["fp64"] = K.lib().kf_trsv_fold(m.ref)
+        ts = []
+        for _ in range(reps):
+            t, hp = timed_solve(m, q0)
+            ts.append(t)
+        if p == "fp64":
+            prof64 = {k: m.profile_get(k) for k in K.Mesh.PROFILED}
+            m.profile(False)
+        paths[p] = summary(hp, sum(ts) / len(ts), f64 if p != "fp64" else None)
+        paths[p]["timed_solves"] = reps
+        log("%s %.3f s" % (p, paths[p]["seconds"]))
+    assert m.last_error() == 0, "device error word set"
+    m.close()
+
+    sizes = (N, E, S, Nf)
     flux_s = sum(prof[k][0] / max(prof[k][1], 1) for k in ("grad", "edge_flux", "residual")) / 1e3
     flux_flops = 360 * E + 48 * S + 193 * Nf
-    counts = {"flux": prof["edge_flux"][1] // max(args.steps, 1), "trsv": prof["trsv_lower"][1] // max(args.steps, 1),
-              "fill": prof["fill"][1] // max(args.steps, 1), "blas_passes": blas_passes(hist, opts.gmres_restart)}
-    host_threads = min(16, len(os.sched_getaffinity(0)))
-    log("solve %.3f s" % max(times))
-    cpu = None if args.no_cpu else cpu_baseline(mesh, LATTICE, counts, host_threads)
-    mixed = None if (args.no_mixed or args.prec != "fp64") else mixed_precision(K, m, mesh, hist)
-    learn = None if (args.no_learned or args.prec != "fp64") else learned(K, mesh, hist, args.learn_rhs,
-                                                                           args.learn_iters, args.learn_prec,
-                                                                           1 if args.progress else 0)
-
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64" if args.prec == "fp64" else "f64 residual/Arnoldi, %s preconditioner" % args.prec,
+        "dtype": "f64" if vprec == "fp64" else "f64 residual/Arnoldi/update, %s preconditioner" % vprec,
         "data": "synthetic",
-        "config": {"workload": "%dx%dx%d Kuhn lattice seed %d (%d vertices), second order, %s preconditioner, "
-                               "GMRES(30) MGS, CFL ramp from 10, solve from free stream to a 1e-10 residual drop"
-                               % (LATTICE + (SEED, N, args.prec)),
-                   "lattice": list(LATTICE), "seed": SEED, "vertices": N, "edges": E},
-        "pseudo_steps": len(hist), "krylov_total": sum(x["krylov"] for x in hist), "residual_drop": drop,
+        "config": {"workload": "%dx%dx%d Kuhn lattice seed %d (%d vertices, %d edges), second order, %s "
+                               "preconditioner (block ILU(0)), GMRES(30) MGS, CFL ramp from 10, one solve from the "
+                               "free stream to a 1e-10 residual drop" % (dims + (seed, N, E, vprec)),
+                   "lattice": list(dims), "seed": seed, "vertices": N, "edges": E, "path": vprec},
+        "pseudo_steps": len(hist), "krylov_total": sum(x["krylov"] for x in hist),
+        "residual_drop": hist[0]["fnorm"] / hist[-1]["fnorm"],
+        "speedup_vs_fp64": paths["fp64"]["seconds"] / value if "fp64" in paths else None,
+        "paths": paths, "warmup_seconds": warm, "setup_s": setup_s,
+        "hbm_footprint": {"fp64_path_bytes": used64 - used0, "all_paths_bytes": used_all - used0,
+                          "device_total_bytes": total,
+                          "note": "hipMemGetInfo deltas: after the first FP64 solve (state, mesh, incidences, "
+                                  "BSR FP64 Jacobian and the exact ILU's side arrays, plans, Krylov basis), and "
+                                  "after the FP32/BF16 solves (their factors and vectors added)"},
         "flux_gflops": flux_flops / flux_s / 1e9, "flux_fp64_frac": flux_flops / flux_s / 1e9 / FP64_PEAK_GFLOPS,
         "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
         "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, args.prec, args.config),
-                     "alg_bytes_per_launch": ab, "avg_launch_ms": ms / n,
-                     "traffic_source": "profiles/traffic_c2_fp64.md (FETCH_SIZE/WRITE_SIZE passes, copy-calibrated)"},
-        "cpu_baseline": cpu,
-        "mixed_precision": mixed,
-        "learned_preconditioner": learn,
+        "roofline": roofline(prof, vprec, sizes, args.config, held, fold[vprec]),
+        "roofline_fp64": roofline(prof64, "fp64", sizes, args.config, held, fold["fp64"]) if prof64 else None,
     }
-    if mixed:
-        for v in mixed.values():
-            v["speedup_vs_fp64"] = value / v["seconds"]
+    if not args.no_seq or not args.no_learned:
+        mesh_c2 = lattice_mesh(K, tmp, "c2")
+    out["exact_parity_cost"] = None if args.no_seq else exact_parity_cost(K, mesh_c2)
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(K, tmp)
+    if args.no_learned:
+        out["learned_preconditioner"] = None
+    else:
+        lmesh = mesh_c2 if args.learn_config == "c2" else mesh
+        lr = learned(K, lmesh, args.learn_rhs, args.learn_iters, args.learn_prec, 1 if args.progress else 0)
+        ldims, lseed = CONFIGS[args.learn_config]
+        lr["mesh"] = "%dx%dx%d lattice seed %d" % (ldims + (lseed,))
+        out["learned_preconditioner"] = lr
     print(json.dumps(out))
-    m.close()
 
 
 if __name__ == "__main__":

@@ -192,6 +192,11 @@ void *kf_stream(const GEOMETRY *g);               /* the context's hipStream_t *
  * ComputeA) in FP64, nnzb x 16 row-major blocks in CSR order of
  * kf_pattern(); and the pattern itself (host copies). */
 size_t kf_nnzb(const GEOMETRY *g);
+/* 1 when GMRES's inner step lets the forward triangular sweep form iJacVec's
+ * finite-difference combination as it loads its right-hand side (chosen
+ * where the sweeps are throughput-bound; KF_TRSV_FOLD=0/1 forces it), 0 when
+ * the combination runs as its own kernel first (for tools and tests) */
+int kf_trsv_fold(const GEOMETRY *g);
 void kf_pattern(const GEOMETRY *g, unsigned int *rowptr, unsigned int *col, unsigned int *diag);
 void kf_get_blocks(const GEOMETRY *g, double *blocks_host);
 

@@ -423,6 +423,15 @@ extern "C" void kf_synchronize(const GEOMETRY *g) { KF_CHECK(hipStreamSynchroniz
 extern "C" void *kf_stream(const GEOMETRY *g) { return (void *)kf_ctx_of(g)->stream; }
 extern "C" size_t kf_nnzb(const GEOMETRY *g) { return kf_ctx_of(g)->nnzb; }
 
+/* whether GMRES's inner step has the forward sweep form the finite-difference
+ * combination at the current precision (where the sweeps are
+ * throughput-bound, or KF_TRSV_FOLD=1) */
+extern "C" int kf_trsv_fold(const GEOMETRY *g)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  return !(c->trsv_fold == 0 || (c->trsv_fold < 0 && !kf_trsv_capped(c)));
+}
+
 extern "C" void kf_pattern(const GEOMETRY *g, unsigned int *rowptr, unsigned int *col, unsigned int *diag)
 {
   kf_ctx *c = kf_ctx_of(g);
