@@ -25,8 +25,10 @@ configuration 1, timed on all physical host cores and on one thread, next to
 the GPU's reference-profile solve of the same mesh.
 
 Also reported: the HBM footprint (hipMemGetInfo deltas), host setup time,
-the cost of exact reference-order reductions (reduce_seq) on the 144x48x48
-lattice, and learned_preconditioner -- configuration 4's recipe on the
+the BSR product's HBM rate (roofline_spmv), the 144x48x48 lattice's solve in
+five modes (default, index-order reductions -- the cost of exact parity --,
+CGS2, flexible GMRES, the first-order BSR operator), and
+learned_preconditioner -- configuration 4's recipe on the
 144x48x48 lattice (seed 1): flexible GMRES with ILU(0) alone against ILU(0)
 plus the learned correction trained online on pairs from the first 10
 pseudo-time steps (the Newton residual and 64 seed-3 Gaussian right-hand
@@ -35,7 +37,7 @@ sides each, solved to 1e-10), 2000 Adam steps at learning rate 1e-3.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c3|c2]
                     [--no-cpu] [--no-mixed] [--no-learned] [--no-seq]
                     [--learn-rhs R] [--learn-iters T] [--learn-config c2|c5]
-                    [--kernel solve|step|copy]
+                    [--kernel solve|step|solve1|copy]
 """
 import argparse
 import ctypes as C
@@ -73,12 +75,14 @@ def alg_bytes(kind, N, E, S, Nf, nnzb, s):
         "ilu": 2 * 16 * s * nnzb + 4 * nnzb,
         "trsv_lower": E * (16 * s + 4) + 64 * N,  # lower blocks + cols; b in, x out
         "trsv_upper": (E + N) * (16 * s + 4) + 64 * N,
+        "spmv": (N + 2 * E) * (16 * s + 4) + 64 * N,   # every block + col; x in, y out
     }[kind]
 
 
 PROFILE_TO_KIND = {"grad": "grad", "edge_flux": "edge_flux", "residual": "residual_gather", "timestep": "timestep",
-                   "fill": "fill", "ilu": "ilu", "trsv_lower": "trsv_lower", "trsv_upper": "trsv_upper"}
-MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper")
+                   "fill": "fill", "ilu": "ilu", "trsv_lower": "trsv_lower", "trsv_upper": "trsv_upper",
+                   "spmv": "spmv"}
+MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper", "spmv")
 TYPE = {"fp64": ("double", "double"), "fp32": ("float", "float"), "bf16": ("unsigned short", "float")}
 
 
@@ -94,6 +98,8 @@ def kernel_name(dom, prec, held, fold):
         return "k_ilu_rows<%s, %s>" % (("double", "true") if prec == "fp64" else ("float", "false"))
     if dom == "fill":
         return "k_fill<%s>" % ("double" if prec == "fp64" else "float")
+    if dom == "spmv":
+        return "k_bsr_spmv<%s>" % ("double" if prec == "fp64" else "float")
     return {"grad": "k_grad", "edge_flux": "k_edge_flux<false>", "residual": "k_residual<false>",
             "timestep": "k_timestep"}[dom]
 
@@ -200,6 +206,33 @@ def roofline(prof, prec, sizes, config, held, fold):
             "avg_launch_ms": ms / n, "launches": n, "traffic_source": src}
 
 
+def spmv_rate(K, m, q0, sizes, config, reps=20):
+    """the BSR product y = A x (the first-order operator's kernel) with the
+    FP64 Jacobian of the free stream at CFL 10 on the same mesh: reps
+    profiled launches after three unprofiled ones"""
+    import numpy as np
+    m.set_options(K.config_options(prec="fp64", bsr_operator=1))
+    m.set_state(q0)
+    m.timestep(10.0)
+    m.fill()
+    xs = K.DeviceVec(m.sz, np.random.default_rng(0).standard_normal(m.sz))
+    ys = K.DeviceVec(m.sz)
+    for _ in range(3):
+        m.spmv(xs, ys)
+    m.profile(True)
+    for _ in range(reps):
+        m.spmv(xs, ys)
+    prof = {"spmv": m.profile_get("spmv")}
+    m.profile(False)
+    assert m.last_error() == 0
+    xs.free()
+    ys.free()
+    m.set_options(K.config_options(prec="fp64"))
+    out = roofline(prof, "fp64", sizes, config, 7, False)
+    log("BSR SpMV %.3f ms, %.0f GB/s" % (out["avg_launch_ms"], out["achieved"]))
+    return out
+
+
 def physical_cores():
     """(physical cores this process may run on, the host's physical cores by
     lscpu, CPUs in the affinity mask): the first counts distinct (package,
@@ -260,22 +293,37 @@ def cpu_baseline(K, tmp):
             "gpu_krylov_total": sum(x["krylov"] for x in h), "cpu_forces": f, "gpu_forces": h[-1]["forces"]}
 
 
-def exact_parity_cost(K, mesh_c2):
-    """the configuration-profile solve of the 144x48x48 lattice with the
-    default tree reductions and with index-order reductions (reduce_seq: the
-    reference's one-thread sums, the mode whose history equals the
-    reference's bit for bit)"""
-    out = {}
-    for name, seq in (("tree", 0), ("reduce_seq", 1)):
-        m = K.Mesh(mesh_c2, K.config_options(reduce_seq=seq))
-        q0 = m.state()
-        m.kernel()
+C2_VARIANTS = [("default", {}), ("reduce_seq", {"reduce_seq": 1}), ("cgs2", {"ortho": "cgs2"}),
+               ("fgmres", {"krylov": "fgmres"}), ("bsr_operator", {"bsr_operator": 1})]
+
+
+def c2_variants(K, mesh_c2):
+    """The configuration-profile solve of the 144x48x48 lattice (FP64) in the
+    default mode and with one option changed each: index-order reductions
+    (reduce_seq, the mode whose history equals the reference's bit for bit:
+    the cost of exact parity), batched classical Gram-Schmidt with
+    reorthogonalisation, flexible GMRES (the learned correction's frame) and
+    the assembled first-order Jacobian as the operator.  One warm-up and one
+    timed solve each, on one context."""
+    import numpy as np
+    m = K.Mesh(mesh_c2, K.config_options())
+    q0 = m.state()
+    out, f_ref = {}, None
+    for name, kw in C2_VARIANTS:
+        m.set_options(K.config_options(**kw))
+        timed_solve(m, q0)
         t, h = timed_solve(m, q0)
-        m.close()
-        out[name] = {"seconds": t, "krylov_total": sum(x["krylov"] for x in h), "pseudo_steps": len(h)}
-        log("exact-parity cost: %s %.3f s" % (name, t))
-    out["ratio"] = out["reduce_seq"]["seconds"] / out["tree"]["seconds"]
-    out["workload"] = "144x48x48 lattice seed 1, FP64, configuration profile, one solve to a 1e-10 drop"
+        assert m.last_error() == 0, name
+        f = np.array(h[-1]["forces"])
+        if f_ref is None:
+            f_ref = f
+        out[name] = {"seconds": t, "krylov_total": sum(x["krylov"] for x in h), "pseudo_steps": len(h),
+                     "residual_drop": h[0]["fnorm"] / h[-1]["fnorm"],
+                     "force_rel_err_vs_default": float((np.abs(f - f_ref) / np.abs(f_ref)).max())}
+        log("144x48x48 %s: %.3f s, %d Krylov, %d steps" % (name, t, out[name]["krylov_total"], len(h)))
+    m.close()
+    out["exact_parity_cost"] = out["reduce_seq"]["seconds"] / out["default"]["seconds"]
+    out["workload"] = "144x48x48 lattice seed 1, FP64 preconditioner, configuration profile, one solve to a 1e-10 drop"
     return out
 
 
@@ -354,17 +402,23 @@ def copy_probe(reps):
     print(json.dumps({"probe": "copy", "bytes_per_copy": COPY_BYTES, "copies": max(reps, 1)}))
 
 
-def step_probe(K, mesh, prec, reps, warmup):
-    """reps first pseudo-time steps (configuration profile, ts_max 1: residual,
-    time step, fill, ILU(0), one GMRES solve, update, forces) from the free
-    stream, for hardware-counter passes over every kernel of the path"""
-    m = K.Mesh(mesh, K.config_options(prec=prec, ts_max=1))
+def step_probe(K, mesh, prec, reps, warmup, ts_max):
+    """reps solves from the free stream (configuration profile) of ts_max
+    pseudo-time steps (1: residual, time step, fill, ILU(0), one GMRES solve,
+    update, forces -- every kernel of the path, for hardware-counter passes;
+    500: whole solves, for kernel traces), nothing else timed or run"""
+    m = K.Mesh(mesh, K.config_options(prec=prec, ts_max=ts_max))
     q0 = m.state()
     for _ in range(warmup):
         timed_solve(m, q0)
-    ts = [timed_solve(m, q0)[0] for _ in range(max(reps, 1))]
+    ts, h = [], None
+    for _ in range(max(reps, 1)):
+        t, h = timed_solve(m, q0)
+        ts.append(t)
     assert m.last_error() == 0
-    print(json.dumps({"probe": "step", "prec": prec, "steps": len(ts), "seconds": ts, "source_hash": source_hash()}))
+    print(json.dumps({"probe": "step" if ts_max == 1 else "solve", "prec": prec, "solves": len(ts), "seconds": ts,
+                      "pseudo_steps": len(h), "krylov_total": sum(x["krylov"] for x in h),
+                      "source_hash": source_hash()}))
     m.close()
 
 
@@ -380,7 +434,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
-    ap.add_argument("--no-seq", action="store_true", help="skip the exact-parity (reduce_seq) cost")
+    ap.add_argument("--no-seq", action="store_true",
+                    help="skip the 144x48x48 variants (exact-parity cost, CGS2, FGMRES, BSR operator)")
     ap.add_argument("--learn-rhs", type=int, default=64, help="Gaussian right-hand sides per collecting step")
     ap.add_argument("--learn-iters", type=int, default=2000, help="Adam steps")
     ap.add_argument("--learn-config", default="c2", choices=["c2", "c5"],
@@ -389,9 +444,10 @@ def main():
                     help="preconditioner precision of the learned leg (fp32: the combined form)")
     ap.add_argument("--progress", action="store_true",
                     help="also the solver's per-step lines in the learned leg (on stdout, before the JSON line)")
-    ap.add_argument("--kernel", default="solve", choices=["solve", "step", "copy"],
-                    help="solve: the benchmark; step / copy: short runs for hardware-counter passes (first "
-                         "pseudo-time steps on the same mesh; a device-to-device copy of known size)")
+    ap.add_argument("--kernel", default="solve", choices=["solve", "step", "solve1", "copy"],
+                    help="solve: the benchmark; step / solve1 / copy: short runs for profiles (first pseudo-time "
+                         "steps, or whole solves, of one precision on the same mesh; a device-to-device copy of "
+                         "known size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path's multi-GPU sharding is a design, "
@@ -403,8 +459,9 @@ def main():
     import kfun3d_amd as K
     tmp = tempfile.mkdtemp(prefix="kfbench_")
     mesh = lattice_mesh(K, tmp, args.config)
-    if args.kernel == "step":
-        return step_probe(K, mesh, "fp64" if args.prec == "auto" else args.prec, args.steps, args.warmup)
+    if args.kernel in ("step", "solve1"):
+        return step_probe(K, mesh, "fp64" if args.prec == "auto" else args.prec, args.steps, args.warmup,
+                          1 if args.kernel == "step" else 500)
     dims, seed = CONFIGS[args.config]
 
     used0, total = mem_used()
@@ -473,6 +530,7 @@ def main():
         paths[p]["timed_solves"] = reps
         log("%s %.3f s" % (p, paths[p]["seconds"]))
     assert m.last_error() == 0, "device error word set"
+    spmv = spmv_rate(K, m, q0, sizes=(N, E, S, Nf), config=args.config)
     m.close()
 
     sizes = (N, E, S, Nf)
@@ -501,10 +559,11 @@ def main():
         "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
         "roofline": roofline(prof, vprec, sizes, args.config, held, fold[vprec]),
         "roofline_fp64": roofline(prof64, "fp64", sizes, args.config, held, fold["fp64"]) if prof64 else None,
+        "roofline_spmv": spmv,
     }
     if not args.no_seq or not args.no_learned:
         mesh_c2 = lattice_mesh(K, tmp, "c2")
-    out["exact_parity_cost"] = None if args.no_seq else exact_parity_cost(K, mesh_c2)
+    out["variants_144x48x48"] = None if args.no_seq else c2_variants(K, mesh_c2)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(K, tmp)
     if args.no_learned:
         out["learned_preconditioner"] = None

@@ -157,6 +157,15 @@ typedef struct kf_options {
                            like the reference's one-thread loops (one
                            wavefront; for parity runs); 0 = fixed-order tree */
   kf_krylov krylov;     /* Krylov method (the reference: left GMRES)        */
+  int    bsr_operator;  /* 0: the Krylov operator is the matrix-free finite-
+                           difference product of the second-order residual
+                           (src/kernel.c:24-45, the reference); 1: the
+                           assembled first-order Jacobian A = J1 + diag(cdt)
+                           by BSR SpMV (kf_bsr_spmv; ComputeA then keeps a
+                           copy of A, which the ILU does not overwrite).  The
+                           residual stays second order, so both converge to
+                           the same steady state; 1 takes more, cheaper
+                           Newton steps (defect correction). */
 } kf_options;
 
 typedef struct kf_step_record {
@@ -197,6 +206,13 @@ size_t kf_nnzb(const GEOMETRY *g);
  * where the sweeps are throughput-bound; KF_TRSV_FOLD=0/1 forces it), 0 when
  * the combination runs as its own kernel first (for tools and tests) */
 int kf_trsv_fold(const GEOMETRY *g);
+/* y = A x (device vectors of 4N doubles, not aliased) with the assembled
+ * Jacobian of the last ComputeA -- FP64, or FP32 with a single-precision
+ * preconditioner -- which ComputeA keeps when kf_options.bsr_operator is
+ * set (aborts otherwise).  Per row, the blocks in ascending column, each
+ * component summed a[c][0] x0 + ... + a[c][3] x3 in that order, in FP64.
+ * No counterpart in the reference (it never multiplies by A). */
+void kf_bsr_spmv(const GEOMETRY *g, const double *x, double *y);
 void kf_pattern(const GEOMETRY *g, unsigned int *rowptr, unsigned int *col, unsigned int *diag);
 void kf_get_blocks(const GEOMETRY *g, double *blocks_host);
 
@@ -246,7 +262,8 @@ int  kf_last_error(const GEOMETRY *g);
 /* Per-kernel timing with HIP events on the context stream.  kf_profile(g, 1)
  * clears and starts it; kf_profile_get returns the total milliseconds and the
  * launch count of one kernel class ("grad", "edge_flux", "residual",
- * "timestep", "fill", "ilu", "trsv_lower", "trsv_upper"); -1 if unknown. */
+ * "timestep", "fill", "ilu", "trsv_lower", "trsv_upper", "spmv"); -1 if
+ * unknown. */
 void kf_profile(GEOMETRY *g, int on);
 int  kf_profile_get(GEOMETRY *g, const char *name, double *total_ms, long *count);
 

@@ -85,7 +85,7 @@ class Options(C.Structure):
                 ("ts_max", C.c_int), ("fnorm_ratio", C.c_double), ("gmres_restart", C.c_int),
                 ("gmres_rtol", C.c_double), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
                 ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int), ("ilu_exact", C.c_int),
-                ("reduce_seq", C.c_int), ("krylov", C.c_int)]
+                ("reduce_seq", C.c_int), ("krylov", C.c_int), ("bsr_operator", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -135,6 +135,7 @@ def _load():
         "kf_stream": (C.c_void_p, [GP]),
         "kf_nnzb": (C.c_size_t, [GP]),
         "kf_trsv_fold": (C.c_int, [GP]),
+        "kf_bsr_spmv": (None, [GP, DP, DP]),
         "kf_pattern": (None, [GP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "kf_get_blocks": (None, [GP, C.c_void_p]),
         "kf_write_lattice_msh": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_uint64]),
@@ -204,7 +205,7 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "ComputeSparseTriangularSolve", "iJacVec", "Kernel", "Compute2ndNorm", "ComputeAXPY",
                "ComputeNewAXPY", "Normalize", "Setup", "kf_default_options", "kf_set_options", "kf_get_options",
                "kf_kernel", "kf_set_state", "kf_get_state", "kf_memcpy_d2h", "kf_memcpy_h2d", "kf_device_alloc",
-               "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_trsv_fold", "kf_pattern", "kf_get_blocks",
+               "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_trsv_fold", "kf_bsr_spmv", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
                "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
                "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_init_params", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
@@ -380,6 +381,10 @@ class Mesh:
     def sptrsv(self, b_dev, x_dev):
         lib().ComputeSparseTriangularSolve(self.ref, b_dev.ptr, x_dev.ptr)
 
+    def spmv(self, x_dev, y_dev):
+        """y = A x with the Jacobian the last fill kept (options bsr_operator=1)"""
+        lib().kf_bsr_spmv(self.ref, x_dev.ptr, y_dev.ptr)
+
     def forces(self):
         f = (C.c_double * 3)()
         lib().ComputeForces(self.ref, f)
@@ -468,7 +473,7 @@ class Mesh:
             raise KeyError(name)
         return ms.value, n.value
 
-    PROFILED = ["grad", "edge_flux", "residual", "timestep", "fill", "ilu", "trsv_lower", "trsv_upper"]
+    PROFILED = ["grad", "edge_flux", "residual", "timestep", "fill", "ilu", "trsv_lower", "trsv_upper", "spmv"]
 
     def synchronize(self):
         lib().kf_synchronize(self.ref)

@@ -71,46 +71,15 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   d.y = dupload(g->n->xyz->x1, N);
   d.z = dupload(g->n->xyz->x2, N);
 
-  /* sliced ELL of the incidence lists (slice height 64) */
-  const uint32_t nslice = (uint32_t)((N + 63) / 64);
+  /* sliced ELL of the incidence lists (slice height 64), built on the host */
+  kf_sell sl;
+  kf_build_sell(g, p, &sl);
+  const uint32_t nslice = sl.nslice;
   d.nslice = nslice;
-  uint32_t *sptr = (uint32_t *)kf_xmalloc((nslice + 1) * sizeof(uint32_t));
-  uint32_t *deg = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
-  size_t rows = 0;
-  for (uint32_t s = 0; s < nslice; s++) {
-    uint32_t len = 0;
-    for (size_t v = (size_t)s * 64; v < N && v < (size_t)s * 64 + 64; v++) {
-      deg[v] = p->vptr[v + 1] - p->vptr[v];
-      if (deg[v] > len) len = deg[v];
-    }
-    sptr[s] = (uint32_t)rows;
-    rows += len;
-  }
-  sptr[nslice] = (uint32_t)rows;
-  const size_t ne = rows * 64;
-  uint32_t *nbr = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
-  uint32_t *ent = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
-  uint32_t *blk = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
-  double *wx = (double *)kf_xcalloc(ne + 1, sizeof(double));
-  double *wy = (double *)kf_xcalloc(ne + 1, sizeof(double));
-  double *wz = (double *)kf_xcalloc(ne + 1, sizeof(double));
-  const uint32_t *n0 = g->e->eptr->n0, *n1 = g->e->eptr->n1;
-  for (size_t v = 0; v < N; v++) {
-    const size_t base = (size_t)sptr[v >> 6] * 64 + (v & 63);
-    for (uint32_t k = 0; k < deg[v]; k++) {
-      const uint32_t s = p->vptr[v] + k, en = p->vent[s], e = en >> 1;
-      const size_t i = base + (size_t)k * 64;
-      ent[i] = en;
-      blk[i] = p->vblk[s];
-      if (en & 1u) {
-        nbr[i] = n0[e];
-        wx[i] = g->e->w->w1->x0[e]; wy[i] = g->e->w->w1->x1[e]; wz[i] = g->e->w->w1->x2[e];
-      } else {
-        nbr[i] = n1[e];
-        wx[i] = g->e->w->w0->x0[e]; wy[i] = g->e->w->w0->x1[e]; wz[i] = g->e->w->w0->x2[e];
-      }
-    }
-  }
+  const size_t ne = sl.ne;
+  uint32_t *sptr = sl.sptr, *deg = sl.deg, *nbr = sl.nbr, *ent = sl.ent, *blk = sl.blk;
+  double *wx = sl.wx, *wy = sl.wy, *wz = sl.wz;
+  kf_phase("sliced ELL (host)");
   d.sell_ptr = dupload(sptr, nslice + 1);
   d.deg = dupload(deg, N);
   d.sell_nbr = dupload(nbr, ne);
@@ -153,9 +122,11 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   /* GMRES's combination formed by the forward sweep: -1 where the sweeps are
    * throughput-bound, 0 never, 1 always */
   c->trsv_fold = (ev = getenv("KF_TRSV_FOLD")) ? (atoi(ev) != 0) : -1;
+  kf_phase("uploads");
   {
     kf_ilu_plan ip;
     kf_plan_ilu(p, &ip);
+    kf_phase("ILU plan");
     c->ilu_rec = dupload(ip.rec, 8 * N);
     c->ilu_ldiag = dupload(ip.ldiag, p->nnzb);
     c->ilu_moff = dupload(ip.moff, p->nnzb);
@@ -181,7 +152,9 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     /* ranges clustered across levels into tasks (1) or tasks of one level (0) */
     const int cluster = (ev = getenv("KF_TRSV_CLUSTER")) ? atoi(ev) != 0 : 1;
     kf_trsv_plan t, tu;
+    kf_phase("ILU plan upload");
     kf_plan_trsv_both(p, R, cluster, &t, &tu);
+    kf_phase("solve plans");
     c->trow_l = dupload(t.trow, (size_t)t.ntask * KF_GROUP * 3);
     c->tlvl_l = dupload(t.tlvl, (size_t)t.ntask * KF_GROUP);
     c->tdep_l = dupload(t.tdep, t.ntask);
@@ -267,13 +240,19 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
   kf_ctx *c = (kf_ctx *)g->matrix;
   kfs_learn_detach(g);
   KF_CHECK(hipStreamSynchronize(c->stream));
+  if (c->fstream) {
+    KF_CHECK(hipStreamSynchronize(c->fstream));
+    KF_CHECK(hipEventDestroy(c->ev_upd));
+    KF_CHECK(hipEventDestroy(c->ev_forces));
+    KF_CHECK(hipStreamDestroy(c->fstream));
+  }
   kf_dv &d = c->d;
   void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->rdone,
                   c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
-                  c->abf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
+                  c->abf, c->jop64, c->jop32, c->ftbuf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));
   if (c->mgs_ev) KF_CHECK(hipEventDestroy(c->mgs_ev));
@@ -306,6 +285,7 @@ extern "C" kf_options kf_default_options(void)
   o.ilu_exact = 1;
   o.reduce_seq = 0;
   o.krylov = KF_KRYLOV_GMRES;
+  o.bsr_operator = 0;
   return o;
 }
 
@@ -344,6 +324,7 @@ static void set_free_stream_state(kf_ctx *c)
 
 extern "C" void kf_setup_host(const char *fpath, GEOMETRY *g)
 {
+  kf_phase(NULL);
   memset(g, 0, sizeof(*g));
   kf_read_msh(fpath, g);
   kf_host_setup(g);
@@ -359,6 +340,7 @@ extern "C" void kf_setup_ex(const char *fpath, GEOMETRY *g, const kf_options *o)
   kf_ctx_create(g, &p, o);
   kf_free_prep(&p);
   kf_bind(g);
+  kf_phase("device context");
 }
 
 extern "C" void Setup(const char *fpath, GEOMETRY *g)
@@ -487,7 +469,34 @@ extern "C" void kfs_timestep_spectral(GEOMETRY *g, double cfl)
 
 extern "C" void ComputeForces(const GEOMETRY *g, double *forces) { kf_launch_forces(kf_ctx_of(g), forces); }
 
-extern "C" void ComputeA(GEOMETRY *g) { kf_launch_fill(kf_ctx_of(g)); }
+/* Kernel's forces: started after an update on a second stream, collected
+ * (bits of ComputeForces) before the state changes again */
+extern "C" void kfs_forces_async(GEOMETRY *g) { kf_forces_async(kf_ctx_of(g)); }
+extern "C" void kfs_forces_collect(GEOMETRY *g, double *forces) { kf_forces_collect(kf_ctx_of(g), forces); }
+
+extern "C" void ComputeA(GEOMETRY *g)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  kf_launch_fill(c);
+  if (c->opt.bsr_operator) kf_keep_jacobian(c);   /* the ILU factors the fill's array in place */
+}
+
+/* y = A x with the Jacobian of the last ComputeA (kept when bsr_operator is
+ * set); no counterpart in the reference, whose Krylov operator is the
+ * matrix-free product of src/kernel.c:24-45 */
+extern "C" void kf_bsr_spmv(const GEOMETRY *g, const double *x, double *y)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (!c->jop64 && !c->jop32) {
+    fprintf(stderr, "kf: kf_bsr_spmv needs a Jacobian kept by ComputeA (set kf_options.bsr_operator = 1 first)\n");
+    abort();
+  }
+  if (x == y) {
+    fprintf(stderr, "kf: kf_bsr_spmv: x and y must not alias\n");
+    abort();
+  }
+  kf_launch_spmv(c, x, y);
+}
 
 extern "C" void ComputeNumericalILU(GEOMETRY *g) { kf_launch_ilu(kf_ctx_of(g)); }
 
@@ -580,7 +589,7 @@ extern "C" double Normalize(const size_t sz, double *x)
 /* ---- per-kernel timing with HIP events on the context stream ----------------- */
 
 static const char *kf_prof_names[KP_N] = {"grad", "edge_flux", "residual", "timestep", "fill", "ilu",
-                                          "trsv_lower", "trsv_upper"};
+                                          "trsv_lower", "trsv_upper", "spmv"};
 
 static void prof_drain(kf_ctx *c)
 {

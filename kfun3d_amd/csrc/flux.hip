@@ -521,3 +521,45 @@ void kf_launch_forces(kf_ctx *c, double *forces_host)
   KF_LAUNCH_CHECK();
   kf_scal_fetch(c, KF_NSCAL - 3, 3, forces_host);
 }
+
+/* The forces of the state as it stands once the work queued so far is done
+ * (Kernel: right after a pseudo-time step's update), computed on a second
+ * stream so that they overlap the next step's work: the same two kernels as
+ * kf_launch_forces, with their own term buffer and scalar slots, the result
+ * copied to the pinned host mirror.  kf_forces_collect waits for them and
+ * orders the main stream after them; it must run before anything changes
+ * the state again. */
+void kf_forces_async(kf_ctx *c)
+{
+  const kf_dv &d = c->d;
+  if (!c->fstream) {
+    KF_CHECK(hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking));
+    KF_CHECK(hipEventCreateWithFlags(&c->ev_upd, hipEventDisableTiming));
+    KF_CHECK(hipEventCreateWithFlags(&c->ev_forces, hipEventDisableTiming));
+  }
+  if (!c->ftbuf) KF_CHECK(hipMalloc((void **)&c->ftbuf, (6 * (size_t)d.S + 1) * sizeof(double)));
+  KF_CHECK(hipEventRecord(c->ev_upd, c->stream));
+  KF_CHECK(hipStreamWaitEvent(c->fstream, c->ev_upd, 0));
+  if (d.S) {
+    hipLaunchKernelGGL(k_force_terms, dim3(kf_blocks_for(d.S, 256)), dim3(256), 0, c->fstream, d, c->q, c->ftbuf);
+    KF_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(256), 0, c->fstream, d, c->ftbuf, d.scal + KF_SLOT_FASYNC);
+  KF_LAUNCH_CHECK();
+  KF_CHECK(hipMemcpyAsync(c->hscal + KF_SLOT_FASYNC, d.scal + KF_SLOT_FASYNC, 3 * sizeof(double),
+                          hipMemcpyDeviceToHost, c->fstream));
+  KF_CHECK(hipEventRecord(c->ev_forces, c->fstream));
+  c->forces_pending = 1;
+}
+
+void kf_forces_collect(kf_ctx *c, double *forces_host)
+{
+  if (!c->forces_pending) {
+    fprintf(stderr, "kf: no forces pending\n");
+    abort();
+  }
+  KF_CHECK(hipStreamWaitEvent(c->stream, c->ev_forces, 0));
+  KF_CHECK(hipEventSynchronize(c->ev_forces));
+  for (int k = 0; k < 3; k++) forces_host[k] = c->hscal[KF_SLOT_FASYNC + k];
+  c->forces_pending = 0;
+}

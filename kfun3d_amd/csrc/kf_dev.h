@@ -45,6 +45,7 @@
 #define KF_MGS_MAXV 64        /* basis vectors + 1 of one resident MGS launch, at most */
 #define KF_MGS_ARRIVE 4096    /* polls a resident MGS workgroup waits for the others to arrive */
 #define KF_GROUP 16           /* rows per triangular-solve group */
+#define KF_SLOT_FASYNC (KF_NSCAL - 6)   /* scalar slots of the overlapped forces (3) */
 
 /* device error word bits */
 #define KF_ERR_NONFINITE_PIVOT 1u
@@ -75,7 +76,7 @@ struct kf_dv {
 };
 
 /* per-kernel HIP-event timing on the context stream (kf_profile) */
-enum kf_prof_id { KP_GRAD, KP_EDGE, KP_RES, KP_TSTEP, KP_FILL, KP_ILU, KP_TRSV_L, KP_TRSV_U, KP_N };
+enum kf_prof_id { KP_GRAD, KP_EDGE, KP_RES, KP_TSTEP, KP_FILL, KP_ILU, KP_TRSV_L, KP_TRSV_U, KP_SPMV, KP_N };
 #define KF_PROF_POOL 512
 
 struct kf_prof {
@@ -143,6 +144,8 @@ struct kf_ctx {
   double *a64;
   float *a32;
   uint16_t *abf;
+  double *jop64;            /* the assembled Jacobian kept for the BSR operator (FP64) ... */
+  float *jop32;             /* ... or FP32 with a single-precision preconditioner */
   double *lold;             /* previous sweep's L blocks (exact ILU) */
   double *ud;               /* non-inverted U(i,i) (exact ILU) */
   /* state */
@@ -153,6 +156,11 @@ struct kf_ctx {
   int nvecs;
   float *f32a, *f32b;       /* single-precision forward / backward outputs */
   double *eterm, *sr;       /* per-edge time-step terms, per-vertex time-step sums */
+  /* forces overlapped with the next pseudo-time step (kf_forces_async) */
+  hipStream_t fstream;      /* default priority, like the main stream */
+  hipEvent_t ev_upd, ev_forces;
+  double *ftbuf;            /* per-face force terms */
+  int forces_pending;
   void *learn;              /* learned-preconditioner state of the driver (solver.c) */
 };
 
@@ -197,6 +205,8 @@ void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *
 void kf_launch_timestep(kf_ctx *c, double cfl);
 void kf_launch_timestep_div(kf_ctx *c, const double *sr, double cfl);
 void kf_launch_forces(kf_ctx *c, double *forces_host);
+void kf_forces_async(kf_ctx *c);
+void kf_forces_collect(kf_ctx *c, double *forces_host);
 void kf_launch_fill(kf_ctx *c);
 void kf_launch_ilu(kf_ctx *c);
 /* the finite-difference combination applied by the forward sweep as it
@@ -207,6 +217,9 @@ struct kf_fdb {
   double h;
 };
 void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x, const kf_fdb *fd);
+/* y = A x with the kept copy of the assembled Jacobian (spmv.hip) */
+void kf_launch_spmv(kf_ctx *c, const double *x, double *y);
+void kf_keep_jacobian(kf_ctx *c);
 bool kf_trsv_capped(const kf_ctx *c);
 
 /* BLAS-1 on the context stream; reductions are deterministic (fixed

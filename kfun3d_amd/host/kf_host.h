@@ -16,6 +16,8 @@ extern "C" {
 
 void *kf_xmalloc(size_t bytes);
 void *kf_xcalloc(size_t n, size_t size);
+/* KF_SETUP_TIMES=1: prints the time since the previous call (NULL: resets) */
+void kf_phase(const char *name);
 
 /* Derived host arrays the device context is built from.  All orders are
  * the reference's: incidences ascend in edge (resp. face, node-list) index,
@@ -73,6 +75,18 @@ typedef struct kf_ilu_plan {
 void kf_plan_ilu(const kf_prep *p, kf_ilu_plan *out);
 void kf_free_ilu_plan(kf_ilu_plan *t);
 
+/* the incidence lists as a sliced ELL (slice height 64: one wavefront):
+ * vertex v of slice s = v / 64, lane v % 64, entry k at
+ * (sptr[s] + k) * 64 + v % 64; ascending edge id; padding zero */
+typedef struct kf_sell {
+  uint32_t nslice;
+  size_t ne;                        /* entries including padding */
+  uint32_t *sptr, *deg;             /* [nslice + 1], [N] */
+  uint32_t *nbr, *ent, *blk;        /* neighbour, edge << 1 | side, BSR block of (v, neighbour) */
+  double *wx, *wy, *wz;             /* LSQ term of the edge seen from v */
+} kf_sell;
+void kf_build_sell(const GEOMETRY *g, const kf_prep *p, kf_sell *out);
+
 /* .msh reader (src/mesh.c:22-154 without the device part) */
 void kf_read_msh(const char *fpath, GEOMETRY *g);
 /* host preprocessing of src/mesh.c:113-153 with one owner */
@@ -87,6 +101,8 @@ void kf_free_lsq(GEOMETRY *g);
 void kf_index_sort(size_t sz, const uint32_t *key, uint32_t *perm);       /* src/index.c:90-112 */
 void kf_color_solid_faces(GEOMETRY *g);                                    /* src/sbface_coloring.c:11-93 */
 void kf_lsq_weights(GEOMETRY *g);                                          /* src/weights.c:299-398 */
+/* vertex -> incident edges in ascending edge id, ent = edge << 1 | side */
+void kf_incidence(size_t N, size_t E, const uint32_t *n0, const uint32_t *n1, uint32_t **vptr, uint32_t **vent);
 void kf_free_stream(double alpha_deg, double *U, double *V);               /* inc/phy.h:8-9 */
 
 /* device context (HIP side) */
@@ -142,6 +158,11 @@ void kfs_cgs_update(GEOMETRY *g, size_t n, const double *vbase, int nv, int slot
  * (src/timestep.c folded into the residual pass; q must not change between
  * the two calls) */
 void kfs_flux_spectral(GEOMETRY *g, const double *q, GRADIENT *grad, double *r);
+/* the forces of the state once the queued work is done, on a second stream
+ * (overlapping what is queued next); collect them (the bits of
+ * ComputeForces) before anything changes the state again */
+void kfs_forces_async(GEOMETRY *g);
+void kfs_forces_collect(GEOMETRY *g, double *forces);
 void kfs_timestep_spectral(GEOMETRY *g, double cfl);
 /* x[i] ~ N(0, 1) from a counter-based generator keyed by (seed, stream, i) */
 void kfs_gaussian(GEOMETRY *g, size_t n, uint64_t seed, uint64_t stream, double *x);

@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "kf_host.h"
 
@@ -28,9 +29,34 @@ void *kf_xmalloc(size_t bytes)
 
 void *kf_xcalloc(size_t n, size_t size)
 {
-  void *p = kf_xmalloc(n * size);
-  memset(p, 0, n * size);
+  const size_t bytes = n * size, chunk = (size_t)8 << 20;
+  void *p = kf_xmalloc(bytes);
+  if (bytes <= 4 * chunk) {
+    memset(p, 0, bytes);
+    return p;
+  }
+  /* large: zeroed by all threads, which also spreads the first-touch page faults */
+  const size_t nchunk = (bytes + chunk - 1) / chunk;
+#pragma omp parallel for schedule(static)
+  for (size_t c = 0; c < nchunk; c++) {
+    const size_t off = c * chunk;
+    memset((char *)p + off, 0, bytes - off < chunk ? bytes - off : chunk);
+  }
   return p;
+}
+
+/* KF_SETUP_TIMES=1: the wall time of each setup phase on stderr */
+void kf_phase(const char *name)
+{
+  static int on = -1;
+  static double last = 0.0;
+  if (on < 0) on = getenv("KF_SETUP_TIMES") && atoi(getenv("KF_SETUP_TIMES")) != 0;
+  if (!on) return;
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  const double t = (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+  if (name) fprintf(stderr, "kf setup: %-24s %7.3f s\n", name, t - last);
+  last = t;
 }
 
 static uint32_t rd_u32(const unsigned char *b)
@@ -138,6 +164,7 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
     abort();
   }
   fclose(f);
+  kf_phase("read file");
   cursor c = {buf, 0, fsz, fpath};
 
   uint32_t p[13];
@@ -153,8 +180,10 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
       abort();
     }
   double *b1 = take_f64(&c, 4 * E);
+  kf_phase("decode edges");
   uint32_t *perm = (uint32_t *)kf_xmalloc((E + 1) * sizeof(uint32_t));
   kf_index_sort(E, b0, perm);
+  kf_phase("edge index sort");
   struct etbl *e = (struct etbl *)kf_xcalloc(1, sizeof(struct etbl));
   e->sz = E;
   e->eptr = (struct edge *)kf_xmalloc(sizeof(struct edge));
@@ -176,6 +205,7 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
   }
   free(b0); free(b1); free(perm);
   g->e = e;
+  kf_phase("edge reorder");
 
   /* nodes: src/nodes.c:19-56 */
   double *nb = take_f64(&c, 4 * N);
@@ -222,4 +252,5 @@ void kf_read_msh(const char *fpath, GEOMETRY *g)
     abort();
   }
   free(buf);
+  kf_phase("nodes, boundaries");
 }

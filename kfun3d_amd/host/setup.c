@@ -67,8 +67,16 @@ static void index_sort_rec(const uint32_t *key, uint32_t *l, uint32_t *h, uint32
 
 void kf_index_sort(size_t sz, const uint32_t *key, uint32_t *perm)
 {
-  for (size_t i = 0; i < sz; i++) perm[i] = (uint32_t)i;
-  if (sz == 0) return;
+  int sorted = 1;
+#pragma omp parallel for schedule(static) reduction(&& : sorted)
+  for (size_t i = 0; i < sz; i++) {
+    perm[i] = (uint32_t)i;
+    if (i && key[i - 1] > key[i]) sorted = 0;
+  }
+  /* on keys already in order the sort is the identity: the leaf exchange
+   * sort swaps only on a strictly greater key, the merge takes from the
+   * right run only on a strictly smaller one */
+  if (sz == 0 || sorted) return;
   uint32_t *tmp = (uint32_t *)kf_xmalloc(sz * sizeof(uint32_t));
 #pragma omp parallel
 #pragma omp single
@@ -109,69 +117,121 @@ void kf_color_solid_faces(GEOMETRY *g)
   g->t->sz = c;
 }
 
+/* vertex -> incident edges in ascending edge id, ent = edge << 1 | side
+ * (side 1: the vertex is n1): counts and slots drawn with atomics, then
+ * each vertex's short list sorted, so the result is deterministic */
+void kf_incidence(size_t N, size_t E, const uint32_t *n0, const uint32_t *n1, uint32_t **vptr, uint32_t **vent)
+{
+  uint32_t *pt = (uint32_t *)kf_xcalloc(N + 1, sizeof(uint32_t));
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < E; i++) {
+#pragma omp atomic
+    pt[n0[i] + 1]++;
+#pragma omp atomic
+    pt[n1[i] + 1]++;
+  }
+  for (size_t i = 0; i < N; i++) pt[i + 1] += pt[i];
+  uint32_t *fillp = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
+  memcpy(fillp, pt, (N + 1) * sizeof(uint32_t));
+  uint32_t *en = (uint32_t *)kf_xmalloc((2 * E + 1) * sizeof(uint32_t));
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < E; i++) {
+    uint32_t s0, s1;
+#pragma omp atomic capture
+    s0 = fillp[n0[i]]++;
+#pragma omp atomic capture
+    s1 = fillp[n1[i]]++;
+    en[s0] = (uint32_t)(i << 1);
+    en[s1] = (uint32_t)(i << 1) | 1u;
+  }
+  free(fillp);
+#pragma omp parallel for schedule(static)
+  for (size_t v = 0; v < N; v++)
+    for (uint32_t s = pt[v] + 1; s < pt[v + 1]; s++) {   /* insertion sort: ~14 entries */
+      const uint32_t x = en[s];
+      uint32_t t = s;
+      for (; t > pt[v] && en[t - 1] > x; t--) en[t] = en[t - 1];
+      en[t] = x;
+    }
+  *vptr = pt;
+  *vent = en;
+}
+
 /* ---- src/weights.c:9-398, one owner: every edge updates both ends -------- */
 
+/* The reference scatters each edge's terms to both ends, edge by edge, in
+ * three passes (src/weights.c:103-297), each pass reading only the end's own
+ * sums of the earlier passes.  So a vertex's sums see its incident edges in
+ * ascending edge id, pass after pass, whatever the other vertices do: here
+ * each vertex gathers them in that order (the same operations, the same
+ * bits), all vertices in parallel. */
 void kf_lsq_weights(GEOMETRY *g)
 {
   const size_t N = g->n->sz, E = g->e->sz;
   const uint32_t *n0 = g->e->eptr->n0, *n1 = g->e->eptr->n1;
   const double *x0 = g->n->xyz->x0, *x1 = g->n->xyz->x1, *x2 = g->n->xyz->x2;
   double *w = (double *)kf_xcalloc(7 * N, sizeof(double));
+  uint32_t *vptr, *vent;
+  kf_incidence(N, E, n0, n1, &vptr, &vent);
 
-  /* w00, w01, w02: src/weights.c:237-297 */
-  for (size_t i = 0; i < E; i++) {
-    const uint32_t a = n0[i], b = n1[i];
-    double rx = x0[b] - x0[a], ry = x1[b] - x1[a], rz = x2[b] - x2[a];
-    w[a * 7 + 0] += rx * rx; w[a * 7 + 1] += rx * ry; w[a * 7 + 2] += rx * rz;
-    rx = x0[a] - x0[b]; ry = x1[a] - x1[b]; rz = x2[a] - x2[b];
-    w[b * 7 + 0] += rx * rx; w[b * 7 + 1] += rx * ry; w[b * 7 + 2] += rx * rz;
-  }
-  for (size_t i = 0; i < N; i++) {
-    w[i * 7 + 0] = sqrt(w[i * 7 + 0]);
-    w[i * 7 + 1] /= w[i * 7 + 0];
-    w[i * 7 + 2] /= w[i * 7 + 0];
-  }
-  /* w11, w12: src/weights.c:173-232 */
-  for (size_t i = 0; i < E; i++) {
-    const uint32_t a = n0[i], b = n1[i];
-    const double dx = x0[b] - x0[a], dy = x1[b] - x1[a], dz = x2[b] - x2[a];
-    double d = w[a * 7 + 1] / w[a * 7 + 0];
-    double d_ = dy - dx * d;
-    w[a * 7 + 3] += d_ * d_;
-    w[a * 7 + 4] += d_ * dz;
-    d = w[b * 7 + 1] / w[b * 7 + 0];
-    d_ = -dy + dx * d;
-    w[b * 7 + 3] += d_ * d_;
-    w[b * 7 + 4] -= d_ * dz;
-  }
-  for (size_t i = 0; i < N; i++) {
-    w[i * 7 + 3] = sqrt(w[i * 7 + 3]);
-    w[i * 7 + 4] /= w[i * 7 + 3];
-  }
-  /* w22: src/weights.c:103-170 */
-  for (size_t i = 0; i < E; i++) {
-    const uint32_t a = n0[i], b = n1[i];
-    const double dx = x0[b] - x0[a], dy = x1[b] - x1[a], dz = x2[b] - x2[a];
-    {
-      const double d0 = w[a * 7 + 1] / w[a * 7 + 0];
-      const double d0_ = dy - dx * d0;
-      const double d1 = w[a * 7 + 2] / w[a * 7 + 0];
-      const double d1_ = dz - dx * d1;
-      const double d2 = w[a * 7 + 4] / w[a * 7 + 3];
-      const double d2_ = d1_ - d2 * d0_;
-      w[a * 7 + 5] += d2_ * d2_;
+#pragma omp parallel for schedule(static)
+  for (size_t v = 0; v < N; v++) {
+    double *wv = w + v * 7;
+    /* w00, w01, w02: src/weights.c:237-297 */
+    for (uint32_t s = vptr[v]; s < vptr[v + 1]; s++) {
+      const uint32_t i = vent[s] >> 1, a = n0[i], b = n1[i];
+      if (!(vent[s] & 1u)) {
+        const double rx = x0[b] - x0[a], ry = x1[b] - x1[a], rz = x2[b] - x2[a];
+        wv[0] += rx * rx; wv[1] += rx * ry; wv[2] += rx * rz;
+      } else {
+        const double rx = x0[a] - x0[b], ry = x1[a] - x1[b], rz = x2[a] - x2[b];
+        wv[0] += rx * rx; wv[1] += rx * ry; wv[2] += rx * rz;
+      }
     }
-    {
-      const double d0 = w[b * 7 + 1] / w[b * 7 + 0];
-      const double d0_ = -dy + dx * d0;
-      const double d1 = w[b * 7 + 2] / w[b * 7 + 0];
-      const double d1_ = -dz + dx * d1;
-      const double d2 = w[b * 7 + 4] / w[b * 7 + 3];
-      const double d2_ = d1_ - d2 * d0_;
-      w[b * 7 + 5] += d2_ * d2_;
+    wv[0] = sqrt(wv[0]);
+    wv[1] /= wv[0];
+    wv[2] /= wv[0];
+    /* w11, w12: src/weights.c:173-232 */
+    for (uint32_t s = vptr[v]; s < vptr[v + 1]; s++) {
+      const uint32_t i = vent[s] >> 1, a = n0[i], b = n1[i];
+      const double dx = x0[b] - x0[a], dy = x1[b] - x1[a], dz = x2[b] - x2[a];
+      const double d = wv[1] / wv[0];
+      if (!(vent[s] & 1u)) {
+        const double d_ = dy - dx * d;
+        wv[3] += d_ * d_;
+        wv[4] += d_ * dz;
+      } else {
+        const double d_ = -dy + dx * d;
+        wv[3] += d_ * d_;
+        wv[4] -= d_ * dz;
+      }
+    }
+    wv[3] = sqrt(wv[3]);
+    wv[4] /= wv[3];
+    /* w22: src/weights.c:103-170 */
+    for (uint32_t s = vptr[v]; s < vptr[v + 1]; s++) {
+      const uint32_t i = vent[s] >> 1, a = n0[i], b = n1[i];
+      const double dx = x0[b] - x0[a], dy = x1[b] - x1[a], dz = x2[b] - x2[a];
+      const double d0 = wv[1] / wv[0];
+      const double d1 = wv[2] / wv[0];
+      const double d2 = wv[4] / wv[3];
+      if (!(vent[s] & 1u)) {
+        const double d0_ = dy - dx * d0;
+        const double d1_ = dz - dx * d1;
+        const double d2_ = d1_ - d2 * d0_;
+        wv[5] += d2_ * d2_;
+      } else {
+        const double d0_ = -dy + dx * d0;
+        const double d1_ = -dz + dx * d1;
+        const double d2_ = d1_ - d2 * d0_;
+        wv[5] += d2_ * d2_;
+      }
     }
   }
+  free(vptr);
+  free(vent);
   /* magnitudes: src/weights.c:313-366 */
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; i++) {
     double *v = w + i * 7;
     v[5] = sqrt(v[5]);
@@ -188,12 +248,12 @@ void kf_lsq_weights(GEOMETRY *g)
   struct weights *wt = (struct weights *)kf_xmalloc(sizeof(struct weights));
   wt->w0 = (struct xyz *)kf_xmalloc(sizeof(struct xyz));
   wt->w1 = (struct xyz *)kf_xmalloc(sizeof(struct xyz));
-  wt->w0->x0 = (double *)kf_xcalloc(E + 1, sizeof(double));
-  wt->w0->x1 = (double *)kf_xcalloc(E + 1, sizeof(double));
-  wt->w0->x2 = (double *)kf_xcalloc(E + 1, sizeof(double));
-  wt->w1->x0 = (double *)kf_xcalloc(E + 1, sizeof(double));
-  wt->w1->x1 = (double *)kf_xcalloc(E + 1, sizeof(double));
-  wt->w1->x2 = (double *)kf_xcalloc(E + 1, sizeof(double));
+  double **wa[6] = {&wt->w0->x0, &wt->w0->x1, &wt->w0->x2, &wt->w1->x0, &wt->w1->x1, &wt->w1->x2};
+  for (int k = 0; k < 6; k++) {   /* every entry but the spare last one is written below */
+    *wa[k] = (double *)kf_xmalloc((E + 1) * sizeof(double));
+    (*wa[k])[E] = 0.0;
+  }
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < E; i++) {
     const uint32_t a = n0[i], b = n1[i];
     const double dx = x0[b] - x0[a], dy = x1[b] - x1[a], dz = x2[b] - x2[a];
@@ -248,17 +308,13 @@ void kf_host_setup(GEOMETRY *g)
   g->s->i[1] = (uint32_t)g->e->sz;
   g->t = (struct stbl *)kf_xcalloc(1, sizeof(struct stbl));
   kf_color_solid_faces(g);
+  kf_phase("face colouring");
   kf_lsq_weights(g);
+  kf_phase("LSQ weights");
   g->q = (struct qtbl *)kf_xcalloc(1, sizeof(struct qtbl));
 }
 
 /* ---- derived arrays ------------------------------------------------------- */
-
-static int cmp_u32(const void *a, const void *b)
-{
-  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
-  return (x > y) - (x < y);
-}
 
 static uint32_t find_block(const kf_prep *p, uint32_t row, uint32_t col)
 {
@@ -326,41 +382,42 @@ void kf_build_prep(const GEOMETRY *g, kf_prep *p)
       abort();
     }
 
-  /* pattern: matrix/init.cpp:7-74 (self + both edge directions, sorted) */
-  p->rowptr = (uint32_t *)kf_xcalloc(N + 1, sizeof(uint32_t));
-  for (size_t i = 0; i < E; i++) { p->rowptr[n0[i] + 1]++; p->rowptr[n1[i] + 1]++; }
-  for (size_t i = 0; i < N; i++) p->rowptr[i + 1] += p->rowptr[i] + 1;
+  /* vertex -> incident edges, ascending edge id */
+  kf_incidence(N, E, n0, n1, &p->vptr, &p->vent);
+
+  /* pattern: matrix/init.cpp:7-74 (self + both edge directions, sorted):
+   * row i holds i and the other end of each incident edge */
+  p->rowptr = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i <= N; i++) p->rowptr[i] = p->vptr[i] + (uint32_t)i;
   p->nnzb = p->rowptr[N];
   p->col = (uint32_t *)kf_xmalloc((p->nnzb + 1) * sizeof(uint32_t));
-  uint32_t *fillp = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
-  for (size_t i = 0; i < N; i++) { p->col[p->rowptr[i]] = (uint32_t)i; fillp[i] = p->rowptr[i] + 1; }
-  for (size_t i = 0; i < E; i++) {
-    p->col[fillp[n0[i]]++] = n1[i];
-    p->col[fillp[n1[i]]++] = n0[i];
-  }
-  free(fillp);
   p->diag = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; i++) {
-    qsort(p->col + p->rowptr[i], p->rowptr[i + 1] - p->rowptr[i], sizeof(uint32_t), cmp_u32);
-    for (uint32_t j = p->rowptr[i] + 1; j < p->rowptr[i + 1]; j++)
-      if (p->col[j] == p->col[j - 1]) {
-        fprintf(stderr, "kf: duplicate edge (%zu,%u)\n", i, p->col[j]);
+    uint32_t *c = p->col + p->rowptr[i];
+    const uint32_t len = p->rowptr[i + 1] - p->rowptr[i];
+    c[0] = (uint32_t)i;
+    for (uint32_t s = p->vptr[i]; s < p->vptr[i + 1]; s++) {
+      const uint32_t e = p->vent[s] >> 1;
+      c[1 + s - p->vptr[i]] = (p->vent[s] & 1) ? n0[e] : n1[e];
+    }
+    for (uint32_t a = 1; a < len; a++) {   /* insertion sort: ~15 entries */
+      const uint32_t x = c[a];
+      uint32_t b = a;
+      for (; b > 0 && c[b - 1] > x; b--) c[b] = c[b - 1];
+      c[b] = x;
+    }
+    for (uint32_t j = 1; j < len; j++)
+      if (c[j] == c[j - 1]) {
+        fprintf(stderr, "kf: duplicate edge (%zu,%u)\n", i, c[j]);
         abort();
       }
     p->diag[i] = find_block(p, (uint32_t)i, (uint32_t)i);
   }
 
-  /* vertex -> incident edges, ascending edge id */
+  kf_phase("pattern");
   {
-    uint32_t *owner = (uint32_t *)kf_xmalloc((2 * E + 1) * sizeof(uint32_t));
-    uint32_t *pay = (uint32_t *)kf_xmalloc((2 * E + 1) * sizeof(uint32_t));
-    for (size_t i = 0; i < E; i++) {
-      owner[2 * i] = n0[i]; pay[2 * i] = (uint32_t)(i << 1);
-      owner[2 * i + 1] = n1[i]; pay[2 * i + 1] = (uint32_t)(i << 1) | 1u;
-    }
-    bucket(N, 2 * E, owner, pay, &p->vptr, &p->vent);
-    free(owner); free(pay);
     p->vblk = (uint32_t *)kf_xmalloc((2 * E + 1) * sizeof(uint32_t));
 #pragma omp parallel for schedule(static)
     for (size_t v = 0; v < N; v++)
@@ -371,9 +428,11 @@ void kf_build_prep(const GEOMETRY *g, kf_prep *p)
       }
   }
 
+  kf_phase("incidences");
   levels(p, 0, &p->nlev_l, &p->lptr, &p->lrows);
   levels(p, 1, &p->nlev_u, &p->uptr, &p->urows);
 
+  kf_phase("level sets");
   /* boundary incidences */
   {
     const size_t S = g->b->fc->sz;
@@ -396,6 +455,62 @@ void kf_build_prep(const GEOMETRY *g, kf_prep *p)
     bucket(N, nf, g->b->f->nptr, idx, &p->fnptr, &p->fnent);
     free(idx);
   }
+  kf_phase("boundary incidences");
+}
+
+void kf_build_sell(const GEOMETRY *g, const kf_prep *p, kf_sell *out)
+{
+  const size_t N = p->N;
+  const uint32_t nslice = (uint32_t)((N + 63) / 64);
+  uint32_t *sptr = (uint32_t *)kf_xmalloc((nslice + 1) * sizeof(uint32_t));
+  uint32_t *deg = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
+  uint32_t *len = (uint32_t *)kf_xmalloc((nslice + 1) * sizeof(uint32_t));
+#pragma omp parallel for schedule(static)
+  for (uint32_t s = 0; s < nslice; s++) {
+    uint32_t l = 0;
+    for (size_t v = (size_t)s * 64; v < N && v < (size_t)s * 64 + 64; v++) {
+      deg[v] = p->vptr[v + 1] - p->vptr[v];
+      if (deg[v] > l) l = deg[v];
+    }
+    len[s] = l;
+  }
+  size_t rows = 0;
+  for (uint32_t s = 0; s < nslice; s++) {
+    sptr[s] = (uint32_t)rows;
+    rows += len[s];
+  }
+  sptr[nslice] = (uint32_t)rows;
+  free(len);
+  const size_t ne = rows * 64;
+  uint32_t *nbr = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
+  uint32_t *ent = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
+  uint32_t *blk = (uint32_t *)kf_xcalloc(ne + 1, sizeof(uint32_t));
+  double *wx = (double *)kf_xcalloc(ne + 1, sizeof(double));
+  double *wy = (double *)kf_xcalloc(ne + 1, sizeof(double));
+  double *wz = (double *)kf_xcalloc(ne + 1, sizeof(double));
+  const uint32_t *n0 = g->e->eptr->n0, *n1 = g->e->eptr->n1;
+  const struct weights *w = g->e->w;
+#pragma omp parallel for schedule(static)
+  for (size_t v = 0; v < N; v++) {
+    const size_t base = (size_t)sptr[v >> 6] * 64 + (v & 63);
+    for (uint32_t k = 0; k < deg[v]; k++) {
+      const uint32_t s = p->vptr[v] + k, en = p->vent[s], e = en >> 1;
+      const size_t i = base + (size_t)k * 64;
+      ent[i] = en;
+      blk[i] = p->vblk[s];
+      if (en & 1u) {
+        nbr[i] = n0[e];
+        wx[i] = w->w1->x0[e]; wy[i] = w->w1->x1[e]; wz[i] = w->w1->x2[e];
+      } else {
+        nbr[i] = n1[e];
+        wx[i] = w->w0->x0[e]; wy[i] = w->w0->x1[e]; wz[i] = w->w0->x2[e];
+      }
+    }
+  }
+  out->nslice = nslice;
+  out->ne = ne;
+  out->sptr = sptr; out->deg = deg; out->nbr = nbr; out->ent = ent; out->blk = blk;
+  out->wx = wx; out->wy = wy; out->wz = wz;
 }
 
 void kf_free_prep(kf_prep *p)

@@ -198,9 +198,11 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
   else kfs_newaxpy(g, sz, 0.0, b, b, VEC(0)); /* v = b (x = 0) */
   double tol = 0.0;
   int total = 0, flag = 0;
+  const int bsr = o->bsr_operator;   /* the assembled first-order Jacobian as the operator */
   while (total <= maxit) {
     if (total > 0) {
-      iJacVec(g, x, K->r, K->qnorm, &K->grad, w, t);
+      if (bsr) kf_bsr_spmv(g, x, t);
+      else iJacVec(g, x, K->r, K->qnorm, &K->grad, w, t);
       if (!flexible) {
         kfs_newaxpy(g, sz, -1.0, t, b, w);
         ComputeSparseTriangularSolve(g, w, VEC(0));
@@ -215,11 +217,17 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
     int it = 0, have_xnorm = 0;   /* |VEC(it)| left by the previous orthogonalisation */
     while (it < restart) {
       if (!flexible) {
-        ijacvec_solve(g, VEC(it), K->r, K->qnorm, &K->grad, w, t, have_xnorm, VEC(it + 1));
+        if (bsr) {
+          kf_bsr_spmv(g, VEC(it), t);
+          ComputeSparseTriangularSolve(g, t, VEC(it + 1));
+        } else {
+          ijacvec_solve(g, VEC(it), K->r, K->qnorm, &K->grad, w, t, have_xnorm, VEC(it + 1));
+        }
       } else {
         ComputeSparseTriangularSolve(g, VEC(it), ZVEC(it)); /* z = M(v) */
         if (use_net) add_correction(K, ZVEC(it));
-        iJacVec(g, ZVEC(it), K->r, K->qnorm, &K->grad, w, VEC(it + 1));
+        if (bsr) kf_bsr_spmv(g, ZVEC(it), VEC(it + 1));
+        else iJacVec(g, ZVEC(it), K->r, K->qnorm, &K->grad, w, VEC(it + 1));
       }
       double *hh = ghh + (size_t)it * (nb + 1);
       if (o->ortho == KF_ORTHO_CGS2) {
@@ -239,8 +247,9 @@ static int ksolve(kspace *K, const double *b, double *x, double rtol, int maxit,
         /* modified Gram-Schmidt, src/kernel.c:129-141 */
         /* then v = w / |w|, and |v| and the perturbed state for the next
          * iJacVec */
+        const int fd_next = !flexible && !bsr;   /* the next operator is iJacVec on this vector */
         have_xnorm = kfs_mgs_unit(g, sz, VEC(0), sz, it + 1, VEC(it + 1), KFS_SLOT_H, hh,
-                                  flexible ? -1 : KFS_SLOT_XNORM, FD_EPS * K->qnorm, flexible ? NULL : w);
+                                  fd_next ? KFS_SLOT_XNORM : -1, FD_EPS * K->qnorm, fd_next ? w : NULL);
       }
       /* Givens, src/kernel.c:143-173 */
       for (int i = 0; i < it; i++) {
@@ -394,6 +403,27 @@ static void train(kspace *K)
 
 /* ---- the driver ------------------------------------------------------------------ */
 
+/* a pseudo-time step whose forces are still being computed */
+typedef struct {
+  int valid, idx;
+  kf_step_record r;
+} step_pending;
+
+/* the step's forces (the bits of ComputeForces on the updated state), then
+ * the reference's per-step lines (src/kernel.c:198-212) and the record */
+static void finish_step(GEOMETRY *g, const kf_options *o, step_pending *p, kf_step_record *rec, int max_rec)
+{
+  kfs_forces_collect(g, p->r.forces);
+  if (o->verbose) {
+    printf(">> GMRES: TOTAL ITERATIONS %d\n", p->r.krylov);
+    printf("STEP: %d CFL: %g NORM: %g\n", p->r.step, p->r.cfl, p->r.fnorm);
+    printf("LIFT: %g DRAG: %g MOMN: %g\n", p->r.forces[0], p->r.forces[1], p->r.forces[2]);
+    fflush(stdout);
+  }
+  if (rec && p->idx < max_rec) rec[p->idx] = p->r;
+  p->valid = 0;
+}
+
 int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
 {
   const kf_options o = kf_get_options(g);
@@ -419,6 +449,8 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
     if (L->lo.judge_every < 1) L->lo.judge_every = 1;
     x2 = (double *)kf_device_alloc(sz * sizeof(double));
   }
+  step_pending pend;
+  memset(&pend, 0, sizeof(pend));
   double fnorm_init = 0.0, cfl = o.cfl_init;
   int its = 0, nrec = 0;
   while (its < o.ts_max) {
@@ -507,30 +539,26 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
               "before the update\n", its, kf_last_error(g));
       break;
     }
+    /* the previous step's forces (src/kernel.c:202) are collected before the
+     * state changes again; they ran beside this step's work */
+    if (pend.valid) finish_step(g, &o, &pend, rec, max_rec);
     kfs_axpy(g, sz, -1.0, x, g->q->q);
-    double forces[3];
-    ComputeForces(g, forces);
-    if (o.verbose) {
-      printf(">> GMRES: TOTAL ITERATIONS %d\n", total);
-      printf("STEP: %d CFL: %g NORM: %g\n", its, cfl, fnorm);
-      printf("LIFT: %g DRAG: %g MOMN: %g\n", forces[0], forces[1], forces[2]);
-      fflush(stdout);
-    }
-    if (rec && nrec < max_rec) {
-      rec[nrec].step = its;
-      rec[nrec].cfl = cfl;
-      rec[nrec].fnorm = fnorm;
-      rec[nrec].krylov = total;
-      memcpy(rec[nrec].forces, forces, sizeof(forces));
-      rec[nrec].learned = learned;
-      rec[nrec].krylov_ilu = k_ilu;
-      rec[nrec].krylov_net = k_net;
-    }
+    kfs_forces_async(g);
+    pend.valid = 1;
+    pend.idx = nrec;
+    pend.r.step = its;
+    pend.r.cfl = cfl;
+    pend.r.fnorm = fnorm;
+    pend.r.krylov = total;
+    pend.r.learned = learned;
+    pend.r.krylov_ilu = k_ilu;
+    pend.r.krylov_net = k_net;
     nrec++;
     /* src/kernel.c:214 rounds the ratio; compared in double (no uint32 wrap) */
     if (round(fnorm_init / fnorm) >= o.fnorm_ratio) break;
     its++;
   }
+  if (pend.valid) finish_step(g, &o, &pend, rec, max_rec);
   if (x2) kf_device_free(x2);
   free(K->ghh);
   return nrec;

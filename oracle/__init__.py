@@ -55,6 +55,7 @@ def lib():
         L.or_fill.argtypes = [vp]
         L.or_ilu.argtypes = [vp]
         L.or_sptrsv.argtypes = [vp, dp, dp]
+        L.or_spmv.argtypes = [vp, dp, dp]
         L.or_forces.argtypes = [vp, dp]
         L.or_ijacvec.argtypes = [vp, dp, dp, C.c_double, dp, dp, dp, dp, dp]
         L.or_norm.restype = C.c_double
@@ -154,6 +155,13 @@ class Oracle:
         x = np.empty(self.sz)
         lib().or_sptrsv(self.c, _p(b), _p(x))
         return x
+
+    def spmv(self, x):
+        """y = A x with the current blocks (after fill, before ilu)"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.sz)
+        lib().or_spmv(self.c, _p(x), _p(y))
+        return y
 
     def forces(self):
         f = np.empty(3)

@@ -856,6 +856,28 @@ void or_sptrsv(or_ctx *c, const double *b, double *x)
     }
 }
 
+/* y = A x with the blocks as they stand (the fill's output before or_ilu).
+ * No counterpart in the reference (its Krylov operator is matrix-free); the
+ * order is the device product's: per row, the blocks in ascending column
+ * (the CSR order of matrix/init.cpp:7-74), each component summed
+ * a[c][0] x0 + a[c][1] x1 + a[c][2] x2 + a[c][3] x3 into one running sum. */
+void or_spmv(const or_ctx *c, const double *x, double *y)
+{
+  const or_mat *A = &c->A;
+  for (size_t row = 0; row < c->N; row++)
+    for (int k = 0; k < 4; k++) {
+      double acc = 0.0;
+      for (uint32_t j = A->i[row]; j < A->i[row + 1]; j++) {
+        const double *a = A->a + (size_t)j * 16 + 4 * k, *xc = x + 4 * (size_t)A->j[j];
+        acc = acc + a[0] * xc[0];
+        acc = acc + a[1] * xc[1];
+        acc = acc + a[2] * xc[2];
+        acc = acc + a[3] * xc[3];
+      }
+      y[4 * row + k] = acc;
+    }
+}
+
 /* ---- src/forces.c:8-99 ------------------------------------------------------ */
 
 void or_forces(or_ctx *c, double *forces)

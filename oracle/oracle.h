@@ -74,6 +74,8 @@ void or_timestep(or_ctx *c, double cfl);
 void or_fill(or_ctx *c);
 void or_ilu(or_ctx *c);
 void or_sptrsv(or_ctx *c, const double *b, double *x);
+/* y = A x with the current blocks (after or_fill, before or_ilu) */
+void or_spmv(const or_ctx *c, const double *x, double *y);
 void or_forces(or_ctx *c, double *forces);
 void or_ijacvec(or_ctx *c, const double *x, const double *r, double qnorm, double *gx, double *gy, double *gz,
                 double *w, double *y);

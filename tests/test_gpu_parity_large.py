@@ -5,7 +5,8 @@ The 144x48x48 lattice (seed 1, 331,776 vertices, 2,258,399 edges) and the
 seeded state (free stream plus 0.05 U(-1, 1) per entry, second order):
   * ComputeFlux (LSQ gradient and residual, src/flux.c:60-779),
     ComputeTimeStep (src/timestep.c:21-188), ComputeA (src/fill.c:115-917),
-    the reference's one-sweep ILU (matrix/ilu.cpp:49-138), one
+    the BSR product y = A x of the assembled Jacobian (kf_bsr_spmv), the
+    reference's one-sweep ILU (matrix/ilu.cpp:49-138), one
     ComputeSparseTriangularSolve (matrix/sptrsv.cpp:34-73) and
     ComputeForces (src/forces.c:8-120): bit for bit;
   * on the 144x48x48 lattice, the first two pseudo-time steps of the
@@ -42,7 +43,7 @@ def big(request, mesh_dir):
     (nx, ny, nz), seed = LATTICES[request.param]
     print("lattice %s %dx%dx%d seed %d: setup" % (request.param, nx, ny, nz, seed), flush=True)
     path = lattice(mesh_dir, nx, ny, nz, seed)
-    m = K.Mesh(path, K.options())
+    m = K.Mesh(path, K.options(bsr_operator=1))   # the fill also keeps A for the BSR product
     orc = O.Oracle(m)
     yield request.param, m, orc
     orc.close()
@@ -76,6 +77,15 @@ def test_kernels_bitwise_at_size(big):
     A, Ao = m.blocks().ravel(), orc.blocks().ravel()
     assert _bitwise(A, Ao), "%s fill: %s" % (name, _describe(A, Ao))
     del A, Ao
+    print("%s: BSR product" % name, flush=True)
+    xs = np.random.default_rng(32).standard_normal(m.sz)
+    xsd, ysd = K.DeviceVec(m.sz, xs), K.DeviceVec(m.sz)
+    m.spmv(xsd, ysd)
+    y, yo = ysd.numpy(), orc.spmv(xs)
+    assert _bitwise(y, yo), "%s spmv: %s" % (name, _describe(y, yo))
+    xsd.free()
+    ysd.free()
+    del y, yo, xs
 
     print("%s: ILU(0)" % name, flush=True)
     m.ilu()

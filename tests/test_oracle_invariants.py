@@ -97,6 +97,51 @@ def test_jacobian_matches_central_differences_at_uniform_state(small):
     assert np.abs(Ad[sel] - fd[sel]).max() <= 1e-7 * np.abs(fd[sel]).max()
 
 
+def _filled(small, seed):
+    orc = O.Oracle(small)
+    q = freestream(small.N) + 0.05 * np.random.default_rng(seed).standard_normal(small.sz)
+    orc.set_state(q)
+    orc.timestep(20.0)
+    orc.fill()
+    return orc
+
+
+def test_bsr_product_matches_scipy(small):
+    """or_spmv (the device product's order) against scipy's BSR product of
+    the same blocks, pattern and vector: the sums differ only in order."""
+    import scipy.sparse as sp
+    orc = _filled(small, 4)
+    rp, col = orc.pattern()
+    A = sp.bsr_matrix((orc.blocks(), col.astype(np.int64), rp.astype(np.int64)), shape=(small.sz, small.sz))
+    x = np.random.default_rng(5).standard_normal(small.sz)
+    y = orc.spmv(x)
+    orc.close()
+    want = A @ x
+    assert np.abs(y - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_bsr_product_extracts_columns_exactly(small):
+    """With x a unit vector every row sums exact zeros and one product a * 1,
+    so y is column k of A bit for bit: a transposed block or a wrong column
+    index would move the entries."""
+    orc = _filled(small, 6)
+    rp, col = orc.pattern()
+    B = orc.blocks()
+    rng = np.random.default_rng(7)
+    for k in rng.choice(small.sz, 12, replace=False):
+        x = np.zeros(small.sz)
+        x[k] = 1.0
+        y = orc.spmv(x)
+        want = np.zeros(small.sz)
+        v, cc = divmod(int(k), 4)
+        for i in range(small.N):
+            for j in range(rp[i], rp[i + 1]):
+                if col[j] == v:
+                    want[4 * i:4 * i + 4] = B[j][:, cc]
+        assert np.array_equal(y, want), k
+    orc.close()
+
+
 def test_ilu0_defining_property_and_triangular_solve(small):
     orc = O.Oracle(small, ilu_textbook=1)
     q = freestream(small.N) + 0.05 * np.random.default_rng(1).standard_normal(small.sz)
