@@ -572,6 +572,9 @@ def main():
         lr = learned(K, lmesh, args.learn_rhs, args.learn_iters, args.learn_prec, 1 if args.progress else 0)
         ldims, lseed = CONFIGS[args.learn_config]
         lr["mesh"] = "%dx%dx%d lattice seed %d" % (ldims + (lseed,))
+        if args.learn_config == "c2" and out["variants_144x48x48"]:
+            # configuration 4 reports against plain (left-preconditioned) ILU(0)-GMRES on the same mesh
+            lr["plain_ilu_gmres"] = out["variants_144x48x48"]["default"]
         out["learned_preconditioner"] = lr
     print(json.dumps(out))
 

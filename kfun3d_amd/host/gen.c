@@ -112,6 +112,35 @@ static void put_f64(FILE *f, double d)
   fwrite(b, 1, 8, f);
 }
 
+/* n big-endian words v[i] + add (or add + i when v is NULL), encoded in
+ * parallel into one buffer and written at once */
+static void put_u32s(FILE *f, const uint32_t *v, size_t n, uint32_t add)
+{
+  unsigned char *b = (unsigned char *)kf_xmalloc(4 * n + 1);
+#pragma omp parallel for schedule(static) if (n > 65536)
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t x = (v ? v[i] : (uint32_t)i) + add;
+    b[4 * i] = (unsigned char)(x >> 24); b[4 * i + 1] = (unsigned char)(x >> 16);
+    b[4 * i + 2] = (unsigned char)(x >> 8); b[4 * i + 3] = (unsigned char)x;
+  }
+  fwrite(b, 1, 4 * n, f);
+  free(b);
+}
+
+/* n big-endian doubles v[i * stride] */
+static void put_f64s(FILE *f, const double *v, size_t n, size_t stride)
+{
+  unsigned char *b = (unsigned char *)kf_xmalloc(8 * n + 1);
+#pragma omp parallel for schedule(static) if (n > 65536)
+  for (size_t i = 0; i < n; i++) {
+    uint64_t x;
+    memcpy(&x, v + i * stride, 8);
+    for (int k = 0; k < 8; k++) b[8 * i + k] = (unsigned char)(x >> (56 - 8 * k));
+  }
+  fwrite(b, 1, 8 * n, f);
+  free(b);
+}
+
 /* writes one boundary section: group table, faces, node ids, node normals */
 static void write_section(FILE *f, const lattice *L, const trilist *tl, const uint32_t *nodes, size_t nn,
                           const uint32_t *local, const double *nrm)
@@ -300,20 +329,26 @@ int kf_write_lattice_msh(const char *fpath, int nx, int ny, int nz, uint64_t see
   uint32_t p[13] = {(uint32_t)L.ncell, (uint32_t)L.N, (uint32_t)L.E, 1, 0, 1, (uint32_t)solid.n, 0,
                     (uint32_t)freeb.n, (uint32_t)ns, 0, (uint32_t)nf, 0};
   for (int i = 0; i < 13; i++) put_u32(f, p[i]);
-  for (size_t e = 0; e < E; e++) put_u32(f, L.n0[e] + 1);
-  for (size_t e = 0; e < E; e++) put_u32(f, L.n1[e] + 1);
+  put_u32s(f, L.n0, E, 1);
+  put_u32s(f, L.n1, E, 1);
   double *area = (double *)kf_xmalloc(E * sizeof(double));
+  double *unit = (double *)kf_xmalloc(E * sizeof(double));
+#pragma omp parallel for schedule(static)
   for (size_t e = 0; e < E; e++) {
     const double *n = L.en + 3 * e;
     area[e] = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   }
-  for (int d = 0; d < 3; d++)
-    for (size_t e = 0; e < E; e++) put_f64(f, L.en[3 * e + d] / area[e]);
-  for (size_t e = 0; e < E; e++) put_f64(f, area[e]);
-  for (size_t v = 0; v < L.N; v++) put_f64(f, L.x[v]);
-  for (size_t v = 0; v < L.N; v++) put_f64(f, L.y[v]);
-  for (size_t v = 0; v < L.N; v++) put_f64(f, L.z[v]);
-  for (size_t v = 0; v < L.N; v++) put_f64(f, L.vol[v]);
+  for (int d = 0; d < 3; d++) {
+#pragma omp parallel for schedule(static)
+    for (size_t e = 0; e < E; e++) unit[e] = L.en[3 * e + d] / area[e];
+    put_f64s(f, unit, E, 1);
+  }
+  free(unit);
+  put_f64s(f, area, E, 1);
+  put_f64s(f, L.x, L.N, 1);
+  put_f64s(f, L.y, L.N, 1);
+  put_f64s(f, L.z, L.N, 1);
+  put_f64s(f, L.vol, L.N, 1);
   write_section(f, &L, &solid, snodes, ns, sloc, sn);
   write_section(f, &L, &freeb, fnodes, nf, floc, fn);
   int rc = ferror(f) ? -3 : 0;
