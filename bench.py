@@ -76,13 +76,19 @@ def alg_bytes(kind, N, E, S, Nf, nnzb, s):
         "trsv_lower": E * (16 * s + 4) + 64 * N,  # lower blocks + cols; b in, x out
         "trsv_upper": (E + N) * (16 * s + 4) + 64 * N,
         "spmv": (N + 2 * E) * (16 * s + 4) + 64 * N,   # every block + col; x in, y out
+        # one Jacobi sweep (per launch), the compulsory bytes: the half's blocks + cols, row bounds and
+        # diagonal index, b (FP64) or y in, the new vector out, vectors in the sweep's precision (8 bytes
+        # FP64, 4 otherwise); the old vector's gathers, the fused D^-1 y of the last forward sweep and
+        # the FP64 store of the last backward one are not counted (a lower bound)
+        "jacobi_lower": E * (16 * s + 4) + 8 * N + 4 * N * (8 + (8 if s == 8 else 4)),
+        "jacobi_upper": (E + N) * (16 * s + 4) + 8 * N + 4 * N * 2 * (8 if s == 8 else 4),
     }[kind]
 
 
 PROFILE_TO_KIND = {"grad": "grad", "edge_flux": "edge_flux", "residual": "residual_gather", "timestep": "timestep",
                    "fill": "fill", "ilu": "ilu", "trsv_lower": "trsv_lower", "trsv_upper": "trsv_upper",
-                   "spmv": "spmv"}
-MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper", "spmv")
+                   "spmv": "spmv", "jacobi_lower": "jacobi_lower", "jacobi_upper": "jacobi_upper"}
+MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper", "spmv", "jacobi_lower", "jacobi_upper")
 TYPE = {"fp64": ("double", "double"), "fp32": ("float", "float"), "bf16": ("unsigned short", "float")}
 
 
@@ -100,6 +106,10 @@ def kernel_name(dom, prec, held, fold):
         return "k_fill<%s>" % ("double" if prec == "fp64" else "float")
     if dom == "spmv":
         return "k_bsr_spmv<%s>" % ("double" if prec == "fp64" else "float")
+    if dom == "jacobi_lower":   # the inner sweeps' instantiation (the class also holds the first and last)
+        return "k_jac_lower<%s, %s, %s, false>" % (tm, tv, tv)
+    if dom == "jacobi_upper":
+        return "k_jac_upper<%s, %s, %s>" % (tm, tv, tv)
     return {"grad": "k_grad", "edge_flux": "k_edge_flux<false>", "residual": "k_residual<false>",
             "timestep": "k_timestep"}[dom]
 
@@ -168,6 +178,20 @@ def lattice_mesh(K, tmp, config):
         log("writing the %dx%dx%d lattice (seed %d)" % (dims + (seed,)))
         K.write_lattice(path, *dims, seed)
     return path
+
+
+def path_options(K, path, **kw):
+    """configuration-profile options of a path name: fp64, fp32, bf16, or
+    <prec>_jacobi<K> (K Jacobi sweeps each way instead of the exact
+    triangular solves)"""
+    prec, _, jac = path.partition("_jacobi")
+    return K.config_options(prec=prec, jacobi_sweeps=int(jac) if jac else 0, **kw)
+
+
+def path_dtype(path):
+    prec = path.partition("_jacobi")[0]
+    return {"fp64": "f64", "fp32": "f64 residual/Arnoldi/update, f32 preconditioner",
+            "bf16": "f64 residual/Arnoldi/update, f32 preconditioner arithmetic on bf16 factors"}[prec]
 
 
 def timed_solve(m, q0):
@@ -429,8 +453,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c5", choices=["c2", "c3", "c5"],
                     help="lattice: c5 288x96x96 seed 4 (the headline), c3 216x72x72 seed 2, c2 144x48x48 seed 1")
-    ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16"],
-                    help="the path `value` times: auto = the faster of FP32 and BF16 in the warm-up solves")
+    ap.add_argument("--prec", default="auto",
+                    help="the path `value` times: fp64, fp32, bf16, or one of those with _jacobiK (K Jacobi "
+                         "sweeps each way instead of the exact triangular solves); auto = the fastest "
+                         "single-precision path of the warm-up solves whose forces are within 1e-8 of FP64")
+    ap.add_argument("--jacobi", default="2,3",
+                    help="Jacobi sweep counts of the extra single-precision paths (comma list; empty: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
@@ -481,17 +509,25 @@ def main():
     f64 = h64[-1]["forces"]
     log("FP64 %.3f s" % t64w)
 
-    precs = ["fp32", "bf16"] if not args.no_mixed else []
-    warm = {"fp64": t64w}
+    precs = []
+    if not args.no_mixed:
+        precs = ["fp32", "bf16"]
+        for k in [int(x) for x in args.jacobi.split(",") if x.strip()]:
+            precs += ["fp32_jacobi%d" % k, "bf16_jacobi%d" % k]
+    warm, warm_err = {"fp64": t64w}, {"fp64": 0.0}
     for p in precs:
-        m.set_options(K.config_options(prec=p))
-        warm[p], _ = timed_solve(m, q0)
-        log("%s warm-up %.3f s" % (p, warm[p]))
+        m.set_options(path_options(K, p))
+        warm[p], hw = timed_solve(m, q0)
+        f = np.array(hw[-1]["forces"])
+        warm_err[p] = float((np.abs(f - np.array(f64)) / np.abs(np.array(f64))).max())
+        log("%s warm-up %.3f s, %d Krylov, forces within %.1e of FP64" % (p, warm[p], sum(x["krylov"] for x in hw),
+                                                                          warm_err[p]))
     used_all, _ = mem_used()
-    vprec = args.prec if args.prec != "auto" else (min(precs, key=lambda p: warm[p]) if precs else "fp64")
+    ok = [p for p in precs if warm_err[p] <= 1e-8]
+    vprec = args.prec if args.prec != "auto" else (min(ok, key=lambda p: warm[p]) if ok else "fp64")
 
     # the value path: W more warm-up solves, then K timed, profiled
-    m.set_options(K.config_options(prec=vprec))
+    m.set_options(path_options(K, vprec))
     for _ in range(args.warmup):
         timed_solve(m, q0)
     log("timed solves (%s)" % vprec)
@@ -514,7 +550,7 @@ def main():
     for p in ["fp64"] + [x for x in precs if x != vprec]:
         if p == vprec:
             continue
-        m.set_options(K.config_options(prec=p))
+        m.set_options(path_options(K, p))
         reps = min(args.steps, 3) if p == "fp64" else 1
         if p == "fp64":
             m.profile(True)
@@ -539,11 +575,14 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64" if vprec == "fp64" else "f64 residual/Arnoldi/update, %s preconditioner" % vprec,
+        "dtype": path_dtype(vprec),
         "data": "synthetic",
         "config": {"workload": "%dx%dx%d Kuhn lattice seed %d (%d vertices, %d edges), second order, %s "
-                               "preconditioner (block ILU(0)), GMRES(30) MGS, CFL ramp from 10, one solve from the "
-                               "free stream to a 1e-10 residual drop" % (dims + (seed, N, E, vprec)),
+                               "preconditioner (block ILU(0)%s), GMRES(30) MGS, CFL ramp from 10, one solve from the "
+                               "free stream to a 1e-10 residual drop"
+                               % (dims + (seed, N, E, vprec.partition("_jacobi")[0],
+                                          ", %s Jacobi sweeps each way" % vprec.partition("_jacobi")[2]
+                                          if "_jacobi" in vprec else "")),
                    "lattice": list(dims), "seed": seed, "vertices": N, "edges": E, "path": vprec},
         "pseudo_steps": len(hist), "krylov_total": sum(x["krylov"] for x in hist),
         "residual_drop": hist[0]["fnorm"] / hist[-1]["fnorm"],
@@ -557,7 +596,7 @@ def main():
         "flux_gflops": flux_flops / flux_s / 1e9, "flux_fp64_frac": flux_flops / flux_s / 1e9 / FP64_PEAK_GFLOPS,
         "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
         "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
-        "roofline": roofline(prof, vprec, sizes, args.config, held, fold[vprec]),
+        "roofline": roofline(prof, vprec.partition("_jacobi")[0], sizes, args.config, held, fold[vprec]),
         "roofline_fp64": roofline(prof64, "fp64", sizes, args.config, held, fold["fp64"]) if prof64 else None,
         "roofline_spmv": spmv,
     }

@@ -166,6 +166,14 @@ typedef struct kf_options {
                            residual stays second order, so both converge to
                            the same steady state; 1 takes more, cheaper
                            Newton steps (defect correction). */
+  int    jacobi_sweeps; /* 0: exact level-scheduled triangular solves
+                           (matrix/sptrsv.cpp:34-73, the reference's
+                           default); k in 1..64: k Jacobi sweeps each way
+                           over the same factors instead
+                           (fun3d::Matrix::sptrsv_Jacobi, matrix/sptrsv.cpp:
+                           75-132, its jacobi_sweeps): an approximate,
+                           fixed linear preconditioner whose sweeps are
+                           independent HBM-bound passes */
 } kf_options;
 
 typedef struct kf_step_record {

@@ -85,7 +85,8 @@ class Options(C.Structure):
                 ("ts_max", C.c_int), ("fnorm_ratio", C.c_double), ("gmres_restart", C.c_int),
                 ("gmres_rtol", C.c_double), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
                 ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int), ("ilu_exact", C.c_int),
-                ("reduce_seq", C.c_int), ("krylov", C.c_int), ("bsr_operator", C.c_int)]
+                ("reduce_seq", C.c_int), ("krylov", C.c_int), ("bsr_operator", C.c_int),
+                ("jacobi_sweeps", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -473,7 +474,8 @@ class Mesh:
             raise KeyError(name)
         return ms.value, n.value
 
-    PROFILED = ["grad", "edge_flux", "residual", "timestep", "fill", "ilu", "trsv_lower", "trsv_upper", "spmv"]
+    PROFILED = ["grad", "edge_flux", "residual", "timestep", "fill", "ilu", "trsv_lower", "trsv_upper", "spmv",
+                "jacobi_lower", "jacobi_upper"]
 
     def synchronize(self):
         lib().kf_synchronize(self.ref)

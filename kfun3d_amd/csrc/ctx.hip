@@ -28,6 +28,7 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
 {
   kf_ctx *c = new kf_ctx();
   memset((void *)c, 0, sizeof(*c));
+  kf_check_options(o);
   KF_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->opt = *o;
   kf_dv &d = c->d;
@@ -252,7 +253,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->rdone,
                   c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
-                  c->abf, c->jop64, c->jop32, c->ftbuf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
+                  c->abf, c->jop64, c->jop32, c->ftbuf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->jac64, c->jac32, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));
   if (c->mgs_ev) KF_CHECK(hipEventDestroy(c->mgs_ev));
@@ -286,18 +287,28 @@ extern "C" kf_options kf_default_options(void)
   o.reduce_seq = 0;
   o.krylov = KF_KRYLOV_GMRES;
   o.bsr_operator = 0;
+  o.jacobi_sweeps = 0;
   return o;
 }
 
 static void set_free_stream_state(kf_ctx *c);
 
-extern "C" void kf_set_options(GEOMETRY *g, const kf_options *o)
+void kf_check_options(const kf_options *o)
 {
-  kf_ctx *c = kf_ctx_of(g);
   if (o->gmres_restart < 1 || o->gmres_restart > 31) {
     fprintf(stderr, "kf: gmres_restart must be in [1, 31]\n");
     abort();
   }
+  if (o->jacobi_sweeps < 0 || o->jacobi_sweeps > KF_JACOBI_MAX) {
+    fprintf(stderr, "kf: jacobi_sweeps must be in [0, %d]\n", KF_JACOBI_MAX);
+    abort();
+  }
+}
+
+extern "C" void kf_set_options(GEOMETRY *g, const kf_options *o)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  kf_check_options(o);
   const int alpha_changed = o->alpha_deg != c->opt.alpha_deg;
   c->opt = *o;
   if (alpha_changed) {
@@ -411,6 +422,7 @@ extern "C" size_t kf_nnzb(const GEOMETRY *g) { return kf_ctx_of(g)->nnzb; }
 extern "C" int kf_trsv_fold(const GEOMETRY *g)
 {
   kf_ctx *c = kf_ctx_of(g);
+  if (c->opt.jacobi_sweeps > 0) return 0;   /* the combination runs first (every forward sweep reads b) */
   return !(c->trsv_fold == 0 || (c->trsv_fold < 0 && !kf_trsv_capped(c)));
 }
 
@@ -506,7 +518,9 @@ extern "C" void ComputeSparseTriangularSolve(const GEOMETRY *g, const double *b,
     fprintf(stderr, "kf: ComputeSparseTriangularSolve: b and x must not alias\n");
     abort();
   }
-  kf_launch_sptrsv(kf_ctx_of(g), b, x, NULL);
+  kf_ctx *c = kf_ctx_of(g);
+  if (c->opt.jacobi_sweeps > 0) kf_launch_jacobi(c, b, x);   /* matrix/sptrsv.cpp:75-132 */
+  else kf_launch_sptrsv(c, b, x, NULL);
 }
 
 /* GMRES's inner step: x = U^-1 L^-1 y with y = ((R(w) - r) + cdt (w - q))
@@ -528,6 +542,11 @@ extern "C" void kfs_sptrsv_fd(GEOMETRY *g, const double *b, double *x, const dou
     abort();
   }
   kf_check_aligned(r, "r"); kf_check_aligned(w, "w");
+  if (c->opt.jacobi_sweeps > 0) {   /* every forward sweep reads b: form it once */
+    kf_fd_combine(c, r, w, h, slot_xnorm, (double *)b);
+    kf_launch_jacobi(c, b, x);
+    return;
+  }
   if (c->trsv_fold == 0 || (c->trsv_fold < 0 && !kf_trsv_capped(c))) {
     /* latency-bound sweeps: the four extra loads at each task's start
      * would lengthen the chain (0.748 against 0.754 s per solve on the
@@ -589,7 +608,7 @@ extern "C" double Normalize(const size_t sz, double *x)
 /* ---- per-kernel timing with HIP events on the context stream ----------------- */
 
 static const char *kf_prof_names[KP_N] = {"grad", "edge_flux", "residual", "timestep", "fill", "ilu",
-                                          "trsv_lower", "trsv_upper", "spmv"};
+                                          "trsv_lower", "trsv_upper", "spmv", "jacobi_lower", "jacobi_upper"};
 
 static void prof_drain(kf_ctx *c)
 {

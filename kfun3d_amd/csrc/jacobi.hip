@@ -1,0 +1,253 @@
+/*
+ * jacobi.hip -- x ~ U^-1 L^-1 b by Jacobi sweeps over the in-place ILU(0)
+ * factors: fun3d::Matrix::sptrsv_Jacobi (matrix/sptrsv.cpp:75-132), the
+ * reference's alternative to its level-scheduled solve (selected at
+ * matrix/wrapper.cpp:86-91; its sweep count is Matrix::jacobi_sweeps,
+ * matrix/matrix.h:35).
+ *
+ * With k sweeps (k >= 1; the reference's do-while runs at least one):
+ *   forward   y0 = b;     y_s[r] = b[r] - sum_{j < r} L(r,j) y_{s-1}[j]
+ *   backward  x0 = D^-1 y_k;  x_s[r] = D^-1 (y_k[r] - sum_{j > r} U(r,j) x_{s-1}[j])
+ * (D^-1 is the inverted diagonal block the factorisation leaves in place).
+ * Each sweep reads every row's old values only, so the rows are independent:
+ * no level schedule, no waits, every sweep one HBM-bound pass over its half
+ * of the factors.  The result approaches the exact triangular solves as k
+ * grows and equals them (up to rounding) once k reaches the depth of the
+ * dependency DAG.
+ *
+ * Layout and order follow k_bsr_spmv (spmv.hip): a quad of lanes per row,
+ * lane c forming component c; per row the reference's order exactly:
+ *   t = dgemv(block, x_col)   t_c = ((0 + a_c0 x0) + a_c1 x1) + a_c2 x2) + a_c3 x3
+ *   s = t + s                 over the row's entries in ascending column, s from 0
+ *   forward   y = (-s) + b
+ *   backward  u = (-s) + y,   x = dgemv(D^-1, u)   (the quad exchanges u)
+ * so the FP64 mode is bit-identical to a one-thread run of the reference's
+ * loops.  Single-precision modes: FP32 or BF16 factors, FP32 vectors and
+ * arithmetic, b widened from FP64 as read, the last backward sweep writing
+ * FP64.  The last forward sweep also writes x0 = D^-1 y_k (fused; the
+ * reference's separate dgemv pass).  Workgroups are remapped XCD by XCD.
+ * Bytes per sweep: (half the factor blocks) (16 s + 4) + vector traffic.
+ */
+#include "kf_dev.h"
+
+#define JAC_WG 256   /* 64 rows per workgroup */
+
+namespace {
+
+__device__ __forceinline__ double jmat(const double *a, size_t i) { return a[i]; }
+__device__ __forceinline__ float jmat(const float *a, size_t i) { return a[i]; }
+__device__ __forceinline__ float jmat(const uint16_t *a, size_t i) { return __uint_as_float((uint32_t)a[i] << 16); }
+
+template <typename TM, typename TV>
+__device__ __forceinline__ void jrow(const TM *__restrict__ a, uint32_t j, int c, TV m[4])
+{
+  const size_t o = 16 * (size_t)j + 4 * c;
+  m[0] = (TV)jmat(a, o);
+  m[1] = (TV)jmat(a, o + 1);
+  m[2] = (TV)jmat(a, o + 2);
+  m[3] = (TV)jmat(a, o + 3);
+}
+
+template <typename TV, typename TX>
+__device__ __forceinline__ void jvec(const TX *__restrict__ x, uint32_t col, TV v[4])
+{
+  const TX *p = x + 4 * (size_t)col;
+  v[0] = (TV)p[0];
+  v[1] = (TV)p[1];
+  v[2] = (TV)p[2];
+  v[3] = (TV)p[3];
+}
+
+/* dgemv row: summed from zero in column order (matrix/init.cpp:96-103) */
+template <typename TV>
+__device__ __forceinline__ TV jdot(const TV m[4], const TV x[4])
+{
+  TV s = TV(0);
+  s += m[0] * x[0];
+  s += m[1] * x[1];
+  s += m[2] * x[2];
+  s += m[3] * x[3];
+  return s;
+}
+
+template <int I>
+__device__ __forceinline__ float jquad(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), I * 0x55, 0xf, 0xf, false));
+}
+
+template <int I>
+__device__ __forceinline__ double jquad(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), I * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), I * 0x55, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+/* s = sum over entries [j0, j1) of the row, in ascending column:
+ * s = dgemv(A_j, x_col) + s (matrix/sptrsv.cpp:91-96 / 116-121) */
+template <typename TM, typename TV, typename TX>
+__device__ __forceinline__ TV jsum(const uint32_t *__restrict__ col, const TM *__restrict__ a, const TX *__restrict__ x,
+                                   uint32_t j0, uint32_t j1, int c)
+{
+  TV s = TV(0);
+  uint32_t j = j0;
+  for (; j + 1 < j1; j += 2) {   /* two blocks in flight */
+    const uint32_t k0 = col[j], k1 = col[j + 1];
+    TV m0[4], m1[4], x0[4], x1[4];
+    jrow<TM, TV>(a, j, c, m0);
+    jrow<TM, TV>(a, j + 1, c, m1);
+    jvec<TV, TX>(x, k0, x0);
+    jvec<TV, TX>(x, k1, x1);
+    const TV t0 = jdot(m0, x0);
+    s = t0 + s;
+    const TV t1 = jdot(m1, x1);
+    s = t1 + s;
+  }
+  if (j < j1) {
+    TV m0[4], x0[4];
+    jrow<TM, TV>(a, j, c, m0);
+    jvec<TV, TX>(x, col[j], x0);
+    const TV t0 = jdot(m0, x0);
+    s = t0 + s;
+  }
+  return s;
+}
+
+/* quad exchange, then dgemv(D^-1, u) row c */
+template <typename TM, typename TV>
+__device__ __forceinline__ TV jdinv(const TM *__restrict__ a, uint32_t jd, int c, TV u)
+{
+  TV dg[4], uv[4];
+  jrow<TM, TV>(a, jd, c, dg);
+  uv[0] = jquad<0>(u);
+  uv[1] = jquad<1>(u);
+  uv[2] = jquad<2>(u);
+  uv[3] = jquad<3>(u);
+  return jdot(dg, uv);
+}
+
+}  // namespace
+
+/* One forward sweep: y[r] = b[r] - sum_{j < r} L(r,j) xin[j]; with DIAG also
+ * x0[r] = D^-1 y[r].  xin is b itself (TX = double) on the first sweep. */
+template <typename TM, typename TV, typename TX, bool DIAG>
+__global__ __launch_bounds__(JAC_WG) void k_jac_lower(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                      const uint32_t *__restrict__ col,
+                                                      const uint32_t *__restrict__ diag, const TM *__restrict__ a,
+                                                      const double *__restrict__ b, const TX *__restrict__ xin,
+                                                      TV *__restrict__ y, TV *__restrict__ x0)
+{
+  const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;   /* whole quads leave together (N rows of 4 lanes) */
+  const uint32_t jd = diag[r];
+  const TV s = jsum<TM, TV, TX>(col, a, xin, rowptr[r], jd, c);
+  const TV v = (-s) + (TV)b[4 * (size_t)r + c];
+  y[4 * (size_t)r + c] = v;
+  if (DIAG) x0[4 * (size_t)r + c] = jdinv<TM, TV>(a, jd, c, v);
+}
+
+/* One backward sweep: x[r] = D^-1 (y[r] - sum_{j > r} U(r,j) xin[j]) */
+template <typename TM, typename TV, typename TO>
+__global__ __launch_bounds__(JAC_WG) void k_jac_upper(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                      const uint32_t *__restrict__ col,
+                                                      const uint32_t *__restrict__ diag, const TM *__restrict__ a,
+                                                      const TV *__restrict__ y, const TV *__restrict__ xin,
+                                                      TO *__restrict__ x)
+{
+  const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;
+  const uint32_t jd = diag[r];
+  const TV s = jsum<TM, TV, TV>(col, a, xin, jd + 1, rowptr[r + 1], c);
+  const TV u = (-s) + y[4 * (size_t)r + c];
+  x[4 * (size_t)r + c] = (TO)jdinv<TM, TV>(a, jd, c, u);
+}
+
+/* The k sweeps each way.  Work vectors (4N of TV each): y (the forward
+ * result, kept for the backward sweeps) and two ping-pong buffers p0, p1.
+ * x (FP64) receives the last backward sweep. */
+template <typename TM, typename TV>
+static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV *y, TV *p0, TV *p1, int k)
+{
+  const kf_dv &d = c->d;
+  const uint32_t nb = kf_blocks_for(d.N, JAC_WG / 4);
+  const dim3 grid(nb), block(JAC_WG);
+  /* forward: sweeps 1 .. k-1 write p0, p1, p0, ...; sweep k reads the last
+   * of them (b when k == 1), writes y, and writes the backward start D^-1 y
+   * into the buffer it does not read */
+  TV *const xread = k >= 2 ? (((k - 1) & 1) ? p0 : p1) : NULL;
+  TV *cur = xread == p0 ? p1 : p0;
+  int s = kf_prof_begin(c, KP_JAC_L);
+  const TV *prev = NULL;
+  for (int sw = 1; sw <= k; sw++) {
+    const bool last = sw == k;
+    TV *out = last ? y : ((sw & 1) ? p0 : p1);
+    if (sw == 1 && last)
+      hipLaunchKernelGGL((k_jac_lower<TM, TV, double, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
+                         a, b, b, out, cur);
+    else if (sw == 1)
+      hipLaunchKernelGGL((k_jac_lower<TM, TV, double, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
+                         a, b, b, out, (TV *)NULL);
+    else if (last)
+      hipLaunchKernelGGL((k_jac_lower<TM, TV, TV, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a,
+                         b, prev, out, cur);
+    else
+      hipLaunchKernelGGL((k_jac_lower<TM, TV, TV, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a,
+                         b, prev, out, (TV *)NULL);
+    KF_LAUNCH_CHECK();
+    prev = out;
+  }
+  kf_prof_end(c, s);
+  /* backward: from cur, ping-pong with the other buffer, the last sweep
+   * into x */
+  TV *other = cur == p0 ? p1 : p0;
+  s = kf_prof_begin(c, KP_JAC_U);
+  for (int sw = 1; sw <= k; sw++) {
+    if (sw == k) {
+      hipLaunchKernelGGL((k_jac_upper<TM, TV, double>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a, y,
+                         cur, x);
+    } else {
+      hipLaunchKernelGGL((k_jac_upper<TM, TV, TV>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a, y, cur,
+                         other);
+      TV *t = cur;
+      cur = other;
+      other = t;
+    }
+    KF_LAUNCH_CHECK();
+  }
+  kf_prof_end(c, s);
+}
+
+/* x ~ U^-1 L^-1 b with c->opt.jacobi_sweeps sweeps each way, in the
+ * preconditioner's precision; b, x: device, 4N doubles, not aliased */
+void kf_launch_jacobi(kf_ctx *c, const double *b, double *x)
+{
+  kf_check_aligned(b, "b");
+  kf_check_aligned(x, "x");
+  if (b == x) {
+    fprintf(stderr, "kf: Jacobi triangular sweeps: b and x must not alias\n");
+    abort();
+  }
+  const int k = c->opt.jacobi_sweeps;
+  if (k < 1 || k > KF_JACOBI_MAX) {
+    fprintf(stderr, "kf: jacobi_sweeps %d outside 1..%d\n", k, KF_JACOBI_MAX);
+    abort();
+  }
+  const size_t n = 4 * (size_t)c->d.N;
+  if (c->opt.prec == KF_PREC_FP64) {
+    if (!c->ty64) KF_CHECK(hipMalloc((void **)&c->ty64, n * sizeof(double)));
+    if (!c->jac64) KF_CHECK(hipMalloc((void **)&c->jac64, 2 * n * sizeof(double)));
+    jacobi_solve<double, double>(c, c->a64, b, x, c->ty64, c->jac64, c->jac64 + n, k);
+    c->ystate[0] = ~(uint64_t)0;   /* y no longer holds the level-scheduled sweep's sentinels */
+    return;
+  }
+  if (!c->f32a) KF_CHECK(hipMalloc((void **)&c->f32a, n * sizeof(float)));
+  if (!c->jac32) KF_CHECK(hipMalloc((void **)&c->jac32, 2 * n * sizeof(float)));
+  if (c->opt.prec == KF_PREC_FP32)
+    jacobi_solve<float, float>(c, c->a32, b, x, c->f32a, c->jac32, c->jac32 + n, k);
+  else
+    jacobi_solve<uint16_t, float>(c, c->abf, b, x, c->f32a, c->jac32, c->jac32 + n, k);
+  c->ystate[1] = ~(uint64_t)0;
+}

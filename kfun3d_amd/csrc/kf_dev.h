@@ -46,6 +46,7 @@
 #define KF_MGS_ARRIVE 4096    /* polls a resident MGS workgroup waits for the others to arrive */
 #define KF_GROUP 16           /* rows per triangular-solve group */
 #define KF_SLOT_FASYNC (KF_NSCAL - 6)   /* scalar slots of the overlapped forces (3) */
+#define KF_JACOBI_MAX 64      /* Jacobi triangular sweeps each way, at most */
 
 /* device error word bits */
 #define KF_ERR_NONFINITE_PIVOT 1u
@@ -76,7 +77,8 @@ struct kf_dv {
 };
 
 /* per-kernel HIP-event timing on the context stream (kf_profile) */
-enum kf_prof_id { KP_GRAD, KP_EDGE, KP_RES, KP_TSTEP, KP_FILL, KP_ILU, KP_TRSV_L, KP_TRSV_U, KP_SPMV, KP_N };
+enum kf_prof_id { KP_GRAD, KP_EDGE, KP_RES, KP_TSTEP, KP_FILL, KP_ILU, KP_TRSV_L, KP_TRSV_U, KP_SPMV, KP_JAC_L, KP_JAC_U,
+                  KP_N };
 #define KF_PROF_POOL 512
 
 struct kf_prof {
@@ -155,6 +157,8 @@ struct kf_ctx {
   double *gx, *gy, *gz, *r, *w, *t, *xk, *vecs;
   int nvecs;
   float *f32a, *f32b;       /* single-precision forward / backward outputs */
+  double *jac64;            /* Jacobi sweeps' ping-pong buffers (2 x 4N, FP64 ...) */
+  float *jac32;             /* ... or FP32) */
   double *eterm, *sr;       /* per-edge time-step terms, per-vertex time-step sums */
   /* forces overlapped with the next pseudo-time step (kf_forces_async) */
   hipStream_t fstream;      /* default priority, like the main stream */
@@ -221,6 +225,9 @@ void kf_launch_sptrsv(kf_ctx *c, const double *b, double *x, const kf_fdb *fd);
 void kf_launch_spmv(kf_ctx *c, const double *x, double *y);
 void kf_keep_jacobian(kf_ctx *c);
 bool kf_trsv_capped(const kf_ctx *c);
+/* x ~ U^-1 L^-1 b by opt.jacobi_sweeps Jacobi sweeps each way (jacobi.hip) */
+void kf_launch_jacobi(kf_ctx *c, const double *b, double *x);
+void kf_check_options(const kf_options *o);
 
 /* BLAS-1 on the context stream; reductions are deterministic (fixed
  * partition, fixed tree) and leave their result in d.scal[slot] */

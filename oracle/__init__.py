@@ -55,6 +55,7 @@ def lib():
         L.or_fill.argtypes = [vp]
         L.or_ilu.argtypes = [vp]
         L.or_sptrsv.argtypes = [vp, dp, dp]
+        L.or_sptrsv_jacobi.argtypes = [vp, dp, dp, C.c_int]
         L.or_spmv.argtypes = [vp, dp, dp]
         L.or_forces.argtypes = [vp, dp]
         L.or_ijacvec.argtypes = [vp, dp, dp, C.c_double, dp, dp, dp, dp, dp]
@@ -154,6 +155,13 @@ class Oracle:
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.empty(self.sz)
         lib().or_sptrsv(self.c, _p(b), _p(x))
+        return x
+
+    def sptrsv_jacobi(self, b, sweeps):
+        """x ~ U^-1 L^-1 b by `sweeps` Jacobi sweeps each way over the factors"""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.sz)
+        lib().or_sptrsv_jacobi(self.c, _p(b), _p(x), int(sweeps))
         return x
 
     def spmv(self, x):

@@ -856,6 +856,56 @@ void or_sptrsv(or_ctx *c, const double *b, double *x)
     }
 }
 
+/* fun3d::Matrix::sptrsv_Jacobi (matrix/sptrsv.cpp:75-132) on the in-place
+ * factors: L = the strictly lower blocks, U = the strictly upper ones, D^-1
+ * the inverted diagonal block.  (The reference's routine reads separate L
+ * and U copies; the copy-back branch of its ILU that would fill U,
+ * matrix/ilu.cpp:106-119, is compiled out, so its factors live in A as
+ * here.)  `sweeps` >= 1 passes each way, as its do-while:
+ *   x = b;  repeat: c_i = -1 * (sum_j dgemv(L_ij, x_j)) + b_i;  x = c
+ *   x' = dgemv(D_i^-1, x_i);  repeat: c_i = dgemv(D_i^-1, -1 * (sum_j
+ *   dgemv(U_ij, x'_j)) + x_i);  x' = c
+ * every sum starting from zero and taking the row's blocks in ascending
+ * column, each term added as axpy(1, t, sum) (matrix/init.cpp:96-142). */
+void or_sptrsv_jacobi(or_ctx *c, const double *b, double *x, int sweeps)
+{
+  const or_mat *A = &c->A;
+  const size_t n4 = 4 * c->N;
+  double *xl = (double *)malloc(n4 * sizeof(double)), *cc = (double *)malloc(n4 * sizeof(double));
+  if (!xl || !cc) abort();
+  double t[4], s[4];
+  memcpy(xl, b, n4 * sizeof(double));
+  int sweep = 0;
+  do {
+    for (size_t i = 0; i < c->N; i++) {
+      memset(s, 0, sizeof(s));
+      for (uint32_t j = A->i[i]; j < A->d[i]; j++) {
+        dgemv4(A->a + (size_t)j * 16, xl + 4 * (size_t)A->j[j], t);
+        axpy4(1.0, t, s);
+      }
+      for (int k = 0; k < 4; k++) { const double ax = -1.0 * s[k]; cc[4 * i + k] = ax + b[4 * i + k]; }
+    }
+    memcpy(xl, cc, n4 * sizeof(double));
+  } while (++sweep < sweeps);
+  for (size_t i = 0; i < c->N; i++) dgemv4(A->a + (size_t)A->d[i] * 16, xl + 4 * i, x + 4 * i);
+  sweep = 0;
+  do {
+    for (size_t i = 0; i < c->N; i++) {
+      memset(s, 0, sizeof(s));
+      for (uint32_t j = A->d[i] + 1; j < A->i[i + 1]; j++) {
+        dgemv4(A->a + (size_t)j * 16, x + 4 * (size_t)A->j[j], t);
+        axpy4(1.0, t, s);
+      }
+      double buf[4];
+      for (int k = 0; k < 4; k++) { const double ax = -1.0 * s[k]; buf[k] = ax + xl[4 * i + k]; }
+      dgemv4(A->a + (size_t)A->d[i] * 16, buf, cc + 4 * i);
+    }
+    memcpy(x, cc, n4 * sizeof(double));
+  } while (++sweep < sweeps);
+  free(xl);
+  free(cc);
+}
+
 /* y = A x with the blocks as they stand (the fill's output before or_ilu).
  * No counterpart in the reference (its Krylov operator is matrix-free); the
  * order is the device product's: per row, the blocks in ascending column

@@ -74,6 +74,7 @@ void or_timestep(or_ctx *c, double cfl);
 void or_fill(or_ctx *c);
 void or_ilu(or_ctx *c);
 void or_sptrsv(or_ctx *c, const double *b, double *x);
+void or_sptrsv_jacobi(or_ctx *c, const double *b, double *x, int sweeps);
 /* y = A x with the current blocks (after or_fill, before or_ilu) */
 void or_spmv(const or_ctx *c, const double *x, double *y);
 void or_forces(or_ctx *c, double *forces);

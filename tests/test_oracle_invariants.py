@@ -207,3 +207,75 @@ def test_wall_forces_vanish_at_unit_pressure(small):
     orc.set_state(q)
     assert np.abs(orc.forces()).max() > 0
     orc.close()
+
+
+def _factor_parts(orc, N):
+    """dense strictly-lower L, strictly-upper U and block-diagonal D^-1 of the
+    in-place factors (the diagonal blocks hold D^-1 already)"""
+    rp, col = orc.pattern()
+    F = orc.blocks()
+    L = np.zeros((4 * N, 4 * N))
+    U = np.zeros((4 * N, 4 * N))
+    Dinv = np.zeros((4 * N, 4 * N))
+    for i in range(N):
+        for j in range(rp[i], rp[i + 1]):
+            c = col[j]
+            dst = L if c < i else (Dinv if c == i else U)
+            dst[4 * i:4 * i + 4, 4 * c:4 * c + 4] = F[j]
+    return L, U, Dinv
+
+
+@pytest.fixture(scope="module")
+def factored(small):
+    orc = O.Oracle(small)
+    q = freestream(small.N) + 0.05 * np.random.default_rng(5).standard_normal(small.sz)
+    orc.set_state(q)
+    orc.timestep(20.0)
+    orc.fill()
+    orc.ilu()
+    yield orc, _factor_parts(orc, small.N)
+    orc.close()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_jacobi_sweeps_closed_form(small, factored, sweeps):
+    """k Jacobi sweeps each way (matrix/sptrsv.cpp:75-132) written with dense
+    matrices: y_s = b - L y_{s-1} from y_0 = b; x_0 = D^-1 y_k;
+    x_s = D^-1 (y_k - U x_{s-1})."""
+    orc, (L, U, Dinv) = factored
+    b = np.random.default_rng(6).standard_normal(small.sz)
+    y = b.copy()
+    for _ in range(sweeps):
+        y = b - L @ y
+    x = Dinv @ y
+    for _ in range(sweeps):
+        x = Dinv @ (y - U @ x)
+    got = orc.sptrsv_jacobi(b, sweeps)
+    assert np.abs(got - x).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_jacobi_sweeps_reach_the_exact_solve(small, factored):
+    """Sweep s fixes every row whose dependency chain is at most s long, so
+    once s reaches the depth of both triangular DAGs the sweeps give the
+    level-scheduled solve (up to the order of the sums); one sweep short of
+    that they do not."""
+    orc, (L, U, Dinv) = factored
+    N = small.N
+    rp, col = orc.pattern()
+    depth_l = np.zeros(N, int)
+    for i in range(N):
+        lo = [depth_l[c] for c in col[rp[i]:rp[i + 1]] if c < i]
+        depth_l[i] = 1 + max(lo, default=0)
+    depth_u = np.zeros(N, int)
+    for i in range(N - 1, -1, -1):
+        hi = [depth_u[c] for c in col[rp[i]:rp[i + 1]] if c > i]
+        depth_u[i] = 1 + max(hi, default=0)
+    k = int(max(depth_l.max(), depth_u.max()))
+    b = np.random.default_rng(7).standard_normal(small.sz)
+    exact = orc.sptrsv(b)
+    scale = np.abs(exact).max()
+    got = orc.sptrsv_jacobi(b, k)
+    assert np.abs(got - exact).max() <= 1e-14 * scale
+    # the error falls with every pair of sweeps and is gone only at the depth
+    errs = [np.abs(orc.sptrsv_jacobi(b, s) - exact).max() / scale for s in (1, 3, k - 2)]
+    assert errs[0] > 1e-3 and errs[0] > errs[1] > errs[2] > 1e-12, errs
