@@ -26,8 +26,9 @@ the GPU's reference-profile solve of the same mesh.
 
 Also reported: the HBM footprint (hipMemGetInfo deltas), host setup time,
 the BSR product's HBM rate (roofline_spmv), the 144x48x48 lattice's solve in
-five modes (default, index-order reductions -- the cost of exact parity --,
-CGS2, flexible GMRES, the first-order BSR operator), and
+four modes (default, CGS2, flexible GMRES, the first-order BSR operator),
+the cost of exact parity (exact_parity_cost: the first pseudo-time steps of
+that lattice with index-order reductions against the default tree), and
 learned_preconditioner -- configuration 4's recipe on the
 144x48x48 lattice (seed 1): flexible GMRES with ILU(0) alone against ILU(0)
 plus the learned correction trained online on pairs from the first 10
@@ -36,7 +37,7 @@ sides each, solved to 1e-10), 2000 Adam steps at learning rate 1e-3.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c3|c2]
                     [--no-cpu] [--no-mixed] [--no-learned] [--no-seq]
-                    [--learn-rhs R] [--learn-iters T] [--learn-config c2|c5]
+                    [--jacobi K,..] [--jacobi-fp32] [--learn-rhs R] [--learn-iters T] [--learn-config c2|c5]
                     [--kernel solve|step|solve1|copy]
 """
 import argparse
@@ -317,18 +318,16 @@ def cpu_baseline(K, tmp):
             "gpu_krylov_total": sum(x["krylov"] for x in h), "cpu_forces": f, "gpu_forces": h[-1]["forces"]}
 
 
-C2_VARIANTS = [("default", {}), ("reduce_seq", {"reduce_seq": 1}), ("cgs2", {"ortho": "cgs2"}),
-               ("fgmres", {"krylov": "fgmres"}), ("bsr_operator", {"bsr_operator": 1})]
+C2_VARIANTS = [("default", {}), ("cgs2", {"ortho": "cgs2"}), ("fgmres", {"krylov": "fgmres"}),
+               ("bsr_operator", {"bsr_operator": 1})]
 
 
 def c2_variants(K, mesh_c2):
     """The configuration-profile solve of the 144x48x48 lattice (FP64) in the
-    default mode and with one option changed each: index-order reductions
-    (reduce_seq, the mode whose history equals the reference's bit for bit:
-    the cost of exact parity), batched classical Gram-Schmidt with
-    reorthogonalisation, flexible GMRES (the learned correction's frame) and
-    the assembled first-order Jacobian as the operator.  One warm-up and one
-    timed solve each, on one context."""
+    default mode and with one option changed each: batched classical
+    Gram-Schmidt with reorthogonalisation, flexible GMRES (the learned
+    correction's frame) and the assembled first-order Jacobian as the
+    operator.  One warm-up and one timed solve each, on one context."""
     import numpy as np
     m = K.Mesh(mesh_c2, K.config_options())
     q0 = m.state()
@@ -346,12 +345,57 @@ def c2_variants(K, mesh_c2):
                      "force_rel_err_vs_default": float((np.abs(f - f_ref) / np.abs(f_ref)).max())}
         log("144x48x48 %s: %.3f s, %d Krylov, %d steps" % (name, t, out[name]["krylov_total"], len(h)))
     m.close()
-    out["exact_parity_cost"] = out["reduce_seq"]["seconds"] / out["default"]["seconds"]
     out["workload"] = "144x48x48 lattice seed 1, FP64 preconditioner, configuration profile, one solve to a 1e-10 drop"
     return out
 
 
-def learned(K, mesh, rhs_per_step, train_iters, prec="fp64", verbose=0):
+def exact_parity_cost(K, mesh_c2, ts=2):
+    """The cost of bit-exact parity: the first `ts` pseudo-time steps of the
+    144x48x48 lattice's configuration-profile solve with the default
+    fixed-order tree reductions, and with index-order reductions (reduce_seq:
+    every dot and norm a sequential sum over the 1.3M entries in one wave, as
+    the reference's one-thread loops add them; the mode whose whole history
+    equals the reference's bit for bit).  A whole solve in the index-order
+    mode takes about a minute on this lattice, so the comparison runs on its
+    first steps.  One warm-up and one timed run each."""
+    m = K.Mesh(mesh_c2, K.config_options(ts_max=ts))
+    q0 = m.state()
+    out = {}
+    for name, kw in (("tree", {}), ("index_order", {"reduce_seq": 1})):
+        m.set_options(K.config_options(ts_max=ts, **kw))
+        timed_solve(m, q0)
+        t, h = timed_solve(m, q0)
+        assert m.last_error() == 0, name
+        out[name] = {"seconds": t, "krylov_total": sum(x["krylov"] for x in h), "pseudo_steps": len(h),
+                     "fnorm": [x["fnorm"] for x in h]}
+        log("exact-parity cost, first %d pseudo-steps of 144x48x48, %s reductions: %.3f s, %d Krylov"
+            % (ts, name, t, out[name]["krylov_total"]))
+    m.close()
+    out["ratio"] = out["index_order"]["seconds"] / out["tree"]["seconds"]
+    out["workload"] = ("144x48x48 lattice seed 1, FP64, configuration profile, the first %d pseudo-time steps "
+                       "(same Krylov counts in both modes)" % ts)
+    return out
+
+
+class stdout_to_stderr:
+    """file descriptor 1 pointed at descriptor 2 for a block (the library's
+    printf lines land on stderr)"""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        C.CDLL(None).fflush(None)
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        C.CDLL(None).fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
+def learned(K, mesh, rhs_per_step, train_iters, prec="fp64"):
     """The learned correction on one mesh: flexible GMRES with ILU(0) alone
     (the method without the network), then with the network trained online
     -- pairs from the first 10 pseudo-time steps (the Newton residual and
@@ -368,16 +412,18 @@ def learned(K, mesh, rhs_per_step, train_iters, prec="fp64", verbose=0):
     log("learned leg: ILU(0)-FGMRES %.3f s; collecting pairs (%d per step), training (%d Adam steps), solving"
         % (t_ilu, rhs_per_step + 1, train_iters))
     m.set_state(q0)
-    if verbose:
-        m.set_options(K.config_options(krylov="fgmres", prec=prec, verbose=1))
+    # the solver's per-step lines (and its collection and training lines) are
+    # the leg's progress: printed, but on stderr (stdout carries the JSON line)
+    m.set_options(K.config_options(krylov="fgmres", prec=prec, verbose=1))
     net = K.Network(m, K.init_params(seed=0), nslot=11)
     m.attach(net, collect_steps=10, rhs_per_step=rhs_per_step, pair_rtol=1e-10, pair_maxit=300,
              train_iters=train_iters, lr=1e-3, judge_every=5, holdout=0.1, seed=3)
     m.synchronize()
-    t0 = time.perf_counter()
-    hl = m.kernel()
-    m.synchronize()
-    t_l = time.perf_counter() - t0
+    with stdout_to_stderr():
+        t0 = time.perf_counter()
+        hl = m.kernel()
+        m.synchronize()
+        t_l = time.perf_counter() - t0
     st = m.learn_stats()
     log("learned leg: %.3f s" % t_l)
     assert m.last_error() == 0, "device error word set (learned)"
@@ -457,21 +503,20 @@ def main():
                     help="the path `value` times: fp64, fp32, bf16, or one of those with _jacobiK (K Jacobi "
                          "sweeps each way instead of the exact triangular solves); auto = the fastest "
                          "single-precision path of the warm-up solves whose forces are within 1e-8 of FP64")
-    ap.add_argument("--jacobi", default="2,3",
-                    help="Jacobi sweep counts of the extra single-precision paths (comma list; empty: none)")
+    ap.add_argument("--jacobi", default="3",
+                    help="Jacobi sweep counts of the extra BF16-factor paths (comma list; empty: none)")
+    ap.add_argument("--jacobi-fp32", action="store_true", help="also FP32-factor paths with the Jacobi sweeps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
     ap.add_argument("--no-seq", action="store_true",
-                    help="skip the 144x48x48 variants (exact-parity cost, CGS2, FGMRES, BSR operator)")
+                    help="skip the 144x48x48 legs (CGS2, FGMRES, BSR operator; the exact-parity cost)")
     ap.add_argument("--learn-rhs", type=int, default=64, help="Gaussian right-hand sides per collecting step")
     ap.add_argument("--learn-iters", type=int, default=2000, help="Adam steps")
     ap.add_argument("--learn-config", default="c2", choices=["c2", "c5"],
                     help="mesh of the learned leg: c2 (configuration 4) or c5 (the combined form)")
     ap.add_argument("--learn-prec", default="fp64", choices=["fp64", "fp32", "bf16"],
                     help="preconditioner precision of the learned leg (fp32: the combined form)")
-    ap.add_argument("--progress", action="store_true",
-                    help="also the solver's per-step lines in the learned leg (on stdout, before the JSON line)")
     ap.add_argument("--kernel", default="solve", choices=["solve", "step", "solve1", "copy"],
                     help="solve: the benchmark; step / solve1 / copy: short runs for profiles (first pseudo-time "
                          "steps, or whole solves, of one precision on the same mesh; a device-to-device copy of "
@@ -513,7 +558,7 @@ def main():
     if not args.no_mixed:
         precs = ["fp32", "bf16"]
         for k in [int(x) for x in args.jacobi.split(",") if x.strip()]:
-            precs += ["fp32_jacobi%d" % k, "bf16_jacobi%d" % k]
+            precs += (["fp32_jacobi%d" % k] if args.jacobi_fp32 else []) + ["bf16_jacobi%d" % k]
     warm, warm_err = {"fp64": t64w}, {"fp64": 0.0}
     for p in precs:
         m.set_options(path_options(K, p))
@@ -603,12 +648,13 @@ def main():
     if not args.no_seq or not args.no_learned:
         mesh_c2 = lattice_mesh(K, tmp, "c2")
     out["variants_144x48x48"] = None if args.no_seq else c2_variants(K, mesh_c2)
+    out["exact_parity_cost"] = None if args.no_seq else exact_parity_cost(K, mesh_c2)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(K, tmp)
     if args.no_learned:
         out["learned_preconditioner"] = None
     else:
         lmesh = mesh_c2 if args.learn_config == "c2" else mesh
-        lr = learned(K, lmesh, args.learn_rhs, args.learn_iters, args.learn_prec, 1 if args.progress else 0)
+        lr = learned(K, lmesh, args.learn_rhs, args.learn_iters, args.learn_prec)
         ldims, lseed = CONFIGS[args.learn_config]
         lr["mesh"] = "%dx%dx%d lattice seed %d" % (ldims + (lseed,))
         if args.learn_config == "c2" and out["variants_144x48x48"]:

@@ -331,6 +331,11 @@ static void collect_pairs(kspace *K, int its)
   }
   kf_synchronize(g);
   L->st.pair_ms += now_ms() - t0;
+  if (K->o.verbose) {
+    printf("LEARN: step %d: %d pairs so far, %ld tight-solve iterations so far, %.1f s collecting\n", its,
+           L->st.npairs, L->st.pair_iters, L->st.pair_ms / 1e3);
+    fflush(stdout);
+  }
 }
 
 static uint64_t mix64(uint64_t *s)
@@ -399,6 +404,12 @@ static void train(kspace *K)
   free(perm);
   L->st.loss_after = heldout_loss(L, sz);
   L->st.train_ms += now_ms() - t0;
+  if (K->o.verbose) {
+    printf("LEARN: %d Adam steps on %d pairs (%d held out) in %.1f s; held-out loss: ILU(0) %g, start %g, "
+           "trained %g\n", L->lo.train_iters, ntrain, nheld, L->st.train_ms / 1e3, L->st.loss_ilu,
+           L->st.loss_before, L->st.loss_after);
+    fflush(stdout);
+  }
 }
 
 /* ---- the driver ------------------------------------------------------------------ */
