@@ -261,6 +261,28 @@ size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, int cluster, 
 size_t kf_ilu_plan_host(const GEOMETRY *g, unsigned *rec, unsigned *ldiag, unsigned *moff, unsigned *mjm,
                         unsigned short *mlo, size_t cap);
 
+/* ---- triangular solve that allocates nothing (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same as
+ * ComputeSparseTriangularSolve with Jacobi sweeps (kf_options.jacobi_sweeps
+ * >= 1): the same kernels and the same bits.  Its three work vectors (4N
+ * values each, in the preconditioner's precision) live in one fixed device
+ * array of KF_NOALLOC_BYTES per GPU, held until the process ends, instead
+ * of context memory allocated at the first call.  So N <= 174,762 vertices
+ * with FP64 factors and N <= 349,525 with FP32 or BF16 factors.  The call
+ * makes no hipMalloc, hipFree or other allocating call; the caller passes b
+ * and x (device, 4N doubles, 32-byte aligned, not aliased) and has run
+ * ComputeA and ComputeNumericalILU in the context's precision.
+ * Returns 0, KF_NOALLOC_ESIZE (the mesh is too large for the array; nothing
+ * launched) or KF_NOALLOC_EMODE (jacobi_sweeps is 0, no factors yet, or b
+ * aliases x).  All calls on one GPU share the array, which keeps the last
+ * call's contents: calls that overlap on one GPU (contexts on different
+ * streams, or several host threads) race, so the caller orders them. */
+#define KF_NOALLOC_BYTES (16u << 20)
+#define KF_NOALLOC_ESIZE 1
+#define KF_NOALLOC_EMODE 2
+int kf_sptrsv_jacobi_noalloc(const GEOMETRY *g, const double *b, double *x);
+
 /* Releases everything Setup allocated, host and device. */
 void kf_geometry_free(GEOMETRY *g);
 

@@ -153,6 +153,7 @@ def _load():
         "ComputeForces": (None, [GP, C.POINTER(C.c_double)]),
         "ComputeNumericalILU": (None, [GP]),
         "ComputeSparseTriangularSolve": (None, [GP, DP, DP]),
+        "kf_sptrsv_jacobi_noalloc": (C.c_int, [GP, DP, DP]),
         "iJacVec": (None, [GP, DP, DP, C.c_double, C.POINTER(Xyz), DP, DP]),
         "Kernel": (None, [GP]),
         "kfs_flux_spectral": (None, [GP, DP, C.POINTER(Xyz), DP]),
@@ -207,7 +208,7 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "ComputeNewAXPY", "Normalize", "Setup", "kf_default_options", "kf_set_options", "kf_get_options",
                "kf_kernel", "kf_set_state", "kf_get_state", "kf_memcpy_d2h", "kf_memcpy_h2d", "kf_device_alloc",
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_trsv_fold", "kf_bsr_spmv", "kf_pattern", "kf_get_blocks",
-               "kf_write_lattice_msh", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
+               "kf_write_lattice_msh", "kf_sptrsv_jacobi_noalloc", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
                "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
                "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_init_params", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
                "kf_mpnn_get_params", "kf_mpnn_take_diag", "kf_mpnn_set_diag", "kf_mpnn_forward",
@@ -381,6 +382,12 @@ class Mesh:
 
     def sptrsv(self, b_dev, x_dev):
         lib().ComputeSparseTriangularSolve(self.ref, b_dev.ptr, x_dev.ptr)
+
+    def sptrsv_noalloc(self, b_dev, x_dev):
+        """ComputeSparseTriangularSolve with Jacobi sweeps, work vectors in the
+        library's fixed device array (no allocation); returns 0 or
+        KF_NOALLOC_ESIZE (1) / KF_NOALLOC_EMODE (2)"""
+        return int(lib().kf_sptrsv_jacobi_noalloc(self.ref, b_dev.ptr, x_dev.ptr))
 
     def spmv(self, x_dev, y_dev):
         """y = A x with the Jacobian the last fill kept (options bsr_operator=1)"""

@@ -223,6 +223,8 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   KF_CHECK(hipMemcpy(c->q, qh, 4 * N * sizeof(double), hipMemcpyHostToDevice));
   free(qh);
 
+  kf_noalloc_setup(c);   /* the device address of the no-allocation solve's fixed array */
+
   g->q->q = c->q;
   g->n->cdt = c->cdt;
   g->matrix = c;

@@ -28,6 +28,8 @@
  * reference's separate dgemv pass).  Workgroups are remapped XCD by XCD.
  * Bytes per sweep: (half the factor blocks) (16 s + 4) + vector traffic.
  */
+#include <mutex>
+
 #include "kf_dev.h"
 
 #define JAC_WG 256   /* 64 rows per workgroup */
@@ -236,6 +238,12 @@ void kf_launch_jacobi(kf_ctx *c, const double *b, double *x)
     abort();
   }
   const size_t n = 4 * (size_t)c->d.N;
+  const void *factors = c->opt.prec == KF_PREC_FP64 ? (const void *)c->a64
+                        : c->opt.prec == KF_PREC_FP32 ? (const void *)c->a32 : (const void *)c->abf;
+  if (!factors) {
+    fprintf(stderr, "kf: Jacobi triangular sweeps: no ILU(0) factors in the context's precision yet\n");
+    abort();
+  }
   if (c->opt.prec == KF_PREC_FP64) {
     if (!c->ty64) KF_CHECK(hipMalloc((void **)&c->ty64, n * sizeof(double)));
     if (!c->jac64) KF_CHECK(hipMalloc((void **)&c->jac64, 2 * n * sizeof(double)));
@@ -250,4 +258,95 @@ void kf_launch_jacobi(kf_ctx *c, const double *b, double *x)
   else
     jacobi_solve<uint16_t, float>(c, c->abf, b, x, c->f32a, c->jac32, c->jac32 + n, k);
   c->ystate[1] = ~(uint64_t)0;
+}
+
+/* ---- the same solve with no allocation: kf_sptrsv_jacobi_noalloc ----------
+ *
+ * The three work vectors (y and the two ping-pong buffers) live in one fixed
+ * device array, `scratch`, instead of context memory allocated at the first
+ * call, so the entry point makes no hipMalloc, hipFree or other allocating
+ * call and can run where the caller cannot allocate (HIP graph capture,
+ * serving loops).  Each GPU the library runs on holds its own copy of the
+ * array (KF_NOALLOC_BYTES) until the process ends.  Its device address is
+ * looked up once per device, at context creation with the context's device
+ * current (kf_noalloc_setup); host code uses that address, never the array's
+ * name.  Every call on one GPU shares the array: calls that overlap on one
+ * GPU (contexts on different streams, or different host threads) race, and
+ * the caller orders them.  Nothing in the array needs to start at zero:
+ * every sweep writes all rows before the next sweep reads them. */
+static __device__ __align__(256) unsigned char scratch[KF_NOALLOC_BYTES];
+
+#define KF_NOALLOC_DEVICES 64
+static std::mutex noalloc_mu;
+static void *noalloc_ptr[KF_NOALLOC_DEVICES];   /* per device: the address of its scratch */
+
+void kf_noalloc_setup(kf_ctx *c)
+{
+  int dev = 0;
+  KF_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= KF_NOALLOC_DEVICES) {
+    c->noalloc = NULL;   /* kf_sptrsv_jacobi_noalloc then reports KF_NOALLOC_EMODE */
+    return;
+  }
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void *p = NULL;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      /* only the no-allocation solve depends on it: it then reports
+       * KF_NOALLOC_EMODE; the error is cleared so later launch checks do
+       * not see it */
+      (void)hipGetLastError();
+      fprintf(stderr, "kf: hipGetSymbolAddress of the fixed work array failed on device %d (%s); "
+                      "kf_sptrsv_jacobi_noalloc is unavailable there\n", dev, hipGetErrorString(e));
+      c->noalloc = NULL;
+      return;
+    }
+    noalloc_ptr[dev] = p;
+  }
+  c->noalloc = noalloc_ptr[dev];
+}
+
+extern "C" int kf_sptrsv_jacobi_noalloc(const GEOMETRY *g, const double *b, double *x)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (b == x) {
+    fprintf(stderr, "kf: kf_sptrsv_jacobi_noalloc: b and x must not alias\n");
+    return KF_NOALLOC_EMODE;
+  }
+  kf_check_aligned(b, "b");
+  kf_check_aligned(x, "x");
+  const int k = c->opt.jacobi_sweeps;
+  if (k < 1 || k > KF_JACOBI_MAX) {
+    fprintf(stderr, "kf: kf_sptrsv_jacobi_noalloc: the context's jacobi_sweeps is %d (needs 1..%d)\n", k,
+            KF_JACOBI_MAX);
+    return KF_NOALLOC_EMODE;
+  }
+  const size_t n = 4 * (size_t)c->d.N;
+  const size_t vb = c->opt.prec == KF_PREC_FP64 ? sizeof(double) : sizeof(float);
+  if (3 * n * vb > (size_t)KF_NOALLOC_BYTES) {
+    fprintf(stderr,
+            "kf: kf_sptrsv_jacobi_noalloc: %u vertices need %zu bytes of work space in this precision, more than "
+            "the fixed array's %u (at most %zu vertices)\n",
+            c->d.N, 3 * n * vb, (unsigned)KF_NOALLOC_BYTES, (size_t)KF_NOALLOC_BYTES / (12 * vb));
+    return KF_NOALLOC_ESIZE;
+  }
+  const void *factors = c->opt.prec == KF_PREC_FP64 ? (const void *)c->a64
+                        : c->opt.prec == KF_PREC_FP32 ? (const void *)c->a32 : (const void *)c->abf;
+  if (!factors || !c->noalloc) {
+    fprintf(stderr, "kf: kf_sptrsv_jacobi_noalloc: no ILU(0) factors in the context's precision yet (call "
+                    "ComputeA and ComputeNumericalILU first)\n");
+    return KF_NOALLOC_EMODE;
+  }
+  if (c->opt.prec == KF_PREC_FP64) {
+    double *y = static_cast<double *>(c->noalloc);
+    jacobi_solve<double, double>(c, c->a64, b, x, y, y + n, y + 2 * n, k);
+  } else {
+    float *y = static_cast<float *>(c->noalloc);
+    if (c->opt.prec == KF_PREC_FP32)
+      jacobi_solve<float, float>(c, c->a32, b, x, y, y + n, y + 2 * n, k);
+    else
+      jacobi_solve<uint16_t, float>(c, c->abf, b, x, y, y + n, y + 2 * n, k);
+  }
+  return 0;
 }

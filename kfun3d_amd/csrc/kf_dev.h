@@ -165,6 +165,7 @@ struct kf_ctx {
   hipEvent_t ev_upd, ev_forces;
   double *ftbuf;            /* per-face force terms */
   int forces_pending;
+  void *noalloc;            /* this device's address of the fixed work array of kf_sptrsv_jacobi_noalloc */
   void *learn;              /* learned-preconditioner state of the driver (solver.c) */
 };
 
@@ -227,6 +228,9 @@ void kf_keep_jacobian(kf_ctx *c);
 bool kf_trsv_capped(const kf_ctx *c);
 /* x ~ U^-1 L^-1 b by opt.jacobi_sweeps Jacobi sweeps each way (jacobi.hip) */
 void kf_launch_jacobi(kf_ctx *c, const double *b, double *x);
+/* at context creation, with the context's device current: the device
+ * address of kf_sptrsv_jacobi_noalloc's fixed work array (jacobi.hip) */
+void kf_noalloc_setup(kf_ctx *c);
 void kf_check_options(const kf_options *o);
 
 /* BLAS-1 on the context stream; reductions are deterministic (fixed
