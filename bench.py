@@ -38,7 +38,7 @@ sides each, solved to 1e-10), 2000 Adam steps at learning rate 1e-3.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c3|c2]
                     [--no-cpu] [--no-mixed] [--no-learned] [--no-seq]
                     [--jacobi K,..] [--jacobi-fp32] [--learn-rhs R] [--learn-iters T] [--learn-config c2|c5]
-                    [--kernel solve|step|solve1|copy]
+                    [--kernel solve|step|solve1|copy|traffic]
 """
 import argparse
 import ctypes as C
@@ -59,7 +59,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_GFLOPS = 78600.0
 # the kernel sources a traffic profile is valid for (hashed into the profile)
 KERNEL_SOURCES = ["kfun3d_amd/csrc/" + f for f in ("flux.hip", "fill.hip", "ilu.hip", "sptrsv.hip", "blas.hip",
-                                                   "handoff.h", "sync.h", "kf_dev.h")]
+                                                   "spmv.hip", "jacobi.hip", "handoff.h", "sync.h", "kf_dev.h")]
 
 
 def alg_bytes(kind, N, E, S, Nf, nnzb, s):
@@ -472,6 +472,103 @@ def copy_probe(reps):
     print(json.dumps({"probe": "copy", "bytes_per_copy": COPY_BYTES, "copies": max(reps, 1)}))
 
 
+def _kernel_key(name):
+    """rocprofv3's kernel name without the return type and the argument list"""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def _pmc_means(root, counter):
+    """per kernel: mean value of one counter per dispatch over every
+    *counter_collection.csv under root, and the dispatch count"""
+    import csv
+    import glob
+    acc = {}
+    files = glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % root)
+    for f in files:
+        for row in csv.DictReader(open(f)):
+            if row.get("Counter_Name") != counter:
+                continue
+            k = _kernel_key(row["Kernel_Name"])
+            a = acc.setdefault(k, {})
+            d = row.get("Dispatch_Id") or row.get("Correlation_Id")
+            a[d] = a.get(d, 0.0) + float(row["Counter_Value"])   # summed over the counter's instances
+    return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
+
+
+def traffic_profile(K, config, root, precs):
+    """profiles/traffic_<config>.json from rocprofv3 --pmc passes: under root,
+    fetch_<prec>/ and write_<prec>/ hold FETCH_SIZE and WRITE_SIZE runs of
+    `--kernel step --prec <prec>`, fetch_copy/ and write_copy/ those of
+    `--kernel copy`.  Each counter is scaled by its reading for the copy
+    kernel of COPY_BYTES read and written.  Prints the summary table."""
+    import numpy as np
+    cf, cw = _pmc_means(os.path.join(root, "fetch_copy"), "FETCH_SIZE"), _pmc_means(os.path.join(root, "write_copy"),
+                                                                                   "WRITE_SIZE")
+    copy_key = [k for k in cf if "copyBuffer" in k or "copy" in k.lower()]
+    if not copy_key:
+        raise SystemExit("no copy kernel in the calibration passes: %s" % sorted(cf))
+    ck = copy_key[0]
+    fscale = COPY_BYTES / (cf[ck][0] * 1024.0)
+    wscale = COPY_BYTES / (cw[ck][0] * 1024.0)
+    tmp = tempfile.mkdtemp(prefix="kftraffic_")
+    m = K.Mesh(lattice_mesh(K, tmp, config), device=False)
+    h = m.host_arrays()
+    N, E, S, Nf = m.N, m.E, len(h["fc_n0"]), len(h["f_nptr"])
+    m.close()
+    nnzb = N + 2 * E
+    kinds = {"k_grad": "grad", "k_edge_flux": "edge_flux", "k_residual": "residual_gather", "k_fill": "fill",
+             "k_ilu_rows": "ilu", "k_trsv_lower": "trsv_lower", "k_trsv_upper": "trsv_upper",
+             "k_bsr_spmv": "spmv", "k_timestep": "timestep"}
+    out = {"workload": "%dx%dx%d Kuhn lattice seed %d: the first pseudo-time step of the configuration-profile "
+                       "solve (bench.py --kernel step), per preconditioner precision" % (CONFIGS[config][0] +
+                                                                                         (CONFIGS[config][1],)),
+           "fetch_scale": fscale, "write_scale": wscale,
+           "calibration": "%s of %d bytes: FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB per dispatch"
+                          % (ck, COPY_BYTES, cf[ck][0], cw[ck][0]),
+           "source_hash": source_hash(), "kernels": {}}
+    rows = []
+    for prec in precs:
+        f = _pmc_means(os.path.join(root, "fetch_%s" % prec), "FETCH_SIZE")
+        w = _pmc_means(os.path.join(root, "write_%s" % prec), "WRITE_SIZE")
+        s = {"fp64": 8, "fp32": 4, "bf16": 2}[prec]
+        for k in sorted(set(f) & set(w)):
+            base = k.split("<")[0]
+            if base not in kinds:
+                continue
+            kind = kinds[base]
+            ss = s if kind in MATRIX_KINDS else 8
+            if base in ("k_fill", "k_ilu_rows") and prec != "fp64":
+                ss = 4   # the fill and the factorisation run in FP32 for both single-precision modes
+            ab = alg_bytes(kind, N, E, S, Nf, nnzb, ss)
+            b = f[k][0] * 1024.0 * fscale + w[k][0] * 1024.0 * wscale
+            if k in out["kernels"]:
+                continue
+            out["kernels"][k] = {"fetch_kib": f[k][0], "write_kib": w[k][0], "bytes": b, "alg_bytes": ab,
+                                 "dispatches": min(f[k][1], w[k][1])}
+            rows.append("| `%s` | %.1f | %.1f | %.0f | %d | %.2f |" % (k, f[k][0], w[k][0], b, ab, b / ab))
+    json.dump(out, open(os.path.join(ROOT, "profiles", "traffic_%s.json" % config), "w"), indent=1)
+    print("Calibration: %s; fetched bytes = FETCH_SIZE x %.4f x 1024, written bytes = WRITE_SIZE x %.4f x 1024."
+          % (out["calibration"], fscale, wscale))
+    print()
+    print("| kernel | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM bytes per launch | algorithmic bytes | ratio |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for r in rows:
+        print(r)
+
+
 def step_probe(K, mesh, prec, reps, warmup, ts_max):
     """reps solves from the free stream (configuration profile) of ts_max
     pseudo-time steps (1: residual, time step, fill, ILU(0), one GMRES solve,
@@ -517,7 +614,9 @@ def main():
                     help="mesh of the learned leg: c2 (configuration 4) or c5 (the combined form)")
     ap.add_argument("--learn-prec", default="fp64", choices=["fp64", "fp32", "bf16"],
                     help="preconditioner precision of the learned leg (fp32: the combined form)")
-    ap.add_argument("--kernel", default="solve", choices=["solve", "step", "solve1", "copy"],
+    ap.add_argument("--traffic-dir", default="gpurun_out/pmc",
+                    help="--kernel traffic: the directory holding the counter passes")
+    ap.add_argument("--kernel", default="solve", choices=["solve", "step", "solve1", "copy", "traffic"],
                     help="solve: the benchmark; step / solve1 / copy: short runs for profiles (first pseudo-time "
                          "steps, or whole solves, of one precision on the same mesh; a device-to-device copy of "
                          "known size)")
@@ -528,6 +627,9 @@ def main():
         sys.exit(2)
     if args.kernel == "copy":
         return copy_probe(args.steps)
+    if args.kernel == "traffic":
+        import kfun3d_amd as K
+        return traffic_profile(K, args.config, os.path.join(ROOT, args.traffic_dir), ["fp64", "bf16"])
     import numpy as np
     import kfun3d_amd as K
     tmp = tempfile.mkdtemp(prefix="kfbench_")

@@ -71,6 +71,17 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   d.x = dupload(g->n->xyz->x0, N);
   d.y = dupload(g->n->xyz->x1, N);
   d.z = dupload(g->n->xyz->x2, N);
+  {
+    double *h = (double *)kf_xmalloc(4 * N * sizeof(double));
+    for (size_t i = 0; i < N; i++) {
+      h[4 * i] = g->n->xyz->x0[i];
+      h[4 * i + 1] = g->n->xyz->x1[i];
+      h[4 * i + 2] = g->n->xyz->x2[i];
+      h[4 * i + 3] = 0.0;
+    }
+    d.xyzw = dupload(h, 4 * N);
+    free(h);
+  }
 
   /* sliced ELL of the incidence lists (slice height 64), built on the host */
   kf_sell sl;
@@ -250,7 +261,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
     KF_CHECK(hipStreamDestroy(c->fstream));
   }
   kf_dv &d = c->d;
-  void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
+  void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.xyzw, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->rdone,

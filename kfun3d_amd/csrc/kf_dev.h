@@ -60,6 +60,7 @@ struct kf_dv {
   uint32_t *n0, *n1;
   double *enx, *eny, *enz, *ea;
   double *x, *y, *z;
+  double *xyzw;                 /* (x, y, z, 0) per vertex: one 32-byte gather for the edge kernel */
   uint32_t *sell_ptr, *deg;
   uint32_t *sell_nbr, *sell_ent, *sell_blk;
   double *sell_wx, *sell_wy, *sell_wz;

@@ -121,6 +121,21 @@ __device__ __forceinline__ bool wait_vec(const TV *v, uint32_t col, typename sen
   }
 }
 
+/* the four values at p in the wave's LDS area (every operand offset is a
+ * multiple of 4 values from a 16-byte aligned base): 128-bit reads */
+__device__ __forceinline__ void lds_get4(const float *p, float x[4])
+{
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+}
+
+__device__ __forceinline__ void lds_get4(const double *p, double x[4])
+{
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+  const double2 u = q[0], v = q[1];
+  x[0] = u.x; x[1] = u.y; x[2] = v.x; x[3] = v.y;
+}
+
 /* Per-wave LDS: the task's row results by slot, the external operands of
  * the prefetched entries by row slot and entry, and a zero block that
  * entries past the end of a row read. */
@@ -167,23 +182,32 @@ __device__ __forceinline__ bool setup_row(const kf_dv &d, const uint8_t *__restr
                                           uint32_t off[PF], TV mp[PF][4], bool aborted)
 {
   static_assert(PF <= KF_TRSV_PF, "the external area holds KF_TRSV_PF operands per row");
-  uint32_t cp[PF], pend = 0;
+  /* every entry's column, slot and block row are loaded unconditionally --
+   * an entry past the end reads position 0, which always exists, and is
+   * then set aside -- so that all loads of the row are in flight together
+   * (one memory round trip, not one per entry) */
+  uint32_t cp[PF], es[PF], pend = 0;
 #pragma unroll
   for (int k = 0; k < PF; k++) {
-    cp[k] = 0;
-    off[k] = KF_TRSV_ZERO;
-    mp[k][0] = mp[k][1] = mp[k][2] = mp[k][3] = TV(0);
-    if (j0 + k < jend) {   /* an empty backward slot has j0 = 1, jend = 0 */
-      cp[k] = d.col[j0 + k];
-      const uint32_t e = eslot[j0 + k];
-      ld_row(a, j0 + k, c, mp[k]);
-      if (e == 0xffu) {
-        pend |= 1u << k;
-        off[k] = KF_TRSV_EXT + 4 * (KF_TRSV_PF * qd + k);
-      } else {
-        off[k] = 4 * e;
-      }
+    const uint32_t j = j0 + k < jend ? j0 + k : 0u;   /* an empty backward slot has j0 = 1, jend = 0 */
+    cp[k] = d.col[j];
+    es[k] = eslot[j];
+    ld_row(a, j, c, mp[k]);
+  }
+  __builtin_amdgcn_sched_barrier(0);   /* every load issued before the first use */
+#pragma unroll
+  for (int k = 0; k < PF; k++) {
+    const bool in = j0 + k < jend;
+    if (!in) {
+      cp[k] = 0;
+      mp[k][0] = mp[k][1] = mp[k][2] = mp[k][3] = TV(0);
     }
+    /* a column is below N < 2^31, so the second test always holds; it makes
+     * the mask depend on the column, whose load then goes out with the
+     * others instead of a second round trip inside the polls' branch */
+    const bool ext = in && es[k] == 0xffu && cp[k] != 0xffffffffu;
+    pend |= ext ? 1u << k : 0u;
+    off[k] = !in ? (uint32_t)KF_TRSV_ZERO : ext ? (uint32_t)(KF_TRSV_EXT + 4 * (KF_TRSV_PF * qd + k)) : 4 * es[k];
   }
   TV xo[PF];
   bool ok = true;
@@ -210,10 +234,7 @@ __device__ __forceinline__ TV row_sweep(const kf_dv &d, const uint8_t *__restric
 {
   TV xin[PF][4], pr[PF];
 #pragma unroll
-  for (int k = 0; k < PF; k++) {
-    const TV *xc = lds + off[k];
-    xin[k][0] = xc[0]; xin[k][1] = xc[1]; xin[k][2] = xc[2]; xin[k][3] = xc[3];
-  }
+  for (int k = 0; k < PF; k++) lds_get4(lds + off[k], xin[k]);
   __builtin_amdgcn_sched_barrier(0);   /* every LDS read issued before the first product */
   /* the 8 products row_dot(mp[k], xin[k]) stage by stage, so that each
    * stage's 8 independent operations hide one another's latency; every
@@ -258,12 +279,15 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
                                                    typename sentinel<TV>::bits sx, const TM *__restrict__ a,
                                                    const double *__restrict__ b, TV *y, TV *xnext, kf_fdb fd)
 {
-  __shared__ TV lds_all[KF_TRSV_GMAX][KF_TRSV_LDS];
+  __shared__ __align__(16) TV lds_all[KF_TRSV_GMAX][KF_TRSV_LDS];
   __shared__ uint32_t sbatch;
   const uint32_t G = blockDim.x >> 6, wv = threadIdx.x >> 6;
   TV *lds = lds_all[wv];
   const uint32_t lane = threadIdx.x & 63u, qd = lane >> 2, c = lane & 3;
   bool aborted = false;
+  /* |x| / h of the finite-difference product: the same for every row (its
+   * norm was formed before this launch) */
+  const double am = FD ? *fd.xnorm / fd.h : 0.0;
   if (lane < 4) lds[KF_TRSV_ZERO + lane] = TV(0);
   if (threadIdx.x == 0) sbatch = atomicAdd(ticket, G) - base;
   __syncthreads();
@@ -278,18 +302,19 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
       const uint32_t r = rt[0], j0 = rt[1], jd = rt[2];
       const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
       const bool active = r != ~0u;
-      TV xr = TV(0);
-      if (active) {
-        const size_t o = 4 * (size_t)r + c;
-        if (FD) {
-          /* b is R(w): the finite-difference product first, as
-           * src/kernel.c:33-44 forms it, ((R(w) - r) + cdt (w - q)) |x| / h */
-          const double am = *fd.xnorm / fd.h, cd = fd.cdt[r];
-          xr = (TV)((b[o] - fd.r[o] + cd * (fd.w[o] - fd.q0[o])) * am);
-        } else {
-          xr = (TV)b[o];
-        }
+      /* the right-hand side, loaded for every slot (an empty one reads row
+       * 0 and keeps 0) beside the row's entries below */
+      const size_t o = 4 * (size_t)(active ? r : 0u) + c;
+      TV xr;
+      if (FD) {
+        /* b is R(w): the finite-difference product first, as
+         * src/kernel.c:33-44 forms it, ((R(w) - r) + cdt (w - q)) |x| / h */
+        const double cd = fd.cdt[active ? r : 0u];
+        xr = (TV)((b[o] - fd.r[o] + cd * (fd.w[o] - fd.q0[o])) * am);
+      } else {
+        xr = (TV)b[o];
       }
+      xr = active ? xr : TV(0);
       if (active) xnext[4 * (size_t)r + c] = sentinel<TV>::val(sx);
       uint32_t off[PF];
       TV mp[PF][4];
@@ -299,7 +324,7 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_lower(kf_dv d, const
       for (uint32_t l = 0; l < dep; l++) {
         if (my == l) {   /* the quad's four lanes take this branch together */
           xr = row_sweep<PF, TM, TV>(d, eslot, a, y, sy, j0, jd, c, off, mp, lds, xr, aborted);
-          xr = __builtin_canonicalize(xr);
+          xr = canon(xr);
           lds[lane] = xr;
         }
         wave_lds_sync();
@@ -323,7 +348,7 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
                                                    typename sentinel<TV>::bits synext, const TM *__restrict__ a,
                                                    TV *y, TV *x, double *__restrict__ xout)
 {
-  __shared__ TV lds_all[KF_TRSV_GMAX][KF_TRSV_LDS];
+  __shared__ __align__(16) TV lds_all[KF_TRSV_GMAX][KF_TRSV_LDS];
   __shared__ uint32_t sbatch;
   const uint32_t G = blockDim.x >> 6, wv = threadIdx.x >> 6;
   TV *lds = lds_all[wv];
@@ -344,13 +369,13 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
       const uint32_t my = tlvl[(size_t)t * KF_GROUP + qd], dep = tdep[t];
       const bool active = rr != ~0u;
       const uint32_t r = active ? rr : 0;
+      /* the diagonal inverse and y, loaded for every slot (an empty one
+       * reads row 0 and discards it) beside the row's entries below */
       TV dg[4];
-      if (active) ld_row(a, jd, c, dg);
-      TV x0 = TV(0);
-      if (active) {
-        x0 = y[4 * (size_t)r + c];
-        y[4 * (size_t)r + c] = sentinel<TV>::val(synext);
-      }
+      ld_row(a, active ? jd : 0u, c, dg);
+      TV x0 = y[4 * (size_t)r + c];
+      x0 = active ? x0 : TV(0);
+      if (active) y[4 * (size_t)r + c] = sentinel<TV>::val(synext);
       uint32_t off[PF];
       TV mp[PF][4];
       if (!setup_row<PF, TM, TV>(d, eslot, a, x, sx, j0, j1, qd, c, lds, off, mp, aborted)) aborted = true;
@@ -362,7 +387,7 @@ __global__ __launch_bounds__(64 * KF_TRSV_GMAX) void k_trsv_upper(kf_dv d, const
           const TV xr = row_sweep<PF, TM, TV>(d, eslot, a, x, sx, j0, j1, c, off, mp, lds, x0, aborted);
           /* x_r = inv(D_r) x_r: the quad exchanges its four components */
           const TV y0 = quad_get<0>(xr), y1 = quad_get<1>(xr), y2 = quad_get<2>(xr), y3 = quad_get<3>(xr);
-          res = __builtin_canonicalize(row_dot(dg, y0, y1, y2, y3));
+          res = canon(row_dot(dg, y0, y1, y2, y3));
           lds[lane] = res;
         }
         wave_lds_sync();
@@ -389,11 +414,11 @@ static void launch_trsv(kf_ctx *c, const TM *a, const double *b, TV *y, int yslo
   /* persistent workgroups of G wavefronts: one atomic per G tickets */
   /* waves: the context's setting, else the schedule's size capped by the
    * resident slots of these kernels -- the FP64 ones holding 8 entries per
-   * row fit two waves per SIMD (188 VGPRs: 2048 on 256 CUs x 4 SIMDs), those
-   * holding 7 three (166 VGPRs: 3072; 0.97 / 1.06 ms on the 2.65M-vertex
-   * lattice against 1.02 / 1.10 ms with 2048), the FP32 ones four, of which
-   * 3072 measured best on that lattice (0.86 / 0.88 ms against 0.95 / 0.95 ms
-   * with 2048 and 0.88 / 0.91 ms with 4096) */
+   * row fit two waves per SIMD (184-186 VGPRs: 2048 on 256 CUs x 4 SIMDs),
+   * those holding 7 three (160-168 VGPRs: 3072; 0.97 / 1.06 ms on the
+   * 2.65M-vertex lattice against 1.02 / 1.10 ms with 2048), the FP32 ones
+   * four, of which 3072 measured best on that lattice (0.86 / 0.88 ms
+   * against 0.95 / 0.95 ms with 2048 and 0.88 / 0.91 ms with 4096) */
   const uint32_t cap = (sizeof(TV) == 8 && PF == KF_TRSV_PF) ? 2048u : 3072u;
   const uint32_t W = c->trsv_waves ? c->trsv_waves : (c->trsv_auto < cap ? c->trsv_auto : cap);
   const uint32_t wl = c->ntask_l < W ? c->ntask_l : W;
