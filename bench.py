@@ -5,10 +5,14 @@ seed 4 (2,654,208 vertices, 18,322,367 edges), second-order residual,
 GMRES(30) with modified Gram-Schmidt, CFL ramp from 10, stop at a 1e-10 drop
 of the nonlinear residual.  One "step" is one complete solve from the free
 stream; ``value`` is its wall-clock in seconds (mesh already resident in
-HBM) for the fastest mixed-precision path: the Jacobian fill, ILU(0)
-factors and triangular solves in FP32 (or with the factors stored in BF16),
-the residual, Arnoldi, least squares and update in FP64.  The plain FP64
-path is timed beside it on the same mesh and context (``paths``,
+HBM) for the fastest path whose final forces are within 1e-8 of the plain
+FP64 path's: the candidates keep the residual, Arnoldi, least squares and
+update in FP64 and change only the preconditioner -- its Jacobian fill,
+ILU(0) factors and triangular solves in FP32 (or with the factors stored in
+BF16), Jacobi sweeps instead of the exact triangular solves, or the
+two-level form (a Galerkin coarse correction over 2x2x2 vertex aggregates
+before the fine ILU(0) solve).  The plain FP64 path, the reference's
+algorithm, is timed beside it on the same mesh and context (``paths``,
 ``speedup_vs_fp64``), with the force error of each path against it.
 
 roofline: the kernel class that takes the most time in the timed solves
@@ -36,7 +40,7 @@ pseudo-time steps (the Newton residual and 64 seed-3 Gaussian right-hand
 sides each, solved to 1e-10), 2000 Adam steps at learning rate 1e-3.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c3|c2]
-                    [--no-cpu] [--no-mixed] [--no-learned] [--no-seq]
+                    [--no-cpu] [--no-mixed] [--no-coarse] [--no-learned] [--no-seq]
                     [--jacobi K,..] [--jacobi-fp32] [--learn-rhs R] [--learn-iters T] [--learn-config c2|c5]
                     [--kernel solve|step|solve1|copy|traffic]
 """
@@ -181,16 +185,36 @@ def lattice_mesh(K, tmp, config):
     return path
 
 
+def parse_path(path):
+    """a path name: <prec>[_jacobi<K>][_coarse] -- the preconditioner's
+    precision (fp64, fp32, bf16), K Jacobi sweeps each way instead of the
+    exact triangular solves, and the two-level (coarse-corrected) form;
+    returns (prec, K, coarse)"""
+    coarse = path.endswith("_coarse")
+    base = path[:-len("_coarse")] if coarse else path
+    prec, _, jac = base.partition("_jacobi")
+    return prec, int(jac) if jac else 0, coarse
+
+
 def path_options(K, path, **kw):
-    """configuration-profile options of a path name: fp64, fp32, bf16, or
-    <prec>_jacobi<K> (K Jacobi sweeps each way instead of the exact
-    triangular solves)"""
-    prec, _, jac = path.partition("_jacobi")
-    return K.config_options(prec=prec, jacobi_sweeps=int(jac) if jac else 0, **kw)
+    """configuration-profile options of a path name (parse_path)"""
+    prec, jac, coarse = parse_path(path)
+    return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=int(coarse), **kw)
+
+
+def path_precond(path):
+    prec, jac, coarse = parse_path(path)
+    s = "%s block ILU(0)" % prec
+    if jac:
+        s += " with %d Jacobi sweeps each way" % jac
+    if coarse:
+        s += (", two-level: a Galerkin coarse correction over 2x2x2 vertex aggregates (coarse block ILU(0) in "
+              "%s) before the fine solve" % ("fp64" if prec == "fp64" else "fp32"))
+    return s
 
 
 def path_dtype(path):
-    prec = path.partition("_jacobi")[0]
+    prec = parse_path(path)[0]
     return {"fp64": "f64", "fp32": "f64 residual/Arnoldi/update, f32 preconditioner",
             "bf16": "f64 residual/Arnoldi/update, f32 preconditioner arithmetic on bf16 factors"}[prec]
 
@@ -218,7 +242,7 @@ def roofline(prof, prec, sizes, config, held, fold):
     """the dominant kernel class of a profile and its roofline object"""
     N, E, S, Nf = sizes
     nnzb = N + 2 * E
-    dom = max(prof, key=lambda k: prof[k][0])
+    dom = max((k for k in prof if k in PROFILE_TO_KIND), key=lambda k: prof[k][0])
     ms, n = prof[dom]
     kind = PROFILE_TO_KIND[dom]
     s = {"fp64": 8, "fp32": 4, "bf16": 2}[prec] if kind in MATRIX_KINDS else 8
@@ -574,7 +598,7 @@ def step_probe(K, mesh, prec, reps, warmup, ts_max):
     pseudo-time steps (1: residual, time step, fill, ILU(0), one GMRES solve,
     update, forces -- every kernel of the path, for hardware-counter passes;
     500: whole solves, for kernel traces), nothing else timed or run"""
-    m = K.Mesh(mesh, K.config_options(prec=prec, ts_max=ts_max))
+    m = K.Mesh(mesh, path_options(K, prec, ts_max=ts_max))
     q0 = m.state()
     for _ in range(warmup):
         timed_solve(m, q0)
@@ -598,11 +622,13 @@ def main():
                     help="lattice: c5 288x96x96 seed 4 (the headline), c3 216x72x72 seed 2, c2 144x48x48 seed 1")
     ap.add_argument("--prec", default="auto",
                     help="the path `value` times: fp64, fp32, bf16, or one of those with _jacobiK (K Jacobi "
-                         "sweeps each way instead of the exact triangular solves); auto = the fastest "
-                         "single-precision path of the warm-up solves whose forces are within 1e-8 of FP64")
+                         "sweeps each way instead of the exact triangular solves) and/or _coarse (the two-level "
+                         "preconditioner); auto = the fastest other path of the warm-up solves whose forces are "
+                         "within 1e-8 of the plain FP64 path's")
     ap.add_argument("--jacobi", default="3",
                     help="Jacobi sweep counts of the extra BF16-factor paths (comma list; empty: none)")
     ap.add_argument("--jacobi-fp32", action="store_true", help="also FP32-factor paths with the Jacobi sweeps")
+    ap.add_argument("--no-coarse", action="store_true", help="skip the two-level (coarse-corrected) paths")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
@@ -661,6 +687,8 @@ def main():
         precs = ["fp32", "bf16"]
         for k in [int(x) for x in args.jacobi.split(",") if x.strip()]:
             precs += (["fp32_jacobi%d" % k] if args.jacobi_fp32 else []) + ["bf16_jacobi%d" % k]
+        if not args.no_coarse:
+            precs += ["fp64_coarse", "fp32_coarse", "bf16_coarse"]
     warm, warm_err = {"fp64": t64w}, {"fp64": 0.0}
     for p in precs:
         m.set_options(path_options(K, p))
@@ -724,12 +752,10 @@ def main():
         "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": path_dtype(vprec),
         "data": "synthetic",
-        "config": {"workload": "%dx%dx%d Kuhn lattice seed %d (%d vertices, %d edges), second order, %s "
-                               "preconditioner (block ILU(0)%s), GMRES(30) MGS, CFL ramp from 10, one solve from the "
+        "config": {"workload": "%dx%dx%d Kuhn lattice seed %d (%d vertices, %d edges), second order, "
+                               "preconditioner %s, GMRES(30) MGS, CFL ramp from 10, one solve from the "
                                "free stream to a 1e-10 residual drop"
-                               % (dims + (seed, N, E, vprec.partition("_jacobi")[0],
-                                          ", %s Jacobi sweeps each way" % vprec.partition("_jacobi")[2]
-                                          if "_jacobi" in vprec else "")),
+                               % (dims + (seed, N, E, path_precond(vprec))),
                    "lattice": list(dims), "seed": seed, "vertices": N, "edges": E, "path": vprec},
         "pseudo_steps": len(hist), "krylov_total": sum(x["krylov"] for x in hist),
         "residual_drop": hist[0]["fnorm"] / hist[-1]["fnorm"],
@@ -743,7 +769,7 @@ def main():
         "flux_gflops": flux_flops / flux_s / 1e9, "flux_fp64_frac": flux_flops / flux_s / 1e9 / FP64_PEAK_GFLOPS,
         "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
         "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
-        "roofline": roofline(prof, vprec.partition("_jacobi")[0], sizes, args.config, held, fold[vprec]),
+        "roofline": roofline(prof, parse_path(vprec)[0], sizes, args.config, held, fold[vprec]),
         "roofline_fp64": roofline(prof64, "fp64", sizes, args.config, held, fold["fp64"]) if prof64 else None,
         "roofline_spmv": spmv,
     }

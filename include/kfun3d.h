@@ -174,6 +174,16 @@ typedef struct kf_options {
                            75-132, its jacobi_sweeps): an approximate,
                            fixed linear preconditioner whose sweeps are
                            independent HBM-bound passes */
+  int    coarse;        /* 0: the preconditioner is block ILU(0) (the
+                           reference's); 1: two-level -- a Galerkin coarse
+                           correction over greedy aggregates of neighbouring
+                           vertices (2 x 2 x 2 on a lattice), factored by
+                           block ILU(0) in the preconditioner's precision
+                           (FP32 for BF16 factors), applied before the fine
+                           ILU(0) solve: z = P Ac^-1 P^T b, then
+                           x = z + (LU)^-1 (b - A z).  No counterpart in the
+                           reference; it cuts the Krylov count on large
+                           meshes.  Needs jacobi_sweeps = 0. */
 } kf_options;
 
 typedef struct kf_step_record {
@@ -260,6 +270,20 @@ size_t kf_trsv_plan_host(const GEOMETRY *g, int upper, unsigned R, int cluster, 
  * column k in row i.  Works on a host-only GEOMETRY. */
 size_t kf_ilu_plan_host(const GEOMETRY *g, unsigned *rec, unsigned *ldiag, unsigned *moff, unsigned *mjm,
                         unsigned short *mlo, size_t cap);
+
+/* The coarse level of the two-level preconditioner (kf_options.coarse; no
+ * counterpart in the reference), for tests.  kf_coarse_plan_host works on a
+ * host-only GEOMETRY: returns the number of aggregates nc and, for non-NULL
+ * arrays, writes *cnnzb, each vertex's aggregate (N entries) and the coarse
+ * BSR pattern (nc + 1 row pointers, cnnzb columns ascending per row).
+ * kf_coarse_matrix needs a GEOMETRY whose ComputeNumericalILU ran with
+ * coarse = 1 (returns 0 otherwise): the same outputs as built on the device,
+ * plus, if blocks is given, the Galerkin blocks P^T A P of the last ComputeA
+ * (16 cnnzb doubles, row-major 4x4 per block; FP32 sums with a
+ * single-precision preconditioner). */
+size_t kf_coarse_plan_host(const GEOMETRY *g, unsigned *agg, size_t *cnnzb, unsigned *crowptr, unsigned *ccol);
+size_t kf_coarse_matrix(const GEOMETRY *g, size_t *cnnzb, unsigned *agg, unsigned *crowptr, unsigned *ccol,
+                        double *blocks);
 
 /* ---- triangular solve that allocates nothing (no counterpart in the reference) ----
  * For callers that cannot allocate device memory: code under HIP graph

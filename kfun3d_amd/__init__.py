@@ -86,7 +86,7 @@ class Options(C.Structure):
                 ("gmres_rtol", C.c_double), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
                 ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int), ("ilu_exact", C.c_int),
                 ("reduce_seq", C.c_int), ("krylov", C.c_int), ("bsr_operator", C.c_int),
-                ("jacobi_sweeps", C.c_int)]
+                ("jacobi_sweeps", C.c_int), ("coarse", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -147,6 +147,9 @@ def _load():
         "kf_trsv_plan_host": (C.c_size_t, [GP, C.c_int, C.c_uint, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_size_t]),
         "kf_ilu_plan_host": (C.c_size_t, [GP, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+        "kf_coarse_plan_host": (C.c_size_t, [GP, C.c_void_p, C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p]),
+        "kf_coarse_matrix": (C.c_size_t, [GP, C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
         "ComputeFlux": (None, [GP, DP, C.POINTER(Xyz), DP]),
         "ComputeTimeStep": (None, [C.c_double, GP]),
         "ComputeA": (None, [GP]),
@@ -210,7 +213,7 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_trsv_fold", "kf_bsr_spmv", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_sptrsv_jacobi_noalloc", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
                "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
-               "kf_ilu_plan_host", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_init_params", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
+               "kf_ilu_plan_host", "kf_coarse_plan_host", "kf_coarse_matrix", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_init_params", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
                "kf_mpnn_get_params", "kf_mpnn_take_diag", "kf_mpnn_set_diag", "kf_mpnn_forward",
                "kf_mpnn_loss_grad", "kf_mpnn_adam", "kf_mpnn_train_step", "kf_mpnn_losses", "kf_mpnn_reset_losses",
                "kf_mpnn_activation", "kf_mpnn_take_matrix", "kf_mpnn_set_loss",
@@ -445,6 +448,34 @@ class Mesh:
         lib().kf_pattern(self.ref, rp.ctypes.data, col.ctypes.data, dg.ctypes.data)
         return rp, col, dg
 
+    def coarse_plan(self):
+        """The two-level preconditioner's coarse level as the host builds it:
+        (aggregate of each vertex, coarse row pointers, coarse columns)."""
+        nz = C.c_size_t(0)
+        nc = int(lib().kf_coarse_plan_host(self.ref, None, C.byref(nz), None, None))
+        agg = np.empty(self.N, dtype=np.uint32)
+        rp = np.empty(nc + 1, dtype=np.uint32)
+        col = np.empty(max(int(nz.value), 1), dtype=np.uint32)
+        lib().kf_coarse_plan_host(self.ref, agg.ctypes.data, C.byref(nz), rp.ctypes.data, col.ctypes.data)
+        return agg, rp, col[:int(nz.value)]
+
+    def coarse_matrix(self):
+        """The device's coarse level after ComputeNumericalILU with coarse=1:
+        (aggregates, coarse row pointers, columns, Galerkin blocks (cnnzb, 4, 4)
+        of the last ComputeA); None before."""
+        nz = C.c_size_t(0)
+        nc = int(lib().kf_coarse_matrix(self.ref, C.byref(nz), None, None, None, None))
+        if nc == 0:
+            return None
+        n = int(nz.value)
+        agg = np.empty(self.N, dtype=np.uint32)
+        rp = np.empty(nc + 1, dtype=np.uint32)
+        col = np.empty(max(n, 1), dtype=np.uint32)
+        blocks = np.empty(max(16 * n, 1))
+        lib().kf_coarse_matrix(self.ref, C.byref(nz), agg.ctypes.data, rp.ctypes.data, col.ctypes.data,
+                               blocks.ctypes.data)
+        return agg, rp, col[:n], blocks[:16 * n].reshape(-1, 4, 4)
+
     def kernel(self, max_rec=1000):
         rec = (StepRecord * max_rec)()
         n = lib().kf_kernel(self.ref, rec, max_rec)
@@ -482,7 +513,7 @@ class Mesh:
         return ms.value, n.value
 
     PROFILED = ["grad", "edge_flux", "residual", "timestep", "fill", "ilu", "trsv_lower", "trsv_upper", "spmv",
-                "jacobi_lower", "jacobi_upper"]
+                "jacobi_lower", "jacobi_upper", "coarse_factor", "coarse"]
 
     def synchronize(self):
         lib().kf_synchronize(self.ref)

@@ -20,8 +20,9 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkfun3d.so")
 ARCH = os.environ.get("KF_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRCS = ["gen.c", "msh.c", "setup.c", "plan.c", "solver.c"]
-HIP_SRCS = ["ctx.hip", "flux.hip", "blas.hip", "fill.hip", "ilu.hip", "sptrsv.hip", "spmv.hip", "jacobi.hip", "mpnn.hip"]
+HOST_SRCS = ["gen.c", "msh.c", "setup.c", "plan.c", "aggregate.c", "solver.c"]
+HIP_SRCS = ["ctx.hip", "flux.hip", "blas.hip", "fill.hip", "ilu.hip", "sptrsv.hip", "spmv.hip", "jacobi.hip", "coarse.hip",
+            "mpnn.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
           "-Wno-unused-parameter", "-I" + os.path.join(ROOT, "include")]

@@ -1,0 +1,286 @@
+/*
+ * coarse.hip -- the two-level preconditioner (kf_options.coarse) on gfx950.
+ *
+ * The reference preconditions GMRES with block ILU(0) alone
+ * (matrix/ilu.cpp:49-138, matrix/sptrsv.cpp:34-73).  Its Krylov count grows
+ * with the mesh: the error modes that ILU(0) leaves are global.  This adds
+ * a coarse level over aggregates of neighbouring vertices (host/aggregate.c):
+ *   Ac = P^T A P                (Galerkin; P piecewise constant per component)
+ *   z  = P Ac^-1 P^T b          (Ac^-1: block ILU(0) of Ac, the same kernels
+ *                                as the fine level, in a matrix-only context)
+ *   x  = z + (LU)^-1 (b - A z)  (the fine ILU(0) solve on the remainder)
+ * a fixed linear preconditioner, so plain left-preconditioned GMRES keeps
+ * working.  A is the fill's Jacobian, kept before the fine factorisation
+ * overwrites it.  Coarse factors are FP64 with an FP64 preconditioner and
+ * FP32 otherwise.
+ *
+ * Kernels (a quad of lanes per block row, lane c forming component c, as
+ * spmv.hip; workgroups remapped XCD by XCD):
+ *   k_galerkin     coarse block = sum of its fine blocks, ascending position
+ *   k_restrict     coarse vector = sum of the aggregate's members, ascending
+ *   k_coarse_resid t = b - A (P z_c): the BSR product reading z_c through the
+ *                  aggregate map (P z_c is never stored)
+ *   k_prolong_add  x += P z_c
+ * Every sum runs in a fixed order, so the preconditioner is deterministic.
+ */
+#include <cstring>
+
+#include "kf_dev.h"
+
+#define CG_WG 256   /* 64 block rows per workgroup */
+
+struct kf_coarse {
+  kf_ctx *cc;          /* the coarse matrix: pattern, plans, values, factors */
+  uint32_t nc;         /* aggregates */
+  size_t cnnzb;
+  uint32_t *agg;       /* [N] aggregate of each fine vertex */
+  uint32_t *mptr, *mlist;   /* members of each aggregate */
+  uint32_t *gptr, *glist;   /* fine block positions of each coarse block */
+  double *rc, *zc;     /* coarse right-hand side and solution, 4 nc */
+  double *t;           /* the fine remainder b - A z, 4 N */
+};
+
+template <typename T>
+static T *cupload(const T *h, size_t n)
+{
+  T *p = NULL;
+  KF_CHECK(hipMalloc((void **)&p, (n ? n : 1) * sizeof(T)));
+  if (n) KF_CHECK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(CG_WG) void k_galerkin(uint32_t nb, const uint32_t *__restrict__ gptr,
+                                                    const uint32_t *__restrict__ glist, const T *__restrict__ a,
+                                                    T *__restrict__ ac)
+{
+  const uint32_t b = kf_xcd_block() * (CG_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (b >= nb) return;
+  T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+  for (uint32_t k = gptr[b]; k < gptr[b + 1]; k++) {
+    const T *p = a + 16 * (size_t)glist[k] + 4 * c;
+    s0 += p[0];
+    s1 += p[1];
+    s2 += p[2];
+    s3 += p[3];
+  }
+  T *o = ac + 16 * (size_t)b + 4 * c;
+  o[0] = s0;
+  o[1] = s1;
+  o[2] = s2;
+  o[3] = s3;
+}
+
+__global__ __launch_bounds__(CG_WG) void k_restrict(uint32_t nc, const uint32_t *__restrict__ mptr,
+                                                    const uint32_t *__restrict__ mlist,
+                                                    const double *__restrict__ v, double *__restrict__ rc)
+{
+  const uint32_t I = kf_xcd_block() * (CG_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (I >= nc) return;
+  double s = 0.0;
+  for (uint32_t m = mptr[I]; m < mptr[I + 1]; m++) s += v[4 * (size_t)mlist[m] + c];
+  rc[4 * (size_t)I + c] = s;
+}
+
+template <typename T>
+__device__ __forceinline__ void cg_row4(const T *a, size_t j, int c, double &a0, double &a1, double &a2, double &a3)
+{
+  const T *p = a + 16 * j + 4 * c;
+  a0 = (double)p[0];
+  a1 = (double)p[1];
+  a2 = (double)p[2];
+  a3 = (double)p[3];
+}
+
+template <typename T>
+__global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                        const uint32_t *__restrict__ col, const T *__restrict__ a,
+                                                        const uint32_t *__restrict__ agg,
+                                                        const double *__restrict__ zc,
+                                                        const double *__restrict__ b, double *__restrict__ t)
+{
+  const uint32_t r = kf_xcd_block() * (CG_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;
+  double acc = 0.0;
+  for (uint32_t j = rowptr[r]; j < rowptr[r + 1]; j++) {
+    double a0, a1, a2, a3;
+    cg_row4<T>(a, j, c, a0, a1, a2, a3);
+    const double *z = zc + 4 * (size_t)agg[col[j]];
+    acc = acc + a0 * z[0];
+    acc = acc + a1 * z[1];
+    acc = acc + a2 * z[2];
+    acc = acc + a3 * z[3];
+  }
+  const size_t o = 4 * (size_t)r + c;
+  t[o] = b[o] - acc;
+}
+
+__global__ __launch_bounds__(CG_WG) void k_prolong_add(size_t n4, const uint32_t *__restrict__ agg,
+                                                       const double *__restrict__ zc, double *__restrict__ x)
+{
+  const size_t i = (size_t)blockIdx.x * CG_WG + threadIdx.x;
+  if (i < n4) x[i] = x[i] + zc[4 * (size_t)agg[i >> 2] + (i & 3)];
+}
+
+/* the coarse level, built on first use from the context's own pattern */
+static kf_coarse *coarse_build(kf_ctx *c)
+{
+  const size_t N = c->d.N, nnzb = c->nnzb;
+  kf_prep fp;
+  memset(&fp, 0, sizeof(fp));
+  fp.N = N;
+  fp.nnzb = nnzb;
+  fp.rowptr = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
+  fp.col = (uint32_t *)kf_xmalloc((nnzb + 1) * sizeof(uint32_t));
+  fp.diag = (uint32_t *)kf_xmalloc((N + 1) * sizeof(uint32_t));
+  KF_CHECK(hipMemcpy(fp.rowptr, c->d.rowptr, (N + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  KF_CHECK(hipMemcpy(fp.col, c->d.col, nnzb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  KF_CHECK(hipMemcpy(fp.diag, c->d.diag, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  kf_coarse_host h;
+  kf_build_coarse(&fp, &h);
+  free(fp.rowptr);
+  free(fp.col);
+  free(fp.diag);
+
+  kf_options co = c->opt;
+  co.prec = c->opt.prec == KF_PREC_FP64 ? KF_PREC_FP64 : KF_PREC_FP32;
+  co.ilu_exact = 0;
+  co.jacobi_sweeps = 0;
+  co.coarse = 0;
+  kf_coarse *k = new kf_coarse();
+  k->cc = kf_ctx_create_matrix(&h.prep, &co, c);
+  k->nc = (uint32_t)h.prep.N;
+  k->cnnzb = h.prep.nnzb;
+  k->agg = cupload(h.agg, N);
+  k->mptr = cupload(h.mptr, (size_t)k->nc + 1);
+  k->mlist = cupload(h.mlist, N);
+  k->gptr = cupload(h.gptr, k->cnnzb + 1);
+  k->glist = cupload(h.glist, nnzb);
+  KF_CHECK(hipMalloc((void **)&k->rc, 4 * (size_t)k->nc * sizeof(double)));
+  KF_CHECK(hipMalloc((void **)&k->zc, 4 * (size_t)k->nc * sizeof(double)));
+  KF_CHECK(hipMalloc((void **)&k->t, 4 * N * sizeof(double)));
+  kf_free_coarse(&h);
+  return k;
+}
+
+/* After ComputeA, before the fine factorisation overwrites the Jacobian:
+ * the Galerkin product into the coarse context and its ILU(0). */
+void kf_coarse_factor(kf_ctx *c)
+{
+  if (!c->coarse) c->coarse = coarse_build(c);
+  kf_coarse *k = (kf_coarse *)c->coarse;
+  kf_ctx *cc = k->cc;
+  const bool f64 = c->opt.prec == KF_PREC_FP64;
+  cc->opt.prec = f64 ? KF_PREC_FP64 : KF_PREC_FP32;   /* follows the fine level's precision */
+  const int s = kf_prof_begin(c, KP_COARSE_FACTOR);
+  const uint32_t nb = kf_blocks_for(k->cnnzb, CG_WG / 4);
+  if (f64) {
+    hipLaunchKernelGGL(k_galerkin<double>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
+                       k->glist, (const double *)c->a64, cc->a64);
+  } else {
+    if (!c->a32) {
+      fprintf(stderr, "kf: two-level preconditioner: no single-precision Jacobian (ComputeA must run first)\n");
+      abort();
+    }
+    if (!cc->a32) KF_CHECK(hipMalloc((void **)&cc->a32, 16 * k->cnnzb * sizeof(float)));
+    hipLaunchKernelGGL(k_galerkin<float>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
+                       k->glist, (const float *)c->a32, cc->a32);
+  }
+  KF_LAUNCH_CHECK();
+  kf_launch_ilu(cc);
+  kf_prof_end(c, s);
+}
+
+/* x = M b with the two-level preconditioner (b, x device, 4N, not aliased) */
+void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
+{
+  kf_coarse *k = (kf_coarse *)c->coarse;
+  if (!k) {
+    fprintf(stderr, "kf: two-level preconditioner: ComputeNumericalILU has not run with coarse = 1\n");
+    abort();
+  }
+  if (!c->jop64 && !c->jop32) {
+    fprintf(stderr, "kf: two-level preconditioner: no kept Jacobian (ComputeA must run with coarse = 1)\n");
+    abort();
+  }
+  kf_check_aligned(b, "b");
+  kf_check_aligned(x, "x");
+  const kf_dv &d = c->d;
+  const int s = kf_prof_begin(c, KP_COARSE);
+  hipLaunchKernelGGL(k_restrict, dim3(kf_blocks_for(k->nc, CG_WG / 4)), dim3(CG_WG), 0, c->stream, k->nc, k->mptr,
+                     k->mlist, b, k->rc);
+  KF_LAUNCH_CHECK();
+  kf_launch_sptrsv(k->cc, k->rc, k->zc, NULL);
+  const uint32_t nb = kf_blocks_for(d.N, CG_WG / 4);
+  if (c->jop64)
+    hipLaunchKernelGGL(k_coarse_resid<double>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, d.rowptr, d.col,
+                       (const double *)c->jop64, k->agg, k->zc, b, k->t);
+  else
+    hipLaunchKernelGGL(k_coarse_resid<float>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, d.rowptr, d.col,
+                       (const float *)c->jop32, k->agg, k->zc, b, k->t);
+  KF_LAUNCH_CHECK();
+  kf_prof_end(c, s);
+  kf_launch_sptrsv(c, k->t, x, NULL);
+  const size_t n4 = 4 * (size_t)d.N;
+  hipLaunchKernelGGL(k_prolong_add, dim3(kf_blocks_for(n4, CG_WG)), dim3(CG_WG), 0, c->stream, n4, k->agg, k->zc, x);
+  KF_LAUNCH_CHECK();
+}
+
+void kf_coarse_destroy(kf_ctx *c)
+{
+  kf_coarse *k = (kf_coarse *)c->coarse;
+  if (!k) return;
+  kf_ctx_destroy_matrix(k->cc);
+  void *ptrs[] = {k->agg, k->mptr, k->mlist, k->gptr, k->glist, k->rc, k->zc, k->t};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
+    if (ptrs[i]) KF_CHECK(hipFree(ptrs[i]));
+  delete k;
+  c->coarse = NULL;
+}
+
+/* the coarse level's size, its aggregate map and its Galerkin blocks (for
+ * tests): returns the number of aggregates; with non-NULL arrays also copies
+ * agg [N], the coarse pattern rowptr [nc + 1], col [cnnzb] and the blocks
+ * [16 cnnzb] as the last factorisation formed them (before their ILU) */
+extern "C" size_t kf_coarse_matrix(const GEOMETRY *g, size_t *cnnzb, unsigned *agg, unsigned *rowptr, unsigned *col,
+                                   double *blocks)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  kf_coarse *k = (kf_coarse *)c->coarse;
+  if (!k) return 0;
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  if (cnnzb) *cnnzb = k->cnnzb;
+  if (agg) KF_CHECK(hipMemcpy(agg, k->agg, (size_t)c->d.N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (rowptr) KF_CHECK(hipMemcpy(rowptr, k->cc->d.rowptr, ((size_t)k->nc + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (col) KF_CHECK(hipMemcpy(col, k->cc->d.col, k->cnnzb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (blocks) {
+    /* the Galerkin product again, into the host array (the coarse values
+     * were factored in place) */
+    const size_t n = 16 * k->cnnzb;
+    const bool f64 = c->opt.prec == KF_PREC_FP64;
+    void *tmp = NULL;
+    KF_CHECK(hipMalloc(&tmp, n * (f64 ? sizeof(double) : sizeof(float))));
+    const uint32_t nb = kf_blocks_for(k->cnnzb, CG_WG / 4);
+    if (f64)
+      hipLaunchKernelGGL(k_galerkin<double>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
+                         k->glist, (const double *)(c->jop64 ? c->jop64 : c->a64), (double *)tmp);
+    else
+      hipLaunchKernelGGL(k_galerkin<float>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
+                         k->glist, (const float *)c->jop32, (float *)tmp);
+    KF_LAUNCH_CHECK();
+    KF_CHECK(hipStreamSynchronize(c->stream));
+    if (f64) {
+      KF_CHECK(hipMemcpy(blocks, tmp, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      float *h = (float *)kf_xmalloc(n * sizeof(float));
+      KF_CHECK(hipMemcpy(h, tmp, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; i++) blocks[i] = (double)h[i];
+      free(h);
+    }
+    KF_CHECK(hipFree(tmp));
+  }
+  return k->nc;
+}

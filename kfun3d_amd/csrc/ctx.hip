@@ -23,104 +23,32 @@ static T *dupload(const T *h, size_t n)
   return p;
 }
 
-/* rows of each level cut into groups of at most KF_GROUP */
-extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o)
+/* launch-shape knobs of the level-scheduled kernels (tuning only) */
+static void ctx_knobs(kf_ctx *c)
 {
-  kf_ctx *c = new kf_ctx();
-  memset((void *)c, 0, sizeof(*c));
-  kf_check_options(o);
-  KF_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  c->opt = *o;
-  kf_dv &d = c->d;
-  /* launch-shape knobs of the level-scheduled kernels (tuning only) */
   const char *ev;
-  /* triangular-solve waves: sized from the schedule below unless set */
+  /* triangular-solve waves: sized from the schedule unless set */
   c->trsv_waves = (ev = getenv("KF_TRSV_WAVES")) ? (uint32_t)atoi(ev) : 0;   /* 0: sized per launch */
   c->ilu_waves = (ev = getenv("KF_ILU_WAVES")) ? (uint32_t)atoi(ev) : 0;   /* 0: by precision */
-  d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
+  c->d.spin_sleep = (ev = getenv("KF_SPIN_SLEEP")) ? (uint32_t)atoi(ev) : 2;
   c->trsv_group = (ev = getenv("KF_TRSV_GROUP")) ? (uint32_t)atoi(ev) : 4;
   c->ilu_group = (ev = getenv("KF_ILU_GROUP")) ? (uint32_t)atoi(ev) : 4;
-
   if (c->trsv_group < 1 || c->trsv_group > 4) c->trsv_group = 4;
   if (c->ilu_group < 1 || c->ilu_group > 4) c->ilu_group = 4;
-  {
-    int dev = 0, lds = 0;
-    KF_CHECK(hipGetDevice(&dev));
-    KF_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    c->lds_max = lds > 0 ? (size_t)lds : 64 * 1024;
-  }
-  const size_t N = p->N, E = p->E;
-  if (N >= (1u << 31) || E >= (1u << 30) || p->nnzb >= (1u << 31)) {
-    fprintf(stderr, "kf: mesh too large for 32-bit indices\n");
-    abort();
-  }
-  d.N = (uint32_t)N;
-  d.E = (uint32_t)E;
-  d.S = (uint32_t)g->b->fc->sz;
-  d.NS = (uint32_t)g->b->s->sz;
-  d.NF = (uint32_t)g->b->f->sz;
-  kf_free_stream(o->alpha_deg, &d.U, &d.V);
-  c->nnzb = p->nnzb;
+  int dev = 0, lds = 0;
+  KF_CHECK(hipGetDevice(&dev));
+  KF_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  c->lds_max = lds > 0 ? (size_t)lds : 64 * 1024;
+}
 
-  d.n0 = dupload(g->e->eptr->n0, E);
-  d.n1 = dupload(g->e->eptr->n1, E);
-  d.enx = dupload(g->e->xyzn->x0, E);
-  d.eny = dupload(g->e->xyzn->x1, E);
-  d.enz = dupload(g->e->xyzn->x2, E);
-  d.ea = dupload(g->e->xyzn->x3, E);
-  d.x = dupload(g->n->xyz->x0, N);
-  d.y = dupload(g->n->xyz->x1, N);
-  d.z = dupload(g->n->xyz->x2, N);
-  {
-    double *h = (double *)kf_xmalloc(4 * N * sizeof(double));
-    for (size_t i = 0; i < N; i++) {
-      h[4 * i] = g->n->xyz->x0[i];
-      h[4 * i + 1] = g->n->xyz->x1[i];
-      h[4 * i + 2] = g->n->xyz->x2[i];
-      h[4 * i + 3] = 0.0;
-    }
-    d.xyzw = dupload(h, 4 * N);
-    free(h);
-  }
-
-  /* sliced ELL of the incidence lists (slice height 64), built on the host */
-  kf_sell sl;
-  kf_build_sell(g, p, &sl);
-  const uint32_t nslice = sl.nslice;
-  d.nslice = nslice;
-  const size_t ne = sl.ne;
-  uint32_t *sptr = sl.sptr, *deg = sl.deg, *nbr = sl.nbr, *ent = sl.ent, *blk = sl.blk;
-  double *wx = sl.wx, *wy = sl.wy, *wz = sl.wz;
-  kf_phase("sliced ELL (host)");
-  d.sell_ptr = dupload(sptr, nslice + 1);
-  d.deg = dupload(deg, N);
-  d.sell_nbr = dupload(nbr, ne);
-  d.sell_ent = dupload(ent, ne);
-  d.sell_blk = dupload(blk, ne);
-  d.sell_wx = dupload(wx, ne);
-  d.sell_wy = dupload(wy, ne);
-  d.sell_wz = dupload(wz, ne);
-  free(sptr); free(deg); free(nbr); free(ent); free(blk); free(wx); free(wy); free(wz);
-
-  /* boundary */
-  d.sf0 = dupload(g->b->fc->fptr->n0, d.S);
-  d.sf1 = dupload(g->b->fc->fptr->n1, d.S);
-  d.sf2 = dupload(g->b->fc->fptr->n2, d.S);
-  d.ncolor = (uint32_t)g->t->sz;
-  d.color = dupload(g->t->i, (size_t)d.ncolor + 1);
-  d.sfptr = dupload(p->sfptr, N + 1);
-  d.sfent = dupload(p->sfent, 3 * (size_t)d.S);
-  d.snx = dupload(g->b->s->xyz->x0, d.NS);
-  d.sny = dupload(g->b->s->xyz->x1, d.NS);
-  d.snz = dupload(g->b->s->xyz->x2, d.NS);
-  d.snptr = dupload(p->snptr, N + 1);
-  d.snent = dupload(p->snent, d.NS);
-  d.fnx = dupload(g->b->f->xyz->x0, d.NF);
-  d.fny = dupload(g->b->f->xyz->x1, d.NF);
-  d.fnz = dupload(g->b->f->xyz->x2, d.NF);
-  d.fnptr = dupload(p->fnptr, N + 1);
-  d.fnent = dupload(p->fnent, d.NF);
-
+/* the block matrix of a context: pattern, ILU(0) and triangular-solve plans,
+ * tickets and the FP64 value array (a mesh's Jacobian, or the coarse
+ * Galerkin matrix of the two-level preconditioner) */
+static void ctx_matrix_setup(kf_ctx *c, const kf_prep *p)
+{
+  kf_dv &d = c->d;
+  const size_t N = p->N;
+  const char *ev;
   /* BSR pattern and level schedule */
   d.rowptr = dupload(p->rowptr, N + 1);
   d.col = dupload(p->col, p->nnzb);
@@ -202,6 +130,93 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   c->a64 = dalloc<double>(16 * p->nnzb);
   KF_CHECK(hipMemset(c->a64, 0, 16 * p->nnzb * sizeof(double)));
 
+}
+
+/* rows of each level cut into groups of at most KF_GROUP */
+extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o)
+{
+  kf_ctx *c = new kf_ctx();
+  memset((void *)c, 0, sizeof(*c));
+  kf_check_options(o);
+  KF_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->opt = *o;
+  kf_dv &d = c->d;
+  const char *ev;
+  ctx_knobs(c);
+  const size_t N = p->N, E = p->E;
+  if (N >= (1u << 31) || E >= (1u << 30) || p->nnzb >= (1u << 31)) {
+    fprintf(stderr, "kf: mesh too large for 32-bit indices\n");
+    abort();
+  }
+  d.N = (uint32_t)N;
+  d.E = (uint32_t)E;
+  d.S = (uint32_t)g->b->fc->sz;
+  d.NS = (uint32_t)g->b->s->sz;
+  d.NF = (uint32_t)g->b->f->sz;
+  kf_free_stream(o->alpha_deg, &d.U, &d.V);
+  c->nnzb = p->nnzb;
+
+  d.n0 = dupload(g->e->eptr->n0, E);
+  d.n1 = dupload(g->e->eptr->n1, E);
+  d.enx = dupload(g->e->xyzn->x0, E);
+  d.eny = dupload(g->e->xyzn->x1, E);
+  d.enz = dupload(g->e->xyzn->x2, E);
+  d.ea = dupload(g->e->xyzn->x3, E);
+  d.x = dupload(g->n->xyz->x0, N);
+  d.y = dupload(g->n->xyz->x1, N);
+  d.z = dupload(g->n->xyz->x2, N);
+  {
+    double *h = (double *)kf_xmalloc(4 * N * sizeof(double));
+    for (size_t i = 0; i < N; i++) {
+      h[4 * i] = g->n->xyz->x0[i];
+      h[4 * i + 1] = g->n->xyz->x1[i];
+      h[4 * i + 2] = g->n->xyz->x2[i];
+      h[4 * i + 3] = 0.0;
+    }
+    d.xyzw = dupload(h, 4 * N);
+    free(h);
+  }
+
+  /* sliced ELL of the incidence lists (slice height 64), built on the host */
+  kf_sell sl;
+  kf_build_sell(g, p, &sl);
+  const uint32_t nslice = sl.nslice;
+  d.nslice = nslice;
+  const size_t ne = sl.ne;
+  uint32_t *sptr = sl.sptr, *deg = sl.deg, *nbr = sl.nbr, *ent = sl.ent, *blk = sl.blk;
+  double *wx = sl.wx, *wy = sl.wy, *wz = sl.wz;
+  kf_phase("sliced ELL (host)");
+  d.sell_ptr = dupload(sptr, nslice + 1);
+  d.deg = dupload(deg, N);
+  d.sell_nbr = dupload(nbr, ne);
+  d.sell_ent = dupload(ent, ne);
+  d.sell_blk = dupload(blk, ne);
+  d.sell_wx = dupload(wx, ne);
+  d.sell_wy = dupload(wy, ne);
+  d.sell_wz = dupload(wz, ne);
+  free(sptr); free(deg); free(nbr); free(ent); free(blk); free(wx); free(wy); free(wz);
+
+  /* boundary */
+  d.sf0 = dupload(g->b->fc->fptr->n0, d.S);
+  d.sf1 = dupload(g->b->fc->fptr->n1, d.S);
+  d.sf2 = dupload(g->b->fc->fptr->n2, d.S);
+  d.ncolor = (uint32_t)g->t->sz;
+  d.color = dupload(g->t->i, (size_t)d.ncolor + 1);
+  d.sfptr = dupload(p->sfptr, N + 1);
+  d.sfent = dupload(p->sfent, 3 * (size_t)d.S);
+  d.snx = dupload(g->b->s->xyz->x0, d.NS);
+  d.sny = dupload(g->b->s->xyz->x1, d.NS);
+  d.snz = dupload(g->b->s->xyz->x2, d.NS);
+  d.snptr = dupload(p->snptr, N + 1);
+  d.snent = dupload(p->snent, d.NS);
+  d.fnx = dupload(g->b->f->xyz->x0, d.NF);
+  d.fny = dupload(g->b->f->xyz->x1, d.NF);
+  d.fnz = dupload(g->b->f->xyz->x2, d.NF);
+  d.fnptr = dupload(p->fnptr, N + 1);
+  d.fnent = dupload(p->fnent, d.NF);
+
+  ctx_matrix_setup(c, p);
+
   /* state and work */
   c->q = dalloc<double>(4 * N);
   c->cdt = dalloc<double>(N);
@@ -248,11 +263,43 @@ static void dfree(void *p)
   if (p) KF_CHECK(hipFree(p));
 }
 
+/* A context holding only a block matrix -- the coarse level of the
+ * two-level preconditioner: pattern, plans, tickets, values and factors.
+ * It shares the parent's stream (so its launches are ordered with the
+ * parent's) and the parent's device error word. */
+kf_ctx *kf_ctx_create_matrix(const kf_prep *p, const kf_options *o, const kf_ctx *parent)
+{
+  kf_ctx *c = new kf_ctx();
+  memset((void *)c, 0, sizeof(*c));
+  c->opt = *o;
+  c->stream = parent->stream;
+  ctx_knobs(c);
+  c->d.N = (uint32_t)p->N;
+  c->d.err = parent->d.err;
+  c->nnzb = p->nnzb;
+  ctx_matrix_setup(c, p);
+  c->trsv_fold = 0;   /* its right-hand sides are formed before each solve */
+  return c;
+}
+
+void kf_ctx_destroy_matrix(kf_ctx *c)
+{
+  if (!c) return;
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  void *ptrs[] = {c->d.rowptr, c->d.col, c->d.diag, c->lrows, c->rdone, c->trow_l, c->trow_u, c->tlvl_l,
+                  c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff,
+                  c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32, c->abf, c->lold,
+                  c->ud, c->f32a, c->f32b, c->jac64, c->jac32};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
+  delete c;
+}
+
 extern "C" void kf_ctx_destroy(GEOMETRY *g)
 {
   if (!g->matrix) return;
   kf_ctx *c = (kf_ctx *)g->matrix;
   kfs_learn_detach(g);
+  kf_coarse_destroy(c);   /* on the context's stream, with its error word */
   KF_CHECK(hipStreamSynchronize(c->stream));
   if (c->fstream) {
     KF_CHECK(hipStreamSynchronize(c->fstream));
@@ -301,6 +348,7 @@ extern "C" kf_options kf_default_options(void)
   o.krylov = KF_KRYLOV_GMRES;
   o.bsr_operator = 0;
   o.jacobi_sweeps = 0;
+  o.coarse = 0;
   return o;
 }
 
@@ -314,6 +362,14 @@ void kf_check_options(const kf_options *o)
   }
   if (o->jacobi_sweeps < 0 || o->jacobi_sweeps > KF_JACOBI_MAX) {
     fprintf(stderr, "kf: jacobi_sweeps must be in [0, %d]\n", KF_JACOBI_MAX);
+    abort();
+  }
+  if (o->coarse != 0 && o->coarse != 1) {
+    fprintf(stderr, "kf: coarse must be 0 or 1\n");
+    abort();
+  }
+  if (o->coarse && o->jacobi_sweeps) {
+    fprintf(stderr, "kf: the two-level preconditioner (coarse = 1) needs exact triangular solves (jacobi_sweeps = 0)\n");
     abort();
   }
 }
@@ -503,7 +559,9 @@ extern "C" void ComputeA(GEOMETRY *g)
 {
   kf_ctx *c = kf_ctx_of(g);
   kf_launch_fill(c);
-  if (c->opt.bsr_operator) kf_keep_jacobian(c);   /* the ILU factors the fill's array in place */
+  /* the ILU factors the fill's array in place: the BSR operator and the
+   * two-level preconditioner's remainder product need a copy */
+  if (c->opt.bsr_operator || c->opt.coarse) kf_keep_jacobian(c);
 }
 
 /* y = A x with the Jacobian of the last ComputeA (kept when bsr_operator is
@@ -523,7 +581,12 @@ extern "C" void kf_bsr_spmv(const GEOMETRY *g, const double *x, double *y)
   kf_launch_spmv(c, x, y);
 }
 
-extern "C" void ComputeNumericalILU(GEOMETRY *g) { kf_launch_ilu(kf_ctx_of(g)); }
+extern "C" void ComputeNumericalILU(GEOMETRY *g)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (c->opt.coarse) kf_coarse_factor(c);   /* from the fill's values, before they are factored */
+  kf_launch_ilu(c);
+}
 
 extern "C" void ComputeSparseTriangularSolve(const GEOMETRY *g, const double *b, double *x)
 {
@@ -532,7 +595,8 @@ extern "C" void ComputeSparseTriangularSolve(const GEOMETRY *g, const double *b,
     abort();
   }
   kf_ctx *c = kf_ctx_of(g);
-  if (c->opt.jacobi_sweeps > 0) kf_launch_jacobi(c, b, x);   /* matrix/sptrsv.cpp:75-132 */
+  if (c->opt.coarse) kf_coarse_apply(c, b, x);
+  else if (c->opt.jacobi_sweeps > 0) kf_launch_jacobi(c, b, x);   /* matrix/sptrsv.cpp:75-132 */
   else kf_launch_sptrsv(c, b, x, NULL);
 }
 
@@ -555,6 +619,11 @@ extern "C" void kfs_sptrsv_fd(GEOMETRY *g, const double *b, double *x, const dou
     abort();
   }
   kf_check_aligned(r, "r"); kf_check_aligned(w, "w");
+  if (c->opt.coarse) {   /* restriction and remainder read b: form it once */
+    kf_fd_combine(c, r, w, h, slot_xnorm, (double *)b);
+    kf_coarse_apply(c, b, x);
+    return;
+  }
   if (c->opt.jacobi_sweeps > 0) {   /* every forward sweep reads b: form it once */
     kf_fd_combine(c, r, w, h, slot_xnorm, (double *)b);
     kf_launch_jacobi(c, b, x);
@@ -621,7 +690,8 @@ extern "C" double Normalize(const size_t sz, double *x)
 /* ---- per-kernel timing with HIP events on the context stream ----------------- */
 
 static const char *kf_prof_names[KP_N] = {"grad", "edge_flux", "residual", "timestep", "fill", "ilu",
-                                          "trsv_lower", "trsv_upper", "spmv", "jacobi_lower", "jacobi_upper"};
+                                          "trsv_lower", "trsv_upper", "spmv", "jacobi_lower", "jacobi_upper",
+                                          "coarse_factor", "coarse"};
 
 static void prof_drain(kf_ctx *c)
 {

@@ -79,7 +79,7 @@ struct kf_dv {
 
 /* per-kernel HIP-event timing on the context stream (kf_profile) */
 enum kf_prof_id { KP_GRAD, KP_EDGE, KP_RES, KP_TSTEP, KP_FILL, KP_ILU, KP_TRSV_L, KP_TRSV_U, KP_SPMV, KP_JAC_L, KP_JAC_U,
-                  KP_N };
+                  KP_COARSE_FACTOR, KP_COARSE, KP_N };
 #define KF_PROF_POOL 512
 
 struct kf_prof {
@@ -167,6 +167,7 @@ struct kf_ctx {
   double *ftbuf;            /* per-face force terms */
   int forces_pending;
   void *noalloc;            /* this device's address of the fixed work array of kf_sptrsv_jacobi_noalloc */
+  void *coarse;             /* the two-level preconditioner's coarse level (coarse.hip), built on first use */
   void *learn;              /* learned-preconditioner state of the driver (solver.c) */
 };
 
@@ -233,6 +234,15 @@ void kf_launch_jacobi(kf_ctx *c, const double *b, double *x);
  * address of kf_sptrsv_jacobi_noalloc's fixed work array (jacobi.hip) */
 void kf_noalloc_setup(kf_ctx *c);
 void kf_check_options(const kf_options *o);
+/* a context holding a matrix only (pattern, plans, values, factors), on its
+ * parent's stream and error word: the coarse level's (ctx.hip) */
+kf_ctx *kf_ctx_create_matrix(const kf_prep *p, const kf_options *o, const kf_ctx *parent);
+void kf_ctx_destroy_matrix(kf_ctx *c);
+/* the two-level preconditioner (coarse.hip): the Galerkin coarse matrix and
+ * its ILU(0) from the kept Jacobian, and x = M b */
+void kf_coarse_factor(kf_ctx *c);
+void kf_coarse_apply(kf_ctx *c, const double *b, double *x);
+void kf_coarse_destroy(kf_ctx *c);
 
 /* BLAS-1 on the context stream; reductions are deterministic (fixed
  * partition, fixed tree) and leave their result in d.scal[slot] */

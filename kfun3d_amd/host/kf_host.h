@@ -94,6 +94,19 @@ void kf_host_setup(GEOMETRY *g);
 /* derived arrays for the device */
 void kf_build_prep(const GEOMETRY *g, kf_prep *p);
 void kf_free_prep(kf_prep *p);
+/* the level sets of a prep whose pattern (N, nnzb, rowptr, col, diag) is set */
+void kf_prep_levels(kf_prep *p);
+
+/* the coarse level of the two-level preconditioner (coarse.c): aggregates,
+ * the Galerkin pattern as a prep of its own, and the gather lists */
+typedef struct kf_coarse_host {
+  kf_prep prep;        /* N = aggregates; rowptr, col, diag, nnzb, level sets */
+  uint32_t *agg;       /* [N fine] aggregate of each vertex */
+  uint32_t *mptr, *mlist;   /* members of each aggregate, ascending */
+  uint32_t *gptr, *glist;   /* fine block positions of each coarse block, ascending */
+} kf_coarse_host;
+void kf_build_coarse(const kf_prep *p, kf_coarse_host *out);
+void kf_free_coarse(kf_coarse_host *h);
 void kf_free_host_geometry(GEOMETRY *g);
 void kf_free_lsq(GEOMETRY *g);
 

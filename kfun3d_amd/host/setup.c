@@ -369,6 +369,12 @@ static void levels(const kf_prep *p, int upper, uint32_t *nlev, uint32_t **lptr,
   free(depth); free(lvl); free(rows);
 }
 
+void kf_prep_levels(kf_prep *p)
+{
+  levels(p, 0, &p->nlev_l, &p->lptr, &p->lrows);
+  levels(p, 1, &p->nlev_u, &p->uptr, &p->urows);
+}
+
 void kf_build_prep(const GEOMETRY *g, kf_prep *p)
 {
   memset(p, 0, sizeof(*p));
