@@ -186,12 +186,12 @@ def lattice_mesh(K, tmp, config):
 
 
 def parse_path(path):
-    """a path name: <prec>[_jacobi<K>][_coarse] -- the preconditioner's
+    """a path name: <prec>[_jacobi<K>][_coarse[L]] -- the preconditioner's
     precision (fp64, fp32, bf16), K Jacobi sweeps each way instead of the
-    exact triangular solves, and the two-level (coarse-corrected) form;
-    returns (prec, K, coarse)"""
-    coarse = path.endswith("_coarse")
-    base = path[:-len("_coarse")] if coarse else path
+    exact triangular solves, and the multilevel form with L coarse levels
+    (1 when L is omitted: two-level); returns (prec, K, L)"""
+    base, _, lv = path.partition("_coarse")
+    coarse = (int(lv) if lv else 1) if "_coarse" in path else 0
     prec, _, jac = base.partition("_jacobi")
     return prec, int(jac) if jac else 0, coarse
 
@@ -199,7 +199,7 @@ def parse_path(path):
 def path_options(K, path, **kw):
     """configuration-profile options of a path name (parse_path)"""
     prec, jac, coarse = parse_path(path)
-    return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=int(coarse), **kw)
+    return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=coarse, **kw)
 
 
 def path_precond(path):
@@ -208,8 +208,10 @@ def path_precond(path):
     if jac:
         s += " with %d Jacobi sweeps each way" % jac
     if coarse:
-        s += (", two-level: a Galerkin coarse correction over 2x2x2 vertex aggregates (coarse block ILU(0) in "
-              "%s) before the fine solve" % ("fp64" if prec == "fp64" else "fp32"))
+        s += (", %d-level: a Galerkin coarse correction over 2x2x2 vertex aggregates before the fine solve, "
+              "%s, coarse arithmetic in %s" % (coarse + 1, "the coarsest system by block ILU(0)" if coarse == 1 else
+                                               "each coarse system by the same scheme, the coarsest by block ILU(0)",
+                                               "fp64" if prec == "fp64" else "fp32"))
     return s
 
 
@@ -688,7 +690,7 @@ def main():
         for k in [int(x) for x in args.jacobi.split(",") if x.strip()]:
             precs += (["fp32_jacobi%d" % k] if args.jacobi_fp32 else []) + ["bf16_jacobi%d" % k]
         if not args.no_coarse:
-            precs += ["fp64_coarse", "fp32_coarse", "bf16_coarse"]
+            precs += ["fp64_coarse", "fp32_coarse", "bf16_coarse", "fp32_coarse2", "fp32_coarse3"]
     warm, warm_err = {"fp64": t64w}, {"fp64": 0.0}
     for p in precs:
         m.set_options(path_options(K, p))

@@ -181,10 +181,14 @@ typedef struct kf_options {
                            block ILU(0) in the preconditioner's precision
                            (FP32 for BF16 factors), applied before the fine
                            ILU(0) solve: z = P Ac^-1 P^T b, then
-                           x = z + (LU)^-1 (b - A z).  No counterpart in the
-                           reference; it cuts the Krylov count on large
-                           meshes.  Needs jacobi_sweeps = 0. */
+                           x = z + (LU)^-1 (b - A z); L in 2..KF_COARSE_MAX:
+                           the coarse system is preconditioned the same way
+                           in turn, L levels below the fine one.  No
+                           counterpart in the reference; it cuts the Krylov
+                           count on large meshes.  Needs jacobi_sweeps = 0. */
 } kf_options;
+
+#define KF_COARSE_MAX 4   /* most levels below the fine one (kf_options.coarse) */
 
 typedef struct kf_step_record {
   int    step;          /* pseudo-time step index                             */

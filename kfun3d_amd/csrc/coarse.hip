@@ -7,19 +7,25 @@
  * a coarse level over aggregates of neighbouring vertices (host/aggregate.c):
  *   Ac = P^T A P                (Galerkin; P piecewise constant per component)
  *   z  = P Ac^-1 P^T b          (Ac^-1: block ILU(0) of Ac, the same kernels
- *                                as the fine level, in a matrix-only context)
+ *                                as the fine level, in a matrix-only context;
+ *                                with coarse = L > 1, the same scheme again
+ *                                on Ac, L levels below the fine one)
  *   x  = z + (LU)^-1 (b - A z)  (the fine ILU(0) solve on the remainder)
  * a fixed linear preconditioner, so plain left-preconditioned GMRES keeps
- * working.  A is the fill's Jacobian, kept before the fine factorisation
- * overwrites it.  Coarse factors are FP64 with an FP64 preconditioner and
- * FP32 otherwise.
+ * working.  A is the fill's Jacobian, read before the fine factorisation
+ * overwrites it, and enters only as A P: the fine rows with their blocks
+ * summed by the aggregate of their column (about 6 blocks a row on a
+ * lattice, against 15 in A).  A P serves both the Galerkin product
+ * (P^T (A P)) and the remainder b - (A P) z_c, so the full Jacobian is
+ * neither kept nor re-read per application.  Coarse factors and A P are
+ * FP64 with an FP64 preconditioner and FP32 otherwise.
  *
  * Kernels (a quad of lanes per block row, lane c forming component c, as
  * spmv.hip; workgroups remapped XCD by XCD):
- *   k_galerkin     coarse block = sum of its fine blocks, ascending position
+ *   k_gather_sum   a block = the sum of listed blocks in ascending position:
+ *                  A P from the fine blocks, then P^T A P from A P
  *   k_restrict     coarse vector = sum of the aggregate's members, ascending
- *   k_coarse_resid t = b - A (P z_c): the BSR product reading z_c through the
- *                  aggregate map (P z_c is never stored)
+ *   k_coarse_resid t = b - (A P) z_c
  *   k_prolong_add  x += P z_c
  * Every sum runs in a fixed order, so the preconditioner is deterministic.
  */
@@ -32,10 +38,14 @@
 struct kf_coarse {
   kf_ctx *cc;          /* the coarse matrix: pattern, plans, values, factors */
   uint32_t nc;         /* aggregates */
-  size_t cnnzb;
+  size_t cnnzb, nap;
   uint32_t *agg;       /* [N] aggregate of each fine vertex */
   uint32_t *mptr, *mlist;   /* members of each aggregate */
-  uint32_t *gptr, *glist;   /* fine block positions of each coarse block */
+  uint32_t *aprp, *apcol;   /* A P pattern: per fine row, its aggregates */
+  uint32_t *apgptr, *apglist;   /* fine block positions of each A P block */
+  uint32_t *gptr, *glist;   /* A P positions of each coarse block */
+  double *ap64;        /* A P values (FP64 preconditioner) ... */
+  float *ap32;         /* ... or FP32 */
   double *rc, *zc;     /* coarse right-hand side and solution, 4 nc */
   double *t;           /* the fine remainder b - A z, 4 N */
 };
@@ -50,7 +60,7 @@ static T *cupload(const T *h, size_t n)
 }
 
 template <typename T>
-__global__ __launch_bounds__(CG_WG) void k_galerkin(uint32_t nb, const uint32_t *__restrict__ gptr,
+__global__ __launch_bounds__(CG_WG) void k_gather_sum(uint32_t nb, const uint32_t *__restrict__ gptr,
                                                     const uint32_t *__restrict__ glist, const T *__restrict__ a,
                                                     T *__restrict__ ac)
 {
@@ -94,10 +104,11 @@ __device__ __forceinline__ void cg_row4(const T *a, size_t j, int c, double &a0,
   a3 = (double)p[3];
 }
 
+/* t = b - (A P) z_c; per row the A P blocks in ascending aggregate, each
+ * component summed a[c][0] z0 + ... + a[c][3] z3 in FP64 */
 template <typename T>
-__global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32_t *__restrict__ rowptr,
-                                                        const uint32_t *__restrict__ col, const T *__restrict__ a,
-                                                        const uint32_t *__restrict__ agg,
+__global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32_t *__restrict__ aprp,
+                                                        const uint32_t *__restrict__ apcol, const T *__restrict__ ap,
                                                         const double *__restrict__ zc,
                                                         const double *__restrict__ b, double *__restrict__ t)
 {
@@ -105,10 +116,10 @@ __global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32
   const int c = threadIdx.x & 3;
   if (r >= N) return;
   double acc = 0.0;
-  for (uint32_t j = rowptr[r]; j < rowptr[r + 1]; j++) {
+  for (uint32_t k = aprp[r]; k < aprp[r + 1]; k++) {
     double a0, a1, a2, a3;
-    cg_row4<T>(a, j, c, a0, a1, a2, a3);
-    const double *z = zc + 4 * (size_t)agg[col[j]];
+    cg_row4<T>(ap, k, c, a0, a1, a2, a3);
+    const double *z = zc + 4 * (size_t)apcol[k];
     acc = acc + a0 * z[0];
     acc = acc + a1 * z[1];
     acc = acc + a2 * z[2];
@@ -149,16 +160,21 @@ static kf_coarse *coarse_build(kf_ctx *c)
   co.prec = c->opt.prec == KF_PREC_FP64 ? KF_PREC_FP64 : KF_PREC_FP32;
   co.ilu_exact = 0;
   co.jacobi_sweeps = 0;
-  co.coarse = 0;
+  co.coarse = c->opt.coarse - 1;   /* further levels below this one */
   kf_coarse *k = new kf_coarse();
   k->cc = kf_ctx_create_matrix(&h.prep, &co, c);
   k->nc = (uint32_t)h.prep.N;
   k->cnnzb = h.prep.nnzb;
+  k->nap = h.nap;
   k->agg = cupload(h.agg, N);
   k->mptr = cupload(h.mptr, (size_t)k->nc + 1);
   k->mlist = cupload(h.mlist, N);
+  k->aprp = cupload(h.aprp, N + 1);
+  k->apcol = cupload(h.apcol, h.nap);
+  k->apgptr = cupload(h.apgptr, h.nap + 1);
+  k->apglist = cupload(h.apglist, nnzb);
   k->gptr = cupload(h.gptr, k->cnnzb + 1);
-  k->glist = cupload(h.glist, nnzb);
+  k->glist = cupload(h.glist, h.nap);
   KF_CHECK(hipMalloc((void **)&k->rc, 4 * (size_t)k->nc * sizeof(double)));
   KF_CHECK(hipMalloc((void **)&k->zc, 4 * (size_t)k->nc * sizeof(double)));
   KF_CHECK(hipMalloc((void **)&k->t, 4 * N * sizeof(double)));
@@ -166,8 +182,17 @@ static kf_coarse *coarse_build(kf_ctx *c)
   return k;
 }
 
-/* After ComputeA, before the fine factorisation overwrites the Jacobian:
- * the Galerkin product into the coarse context and its ILU(0). */
+/* P^T (A P) from A P into a coarse value array */
+template <typename T>
+static void galerkin_from_ap(kf_ctx *c, const kf_coarse *k, const T *ap, T *out)
+{
+  hipLaunchKernelGGL(k_gather_sum<T>, dim3(kf_blocks_for(k->cnnzb, CG_WG / 4)), dim3(CG_WG), 0, c->stream,
+                     (uint32_t)k->cnnzb, k->gptr, k->glist, ap, out);
+  KF_LAUNCH_CHECK();
+}
+
+/* A P from the fill's values (before the fine factorisation overwrites
+ * them), then P^T A P into the coarse context and its ILU(0) */
 void kf_coarse_factor(kf_ctx *c)
 {
   if (!c->coarse) c->coarse = coarse_build(c);
@@ -175,21 +200,39 @@ void kf_coarse_factor(kf_ctx *c)
   kf_ctx *cc = k->cc;
   const bool f64 = c->opt.prec == KF_PREC_FP64;
   cc->opt.prec = f64 ? KF_PREC_FP64 : KF_PREC_FP32;   /* follows the fine level's precision */
+  if (cc->opt.coarse != c->opt.coarse - 1) {   /* the number of levels changed */
+    if (c->opt.coarse == 1) kf_coarse_destroy(cc);
+    cc->opt.coarse = c->opt.coarse - 1;
+  }
+  if (f64 ? k->ap32 != NULL : k->ap64 != NULL) {
+    /* the precision changed: drop the other copy of A P (its last reader is
+     * done once the stream is), so that a stale one is never read */
+    KF_CHECK(hipStreamSynchronize(c->stream));
+    KF_CHECK(hipFree(f64 ? (void *)k->ap32 : (void *)k->ap64));
+    if (f64) k->ap32 = NULL;
+    else k->ap64 = NULL;
+  }
   const int s = kf_prof_begin(c, KP_COARSE_FACTOR);
-  const uint32_t nb = kf_blocks_for(k->cnnzb, CG_WG / 4);
+  const uint32_t nb = kf_blocks_for(k->nap, CG_WG / 4);
   if (f64) {
-    hipLaunchKernelGGL(k_galerkin<double>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
-                       k->glist, (const double *)c->a64, cc->a64);
+    if (!k->ap64) KF_CHECK(hipMalloc((void **)&k->ap64, 16 * k->nap * sizeof(double)));
+    hipLaunchKernelGGL(k_gather_sum<double>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->nap, k->apgptr,
+                       k->apglist, (const double *)c->a64, k->ap64);
+    KF_LAUNCH_CHECK();
+    galerkin_from_ap<double>(c, k, k->ap64, cc->a64);
   } else {
     if (!c->a32) {
       fprintf(stderr, "kf: two-level preconditioner: no single-precision Jacobian (ComputeA must run first)\n");
       abort();
     }
+    if (!k->ap32) KF_CHECK(hipMalloc((void **)&k->ap32, 16 * k->nap * sizeof(float)));
     if (!cc->a32) KF_CHECK(hipMalloc((void **)&cc->a32, 16 * k->cnnzb * sizeof(float)));
-    hipLaunchKernelGGL(k_galerkin<float>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
-                       k->glist, (const float *)c->a32, cc->a32);
+    hipLaunchKernelGGL(k_gather_sum<float>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->nap, k->apgptr,
+                       k->apglist, (const float *)c->a32, k->ap32);
+    KF_LAUNCH_CHECK();
+    galerkin_from_ap<float>(c, k, k->ap32, cc->a32);
   }
-  KF_LAUNCH_CHECK();
+  if (cc->opt.coarse) kf_coarse_factor(cc);   /* the next level, from the Galerkin values */
   kf_launch_ilu(cc);
   kf_prof_end(c, s);
 }
@@ -202,8 +245,10 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
     fprintf(stderr, "kf: two-level preconditioner: ComputeNumericalILU has not run with coarse = 1\n");
     abort();
   }
-  if (!c->jop64 && !c->jop32) {
-    fprintf(stderr, "kf: two-level preconditioner: no kept Jacobian (ComputeA must run with coarse = 1)\n");
+  const bool f64 = c->opt.prec == KF_PREC_FP64;
+  if (f64 ? !k->ap64 : !k->ap32) {
+    fprintf(stderr, "kf: two-level preconditioner: the preconditioner's precision changed since "
+                    "ComputeNumericalILU\n");
     abort();
   }
   kf_check_aligned(b, "b");
@@ -213,14 +258,15 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
   hipLaunchKernelGGL(k_restrict, dim3(kf_blocks_for(k->nc, CG_WG / 4)), dim3(CG_WG), 0, c->stream, k->nc, k->mptr,
                      k->mlist, b, k->rc);
   KF_LAUNCH_CHECK();
-  kf_launch_sptrsv(k->cc, k->rc, k->zc, NULL);
+  if (k->cc->opt.coarse) kf_coarse_apply(k->cc, k->rc, k->zc);   /* the coarse system by the same scheme */
+  else kf_launch_sptrsv(k->cc, k->rc, k->zc, NULL);
   const uint32_t nb = kf_blocks_for(d.N, CG_WG / 4);
-  if (c->jop64)
-    hipLaunchKernelGGL(k_coarse_resid<double>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, d.rowptr, d.col,
-                       (const double *)c->jop64, k->agg, k->zc, b, k->t);
+  if (f64)
+    hipLaunchKernelGGL(k_coarse_resid<double>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+                       (const double *)k->ap64, k->zc, b, k->t);
   else
-    hipLaunchKernelGGL(k_coarse_resid<float>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, d.rowptr, d.col,
-                       (const float *)c->jop32, k->agg, k->zc, b, k->t);
+    hipLaunchKernelGGL(k_coarse_resid<float>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+                       (const float *)k->ap32, k->zc, b, k->t);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   kf_launch_sptrsv(c, k->t, x, NULL);
@@ -233,8 +279,10 @@ void kf_coarse_destroy(kf_ctx *c)
 {
   kf_coarse *k = (kf_coarse *)c->coarse;
   if (!k) return;
+  kf_coarse_destroy(k->cc);
   kf_ctx_destroy_matrix(k->cc);
-  void *ptrs[] = {k->agg, k->mptr, k->mlist, k->gptr, k->glist, k->rc, k->zc, k->t};
+  void *ptrs[] = {k->agg, k->mptr, k->mlist, k->aprp, k->apcol, k->apgptr, k->apglist, k->gptr, k->glist,
+                  k->ap64, k->ap32, k->rc, k->zc, k->t};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
     if (ptrs[i]) KF_CHECK(hipFree(ptrs[i]));
   delete k;
@@ -257,20 +305,18 @@ extern "C" size_t kf_coarse_matrix(const GEOMETRY *g, size_t *cnnzb, unsigned *a
   if (rowptr) KF_CHECK(hipMemcpy(rowptr, k->cc->d.rowptr, ((size_t)k->nc + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (col) KF_CHECK(hipMemcpy(col, k->cc->d.col, k->cnnzb * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (blocks) {
-    /* the Galerkin product again, into the host array (the coarse values
-     * were factored in place) */
+    /* the Galerkin product again from A P (the coarse values were factored
+     * in place), into the host array */
     const size_t n = 16 * k->cnnzb;
     const bool f64 = c->opt.prec == KF_PREC_FP64;
+    if (f64 ? !k->ap64 : !k->ap32) {
+      fprintf(stderr, "kf: kf_coarse_matrix: the preconditioner's precision changed since ComputeNumericalILU\n");
+      abort();
+    }
     void *tmp = NULL;
     KF_CHECK(hipMalloc(&tmp, n * (f64 ? sizeof(double) : sizeof(float))));
-    const uint32_t nb = kf_blocks_for(k->cnnzb, CG_WG / 4);
-    if (f64)
-      hipLaunchKernelGGL(k_galerkin<double>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
-                         k->glist, (const double *)(c->jop64 ? c->jop64 : c->a64), (double *)tmp);
-    else
-      hipLaunchKernelGGL(k_galerkin<float>, dim3(nb), dim3(CG_WG), 0, c->stream, (uint32_t)k->cnnzb, k->gptr,
-                         k->glist, (const float *)c->jop32, (float *)tmp);
-    KF_LAUNCH_CHECK();
+    if (f64) galerkin_from_ap<double>(c, k, k->ap64, (double *)tmp);
+    else galerkin_from_ap<float>(c, k, k->ap32, (float *)tmp);
     KF_CHECK(hipStreamSynchronize(c->stream));
     if (f64) {
       KF_CHECK(hipMemcpy(blocks, tmp, n * sizeof(double), hipMemcpyDeviceToHost));

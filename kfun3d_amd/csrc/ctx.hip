@@ -364,12 +364,12 @@ void kf_check_options(const kf_options *o)
     fprintf(stderr, "kf: jacobi_sweeps must be in [0, %d]\n", KF_JACOBI_MAX);
     abort();
   }
-  if (o->coarse != 0 && o->coarse != 1) {
-    fprintf(stderr, "kf: coarse must be 0 or 1\n");
+  if (o->coarse < 0 || o->coarse > KF_COARSE_MAX) {
+    fprintf(stderr, "kf: coarse must be in [0, %d]\n", KF_COARSE_MAX);
     abort();
   }
   if (o->coarse && o->jacobi_sweeps) {
-    fprintf(stderr, "kf: the two-level preconditioner (coarse = 1) needs exact triangular solves (jacobi_sweeps = 0)\n");
+    fprintf(stderr, "kf: the multilevel preconditioner (coarse > 0) needs exact triangular solves (jacobi_sweeps = 0)\n");
     abort();
   }
 }
@@ -559,9 +559,7 @@ extern "C" void ComputeA(GEOMETRY *g)
 {
   kf_ctx *c = kf_ctx_of(g);
   kf_launch_fill(c);
-  /* the ILU factors the fill's array in place: the BSR operator and the
-   * two-level preconditioner's remainder product need a copy */
-  if (c->opt.bsr_operator || c->opt.coarse) kf_keep_jacobian(c);
+  if (c->opt.bsr_operator) kf_keep_jacobian(c);   /* the ILU factors the fill's array in place */
 }
 
 /* y = A x with the Jacobian of the last ComputeA (kept when bsr_operator is

@@ -4,9 +4,10 @@
  * product P^T A P, and the index lists the device kernels gather with.
  *
  * P is piecewise constant: component c of fine vertex v maps to component c
- * of its aggregate.  So a coarse block (I, J) is the sum of the fine blocks
- * (i, j) with i in aggregate I and j in aggregate J, restriction sums an
- * aggregate's members, and prolongation copies the aggregate's value to
+ * of its aggregate.  So block (i, J) of A P is the sum of the fine blocks
+ * (i, j) of row i with j in aggregate J, a coarse block (I, J) of P^T A P is
+ * the sum of the blocks (i, J) of A P with i in aggregate I, restriction sums
+ * an aggregate's members, and prolongation copies the aggregate's value to
  * each member.  The reference has no counterpart; the fine level is its
  * block ILU(0) (matrix/ilu.cpp), unchanged.
  */
@@ -106,29 +107,65 @@ void kf_build_coarse(const kf_prep *p, kf_coarse_host *out)
     }
   }
 
-  /* the fine blocks of each coarse block, ascending fine position */
-  uint32_t *gpos = (uint32_t *)kf_xmalloc((nnzb + 1) * sizeof(uint32_t));
+  /* A P: per fine row, the aggregates its columns fall in, ascending; each
+   * fine block's slot there; the fine blocks of each A P block ascending */
+  uint32_t *aprp = (uint32_t *)kf_xcalloc(N + 1, sizeof(uint32_t));
+  uint32_t *apcol = (uint32_t *)kf_xmalloc((nnzb + 1) * sizeof(uint32_t));
+  uint32_t *apos = (uint32_t *)kf_xmalloc((nnzb + 1) * sizeof(uint32_t));
+  size_t nap = 0;
   for (size_t i = 0; i < N; i++) {
-    const uint32_t I = agg[i];
+    const size_t start = nap;
     for (uint32_t j = p->rowptr[i]; j < p->rowptr[i + 1]; j++) {
       const uint32_t J = agg[p->col[j]];
+      size_t k = start;
+      while (k < nap && apcol[k] != J) k++;   /* a row holds few aggregates */
+      if (k == nap) apcol[nap++] = J;
+    }
+    qsort(apcol + start, nap - start, sizeof(uint32_t), cmp_u32);
+    aprp[i + 1] = (uint32_t)nap;
+    for (uint32_t j = p->rowptr[i]; j < p->rowptr[i + 1]; j++) {
+      const uint32_t J = agg[p->col[j]];
+      size_t k = start;
+      while (apcol[k] != J) k++;
+      apos[j] = (uint32_t)k;
+    }
+  }
+  uint32_t *apgptr = (uint32_t *)kf_xcalloc(nap + 1, sizeof(uint32_t));
+  uint32_t *apglist = (uint32_t *)kf_xmalloc((nnzb + 1) * sizeof(uint32_t));
+  for (size_t j = 0; j < nnzb; j++) apgptr[apos[j] + 1]++;
+  for (size_t b = 0; b < nap; b++) apgptr[b + 1] += apgptr[b];
+  {
+    uint32_t *fill = (uint32_t *)kf_xmalloc((nap + 1) * sizeof(uint32_t));
+    memcpy(fill, apgptr, (nap + 1) * sizeof(uint32_t));
+    for (size_t j = 0; j < nnzb; j++) apglist[fill[apos[j]]++] = (uint32_t)j;
+    free(fill);
+  }
+  free(apos);
+
+  /* the A P blocks of each coarse block: block (i, J) of A P goes to
+   * (agg[i], J); ascending A P position, i.e. ascending member */
+  uint32_t *gpos = (uint32_t *)kf_xmalloc((nap + 1) * sizeof(uint32_t));
+  for (size_t i = 0; i < N; i++) {
+    const uint32_t I = agg[i];
+    for (uint32_t k = aprp[i]; k < aprp[i + 1]; k++) {
+      const uint32_t J = apcol[k];
       uint32_t lo = crp[I], hi = crp[I + 1];
       while (lo < hi) {
         const uint32_t mid = (lo + hi) / 2;
         if (ccol[mid] < J) lo = mid + 1;
         else hi = mid;
       }
-      gpos[j] = lo;
+      gpos[k] = lo;
     }
   }
   uint32_t *gptr = (uint32_t *)kf_xcalloc(cnnzb + 1, sizeof(uint32_t));
-  uint32_t *glist = (uint32_t *)kf_xmalloc((nnzb + 1) * sizeof(uint32_t));
-  for (size_t j = 0; j < nnzb; j++) gptr[gpos[j] + 1]++;
+  uint32_t *glist = (uint32_t *)kf_xmalloc((nap + 1) * sizeof(uint32_t));
+  for (size_t k = 0; k < nap; k++) gptr[gpos[k] + 1]++;
   for (size_t b = 0; b < cnnzb; b++) gptr[b + 1] += gptr[b];
   {
     uint32_t *fill = (uint32_t *)kf_xmalloc((cnnzb + 1) * sizeof(uint32_t));
     memcpy(fill, gptr, (cnnzb + 1) * sizeof(uint32_t));
-    for (size_t j = 0; j < nnzb; j++) glist[fill[gpos[j]]++] = (uint32_t)j;
+    for (size_t k = 0; k < nap; k++) glist[fill[gpos[k]]++] = (uint32_t)k;
     free(fill);
   }
   free(gpos);
@@ -144,6 +181,11 @@ void kf_build_coarse(const kf_prep *p, kf_coarse_host *out)
   out->agg = agg;
   out->mptr = mptr;
   out->mlist = mlist;
+  out->nap = nap;
+  out->aprp = aprp;
+  out->apcol = apcol;
+  out->apgptr = apgptr;
+  out->apglist = apglist;
   out->gptr = gptr;
   out->glist = glist;
 }
@@ -151,7 +193,9 @@ void kf_build_coarse(const kf_prep *p, kf_coarse_host *out)
 void kf_free_coarse(kf_coarse_host *h)
 {
   kf_free_prep(&h->prep);
-  free(h->agg); free(h->mptr); free(h->mlist); free(h->gptr); free(h->glist);
+  free(h->agg); free(h->mptr); free(h->mlist);
+  free(h->aprp); free(h->apcol); free(h->apgptr); free(h->apglist);
+  free(h->gptr); free(h->glist);
   memset(h, 0, sizeof(*h));
 }
 

@@ -97,13 +97,17 @@ void kf_free_prep(kf_prep *p);
 /* the level sets of a prep whose pattern (N, nnzb, rowptr, col, diag) is set */
 void kf_prep_levels(kf_prep *p);
 
-/* the coarse level of the two-level preconditioner (coarse.c): aggregates,
- * the Galerkin pattern as a prep of its own, and the gather lists */
+/* the coarse level of the two-level preconditioner (aggregate.c):
+ * aggregates, the product A P (fine rows by aggregate columns), the Galerkin
+ * pattern as a prep of its own, and the gather lists of both sums */
 typedef struct kf_coarse_host {
   kf_prep prep;        /* N = aggregates; rowptr, col, diag, nnzb, level sets */
   uint32_t *agg;       /* [N fine] aggregate of each vertex */
   uint32_t *mptr, *mlist;   /* members of each aggregate, ascending */
-  uint32_t *gptr, *glist;   /* fine block positions of each coarse block, ascending */
+  size_t nap;               /* blocks of A P */
+  uint32_t *aprp, *apcol;   /* A P: per fine row, its aggregates ascending */
+  uint32_t *apgptr, *apglist;   /* fine block positions of each A P block, ascending */
+  uint32_t *gptr, *glist;   /* A P positions of each coarse block, ascending (= members ascending) */
 } kf_coarse_host;
 void kf_build_coarse(const kf_prep *p, kf_coarse_host *out);
 void kf_free_coarse(kf_coarse_host *h);

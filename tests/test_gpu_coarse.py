@@ -10,9 +10,15 @@ coarse level adds:
     zc = (Lc Uc)^-1 P^T b with a textbook block ILU(0) of the Galerkin
     matrix (numpy), z = P zc, x = z + (LU)^-1 (b - A z) with the oracle's
     fine factors and solve -- within 1e-9 relative (2-norm) in FP64;
-  * Kernel() with the two-level preconditioner reaches the oracle's steady
-    state (forces within 1e-8 relative, FP64 and FP32 preconditioners) in
-    no more Krylov iterations than block ILU(0) alone.
+  * Kernel() with the two- and three-level preconditioners reaches the
+    oracle's steady state (forces within 1e-8 relative, FP64 and FP32
+    preconditioners) in no more Krylov iterations than block ILU(0) alone;
+  * switching the number of levels on one context rebuilds or drops the
+    deeper levels: a three-level, then a two-level, then a three-level
+    solve on one mesh each repeat their own fresh-context history (with the
+    textbook fine ILU(0): the reference-exact one carries the previous
+    factorisation's L blocks from call to call, as the reference's does, so
+    there a second solve on one context differs from the first).
 """
 import numpy as np
 import pytest
@@ -99,16 +105,17 @@ def test_apply_matches_definition(which, mesh_a, mesh_c1):
         m.close()
 
 
+@pytest.mark.parametrize("levels", [1, 2])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("which", ["mesh_a", "mesh_c1"])
-def test_kernel_two_level(which, prec, mesh_a, mesh_c1):
+def test_kernel_multilevel(which, prec, levels, mesh_a, mesh_c1):
     path = {"mesh_a": mesh_a, "mesh_c1": mesh_c1}[which]
     hm = K.Mesh(path, device=False)
     orc = O.Oracle(hm, **O.config_options())
     ref = orc.kernel()
     orc.close()
     hm.close()
-    m = K.Mesh(path, K.config_options(coarse=1, prec=prec))
+    m = K.Mesh(path, K.config_options(coarse=levels, prec=prec))
     try:
         h = m.kernel()
         assert m.last_error() == 0
@@ -118,5 +125,28 @@ def test_kernel_two_level(which, prec, mesh_a, mesh_c1):
     f, f0 = np.array(h[-1]["forces"]), np.array(ref[-1]["forces"])
     assert np.all(np.abs(f - f0) <= 1e-8 * np.abs(f0)), (f, f0)
     k2, k1 = sum(s["krylov"] for s in h), sum(s["krylov"] for s in ref)
-    print("%s %s: two-level %d pseudo-steps, %d Krylov; ILU(0) %d, %d" % (which, prec, len(h), k2, len(ref), k1))
+    print("%s %s: %d-level %d pseudo-steps, %d Krylov; ILU(0) %d, %d" % (which, prec, levels + 1, len(h), k2, len(ref),
+                                                                         k1))
     assert k2 <= k1
+
+
+def test_levels_switch(mesh_c1):
+    runs = {}
+    for levels in (2, 1):
+        m = K.Mesh(mesh_c1, K.config_options(coarse=levels, ilu_exact=0))
+        try:
+            runs[levels] = m.kernel()
+        finally:
+            m.close()
+    m = K.Mesh(mesh_c1, K.config_options(coarse=2, ilu_exact=0))
+    try:
+        q0 = m.state()
+        for levels in (2, 1, 2):
+            m.set_options(K.config_options(coarse=levels, ilu_exact=0))
+            m.set_state(q0)
+            h = m.kernel()
+            assert [s["krylov"] for s in h] == [s["krylov"] for s in runs[levels]]
+            assert h[-1]["forces"] == runs[levels][-1]["forces"]
+        assert m.last_error() == 0
+    finally:
+        m.close()
