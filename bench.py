@@ -10,10 +10,12 @@ FP64 path's: the candidates keep the residual, Arnoldi, least squares and
 update in FP64 and change only the preconditioner -- its Jacobian fill,
 ILU(0) factors and triangular solves in FP32 (or with the factors stored in
 BF16), Jacobi sweeps instead of the exact triangular solves, or the
-two-level form (a Galerkin coarse correction over 2x2x2 vertex aggregates
-before the fine ILU(0) solve).  The plain FP64 path, the reference's
-algorithm, is timed beside it on the same mesh and context (``paths``,
-``speedup_vs_fp64``), with the force error of each path against it.
+multilevel form (Galerkin coarse corrections over 2x2x2 vertex aggregates,
+applied recursively, before the fine ILU(0) solve; path names
+<prec>[_jacobi<K>][_coarse<L>[j<J>]], see parse_path).  The plain FP64 path,
+the reference's algorithm, is timed beside it on the same mesh and context
+(``paths``, ``speedup_vs_fp64``), with the force error of each path against
+it.
 
 roofline: the kernel class that takes the most time in the timed solves
 (HIP events on the context stream, kf_profile); achieved = its algorithmic
@@ -40,7 +42,7 @@ pseudo-time steps (the Newton residual and 64 seed-3 Gaussian right-hand
 sides each, solved to 1e-10), 2000 Adam steps at learning rate 1e-3.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c3|c2]
-                    [--no-cpu] [--no-mixed] [--no-coarse] [--no-learned] [--no-seq]
+                    [--no-cpu] [--no-mixed] [--no-coarse] [--coarse-paths P,..] [--no-learned] [--no-seq]
                     [--jacobi K,..] [--jacobi-fp32] [--learn-rhs R] [--learn-iters T] [--learn-config c2|c5]
                     [--kernel solve|step|solve1|copy|traffic]
 """
@@ -97,6 +99,18 @@ MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper", "spmv", "jacobi_lower
 TYPE = {"fp64": ("double", "double"), "fp32": ("float", "float"), "bf16": ("unsigned short", "float")}
 
 
+def jacobi_names(dom, prec, sweeps):
+    """the rocprofv3 names of the launches of one Jacobi solve's forward
+    (jacobi_lower) or backward (jacobi_upper) sweeps, in order: the first
+    forward sweep reads b (FP64), the last writes D^-1 y too; the last
+    backward sweep writes FP64"""
+    tm, tv = TYPE[prec]
+    if dom == "jacobi_lower":
+        return ["k_jac_lower<%s, %s, %s, %s>" % (tm, tv, "double" if i == 0 else tv, "true" if i == sweeps - 1 else
+                                                  "false") for i in range(sweeps)]
+    return ["k_jac_upper<%s, %s, %s>" % (tm, tv, "double" if i == sweeps - 1 else tv) for i in range(sweeps)]
+
+
 def kernel_name(dom, prec, held, fold):
     """the rocprofv3 name of the kernel behind a kf_profile class (for the
     forward sweep, the instantiation GMRES's inner steps launch)"""
@@ -130,7 +144,13 @@ def source_hash():
 def measured_traffic(config, name):
     """HBM bytes per launch of one kernel from the committed counter passes
     (profiles/traffic_<config>.json), or None when the profile does not cover
-    this exact kernel built from the current sources."""
+    this exact kernel built from the current sources.  name may be a list of
+    launches (a Jacobi solve's sweeps): their mean."""
+    if isinstance(name, list):
+        got = [measured_traffic(config, x) for x in name]
+        if any(b is None for b, _ in got):
+            return None, "; ".join(src for b, src in got if b is None)
+        return sum(b for b, _ in got) / len(got), got[0][1]
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
     if not os.path.exists(path):
         return None, "no profiles/traffic_%s.json" % config
@@ -186,32 +206,37 @@ def lattice_mesh(K, tmp, config):
 
 
 def parse_path(path):
-    """a path name: <prec>[_jacobi<K>][_coarse[L]] -- the preconditioner's
-    precision (fp64, fp32, bf16), K Jacobi sweeps each way instead of the
-    exact triangular solves, and the multilevel form with L coarse levels
-    (1 when L is omitted: two-level); returns (prec, K, L)"""
+    """a path name: <prec>[_jacobi<K>][_coarse[L][j<J>]] -- the
+    preconditioner's precision (fp64, fp32, bf16), K Jacobi sweeps each way
+    instead of the fine level's exact triangular solves, and the multilevel
+    form with L coarse levels (1 when L is omitted: two-level) whose ILU(0)
+    solves take J Jacobi sweeps each way (exact when omitted); returns
+    (prec, K, L, J)"""
     base, _, lv = path.partition("_coarse")
+    lv, _, cj = lv.partition("j")
     coarse = (int(lv) if lv else 1) if "_coarse" in path else 0
     prec, _, jac = base.partition("_jacobi")
-    return prec, int(jac) if jac else 0, coarse
+    return prec, int(jac) if jac else 0, coarse, int(cj) if cj else 0
 
 
 def path_options(K, path, **kw):
     """configuration-profile options of a path name (parse_path)"""
-    prec, jac, coarse = parse_path(path)
-    return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=coarse, **kw)
+    prec, jac, coarse, cj = parse_path(path)
+    return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=coarse, coarse_sweeps=cj, **kw)
 
 
 def path_precond(path):
-    prec, jac, coarse = parse_path(path)
+    prec, jac, coarse, cj = parse_path(path)
     s = "%s block ILU(0)" % prec
     if jac:
         s += " with %d Jacobi sweeps each way" % jac
     if coarse:
         s += (", %d-level: a Galerkin coarse correction over 2x2x2 vertex aggregates before the fine solve, "
-              "%s, coarse arithmetic in %s" % (coarse + 1, "the coarsest system by block ILU(0)" if coarse == 1 else
-                                               "each coarse system by the same scheme, the coarsest by block ILU(0)",
-                                               "fp64" if prec == "fp64" else "fp32"))
+              "%s, %s, coarse arithmetic in %s"
+              % (coarse + 1, "the coarse system by block ILU(0)" if coarse == 1 else
+                 "each coarse system by the same scheme, the coarsest by block ILU(0)",
+                 "coarse ILU(0) solves by %d Jacobi sweeps each way" % cj if cj else "exact coarse ILU(0) solves",
+                 "fp64" if prec == "fp64" else "fp32"))
     return s
 
 
@@ -240,21 +265,28 @@ def summary(h, t, f_ref=None):
     return out
 
 
-def roofline(prof, prec, sizes, config, held, fold):
-    """the dominant kernel class of a profile and its roofline object"""
+def roofline(prof, path, sizes, config, held, fold):
+    """the dominant kernel class of a profile and its roofline object.  A
+    Jacobi class times all sweeps of one solve as one event pair: its launch
+    average divides by the sweep count, and its kernel is the list of the
+    sweeps' instantiations.  The multilevel preconditioner's coarse chain
+    (class "coarse": many small latency-bound launches) is not a kernel and
+    is not a candidate."""
+    prec, sweeps = parse_path(path)[:2]
     N, E, S, Nf = sizes
     nnzb = N + 2 * E
     dom = max((k for k in prof if k in PROFILE_TO_KIND), key=lambda k: prof[k][0])
     ms, n = prof[dom]
     kind = PROFILE_TO_KIND[dom]
+    per = sweeps if kind in ("jacobi_lower", "jacobi_upper") else 1
     s = {"fp64": 8, "fp32": 4, "bf16": 2}[prec] if kind in MATRIX_KINDS else 8
     ab = alg_bytes(kind, N, E, S, Nf, nnzb, s)
-    achieved = ab / (ms / n / 1e3) / 1e9
-    name = kernel_name(dom, prec, held, fold)
+    achieved = ab / (ms / (n * per) / 1e3) / 1e9
+    name = jacobi_names(dom, prec, sweeps) if per > 1 else kernel_name(dom, prec, held, fold)
     traffic, src = measured_traffic(config, name)
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": ab,
-            "avg_launch_ms": ms / n, "launches": n, "traffic_source": src}
+            "avg_launch_ms": ms / (n * per), "launches": n * per, "traffic_source": src}
 
 
 def spmv_rate(K, m, q0, sizes, config, reps=20):
@@ -514,12 +546,15 @@ def _kernel_key(name):
     return name
 
 
-def _pmc_means(root, counter):
+def _pmc_means(root, counter, finest=False):
     """per kernel: mean value of one counter per dispatch over every
-    *counter_collection.csv under root, and the dispatch count"""
+    *counter_collection.csv under root, and the dispatch count.  finest: only
+    the dispatches with the kernel's largest grid (a multilevel path launches
+    the same kernels on every coarse level; the fine level's grid is the
+    largest)"""
     import csv
     import glob
-    acc = {}
+    acc, grid = {}, {}
     files = glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True)
     if not files:
         raise SystemExit("no counter_collection.csv under %s" % root)
@@ -531,7 +566,15 @@ def _pmc_means(root, counter):
             a = acc.setdefault(k, {})
             d = row.get("Dispatch_Id") or row.get("Correlation_Id")
             a[d] = a.get(d, 0.0) + float(row["Counter_Value"])   # summed over the counter's instances
-    return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
+            grid.setdefault(k, {})[d] = int(row.get("Grid_Size") or row.get("Grid_Size_X") or 0)
+    out = {}
+    for k, v in acc.items():
+        keep = list(v)
+        if finest:
+            top = max(grid[k].values())
+            keep = [d for d in v if grid[k][d] == top]
+        out[k] = (sum(v[d] for d in keep) / len(keep), len(keep))
+    return out
 
 
 def traffic_profile(K, config, root, precs):
@@ -557,7 +600,8 @@ def traffic_profile(K, config, root, precs):
     nnzb = N + 2 * E
     kinds = {"k_grad": "grad", "k_edge_flux": "edge_flux", "k_residual": "residual_gather", "k_fill": "fill",
              "k_ilu_rows": "ilu", "k_trsv_lower": "trsv_lower", "k_trsv_upper": "trsv_upper",
-             "k_bsr_spmv": "spmv", "k_timestep": "timestep"}
+             "k_bsr_spmv": "spmv", "k_timestep": "timestep", "k_jac_lower": "jacobi_lower",
+             "k_jac_upper": "jacobi_upper"}
     out = {"workload": "%dx%dx%d Kuhn lattice seed %d: the first pseudo-time step of the configuration-profile "
                        "solve (bench.py --kernel step), per preconditioner precision" % (CONFIGS[config][0] +
                                                                                          (CONFIGS[config][1],)),
@@ -567,16 +611,19 @@ def traffic_profile(K, config, root, precs):
            "source_hash": source_hash(), "kernels": {}}
     rows = []
     for prec in precs:
-        f = _pmc_means(os.path.join(root, "fetch_%s" % prec), "FETCH_SIZE")
-        w = _pmc_means(os.path.join(root, "write_%s" % prec), "WRITE_SIZE")
-        s = {"fp64": 8, "fp32": 4, "bf16": 2}[prec]
+        multi = parse_path(prec)[2] > 0
+        f = _pmc_means(os.path.join(root, "fetch_%s" % prec), "FETCH_SIZE", finest=multi)
+        w = _pmc_means(os.path.join(root, "write_%s" % prec), "WRITE_SIZE", finest=multi)
+        s = {"fp64": 8, "fp32": 4, "bf16": 2}[parse_path(prec)[0]]
         for k in sorted(set(f) & set(w)):
             base = k.split("<")[0]
             if base not in kinds:
                 continue
             kind = kinds[base]
+            if multi and kind == "ilu":
+                continue   # the fine and the coarse factorisations share the persistent grid: not separable
             ss = s if kind in MATRIX_KINDS else 8
-            if base in ("k_fill", "k_ilu_rows") and prec != "fp64":
+            if base in ("k_fill", "k_ilu_rows") and s != 8:
                 ss = 4   # the fill and the factorisation run in FP32 for both single-precision modes
             ab = alg_bytes(kind, N, E, S, Nf, nnzb, ss)
             b = f[k][0] * 1024.0 * fscale + w[k][0] * 1024.0 * wscale
@@ -630,7 +677,9 @@ def main():
     ap.add_argument("--jacobi", default="3",
                     help="Jacobi sweep counts of the extra BF16-factor paths (comma list; empty: none)")
     ap.add_argument("--jacobi-fp32", action="store_true", help="also FP32-factor paths with the Jacobi sweeps")
-    ap.add_argument("--no-coarse", action="store_true", help="skip the two-level (coarse-corrected) paths")
+    ap.add_argument("--no-coarse", action="store_true", help="skip the multilevel (coarse-corrected) paths")
+    ap.add_argument("--coarse-paths", default="fp64_coarse8,bf16_coarse8,fp64_jacobi2_coarse8j3,bf16_jacobi2_coarse8j3",
+                    help="the multilevel paths timed beside the others (comma list of path names)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
@@ -644,6 +693,8 @@ def main():
                     help="preconditioner precision of the learned leg (fp32: the combined form)")
     ap.add_argument("--traffic-dir", default="gpurun_out/pmc",
                     help="--kernel traffic: the directory holding the counter passes")
+    ap.add_argument("--traffic-paths", default="fp64,bf16,bf16_jacobi2_coarse8j3",
+                    help="--kernel traffic: the paths whose passes it reads (fetch_<path>/, write_<path>/)")
     ap.add_argument("--kernel", default="solve", choices=["solve", "step", "solve1", "copy", "traffic"],
                     help="solve: the benchmark; step / solve1 / copy: short runs for profiles (first pseudo-time "
                          "steps, or whole solves, of one precision on the same mesh; a device-to-device copy of "
@@ -657,7 +708,8 @@ def main():
         return copy_probe(args.steps)
     if args.kernel == "traffic":
         import kfun3d_amd as K
-        return traffic_profile(K, args.config, os.path.join(ROOT, args.traffic_dir), ["fp64", "bf16"])
+        return traffic_profile(K, args.config, os.path.join(ROOT, args.traffic_dir),
+                               [x for x in args.traffic_paths.split(",") if x.strip()])
     import numpy as np
     import kfun3d_amd as K
     tmp = tempfile.mkdtemp(prefix="kfbench_")
@@ -690,7 +742,7 @@ def main():
         for k in [int(x) for x in args.jacobi.split(",") if x.strip()]:
             precs += (["fp32_jacobi%d" % k] if args.jacobi_fp32 else []) + ["bf16_jacobi%d" % k]
         if not args.no_coarse:
-            precs += ["fp64_coarse", "fp32_coarse", "bf16_coarse", "fp32_coarse2", "fp32_coarse3"]
+            precs += [x for x in args.coarse_paths.split(",") if x.strip()]
     warm, warm_err = {"fp64": t64w}, {"fp64": 0.0}
     for p in precs:
         m.set_options(path_options(K, p))
@@ -771,7 +823,7 @@ def main():
         "flux_gflops": flux_flops / flux_s / 1e9, "flux_fp64_frac": flux_flops / flux_s / 1e9 / FP64_PEAK_GFLOPS,
         "kernel_ms": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
         "kernel_share": {k: v[0] / sum(x[0] for x in prof.values()) for k, v in prof.items()},
-        "roofline": roofline(prof, parse_path(vprec)[0], sizes, args.config, held, fold[vprec]),
+        "roofline": roofline(prof, vprec, sizes, args.config, held, fold[vprec]),
         "roofline_fp64": roofline(prof64, "fp64", sizes, args.config, held, fold["fp64"]) if prof64 else None,
         "roofline_spmv": spmv,
     }

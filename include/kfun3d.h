@@ -185,10 +185,14 @@ typedef struct kf_options {
                            the coarse system is preconditioned the same way
                            in turn, L levels below the fine one.  No
                            counterpart in the reference; it cuts the Krylov
-                           count on large meshes.  Needs jacobi_sweeps = 0. */
+                           count on large meshes.  jacobi_sweeps applies to
+                           the fine level's ILU(0) solve. */
+  int    coarse_sweeps; /* coarse > 0: 0 solves every coarse level's ILU(0)
+                           exactly (level-scheduled); k in 1..64 by k Jacobi
+                           sweeps each way (as jacobi_sweeps) */
 } kf_options;
 
-#define KF_COARSE_MAX 4   /* most levels below the fine one (kf_options.coarse) */
+#define KF_COARSE_MAX 8   /* most levels below the fine one (kf_options.coarse) */
 
 typedef struct kf_step_record {
   int    step;          /* pseudo-time step index                             */

@@ -86,7 +86,8 @@ class Options(C.Structure):
                 ("gmres_rtol", C.c_double), ("gmres_max_it", C.c_int), ("first_order", C.c_int),
                 ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int), ("ilu_exact", C.c_int),
                 ("reduce_seq", C.c_int), ("krylov", C.c_int), ("bsr_operator", C.c_int),
-                ("jacobi_sweeps", C.c_int), ("coarse", C.c_int)]
+                ("jacobi_sweeps", C.c_int), ("coarse", C.c_int),
+                ("coarse_sweeps", C.c_int)]
 
 
 class StepRecord(C.Structure):

@@ -7,10 +7,12 @@
  * a coarse level over aggregates of neighbouring vertices (host/aggregate.c):
  *   Ac = P^T A P                (Galerkin; P piecewise constant per component)
  *   z  = P Ac^-1 P^T b          (Ac^-1: block ILU(0) of Ac, the same kernels
- *                                as the fine level, in a matrix-only context;
- *                                with coarse = L > 1, the same scheme again
- *                                on Ac, L levels below the fine one)
- *   x  = z + (LU)^-1 (b - A z)  (the fine ILU(0) solve on the remainder)
+ *                                as the fine level, in a matrix-only context,
+ *                                solved exactly or by coarse_sweeps Jacobi
+ *                                sweeps; with coarse = L > 1, the same scheme
+ *                                again on Ac, L levels below the fine one)
+ *   x  = z + (LU)^-1 (b - A z)  (the fine ILU(0) solve on the remainder,
+ *                                exact or by jacobi_sweeps sweeps)
  * a fixed linear preconditioner, so plain left-preconditioned GMRES keeps
  * working.  A is the fill's Jacobian, read before the fine factorisation
  * overwrites it, and enters only as A P: the fine rows with their blocks
@@ -159,7 +161,7 @@ static kf_coarse *coarse_build(kf_ctx *c)
   kf_options co = c->opt;
   co.prec = c->opt.prec == KF_PREC_FP64 ? KF_PREC_FP64 : KF_PREC_FP32;
   co.ilu_exact = 0;
-  co.jacobi_sweeps = 0;
+  co.jacobi_sweeps = c->opt.coarse_sweeps;   /* the coarse levels' own solves */
   co.coarse = c->opt.coarse - 1;   /* further levels below this one */
   kf_coarse *k = new kf_coarse();
   k->cc = kf_ctx_create_matrix(&h.prep, &co, c);
@@ -200,6 +202,7 @@ void kf_coarse_factor(kf_ctx *c)
   kf_ctx *cc = k->cc;
   const bool f64 = c->opt.prec == KF_PREC_FP64;
   cc->opt.prec = f64 ? KF_PREC_FP64 : KF_PREC_FP32;   /* follows the fine level's precision */
+  cc->opt.jacobi_sweeps = cc->opt.coarse_sweeps = c->opt.coarse_sweeps;
   if (cc->opt.coarse != c->opt.coarse - 1) {   /* the number of levels changed */
     if (c->opt.coarse == 1) kf_coarse_destroy(cc);
     cc->opt.coarse = c->opt.coarse - 1;
@@ -259,6 +262,7 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
                      k->mlist, b, k->rc);
   KF_LAUNCH_CHECK();
   if (k->cc->opt.coarse) kf_coarse_apply(k->cc, k->rc, k->zc);   /* the coarse system by the same scheme */
+  else if (k->cc->opt.jacobi_sweeps > 0) kf_launch_jacobi(k->cc, k->rc, k->zc);
   else kf_launch_sptrsv(k->cc, k->rc, k->zc, NULL);
   const uint32_t nb = kf_blocks_for(d.N, CG_WG / 4);
   if (f64)
@@ -269,7 +273,8 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
                        (const float *)k->ap32, k->zc, b, k->t);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  kf_launch_sptrsv(c, k->t, x, NULL);
+  if (c->opt.jacobi_sweeps > 0) kf_launch_jacobi(c, k->t, x);   /* the fine solve by sweeps */
+  else kf_launch_sptrsv(c, k->t, x, NULL);
   const size_t n4 = 4 * (size_t)d.N;
   hipLaunchKernelGGL(k_prolong_add, dim3(kf_blocks_for(n4, CG_WG)), dim3(CG_WG), 0, c->stream, n4, k->agg, k->zc, x);
   KF_LAUNCH_CHECK();

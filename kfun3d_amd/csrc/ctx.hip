@@ -349,6 +349,7 @@ extern "C" kf_options kf_default_options(void)
   o.bsr_operator = 0;
   o.jacobi_sweeps = 0;
   o.coarse = 0;
+  o.coarse_sweeps = 0;
   return o;
 }
 
@@ -364,12 +365,12 @@ void kf_check_options(const kf_options *o)
     fprintf(stderr, "kf: jacobi_sweeps must be in [0, %d]\n", KF_JACOBI_MAX);
     abort();
   }
-  if (o->coarse < 0 || o->coarse > KF_COARSE_MAX) {
-    fprintf(stderr, "kf: coarse must be in [0, %d]\n", KF_COARSE_MAX);
+  if (o->coarse_sweeps < 0 || o->coarse_sweeps > KF_JACOBI_MAX) {
+    fprintf(stderr, "kf: coarse_sweeps must be in [0, %d]\n", KF_JACOBI_MAX);
     abort();
   }
-  if (o->coarse && o->jacobi_sweeps) {
-    fprintf(stderr, "kf: the multilevel preconditioner (coarse > 0) needs exact triangular solves (jacobi_sweeps = 0)\n");
+  if (o->coarse < 0 || o->coarse > KF_COARSE_MAX) {
+    fprintf(stderr, "kf: coarse must be in [0, %d]\n", KF_COARSE_MAX);
     abort();
   }
 }

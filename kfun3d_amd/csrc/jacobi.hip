@@ -87,26 +87,35 @@ __device__ __forceinline__ double jquad(double v)
 }
 
 /* s = sum over entries [j0, j1) of the row, in ascending column:
- * s = dgemv(A_j, x_col) + s (matrix/sptrsv.cpp:91-96 / 116-121) */
+ * s = dgemv(A_j, x_col) + s (matrix/sptrsv.cpp:91-96 / 116-121).  The first
+ * JAC_HELD entries are loaded in one batch -- column indices, then every
+ * block and vector at once, slots past the row's end reading the diagonal
+ * block jd (always in bounds) -- and summed in order with the empty slots
+ * selected away, so a row's loads overlap instead of chaining; longer rows
+ * finish in a loop. */
+#define JAC_HELD 8
 template <typename TM, typename TV, typename TX>
 __device__ __forceinline__ TV jsum(const uint32_t *__restrict__ col, const TM *__restrict__ a, const TX *__restrict__ x,
-                                   uint32_t j0, uint32_t j1, int c)
+                                   uint32_t j0, uint32_t j1, uint32_t jd, int c)
 {
+  const uint32_t n = j1 - j0;
+  uint32_t jj[JAC_HELD], kk[JAC_HELD];
+#pragma unroll
+  for (int i = 0; i < JAC_HELD; i++) jj[i] = (uint32_t)i < n ? j0 + i : jd;
+#pragma unroll
+  for (int i = 0; i < JAC_HELD; i++) kk[i] = col[jj[i]];
+  TV m[JAC_HELD][4], xv[JAC_HELD][4];
+#pragma unroll
+  for (int i = 0; i < JAC_HELD; i++) jrow<TM, TV>(a, jj[i], c, m[i]);
+#pragma unroll
+  for (int i = 0; i < JAC_HELD; i++) jvec<TV, TX>(x, kk[i], xv[i]);
   TV s = TV(0);
-  uint32_t j = j0;
-  for (; j + 1 < j1; j += 2) {   /* two blocks in flight */
-    const uint32_t k0 = col[j], k1 = col[j + 1];
-    TV m0[4], m1[4], x0[4], x1[4];
-    jrow<TM, TV>(a, j, c, m0);
-    jrow<TM, TV>(a, j + 1, c, m1);
-    jvec<TV, TX>(x, k0, x0);
-    jvec<TV, TX>(x, k1, x1);
-    const TV t0 = jdot(m0, x0);
-    s = t0 + s;
-    const TV t1 = jdot(m1, x1);
-    s = t1 + s;
+#pragma unroll
+  for (int i = 0; i < JAC_HELD; i++) {
+    const TV t = jdot(m[i], xv[i]);
+    s = (uint32_t)i < n ? t + s : s;
   }
-  if (j < j1) {
+  for (uint32_t j = j0 + JAC_HELD; j < j1; j++) {
     TV m0[4], x0[4];
     jrow<TM, TV>(a, j, c, m0);
     jvec<TV, TX>(x, col[j], x0);
@@ -144,7 +153,7 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_lower(uint32_t N, const uint32_t
   const int c = threadIdx.x & 3;
   if (r >= N) return;   /* whole quads leave together (N rows of 4 lanes) */
   const uint32_t jd = diag[r];
-  const TV s = jsum<TM, TV, TX>(col, a, xin, rowptr[r], jd, c);
+  const TV s = jsum<TM, TV, TX>(col, a, xin, rowptr[r], jd, jd, c);
   const TV v = (-s) + (TV)b[4 * (size_t)r + c];
   y[4 * (size_t)r + c] = v;
   if (DIAG) x0[4 * (size_t)r + c] = jdinv<TM, TV>(a, jd, c, v);
@@ -162,7 +171,7 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_upper(uint32_t N, const uint32_t
   const int c = threadIdx.x & 3;
   if (r >= N) return;
   const uint32_t jd = diag[r];
-  const TV s = jsum<TM, TV, TV>(col, a, xin, jd + 1, rowptr[r + 1], c);
+  const TV s = jsum<TM, TV, TV>(col, a, xin, jd + 1, rowptr[r + 1], jd, c);
   const TV u = (-s) + y[4 * (size_t)r + c];
   x[4 * (size_t)r + c] = (TO)jdinv<TM, TV>(a, jd, c, u);
 }
