@@ -289,7 +289,7 @@ void kf_ctx_destroy_matrix(kf_ctx *c)
   void *ptrs[] = {c->d.rowptr, c->d.col, c->d.diag, c->lrows, c->rdone, c->trow_l, c->trow_u, c->tlvl_l,
                   c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff,
                   c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32, c->abf, c->lold,
-                  c->ud, c->f32a, c->f32b, c->jac64, c->jac32};
+                  c->ud, c->f32a, c->f32b, c->jac64, c->jac32, c->jsl_val, c->jsl_col};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   delete c;
 }
@@ -313,7 +313,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->rdone,
                   c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
-                  c->abf, c->jop64, c->jop32, c->ftbuf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->jac64, c->jac32, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
+                  c->abf, c->jop64, c->jop32, c->ftbuf, c->lold, c->ud, c->q, c->cdt, c->gx, c->gy, c->gz, c->r, c->w, c->t, c->xk, c->vecs, c->f32a, c->f32b, c->jac64, c->jac32, c->jsl_val, c->jsl_col, c->eterm, c->sr, c->mgs_part, c->mgs_sync};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   KF_CHECK(hipHostFree(c->hscal));
   if (c->mgs_ev) KF_CHECK(hipEventDestroy(c->mgs_ev));

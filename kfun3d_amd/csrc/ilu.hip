@@ -489,6 +489,7 @@ __global__ __launch_bounds__(256) void k_to_bf16(size_t n, const float *__restri
 
 void kf_launch_ilu(kf_ctx *c)
 {
+  c->jsl_fresh = 0;   /* the Jacobi sweeps' sliced copy is now stale */
   if (c->maxrow > ILU_MAXROW) {
     fprintf(stderr, "kf: ILU supports at most %d blocks per row (mesh has %u)\n", ILU_MAXROW, c->maxrow);
     abort();

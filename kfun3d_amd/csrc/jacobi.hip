@@ -176,11 +176,190 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_upper(uint32_t N, const uint32_t
   x[4 * (size_t)r + c] = (TO)jdinv<TM, TV>(a, jd, c, u);
 }
 
+/* ---- the sliced copy of the factors ------------------------------------------
+ *
+ * The CSR factor array stores a row's blocks side by side, so one load
+ * instruction of a wavefront (16 rows, a quad each) touches 16 blocks a row
+ * apart.  For the sweeps the factors are also packed per slice of 16 rows
+ * (one wavefront) and per entry slot k < K: slot k of the slice's 16 rows
+ * is 16 consecutive blocks (512 bytes in BF16), so that instruction reads
+ * one contiguous run.  Layout, in the factors' type (jsl_val):
+ *   L:  [slice][k][row of slice][16]    the k-th block left of the diagonal
+ *   U:  [slice][k][row of slice][16]    the k-th block right of it
+ *   D:  [slice][row of slice][16]       the inverted diagonal block
+ * and the slots' columns (jsl_col): L then U, [slice][k][row of slice].  A
+ * slot past a row's end holds zeros and the row's own column; the sweeps
+ * select it away (the counts come from rowptr and diag), so the sums are
+ * the CSR kernels' to the bit.  Entries past slot K (rows longer than K,
+ * none on a lattice) are read from the CSR array.  K is the most held
+ * entries of the triangular solves (7, or 8 when a row has more than 7 on a
+ * side).  Packed after each factorisation, on the first sweep that needs
+ * it. */
+template <typename TM, typename TV>
+__device__ __forceinline__ void jld4(const TM *p, TV m[4])
+{
+  m[0] = (TV)jmat(p, 0);
+  m[1] = (TV)jmat(p, 1);
+  m[2] = (TV)jmat(p, 2);
+  m[3] = (TV)jmat(p, 3);
+}
+template <>
+__device__ __forceinline__ void jld4<uint16_t, float>(const uint16_t *p, float m[4])
+{
+  const uint2 v = *reinterpret_cast<const uint2 *>(p);   /* 8-byte aligned: 4 values of a block row */
+  m[0] = __uint_as_float(v.x << 16);
+  m[1] = __uint_as_float(v.x & 0xffff0000u);
+  m[2] = __uint_as_float(v.y << 16);
+  m[3] = __uint_as_float(v.y & 0xffff0000u);
+}
+template <>
+__device__ __forceinline__ void jld4<float, float>(const float *p, float m[4])
+{
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  m[0] = v.x;
+  m[1] = v.y;
+  m[2] = v.z;
+  m[3] = v.w;
+}
+
+/* s over a row's part: slots [0, min(n, K)) from the slice, the rest of the
+ * n entries from the CSR array from position jcsr */
+template <int K, typename TM, typename TV, typename TX>
+__device__ __forceinline__ TV jsum_sl(const TM *__restrict__ sv, const uint32_t *__restrict__ sc, uint32_t base,
+                                      uint32_t q, int c, uint32_t n, const uint32_t *__restrict__ col,
+                                      const TM *__restrict__ a, uint32_t jcsr, const TX *__restrict__ x)
+{
+  uint32_t kk[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) kk[k] = sc[(size_t)(base + k) * 16 + q];
+  TV m[K][4], xv[K][4];
+#pragma unroll
+  for (int k = 0; k < K; k++) jld4<TM, TV>(sv + ((size_t)(base + k) * 16 + q) * 16 + 4 * c, m[k]);
+#pragma unroll
+  for (int k = 0; k < K; k++) jvec<TV, TX>(x, kk[k], xv[k]);
+  TV s = TV(0);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const TV t = jdot(m[k], xv[k]);
+    s = (uint32_t)k < n ? t + s : s;
+  }
+  for (uint32_t k = K; k < n; k++) {
+    TV m0[4], x0[4];
+    jrow<TM, TV>(a, jcsr + k, c, m0);
+    jvec<TV, TX>(x, col[jcsr + k], x0);
+    const TV t0 = jdot(m0, x0);
+    s = t0 + s;
+  }
+  return s;
+}
+
+template <typename TM, typename TV>
+__device__ __forceinline__ TV jdinv_sl(const TM *__restrict__ dv, size_t o, TV u)
+{
+  TV dg[4], uv[4];
+  jld4<TM, TV>(dv + o, dg);
+  uv[0] = jquad<0>(u);
+  uv[1] = jquad<1>(u);
+  uv[2] = jquad<2>(u);
+  uv[3] = jquad<3>(u);
+  return jdot(dg, uv);
+}
+
+/* the sliced layout's parts: nslice slices of K slots */
+struct jsl_view {
+  uint32_t nslice;
+  const void *val;        /* L, U, D */
+  const uint32_t *col;    /* L, U */
+};
+
+template <int K, typename TM, typename TV, typename TX, bool DIAG>
+__global__ __launch_bounds__(JAC_WG) void k_jac_lower_sl(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                         const uint32_t *__restrict__ col,
+                                                         const uint32_t *__restrict__ diag, const TM *__restrict__ a,
+                                                         jsl_view v, const double *__restrict__ b,
+                                                         const TX *__restrict__ xin, TV *__restrict__ y,
+                                                         TV *__restrict__ x0)
+{
+  const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;
+  const uint32_t sl = r >> 4, q = r & 15u, j0 = rowptr[r], jd = diag[r];
+  const TM *sv = static_cast<const TM *>(v.val);
+  const TV s = jsum_sl<K, TM, TV, TX>(sv, v.col, sl * K, q, c, jd - j0, col, a, j0, xin);
+  const TV u = (-s) + (TV)b[4 * (size_t)r + c];
+  y[4 * (size_t)r + c] = u;
+  if (DIAG) {
+    const TM *dv = sv + (size_t)2 * v.nslice * K * 256;
+    x0[4 * (size_t)r + c] = jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
+  }
+}
+
+template <int K, typename TM, typename TV, typename TO>
+__global__ __launch_bounds__(JAC_WG) void k_jac_upper_sl(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                         const uint32_t *__restrict__ col,
+                                                         const uint32_t *__restrict__ diag, const TM *__restrict__ a,
+                                                         jsl_view v, const TV *__restrict__ y,
+                                                         const TV *__restrict__ xin, TO *__restrict__ x)
+{
+  const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;
+  const uint32_t sl = r >> 4, q = r & 15u, jd = diag[r];
+  const TM *sv = static_cast<const TM *>(v.val);
+  const size_t nk = (size_t)v.nslice * K;
+  const TV s = jsum_sl<K, TM, TV, TV>(sv + nk * 256, v.col + nk * 16, sl * K, q, c, rowptr[r + 1] - jd - 1, col, a,
+                                      jd + 1, xin);
+  const TV u = (-s) + y[4 * (size_t)r + c];
+  const TM *dv = sv + 2 * nk * 256;
+  x[4 * (size_t)r + c] = (TO)jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
+}
+
+/* the sliced copy from the CSR factors (a quad per row, lane c copying
+ * block row c) */
+template <int K, typename TM>
+__global__ __launch_bounds__(JAC_WG) void k_jac_pack(uint32_t N, uint32_t nslice, const uint32_t *__restrict__ rowptr,
+                                                     const uint32_t *__restrict__ col,
+                                                     const uint32_t *__restrict__ diag, const TM *__restrict__ a,
+                                                     TM *__restrict__ val, uint32_t *__restrict__ scol)
+{
+  const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;
+  const uint32_t sl = r >> 4, q = r & 15u, j0 = rowptr[r], jd = diag[r], j1 = rowptr[r + 1];
+  const size_t nk = (size_t)nslice * K;
+  for (int part = 0; part < 2; part++) {
+    const uint32_t first = part ? jd + 1 : j0, n = part ? j1 - jd - 1 : jd - j0;
+    TM *pv = val + (size_t)part * nk * 256;
+    uint32_t *pc = scol + (size_t)part * nk * 16;
+    for (int k = 0; k < K; k++) {
+      const size_t slot = (size_t)sl * K + k;
+      TM *o = pv + (slot * 16 + q) * 16 + 4 * c;
+      if ((uint32_t)k < n) {
+        const TM *src = a + 16 * (size_t)(first + k) + 4 * c;
+        o[0] = src[0];
+        o[1] = src[1];
+        o[2] = src[2];
+        o[3] = src[3];
+      } else {
+        o[0] = o[1] = o[2] = o[3] = TM(0);
+      }
+      if (c == 0) pc[slot * 16 + q] = (uint32_t)k < n ? col[first + k] : r;
+    }
+  }
+  TM *o = val + 2 * nk * 256 + ((size_t)sl * 16 + q) * 16 + 4 * c;
+  const TM *src = a + 16 * (size_t)jd + 4 * c;
+  o[0] = src[0];
+  o[1] = src[1];
+  o[2] = src[2];
+  o[3] = src[3];
+}
+
 /* The k sweeps each way.  Work vectors (4N of TV each): y (the forward
  * result, kept for the backward sweeps) and two ping-pong buffers p0, p1.
  * x (FP64) receives the last backward sweep. */
 template <typename TM, typename TV>
-static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV *y, TV *p0, TV *p1, int k)
+static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV *y, TV *p0, TV *p1, int k,
+                         const jsl_view *sl)
 {
   const kf_dv &d = c->d;
   const uint32_t nb = kf_blocks_for(d.N, JAC_WG / 4);
@@ -195,7 +374,36 @@ static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV 
   for (int sw = 1; sw <= k; sw++) {
     const bool last = sw == k;
     TV *out = last ? y : ((sw & 1) ? p0 : p1);
-    if (sw == 1 && last)
+    if (sl) {
+      const jsl_view v = *sl;
+      if (c->jsl_k == 7) {
+        if (sw == 1 && last)
+          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, double, true>), grid, block, 0, c->stream, d.N, d.rowptr,
+                             d.col, d.diag, a, v, b, b, out, cur);
+        else if (sw == 1)
+          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, double, false>), grid, block, 0, c->stream, d.N, d.rowptr,
+                             d.col, d.diag, a, v, b, b, out, (TV *)NULL);
+        else if (last)
+          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, TV, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                             d.diag, a, v, b, prev, out, cur);
+        else
+          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, TV, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                             d.diag, a, v, b, prev, out, (TV *)NULL);
+      } else {
+        if (sw == 1 && last)
+          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, double, true>), grid, block, 0, c->stream, d.N, d.rowptr,
+                             d.col, d.diag, a, v, b, b, out, cur);
+        else if (sw == 1)
+          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, double, false>), grid, block, 0, c->stream, d.N, d.rowptr,
+                             d.col, d.diag, a, v, b, b, out, (TV *)NULL);
+        else if (last)
+          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, TV, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                             d.diag, a, v, b, prev, out, cur);
+        else
+          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, TV, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                             d.diag, a, v, b, prev, out, (TV *)NULL);
+      }
+    } else if (sw == 1 && last)
       hipLaunchKernelGGL((k_jac_lower<TM, TV, double, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
                          a, b, b, out, cur);
     else if (sw == 1)
@@ -216,7 +424,24 @@ static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV 
   TV *other = cur == p0 ? p1 : p0;
   s = kf_prof_begin(c, KP_JAC_U);
   for (int sw = 1; sw <= k; sw++) {
-    if (sw == k) {
+    if (sl && sw == k) {
+      if (c->jsl_k == 7)
+        hipLaunchKernelGGL((k_jac_upper_sl<7, TM, TV, double>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
+                           a, *sl, y, cur, x);
+      else
+        hipLaunchKernelGGL((k_jac_upper_sl<8, TM, TV, double>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
+                           a, *sl, y, cur, x);
+    } else if (sl) {
+      if (c->jsl_k == 7)
+        hipLaunchKernelGGL((k_jac_upper_sl<7, TM, TV, TV>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a,
+                           *sl, y, cur, other);
+      else
+        hipLaunchKernelGGL((k_jac_upper_sl<8, TM, TV, TV>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a,
+                           *sl, y, cur, other);
+      TV *t = cur;
+      cur = other;
+      other = t;
+    } else if (sw == k) {
       hipLaunchKernelGGL((k_jac_upper<TM, TV, double>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a, y,
                          cur, x);
     } else {
@@ -229,6 +454,52 @@ static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV 
     KF_LAUNCH_CHECK();
   }
   kf_prof_end(c, s);
+}
+
+/* the sliced copy of the current factors, packed now unless the last
+ * factorisation's is already there (kf_launch_ilu marks it stale) */
+template <typename TM>
+static void jsl_pack_as(kf_ctx *c, const TM *a, uint32_t nslice)
+{
+  const kf_dv &d = c->d;
+  const dim3 grid(kf_blocks_for(d.N, JAC_WG / 4)), block(JAC_WG);
+  if (c->jsl_k == 7)
+    hipLaunchKernelGGL((k_jac_pack<7, TM>), grid, block, 0, c->stream, d.N, nslice, d.rowptr, d.col, d.diag, a,
+                       (TM *)c->jsl_val, c->jsl_col);
+  else
+    hipLaunchKernelGGL((k_jac_pack<8, TM>), grid, block, 0, c->stream, d.N, nslice, d.rowptr, d.col, d.diag, a,
+                       (TM *)c->jsl_val, c->jsl_col);
+  KF_LAUNCH_CHECK();
+}
+
+static jsl_view jsl_pack(kf_ctx *c)
+{
+  const uint32_t nslice = (c->d.N + 15) / 16;
+  const size_t es = c->opt.prec == KF_PREC_FP64 ? 8 : c->opt.prec == KF_PREC_FP32 ? 4 : 2;
+  c->jsl_k = c->trsv_pf;   /* 7 or 8 */
+  const size_t nk = (size_t)nslice * c->jsl_k;
+  if (c->jsl_val && c->jsl_esize != es) {   /* the precision changed */
+    KF_CHECK(hipStreamSynchronize(c->stream));
+    KF_CHECK(hipFree(c->jsl_val));
+    c->jsl_val = NULL;
+  }
+  if (!c->jsl_val) {
+    KF_CHECK(hipMalloc(&c->jsl_val, (2 * nk + nslice) * 256 * es));
+    c->jsl_esize = es;
+    c->jsl_fresh = 0;
+  }
+  if (!c->jsl_col) KF_CHECK(hipMalloc((void **)&c->jsl_col, 2 * nk * 16 * sizeof(uint32_t)));
+  if (!c->jsl_fresh) {
+    if (es == 8) jsl_pack_as<double>(c, c->a64, nslice);
+    else if (es == 4) jsl_pack_as<float>(c, c->a32, nslice);
+    else jsl_pack_as<uint16_t>(c, c->abf, nslice);
+    c->jsl_fresh = 1;
+  }
+  jsl_view v;
+  v.nslice = nslice;
+  v.val = c->jsl_val;
+  v.col = c->jsl_col;
+  return v;
 }
 
 /* x ~ U^-1 L^-1 b with c->opt.jacobi_sweeps sweeps each way, in the
@@ -253,19 +524,20 @@ void kf_launch_jacobi(kf_ctx *c, const double *b, double *x)
     fprintf(stderr, "kf: Jacobi triangular sweeps: no ILU(0) factors in the context's precision yet\n");
     abort();
   }
+  const jsl_view v = jsl_pack(c);
   if (c->opt.prec == KF_PREC_FP64) {
     if (!c->ty64) KF_CHECK(hipMalloc((void **)&c->ty64, n * sizeof(double)));
     if (!c->jac64) KF_CHECK(hipMalloc((void **)&c->jac64, 2 * n * sizeof(double)));
-    jacobi_solve<double, double>(c, c->a64, b, x, c->ty64, c->jac64, c->jac64 + n, k);
+    jacobi_solve<double, double>(c, c->a64, b, x, c->ty64, c->jac64, c->jac64 + n, k, &v);
     c->ystate[0] = ~(uint64_t)0;   /* y no longer holds the level-scheduled sweep's sentinels */
     return;
   }
   if (!c->f32a) KF_CHECK(hipMalloc((void **)&c->f32a, n * sizeof(float)));
   if (!c->jac32) KF_CHECK(hipMalloc((void **)&c->jac32, 2 * n * sizeof(float)));
   if (c->opt.prec == KF_PREC_FP32)
-    jacobi_solve<float, float>(c, c->a32, b, x, c->f32a, c->jac32, c->jac32 + n, k);
+    jacobi_solve<float, float>(c, c->a32, b, x, c->f32a, c->jac32, c->jac32 + n, k, &v);
   else
-    jacobi_solve<uint16_t, float>(c, c->abf, b, x, c->f32a, c->jac32, c->jac32 + n, k);
+    jacobi_solve<uint16_t, float>(c, c->abf, b, x, c->f32a, c->jac32, c->jac32 + n, k, &v);
   c->ystate[1] = ~(uint64_t)0;
 }
 
@@ -349,13 +621,13 @@ extern "C" int kf_sptrsv_jacobi_noalloc(const GEOMETRY *g, const double *b, doub
   }
   if (c->opt.prec == KF_PREC_FP64) {
     double *y = static_cast<double *>(c->noalloc);
-    jacobi_solve<double, double>(c, c->a64, b, x, y, y + n, y + 2 * n, k);
+    jacobi_solve<double, double>(c, c->a64, b, x, y, y + n, y + 2 * n, k, NULL);
   } else {
     float *y = static_cast<float *>(c->noalloc);
     if (c->opt.prec == KF_PREC_FP32)
-      jacobi_solve<float, float>(c, c->a32, b, x, y, y + n, y + 2 * n, k);
+      jacobi_solve<float, float>(c, c->a32, b, x, y, y + n, y + 2 * n, k, NULL);
     else
-      jacobi_solve<uint16_t, float>(c, c->abf, b, x, y, y + n, y + 2 * n, k);
+      jacobi_solve<uint16_t, float>(c, c->abf, b, x, y, y + n, y + 2 * n, k, NULL);
   }
   return 0;
 }

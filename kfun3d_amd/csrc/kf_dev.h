@@ -160,6 +160,11 @@ struct kf_ctx {
   float *f32a, *f32b;       /* single-precision forward / backward outputs */
   double *jac64;            /* Jacobi sweeps' ping-pong buffers (2 x 4N, FP64 ...) */
   float *jac32;             /* ... or FP32) */
+  void *jsl_val;            /* the factors packed per 16-row slice for the Jacobi sweeps (jacobi.hip) */
+  uint32_t *jsl_col;        /* ... their columns */
+  size_t jsl_esize;         /* bytes per packed value */
+  uint32_t jsl_k;           /* slots per row and side */
+  int jsl_fresh;            /* 1: packed from the current factors */
   double *eterm, *sr;       /* per-edge time-step terms, per-vertex time-step sums */
   /* forces overlapped with the next pseudo-time step (kf_forces_async) */
   hipStream_t fstream;      /* default priority, like the main stream */
