@@ -15,6 +15,9 @@ coarse level adds:
     preconditioners) in no more Krylov iterations than block ILU(0) alone,
     and so does a four-level one whose fine and coarse ILU(0) solves are
     Jacobi sweeps (2 fine, 4 coarse);
+  * on the 144x48x48 lattice, exact FP64 levels and the BF16 path with
+    Jacobi sweeps (9 levels) reach the plain FP64 path's forces within 1e-8
+    in under half its Krylov iterations;
   * switching the number of levels on one context rebuilds or drops the
     deeper levels: a three-level, then a two-level, then a three-level
     solve on one mesh each repeat their own fresh-context history (with the
@@ -152,5 +155,35 @@ def test_levels_switch(mesh_c1):
             assert [s["krylov"] for s in h] == [s["krylov"] for s in runs[levels]]
             assert h[-1]["forces"] == runs[levels][-1]["forces"]
         assert m.last_error() == 0
+    finally:
+        m.close()
+
+
+@pytest.mark.slow
+def test_multilevel_at_144x48x48(mesh_dir):
+    """The 0.33M-vertex lattice (seed 1): the multilevel paths -- exact FP64
+    levels, and BF16 factors with 2 fine and 3 coarse Jacobi sweeps over 9
+    levels -- reach the plain FP64 path's forces within 1e-8 in under half
+    its Krylov iterations (the FP64 path is the reference's algorithm, held
+    to the oracle by the parity tests)."""
+    from conftest import lattice
+    path = lattice(mesh_dir, 144, 48, 48, 1)
+    m = K.Mesh(path, K.config_options())
+    try:
+        q0 = m.state()
+        ref = m.kernel()
+        f0 = np.array(ref[-1]["forces"])
+        k1 = sum(s["krylov"] for s in ref)
+        for opts in (dict(coarse=8), dict(prec="bf16", coarse=8, jacobi_sweeps=2, coarse_sweeps=3)):
+            m.set_options(K.config_options(**opts))
+            m.set_state(q0)
+            h = m.kernel()
+            assert m.last_error() == 0
+            assert h[0]["fnorm"] / h[-1]["fnorm"] >= 1e10
+            f = np.array(h[-1]["forces"])
+            assert np.all(np.abs(f - f0) <= 1e-8 * np.abs(f0)), (opts, f, f0)
+            k2 = sum(s["krylov"] for s in h)
+            print("144x48x48 %s: %d Krylov against %d" % (opts, k2, k1))
+            assert 2 * k2 < k1
     finally:
         m.close()
