@@ -99,16 +99,18 @@ MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper", "spmv", "jacobi_lower
 TYPE = {"fp64": ("double", "double"), "fp32": ("float", "float"), "bf16": ("unsigned short", "float")}
 
 
-def jacobi_names(dom, prec, sweeps):
+def jacobi_names(dom, prec, sweeps, held):
     """the rocprofv3 names of the launches of one Jacobi solve's forward
-    (jacobi_lower) or backward (jacobi_upper) sweeps, in order: the first
-    forward sweep reads b (FP64), the last writes D^-1 y too; the last
-    backward sweep writes FP64"""
+    (jacobi_lower) or backward (jacobi_upper) sweeps over the sliced factors,
+    in order: the first forward sweep reads b (FP64), the last writes D^-1 y
+    too; the last backward sweep writes FP64"""
     tm, tv = TYPE[prec]
     if dom == "jacobi_lower":
-        return ["k_jac_lower<%s, %s, %s, %s>" % (tm, tv, "double" if i == 0 else tv, "true" if i == sweeps - 1 else
-                                                  "false") for i in range(sweeps)]
-    return ["k_jac_upper<%s, %s, %s>" % (tm, tv, "double" if i == sweeps - 1 else tv) for i in range(sweeps)]
+        return ["k_jac_lower_sl<%d, %s, %s, %s, %s>" % (held, tm, tv, "double" if i == 0 else tv,
+                                                         "true" if i == sweeps - 1 else "false")
+                for i in range(sweeps)]
+    return ["k_jac_upper_sl<%d, %s, %s, %s>" % (held, tm, tv, "double" if i == sweeps - 1 else tv)
+            for i in range(sweeps)]
 
 
 def kernel_name(dom, prec, held, fold):
@@ -125,10 +127,10 @@ def kernel_name(dom, prec, held, fold):
         return "k_fill<%s>" % ("double" if prec == "fp64" else "float")
     if dom == "spmv":
         return "k_bsr_spmv<%s>" % ("double" if prec == "fp64" else "float")
-    if dom == "jacobi_lower":   # the inner sweeps' instantiation (the class also holds the first and last)
-        return "k_jac_lower<%s, %s, %s, false>" % (tm, tv, tv)
+    if dom == "jacobi_lower":   # one sweep: reads b, writes D^-1 y too
+        return "k_jac_lower_sl<%d, %s, %s, double, true>" % (held, tm, tv)
     if dom == "jacobi_upper":
-        return "k_jac_upper<%s, %s, %s>" % (tm, tv, tv)
+        return "k_jac_upper_sl<%d, %s, %s, double>" % (held, tm, tv)
     return {"grad": "k_grad", "edge_flux": "k_edge_flux<false>", "residual": "k_residual<false>",
             "timestep": "k_timestep"}[dom]
 
@@ -282,7 +284,7 @@ def roofline(prof, path, sizes, config, held, fold):
     s = {"fp64": 8, "fp32": 4, "bf16": 2}[prec] if kind in MATRIX_KINDS else 8
     ab = alg_bytes(kind, N, E, S, Nf, nnzb, s)
     achieved = ab / (ms / (n * per) / 1e3) / 1e9
-    name = jacobi_names(dom, prec, sweeps) if per > 1 else kernel_name(dom, prec, held, fold)
+    name = jacobi_names(dom, prec, sweeps, held) if per > 1 else kernel_name(dom, prec, held, fold)
     traffic, src = measured_traffic(config, name)
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": ab,
@@ -600,8 +602,8 @@ def traffic_profile(K, config, root, precs):
     nnzb = N + 2 * E
     kinds = {"k_grad": "grad", "k_edge_flux": "edge_flux", "k_residual": "residual_gather", "k_fill": "fill",
              "k_ilu_rows": "ilu", "k_trsv_lower": "trsv_lower", "k_trsv_upper": "trsv_upper",
-             "k_bsr_spmv": "spmv", "k_timestep": "timestep", "k_jac_lower": "jacobi_lower",
-             "k_jac_upper": "jacobi_upper"}
+             "k_bsr_spmv": "spmv", "k_timestep": "timestep", "k_jac_lower_sl": "jacobi_lower",
+             "k_jac_upper_sl": "jacobi_upper"}
     out = {"workload": "%dx%dx%d Kuhn lattice seed %d: the first pseudo-time step of the configuration-profile "
                        "solve (bench.py --kernel step), per preconditioner precision" % (CONFIGS[config][0] +
                                                                                          (CONFIGS[config][1],)),
@@ -622,6 +624,8 @@ def traffic_profile(K, config, root, precs):
             kind = kinds[base]
             if multi and kind == "ilu":
                 continue   # the fine and the coarse factorisations share the persistent grid: not separable
+            if multi and kind in ("jacobi_lower", "jacobi_upper") and ", %s," % TYPE[parse_path(prec)[0]][0] not in k:
+                continue   # a coarse level's sweeps (their factors are FP32 or FP64, not the fine type)
             ss = s if kind in MATRIX_KINDS else 8
             if base in ("k_fill", "k_ilu_rows") and s != 8:
                 ss = 4   # the fill and the factorisation run in FP32 for both single-precision modes
