@@ -208,12 +208,15 @@ def lattice_mesh(K, tmp, config):
 
 
 def parse_path(path):
-    """a path name: <prec>[_jacobi<K>][_coarse[L][j<J>]] -- the
+    """a path name: <prec>[_jacobi<K>][_coarse[L][j<J>]][_lag] -- the
     preconditioner's precision (fp64, fp32, bf16), K Jacobi sweeps each way
-    instead of the fine level's exact triangular solves, and the multilevel
-    form with L coarse levels (1 when L is omitted: two-level) whose ILU(0)
-    solves take J Jacobi sweeps each way (exact when omitted); returns
-    (prec, K, L, J)"""
+    instead of the fine level's exact triangular solves, the multilevel form
+    with L coarse levels (1 when L is omitted: two-level) whose ILU(0)
+    solves take J Jacobi sweeps each way (exact when omitted), and _lag: the
+    preconditioner kept across pseudo-steps at the capped CFL
+    (kf_options.lag_factors); returns (prec, K, L, J) -- path_lag() reads
+    the last"""
+    path = path[:-len("_lag")] if path.endswith("_lag") else path
     base, _, lv = path.partition("_coarse")
     lv, _, cj = lv.partition("j")
     coarse = (int(lv) if lv else 1) if "_coarse" in path else 0
@@ -221,10 +224,15 @@ def parse_path(path):
     return prec, int(jac) if jac else 0, coarse, int(cj) if cj else 0
 
 
+def path_lag(path):
+    return path.endswith("_lag")
+
+
 def path_options(K, path, **kw):
     """configuration-profile options of a path name (parse_path)"""
     prec, jac, coarse, cj = parse_path(path)
-    return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=coarse, coarse_sweeps=cj, **kw)
+    return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=coarse, coarse_sweeps=cj,
+                            lag_factors=int(path_lag(path)), **kw)
 
 
 def path_precond(path):
@@ -239,6 +247,8 @@ def path_precond(path):
                  "each coarse system by the same scheme, the coarsest by block ILU(0)",
                  "coarse ILU(0) solves by %d Jacobi sweeps each way" % cj if cj else "exact coarse ILU(0) solves",
                  "fp64" if prec == "fp64" else "fp32"))
+    if path_lag(path):
+        s += ", kept across pseudo-steps once the CFL ramp reaches its cap"
     return s
 
 
@@ -682,7 +692,8 @@ def main():
                     help="Jacobi sweep counts of the extra BF16-factor paths (comma list; empty: none)")
     ap.add_argument("--jacobi-fp32", action="store_true", help="also FP32-factor paths with the Jacobi sweeps")
     ap.add_argument("--no-coarse", action="store_true", help="skip the multilevel (coarse-corrected) paths")
-    ap.add_argument("--coarse-paths", default="fp64_coarse8,bf16_coarse8,fp64_jacobi2_coarse8j3,bf16_jacobi2_coarse8j3",
+    ap.add_argument("--coarse-paths", default="fp64_coarse8,bf16_coarse8,fp64_jacobi2_coarse8j3,bf16_jacobi2_coarse8j3,"
+                                              "bf16_jacobi2_coarse8j3_lag",
                     help="the multilevel paths timed beside the others (comma list of path names)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")

@@ -87,7 +87,7 @@ class Options(C.Structure):
                 ("prec", C.c_int), ("ortho", C.c_int), ("verbose", C.c_int), ("ilu_exact", C.c_int),
                 ("reduce_seq", C.c_int), ("krylov", C.c_int), ("bsr_operator", C.c_int),
                 ("jacobi_sweeps", C.c_int), ("coarse", C.c_int),
-                ("coarse_sweeps", C.c_int)]
+                ("coarse_sweeps", C.c_int), ("lag_factors", C.c_int)]
 
 
 class StepRecord(C.Structure):

@@ -350,6 +350,7 @@ extern "C" kf_options kf_default_options(void)
   o.jacobi_sweeps = 0;
   o.coarse = 0;
   o.coarse_sweeps = 0;
+  o.lag_factors = 0;
   return o;
 }
 
@@ -363,6 +364,10 @@ void kf_check_options(const kf_options *o)
   }
   if (o->jacobi_sweeps < 0 || o->jacobi_sweeps > KF_JACOBI_MAX) {
     fprintf(stderr, "kf: jacobi_sweeps must be in [0, %d]\n", KF_JACOBI_MAX);
+    abort();
+  }
+  if (o->lag_factors != 0 && o->lag_factors != 1) {
+    fprintf(stderr, "kf: lag_factors must be 0 or 1\n");
     abort();
   }
   if (o->coarse_sweeps < 0 || o->coarse_sweeps > KF_JACOBI_MAX) {

@@ -462,7 +462,7 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
   }
   step_pending pend;
   memset(&pend, 0, sizeof(pend));
-  double fnorm_init = 0.0, cfl = o.cfl_init;
+  double fnorm_init = 0.0, cfl = o.cfl_init, cfl_factored = -1.0;
   int its = 0, nrec = 0;
   while (its < o.ts_max) {
     /* the residual also leaves the time-step sums of this state */
@@ -486,11 +486,17 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
       cfl = (cfl0 < o.cfl_max) ? cfl0 : o.cfl_max;
     }
     kfs_timestep_spectral(g, cfl); /* = ComputeTimeStep(cfl, g) on the same state */
-    ComputeA(g);
+    /* lag_factors: at a repeated CFL (the ramp at its cap) the previous
+     * step's preconditioner is kept; the operator stays exact */
+    const int lag = o.lag_factors && its > 0 && cfl == cfl_factored && !L;
+    if (!lag) ComputeA(g);
     if (L && its < L->lo.collect_steps) kf_mpnn_take_matrix(L->net, 1 + its); /* the loss metric */
     kfs_nrm2(g, sz, g->q->q, KFS_SLOT_QNORM);
     K->qnorm = sqrt(1.0 + fetch1(g, KFS_SLOT_QNORM));
-    ComputeNumericalILU(g);
+    if (!lag) {
+      ComputeNumericalILU(g);
+      cfl_factored = cfl;
+    }
     int total, learned = 0, k_ilu = -1, k_net = -1;
     if (L && L->enabled) {
       kf_mpnn_take_diag(L->net, 0);
