@@ -208,15 +208,16 @@ def lattice_mesh(K, tmp, config):
 
 
 def parse_path(path):
-    """a path name: <prec>[_jacobi<K>][_coarse[L][j<J>]][_lag] -- the
+    """a path name: <prec>[_jacobi<K>][_coarse[L][j<J>]][_lag[R]] -- the
     preconditioner's precision (fp64, fp32, bf16), K Jacobi sweeps each way
     instead of the fine level's exact triangular solves, the multilevel form
     with L coarse levels (1 when L is omitted: two-level) whose ILU(0)
-    solves take J Jacobi sweeps each way (exact when omitted), and _lag: the
-    preconditioner kept across pseudo-steps at the capped CFL
-    (kf_options.lag_factors); returns (prec, K, L, J) -- path_lag() reads
-    the last"""
-    path = path[:-len("_lag")] if path.endswith("_lag") else path
+    solves take J Jacobi sweeps each way (exact when omitted), and _lag[R]:
+    the preconditioner kept across pseudo-steps while the CFL stays below R
+    times the CFL it was factored at (kf_options.lag_factors = R, 1 when
+    omitted: only while unchanged); returns (prec, K, L, J) -- path_lag()
+    reads R"""
+    path = path.rpartition("_lag")[0] if path_lag(path) else path
     base, _, lv = path.partition("_coarse")
     lv, _, cj = lv.partition("j")
     coarse = (int(lv) if lv else 1) if "_coarse" in path else 0
@@ -225,14 +226,18 @@ def parse_path(path):
 
 
 def path_lag(path):
-    return path.endswith("_lag")
+    """R of a _lag[R] suffix, 0 without one"""
+    head, sep, tail = path.rpartition("_lag")
+    if not sep or (tail and not tail.isdigit()):
+        return 0
+    return int(tail) if tail else 1
 
 
 def path_options(K, path, **kw):
     """configuration-profile options of a path name (parse_path)"""
     prec, jac, coarse, cj = parse_path(path)
     return K.config_options(prec=prec, jacobi_sweeps=jac, coarse=coarse, coarse_sweeps=cj,
-                            lag_factors=int(path_lag(path)), **kw)
+                            lag_factors=path_lag(path), **kw)
 
 
 def path_precond(path):
@@ -247,8 +252,10 @@ def path_precond(path):
                  "each coarse system by the same scheme, the coarsest by block ILU(0)",
                  "coarse ILU(0) solves by %d Jacobi sweeps each way" % cj if cj else "exact coarse ILU(0) solves",
                  "fp64" if prec == "fp64" else "fp32"))
-    if path_lag(path):
+    if path_lag(path) == 1:
         s += ", kept across pseudo-steps once the CFL ramp reaches its cap"
+    elif path_lag(path):
+        s += ", kept across pseudo-steps while the CFL stays below %d times the CFL it was formed at" % path_lag(path)
     return s
 
 
@@ -693,7 +700,7 @@ def main():
     ap.add_argument("--jacobi-fp32", action="store_true", help="also FP32-factor paths with the Jacobi sweeps")
     ap.add_argument("--no-coarse", action="store_true", help="skip the multilevel (coarse-corrected) paths")
     ap.add_argument("--coarse-paths", default="fp64_coarse8,bf16_coarse8,fp64_jacobi2_coarse8j3,bf16_jacobi2_coarse8j3,"
-                                              "bf16_jacobi2_coarse8j3_lag",
+                                              "bf16_jacobi2_coarse8j3_lag4",
                     help="the multilevel paths timed beside the others (comma list of path names)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only")

@@ -192,12 +192,14 @@ typedef struct kf_options {
                            sweeps each way (as jacobi_sweeps) */
   int    lag_factors;   /* Kernel(): 0 forms the Jacobian and factors it at
                            every pseudo-time step (src/kernel.c:91-94, the
-                           reference); 1 keeps the previous step's
-                           preconditioner on a step whose CFL equals the
-                           previous step's (the ramp has reached cfl_max).
-                           The Krylov operator stays the exact matrix-free
-                           product; only the preconditioner lags.  No
-                           counterpart in the reference. */
+                           reference); k >= 1 keeps the last factorisation
+                           while the step's CFL is at least the CFL it was
+                           formed at and below k times it (k = 1: while the
+                           CFL is unchanged, the ramp at cfl_max); a lower
+                           CFL refactors.  The Krylov operator stays the
+                           exact matrix-free product; only the
+                           preconditioner lags.  No counterpart in the
+                           reference. */
 } kf_options;
 
 #define KF_COARSE_MAX 8   /* most levels below the fine one (kf_options.coarse) */

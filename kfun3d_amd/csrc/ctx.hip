@@ -366,8 +366,8 @@ void kf_check_options(const kf_options *o)
     fprintf(stderr, "kf: jacobi_sweeps must be in [0, %d]\n", KF_JACOBI_MAX);
     abort();
   }
-  if (o->lag_factors != 0 && o->lag_factors != 1) {
-    fprintf(stderr, "kf: lag_factors must be 0 or 1\n");
+  if (o->lag_factors < 0 || o->lag_factors > 1000) {
+    fprintf(stderr, "kf: lag_factors must be in [0, 1000]\n");
     abort();
   }
   if (o->coarse_sweeps < 0 || o->coarse_sweeps > KF_JACOBI_MAX) {

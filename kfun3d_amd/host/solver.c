@@ -486,9 +486,11 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
       cfl = (cfl0 < o.cfl_max) ? cfl0 : o.cfl_max;
     }
     kfs_timestep_spectral(g, cfl); /* = ComputeTimeStep(cfl, g) on the same state */
-    /* lag_factors: at a repeated CFL (the ramp at its cap) the previous
-     * step's preconditioner is kept; the operator stays exact */
-    const int lag = o.lag_factors && its > 0 && cfl == cfl_factored && !L;
+    /* lag_factors = k: the previous factorisation is kept while the CFL
+     * stays in [its CFL, k times it) -- k = 1: while unchanged (the ramp at
+     * its cap); the operator stays exact */
+    const int lag = o.lag_factors > 0 && its > 0 && !L && cfl >= cfl_factored &&
+                    (cfl == cfl_factored || cfl < o.lag_factors * cfl_factored);
     if (!lag) ComputeA(g);
     if (L && its < L->lo.collect_steps) kf_mpnn_take_matrix(L->net, 1 + its); /* the loss metric */
     kfs_nrm2(g, sz, g->q->q, KFS_SLOT_QNORM);
