@@ -198,8 +198,10 @@ typedef struct kf_options {
                            CFL is unchanged, the ramp at cfl_max); a lower
                            CFL refactors.  The Krylov operator stays the
                            exact matrix-free product; only the
-                           preconditioner lags.  No counterpart in the
-                           reference. */
+                           preconditioner lags.  Ignored with bsr_operator
+                           (whose operator is the fill itself) and with a
+                           learned correction attached.  No counterpart in
+                           the reference. */
 } kf_options;
 
 #define KF_COARSE_MAX 8   /* most levels below the fine one (kf_options.coarse) */

@@ -488,8 +488,9 @@ int kf_kernel(GEOMETRY *g, kf_step_record *rec, int max_rec)
     kfs_timestep_spectral(g, cfl); /* = ComputeTimeStep(cfl, g) on the same state */
     /* lag_factors = k: the previous factorisation is kept while the CFL
      * stays in [its CFL, k times it) -- k = 1: while unchanged (the ramp at
-     * its cap); the operator stays exact */
-    const int lag = o.lag_factors > 0 && its > 0 && !L && cfl >= cfl_factored &&
+     * its cap); the operator stays exact.  Not with the BSR operator, whose
+     * operator is the fill itself. */
+    const int lag = o.lag_factors > 0 && its > 0 && !L && !o.bsr_operator && cfl >= cfl_factored &&
                     (cfl == cfl_factored || cfl < o.lag_factors * cfl_factored);
     if (!lag) ComputeA(g);
     if (L && its < L->lo.collect_steps) kf_mpnn_take_matrix(L->net, 1 + its); /* the loss metric */
