@@ -99,15 +99,17 @@ MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper", "spmv", "jacobi_lower
 TYPE = {"fp64": ("double", "double"), "fp32": ("float", "float"), "bf16": ("unsigned short", "float")}
 
 
-def jacobi_names(dom, prec, sweeps, held):
+def jacobi_names(dom, prec, sweeps, held, multilevel=False):
     """the rocprofv3 names of the launches of one Jacobi solve's forward
     (jacobi_lower) or backward (jacobi_upper) sweeps over the sliced factors,
-    in order: the first forward sweep reads b (FP64), the last writes D^-1 y
-    too; the last backward sweep writes FP64"""
+    in order: the first forward sweep reads b, FP64 (or, as the fine solve of
+    the multilevel preconditioner, in the sweeps' own type), the last writes
+    D^-1 y too; the last backward sweep writes FP64"""
     tm, tv = TYPE[prec]
+    tb = tv if multilevel else "double"
     if dom == "jacobi_lower":
-        return ["k_jac_lower_sl<%d, %s, %s, %s, %s>" % (held, tm, tv, "double" if i == 0 else tv,
-                                                         "true" if i == sweeps - 1 else "false")
+        return ["k_jac_lower_sl<%d, %s, %s, %s, %s, %s>" % (held, tm, tv, tb, tb if i == 0 else tv,
+                                                             "true" if i == sweeps - 1 else "false")
                 for i in range(sweeps)]
     return ["k_jac_upper_sl<%d, %s, %s, %s>" % (held, tm, tv, "double" if i == sweeps - 1 else tv)
             for i in range(sweeps)]
@@ -128,7 +130,7 @@ def kernel_name(dom, prec, held, fold):
     if dom == "spmv":
         return "k_bsr_spmv<%s>" % ("double" if prec == "fp64" else "float")
     if dom == "jacobi_lower":   # one sweep: reads b, writes D^-1 y too
-        return "k_jac_lower_sl<%d, %s, %s, double, true>" % (held, tm, tv)
+        return "k_jac_lower_sl<%d, %s, %s, double, double, true>" % (held, tm, tv)
     if dom == "jacobi_upper":
         return "k_jac_upper_sl<%d, %s, %s, double>" % (held, tm, tv)
     return {"grad": "k_grad", "edge_flux": "k_edge_flux<false>", "residual": "k_residual<false>",
@@ -301,7 +303,8 @@ def roofline(prof, path, sizes, config, held, fold):
     s = {"fp64": 8, "fp32": 4, "bf16": 2}[prec] if kind in MATRIX_KINDS else 8
     ab = alg_bytes(kind, N, E, S, Nf, nnzb, s)
     achieved = ab / (ms / (n * per) / 1e3) / 1e9
-    name = jacobi_names(dom, prec, sweeps, held) if per > 1 else kernel_name(dom, prec, held, fold)
+    name = jacobi_names(dom, prec, sweeps, held, parse_path(path)[2] > 0) if per > 1 else \
+        kernel_name(dom, prec, held, fold)
     traffic, src = measured_traffic(config, name)
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": ab,

@@ -27,8 +27,8 @@
  *   k_gather_sum   a block = the sum of listed blocks in ascending position:
  *                  A P from the fine blocks, then P^T A P from A P
  *   k_restrict     coarse vector = sum of the aggregate's members, ascending
- *   k_coarse_resid t = b - (A P) z_c
- *   k_prolong_add  x += P z_c
+ *   k_coarse_resid t = b - (A P) z_c (FP32 where the fine solve is FP32 sweeps)
+ *   k_prolong_add  x += P z_c (with Jacobi sweeps: inside their last sweep)
  * Every sum runs in a fixed order, so the preconditioner is deterministic.
  */
 #include <cstring>
@@ -50,6 +50,7 @@ struct kf_coarse {
   float *ap32;         /* ... or FP32 */
   double *rc, *zc;     /* coarse right-hand side and solution, 4 nc */
   double *t;           /* the fine remainder b - A z, 4 N */
+  float *t32;          /* ... in FP32, where the fine solve is FP32 sweeps */
 };
 
 template <typename T>
@@ -107,12 +108,13 @@ __device__ __forceinline__ void cg_row4(const T *a, size_t j, int c, double &a0,
 }
 
 /* t = b - (A P) z_c; per row the A P blocks in ascending aggregate, each
- * component summed a[c][0] z0 + ... + a[c][3] z3 in FP64 */
-template <typename T>
+ * component summed a[c][0] z0 + ... + a[c][3] z3 in FP64; stored as TT
+ * (float where the fine sweeps round it to FP32 on reading anyway) */
+template <typename T, typename TT>
 __global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32_t *__restrict__ aprp,
                                                         const uint32_t *__restrict__ apcol, const T *__restrict__ ap,
                                                         const double *__restrict__ zc,
-                                                        const double *__restrict__ b, double *__restrict__ t)
+                                                        const double *__restrict__ b, TT *__restrict__ t)
 {
   const uint32_t r = kf_xcd_block() * (CG_WG / 4) + (threadIdx.x >> 2);
   const int c = threadIdx.x & 3;
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32
     acc = acc + a3 * z[3];
   }
   const size_t o = 4 * (size_t)r + c;
-  t[o] = b[o] - acc;
+  t[o] = (TT)(b[o] - acc);
 }
 
 __global__ __launch_bounds__(CG_WG) void k_prolong_add(size_t n4, const uint32_t *__restrict__ agg,
@@ -265,17 +267,27 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
   else if (k->cc->opt.jacobi_sweeps > 0) kf_launch_jacobi(k->cc, k->rc, k->zc);
   else kf_launch_sptrsv(k->cc, k->rc, k->zc, NULL);
   const uint32_t nb = kf_blocks_for(d.N, CG_WG / 4);
-  if (f64)
-    hipLaunchKernelGGL(k_coarse_resid<double>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+  const size_t n4 = 4 * (size_t)d.N;
+  const bool sweeps = c->opt.jacobi_sweeps > 0;
+  if (f64) {
+    hipLaunchKernelGGL((k_coarse_resid<double, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
                        (const double *)k->ap64, k->zc, b, k->t);
-  else
-    hipLaunchKernelGGL(k_coarse_resid<float>, dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+  } else if (sweeps) {
+    if (!k->t32) KF_CHECK(hipMalloc((void **)&k->t32, n4 * sizeof(float)));
+    hipLaunchKernelGGL((k_coarse_resid<float, float>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+                       (const float *)k->ap32, k->zc, b, k->t32);
+  } else {
+    hipLaunchKernelGGL((k_coarse_resid<float, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
                        (const float *)k->ap32, k->zc, b, k->t);
+  }
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
-  if (c->opt.jacobi_sweeps > 0) kf_launch_jacobi(c, k->t, x);   /* the fine solve by sweeps */
-  else kf_launch_sptrsv(c, k->t, x, NULL);
-  const size_t n4 = 4 * (size_t)d.N;
+  if (sweeps) {
+    /* the fine solve by sweeps, x += P z_c in its last sweep */
+    kf_launch_jacobi_ml(c, f64 ? (const void *)k->t : (const void *)k->t32, x, k->agg, k->zc);
+    return;
+  }
+  kf_launch_sptrsv(c, k->t, x, NULL);
   hipLaunchKernelGGL(k_prolong_add, dim3(kf_blocks_for(n4, CG_WG)), dim3(CG_WG), 0, c->stream, n4, k->agg, k->zc, x);
   KF_LAUNCH_CHECK();
 }
@@ -287,7 +299,7 @@ void kf_coarse_destroy(kf_ctx *c)
   kf_coarse_destroy(k->cc);
   kf_ctx_destroy_matrix(k->cc);
   void *ptrs[] = {k->agg, k->mptr, k->mlist, k->aprp, k->apcol, k->apgptr, k->apglist, k->gptr, k->glist,
-                  k->ap64, k->ap32, k->rc, k->zc, k->t};
+                  k->ap64, k->ap32, k->rc, k->zc, k->t, k->t32};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
     if (ptrs[i]) KF_CHECK(hipFree(ptrs[i]));
   delete k;

@@ -272,11 +272,11 @@ struct jsl_view {
   const uint32_t *col;    /* L, U */
 };
 
-template <int K, typename TM, typename TV, typename TX, bool DIAG>
+template <int K, typename TM, typename TV, typename TB, typename TX, bool DIAG>
 __global__ __launch_bounds__(JAC_WG) void k_jac_lower_sl(uint32_t N, const uint32_t *__restrict__ rowptr,
                                                          const uint32_t *__restrict__ col,
                                                          const uint32_t *__restrict__ diag, const TM *__restrict__ a,
-                                                         jsl_view v, const double *__restrict__ b,
+                                                         jsl_view v, const TB *__restrict__ b,
                                                          const TX *__restrict__ xin, TV *__restrict__ y,
                                                          TV *__restrict__ x0)
 {
@@ -299,7 +299,9 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_upper_sl(uint32_t N, const uint3
                                                          const uint32_t *__restrict__ col,
                                                          const uint32_t *__restrict__ diag, const TM *__restrict__ a,
                                                          jsl_view v, const TV *__restrict__ y,
-                                                         const TV *__restrict__ xin, TO *__restrict__ x)
+                                                         const TV *__restrict__ xin, TO *__restrict__ x,
+                                                         const uint32_t *__restrict__ pagg,
+                                                         const double *__restrict__ pzc)
 {
   const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
   const int c = threadIdx.x & 3;
@@ -311,7 +313,11 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_upper_sl(uint32_t N, const uint3
                                       jd + 1, xin);
   const TV u = (-s) + y[4 * (size_t)r + c];
   const TM *dv = sv + 2 * nk * 256;
-  x[4 * (size_t)r + c] = (TO)jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
+  TO xo = (TO)jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
+  /* the multilevel preconditioner's prolongation x += P z_c, fused into the
+   * last sweep (TO = double; the same add as k_prolong_add) */
+  if (pagg) xo = (TO)((double)xo + pzc[4 * (size_t)pagg[r] + c]);
+  x[4 * (size_t)r + c] = xo;
 }
 
 /* the sliced copy from the CSR factors (a quad per row, lane c copying
@@ -356,10 +362,11 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_pack(uint32_t N, uint32_t nslice
 
 /* The k sweeps each way.  Work vectors (4N of TV each): y (the forward
  * result, kept for the backward sweeps) and two ping-pong buffers p0, p1.
- * x (FP64) receives the last backward sweep. */
+ * x (FP64) receives the last backward sweep.  This is the CSR form (the
+ * no-allocation entry); jacobi_solve_sl below is the sliced one, with the
+ * same buffers, order and bits. */
 template <typename TM, typename TV>
-static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV *y, TV *p0, TV *p1, int k,
-                         const jsl_view *sl)
+static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV *y, TV *p0, TV *p1, int k)
 {
   const kf_dv &d = c->d;
   const uint32_t nb = kf_blocks_for(d.N, JAC_WG / 4);
@@ -374,36 +381,7 @@ static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV 
   for (int sw = 1; sw <= k; sw++) {
     const bool last = sw == k;
     TV *out = last ? y : ((sw & 1) ? p0 : p1);
-    if (sl) {
-      const jsl_view v = *sl;
-      if (c->jsl_k == 7) {
-        if (sw == 1 && last)
-          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, double, true>), grid, block, 0, c->stream, d.N, d.rowptr,
-                             d.col, d.diag, a, v, b, b, out, cur);
-        else if (sw == 1)
-          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, double, false>), grid, block, 0, c->stream, d.N, d.rowptr,
-                             d.col, d.diag, a, v, b, b, out, (TV *)NULL);
-        else if (last)
-          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, TV, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
-                             d.diag, a, v, b, prev, out, cur);
-        else
-          hipLaunchKernelGGL((k_jac_lower_sl<7, TM, TV, TV, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
-                             d.diag, a, v, b, prev, out, (TV *)NULL);
-      } else {
-        if (sw == 1 && last)
-          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, double, true>), grid, block, 0, c->stream, d.N, d.rowptr,
-                             d.col, d.diag, a, v, b, b, out, cur);
-        else if (sw == 1)
-          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, double, false>), grid, block, 0, c->stream, d.N, d.rowptr,
-                             d.col, d.diag, a, v, b, b, out, (TV *)NULL);
-        else if (last)
-          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, TV, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
-                             d.diag, a, v, b, prev, out, cur);
-        else
-          hipLaunchKernelGGL((k_jac_lower_sl<8, TM, TV, TV, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
-                             d.diag, a, v, b, prev, out, (TV *)NULL);
-      }
-    } else if (sw == 1 && last)
+    if (sw == 1 && last)
       hipLaunchKernelGGL((k_jac_lower<TM, TV, double, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
                          a, b, b, out, cur);
     else if (sw == 1)
@@ -424,24 +402,7 @@ static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV 
   TV *other = cur == p0 ? p1 : p0;
   s = kf_prof_begin(c, KP_JAC_U);
   for (int sw = 1; sw <= k; sw++) {
-    if (sl && sw == k) {
-      if (c->jsl_k == 7)
-        hipLaunchKernelGGL((k_jac_upper_sl<7, TM, TV, double>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
-                           a, *sl, y, cur, x);
-      else
-        hipLaunchKernelGGL((k_jac_upper_sl<8, TM, TV, double>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag,
-                           a, *sl, y, cur, x);
-    } else if (sl) {
-      if (c->jsl_k == 7)
-        hipLaunchKernelGGL((k_jac_upper_sl<7, TM, TV, TV>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a,
-                           *sl, y, cur, other);
-      else
-        hipLaunchKernelGGL((k_jac_upper_sl<8, TM, TV, TV>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a,
-                           *sl, y, cur, other);
-      TV *t = cur;
-      cur = other;
-      other = t;
-    } else if (sw == k) {
+    if (sw == k) {
       hipLaunchKernelGGL((k_jac_upper<TM, TV, double>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a, y,
                          cur, x);
     } else {
@@ -452,6 +413,75 @@ static void jacobi_solve(kf_ctx *c, const TM *a, const double *b, double *x, TV 
       other = t;
     }
     KF_LAUNCH_CHECK();
+  }
+  kf_prof_end(c, s);
+}
+
+/* one forward sweep over the sliced factors: the first reads the
+ * right-hand side b (type TB) as its old vector, the last also writes
+ * D^-1 y into cur */
+template <int K, typename TM, typename TV, typename TB>
+static void lower_sl(kf_ctx *c, const TM *a, const jsl_view &v, const TB *b, const TV *prev, TV *out, TV *cur,
+                     bool first, bool last)
+{
+  const kf_dv &d = c->d;
+  const dim3 grid(kf_blocks_for(d.N, JAC_WG / 4)), block(JAC_WG);
+  if (first && last)
+    hipLaunchKernelGGL((k_jac_lower_sl<K, TM, TV, TB, TB, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                       d.diag, a, v, b, b, out, cur);
+  else if (first)
+    hipLaunchKernelGGL((k_jac_lower_sl<K, TM, TV, TB, TB, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                       d.diag, a, v, b, b, out, (TV *)NULL);
+  else if (last)
+    hipLaunchKernelGGL((k_jac_lower_sl<K, TM, TV, TB, TV, true>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                       d.diag, a, v, b, prev, out, cur);
+  else
+    hipLaunchKernelGGL((k_jac_lower_sl<K, TM, TV, TB, TV, false>), grid, block, 0, c->stream, d.N, d.rowptr, d.col,
+                       d.diag, a, v, b, prev, out, (TV *)NULL);
+  KF_LAUNCH_CHECK();
+}
+
+/* one backward sweep over the sliced factors, into a TV buffer or (the
+ * last) into x, optionally adding P z_c */
+template <int K, typename TM, typename TV, typename TO>
+static void upper_sl(kf_ctx *c, const TM *a, const jsl_view &v, const TV *y, const TV *cur, TO *out,
+                     const uint32_t *pagg, const double *pzc)
+{
+  const kf_dv &d = c->d;
+  const dim3 grid(kf_blocks_for(d.N, JAC_WG / 4)), block(JAC_WG);
+  hipLaunchKernelGGL((k_jac_upper_sl<K, TM, TV, TO>), grid, block, 0, c->stream, d.N, d.rowptr, d.col, d.diag, a, v,
+                     y, cur, out, pagg, pzc);
+  KF_LAUNCH_CHECK();
+}
+
+/* jacobi_solve over the sliced factors, the right-hand side in TB (double,
+ * or float where the sweeps compute in FP32 anyway: the same bits), and
+ * with pagg non-NULL the prolongation x += P z_c fused into the last sweep */
+template <int K, typename TM, typename TV, typename TB>
+static void jacobi_solve_sl(kf_ctx *c, const TM *a, const TB *b, double *x, TV *y, TV *p0, TV *p1, int k,
+                            const jsl_view &v, const uint32_t *pagg, const double *pzc)
+{
+  TV *const xread = k >= 2 ? (((k - 1) & 1) ? p0 : p1) : NULL;
+  TV *cur = xread == p0 ? p1 : p0;
+  int s = kf_prof_begin(c, KP_JAC_L);
+  const TV *prev = NULL;
+  for (int sw = 1; sw <= k; sw++) {
+    TV *out = sw == k ? y : ((sw & 1) ? p0 : p1);
+    lower_sl<K, TM, TV, TB>(c, a, v, b, prev, out, cur, sw == 1, sw == k);
+    prev = out;
+  }
+  kf_prof_end(c, s);
+  TV *other = cur == p0 ? p1 : p0;
+  s = kf_prof_begin(c, KP_JAC_U);
+  for (int sw = 1; sw <= k; sw++) {
+    if (sw == k) {
+      upper_sl<K, TM, TV, double>(c, a, v, y, cur, x, pagg, pzc);
+    } else {
+      upper_sl<K, TM, TV, TV>(c, a, v, y, cur, other, NULL, NULL);
+      TV *t = cur;
+      cur = other;
+      other = t;
+    }
   }
   kf_prof_end(c, s);
 }
@@ -502,13 +532,14 @@ static jsl_view jsl_pack(kf_ctx *c)
   return v;
 }
 
-/* x ~ U^-1 L^-1 b with c->opt.jacobi_sweeps sweeps each way, in the
- * preconditioner's precision; b, x: device, 4N doubles, not aliased */
-void kf_launch_jacobi(kf_ctx *c, const double *b, double *x)
+/* the sweeps of kf_launch_jacobi / kf_launch_jacobi_ml in the context's
+ * precision; b is double, or float (TB) with single-precision sweeps */
+template <typename TB>
+static void jacobi_launch(kf_ctx *c, const TB *b, double *x, const uint32_t *pagg, const double *pzc)
 {
   kf_check_aligned(b, "b");
   kf_check_aligned(x, "x");
-  if (b == x) {
+  if ((const void *)b == (const void *)x) {
     fprintf(stderr, "kf: Jacobi triangular sweeps: b and x must not alias\n");
     abort();
   }
@@ -525,20 +556,42 @@ void kf_launch_jacobi(kf_ctx *c, const double *b, double *x)
     abort();
   }
   const jsl_view v = jsl_pack(c);
+  const bool k7 = c->jsl_k == 7;
   if (c->opt.prec == KF_PREC_FP64) {
     if (!c->ty64) KF_CHECK(hipMalloc((void **)&c->ty64, n * sizeof(double)));
     if (!c->jac64) KF_CHECK(hipMalloc((void **)&c->jac64, 2 * n * sizeof(double)));
-    jacobi_solve<double, double>(c, c->a64, b, x, c->ty64, c->jac64, c->jac64 + n, k, &v);
+    if (k7) jacobi_solve_sl<7, double, double, TB>(c, c->a64, b, x, c->ty64, c->jac64, c->jac64 + n, k, v, pagg, pzc);
+    else jacobi_solve_sl<8, double, double, TB>(c, c->a64, b, x, c->ty64, c->jac64, c->jac64 + n, k, v, pagg, pzc);
     c->ystate[0] = ~(uint64_t)0;   /* y no longer holds the level-scheduled sweep's sentinels */
     return;
   }
   if (!c->f32a) KF_CHECK(hipMalloc((void **)&c->f32a, n * sizeof(float)));
   if (!c->jac32) KF_CHECK(hipMalloc((void **)&c->jac32, 2 * n * sizeof(float)));
-  if (c->opt.prec == KF_PREC_FP32)
-    jacobi_solve<float, float>(c, c->a32, b, x, c->f32a, c->jac32, c->jac32 + n, k, &v);
-  else
-    jacobi_solve<uint16_t, float>(c, c->abf, b, x, c->f32a, c->jac32, c->jac32 + n, k, &v);
+  float *y = c->f32a, *p0 = c->jac32, *p1 = c->jac32 + n;
+  if (c->opt.prec == KF_PREC_FP32) {
+    if (k7) jacobi_solve_sl<7, float, float, TB>(c, c->a32, b, x, y, p0, p1, k, v, pagg, pzc);
+    else jacobi_solve_sl<8, float, float, TB>(c, c->a32, b, x, y, p0, p1, k, v, pagg, pzc);
+  } else {
+    if (k7) jacobi_solve_sl<7, uint16_t, float, TB>(c, c->abf, b, x, y, p0, p1, k, v, pagg, pzc);
+    else jacobi_solve_sl<8, uint16_t, float, TB>(c, c->abf, b, x, y, p0, p1, k, v, pagg, pzc);
+  }
   c->ystate[1] = ~(uint64_t)0;
+}
+
+/* x ~ U^-1 L^-1 b with c->opt.jacobi_sweeps sweeps each way, in the
+ * preconditioner's precision; b, x: device, 4N doubles, not aliased */
+void kf_launch_jacobi(kf_ctx *c, const double *b, double *x)
+{
+  jacobi_launch<double>(c, b, x, NULL, NULL);
+}
+
+/* the multilevel preconditioner's fine solve: as kf_launch_jacobi on t
+ * (FP64 with FP64 factors, else FP32: the sweeps' own arithmetic type), then
+ * x += P z_c (agg, zc) in the last sweep */
+void kf_launch_jacobi_ml(kf_ctx *c, const void *t, double *x, const uint32_t *agg, const double *zc)
+{
+  if (c->opt.prec == KF_PREC_FP64) jacobi_launch<double>(c, (const double *)t, x, agg, zc);
+  else jacobi_launch<float>(c, (const float *)t, x, agg, zc);
 }
 
 /* ---- the same solve with no allocation: kf_sptrsv_jacobi_noalloc ----------
@@ -621,13 +674,13 @@ extern "C" int kf_sptrsv_jacobi_noalloc(const GEOMETRY *g, const double *b, doub
   }
   if (c->opt.prec == KF_PREC_FP64) {
     double *y = static_cast<double *>(c->noalloc);
-    jacobi_solve<double, double>(c, c->a64, b, x, y, y + n, y + 2 * n, k, NULL);
+    jacobi_solve<double, double>(c, c->a64, b, x, y, y + n, y + 2 * n, k);
   } else {
     float *y = static_cast<float *>(c->noalloc);
     if (c->opt.prec == KF_PREC_FP32)
-      jacobi_solve<float, float>(c, c->a32, b, x, y, y + n, y + 2 * n, k, NULL);
+      jacobi_solve<float, float>(c, c->a32, b, x, y, y + n, y + 2 * n, k);
     else
-      jacobi_solve<uint16_t, float>(c, c->abf, b, x, y, y + n, y + 2 * n, k, NULL);
+      jacobi_solve<uint16_t, float>(c, c->abf, b, x, y, y + n, y + 2 * n, k);
   }
   return 0;
 }

@@ -235,6 +235,9 @@ void kf_keep_jacobian(kf_ctx *c);
 bool kf_trsv_capped(const kf_ctx *c);
 /* x ~ U^-1 L^-1 b by opt.jacobi_sweeps Jacobi sweeps each way (jacobi.hip) */
 void kf_launch_jacobi(kf_ctx *c, const double *b, double *x);
+/* the multilevel preconditioner's fine solve: the sweeps on t (FP64 with FP64
+ * factors, else FP32), then x += P z_c fused into the last sweep */
+void kf_launch_jacobi_ml(kf_ctx *c, const void *t, double *x, const uint32_t *agg, const double *zc);
 /* at context creation, with the context's device current: the device
  * address of kf_sptrsv_jacobi_noalloc's fixed work array (jacobi.hip) */
 void kf_noalloc_setup(kf_ctx *c);
