@@ -718,7 +718,7 @@ def main():
                     help="preconditioner precision of the learned leg (fp32: the combined form)")
     ap.add_argument("--traffic-dir", default="gpurun_out/pmc",
                     help="--kernel traffic: the directory holding the counter passes")
-    ap.add_argument("--traffic-paths", default="fp64,bf16,bf16_jacobi2_coarse8j3",
+    ap.add_argument("--traffic-paths", default="fp64,bf16,bf16_jacobi2_coarse8j3_lag4",
                     help="--kernel traffic: the paths whose passes it reads (fetch_<path>/, write_<path>/)")
     ap.add_argument("--kernel", default="solve", choices=["solve", "step", "solve1", "copy", "traffic"],
                     help="solve: the benchmark; step / solve1 / copy: short runs for profiles (first pseudo-time "
