@@ -253,7 +253,7 @@ def path_precond(path):
               % (coarse + 1, "the coarse system by block ILU(0)" if coarse == 1 else
                  "each coarse system by the same scheme, the coarsest by block ILU(0)",
                  "coarse ILU(0) solves by %d Jacobi sweeps each way" % cj if cj else "exact coarse ILU(0) solves",
-                 "fp64" if prec == "fp64" else "fp32"))
+                 {"fp64": "fp64", "fp32": "fp32", "bf16": "fp32 on bf16 factors"}[prec]))
     if path_lag(path) == 1:
         s += ", kept across pseudo-steps once the CFL ramp reaches its cap"
     elif path_lag(path):

@@ -179,7 +179,8 @@ typedef struct kf_options {
                            correction over greedy aggregates of neighbouring
                            vertices (2 x 2 x 2 on a lattice), factored by
                            block ILU(0) in the preconditioner's precision
-                           (FP32 for BF16 factors), applied before the fine
+                           (Galerkin sums in FP32 for single-precision
+                           factors), applied before the fine
                            ILU(0) solve: z = P Ac^-1 P^T b, then
                            x = z + (LU)^-1 (b - A z); L in 2..KF_COARSE_MAX:
                            the coarse system is preconditioned the same way

@@ -19,8 +19,10 @@
  * summed by the aggregate of their column (about 6 blocks a row on a
  * lattice, against 15 in A).  A P serves both the Galerkin product
  * (P^T (A P)) and the remainder b - (A P) z_c, so the full Jacobian is
- * neither kept nor re-read per application.  Coarse factors and A P are
- * FP64 with an FP64 preconditioner and FP32 otherwise.
+ * neither kept nor re-read per application.  A P and the Galerkin sums are
+ * FP64 with an FP64 preconditioner and FP32 otherwise; the coarse factors
+ * take the fine level's precision (FP64, FP32, or BF16-stored with FP32
+ * arithmetic).
  *
  * Kernels (a quad of lanes per block row, lane c forming component c, as
  * spmv.hip; workgroups remapped XCD by XCD):
@@ -161,7 +163,7 @@ static kf_coarse *coarse_build(kf_ctx *c)
   free(fp.diag);
 
   kf_options co = c->opt;
-  co.prec = c->opt.prec == KF_PREC_FP64 ? KF_PREC_FP64 : KF_PREC_FP32;
+  co.prec = c->opt.prec;
   co.ilu_exact = 0;
   co.jacobi_sweeps = c->opt.coarse_sweeps;   /* the coarse levels' own solves */
   co.coarse = c->opt.coarse - 1;   /* further levels below this one */
@@ -203,7 +205,7 @@ void kf_coarse_factor(kf_ctx *c)
   kf_coarse *k = (kf_coarse *)c->coarse;
   kf_ctx *cc = k->cc;
   const bool f64 = c->opt.prec == KF_PREC_FP64;
-  cc->opt.prec = f64 ? KF_PREC_FP64 : KF_PREC_FP32;   /* follows the fine level's precision */
+  cc->opt.prec = c->opt.prec;   /* the coarse factors follow the fine level's precision */
   cc->opt.jacobi_sweeps = cc->opt.coarse_sweeps = c->opt.coarse_sweeps;
   if (cc->opt.coarse != c->opt.coarse - 1) {   /* the number of levels changed */
     if (c->opt.coarse == 1) kf_coarse_destroy(cc);
