@@ -29,7 +29,8 @@
  *   k_gather_sum   a block = the sum of listed blocks in ascending position:
  *                  A P from the fine blocks, then P^T A P from A P
  *   k_restrict     coarse vector = sum of the aggregate's members, ascending
- *   k_coarse_resid t = b - (A P) z_c (FP32 where the fine solve is FP32 sweeps)
+ *   k_coarse_resid_sl  t = b - (A P) z_c over a sliced copy of A P (FP32
+ *                  where the fine solve is FP32 sweeps)
  *   k_prolong_add  x += P z_c (with Jacobi sweeps: inside their last sweep)
  * Every sum runs in a fixed order, so the preconditioner is deterministic.
  */
@@ -50,6 +51,10 @@ struct kf_coarse {
   uint32_t *gptr, *glist;   /* A P positions of each coarse block */
   double *ap64;        /* A P values (FP64 preconditioner) ... */
   float *ap32;         /* ... or FP32 */
+  void *apsl;          /* A P packed per 16-row slice and slot for the remainder
+                          product, in the preconditioner's storage type */
+  uint32_t *apsl_col;  /* ... the slots' aggregates */
+  size_t apsl_es;      /* bytes per packed value */
   double *rc, *zc;     /* coarse right-hand side and solution, 4 nc */
   double *t;           /* the fine remainder b - A z, 4 N */
   float *t32;          /* ... in FP32, where the fine solve is FP32 sweeps */
@@ -109,27 +114,107 @@ __device__ __forceinline__ void cg_row4(const T *a, size_t j, int c, double &a0,
   a3 = (double)p[3];
 }
 
-/* t = b - (A P) z_c; per row the A P blocks in ascending aggregate, each
- * component summed a[c][0] z0 + ... + a[c][3] z3 in FP64; stored as TT
- * (float where the fine sweeps round it to FP32 on reading anyway) */
-template <typename T, typename TT>
-__global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32_t *__restrict__ aprp,
-                                                        const uint32_t *__restrict__ apcol, const T *__restrict__ ap,
-                                                        const double *__restrict__ zc,
-                                                        const double *__restrict__ b, TT *__restrict__ t)
+/* ---- A P per 16-row slice, for the remainder product ------------------------
+ *
+ * As the Jacobi sweeps' factors (jacobi.hip): slot k of a slice's 16 rows is
+ * 16 consecutive blocks, so one load instruction of a wavefront reads one
+ * contiguous run; AP_SLOTS = 8 slots a row (a lattice row has 5 or 8 A P
+ * blocks), a slot past the row's end holds zeros and aggregate 0 and is
+ * selected away, entries past the last slot are read from the CSR copy.
+ * Values in the preconditioner's storage type: FP64, FP32, or BF16 (rounded
+ * to nearest even from the FP32 sums) with a BF16 preconditioner. */
+#define AP_SLOTS 8
+
+__device__ __forceinline__ void cg_st(double *o, double v) { *o = v; }
+__device__ __forceinline__ void cg_st(float *o, double v) { *o = (float)v; }
+__device__ __forceinline__ void cg_st(uint16_t *o, double v)
+{
+  uint32_t u = __float_as_uint((float)v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  *o = (uint16_t)(u >> 16);
+}
+
+template <typename TS, typename TA>
+__global__ __launch_bounds__(CG_WG) void k_ap_pack(uint32_t N, const uint32_t *__restrict__ aprp,
+                                                   const uint32_t *__restrict__ apcol, const TA *__restrict__ ap,
+                                                   TS *__restrict__ sv, uint32_t *__restrict__ sc)
 {
   const uint32_t r = kf_xcd_block() * (CG_WG / 4) + (threadIdx.x >> 2);
   const int c = threadIdx.x & 3;
   if (r >= N) return;
+  const uint32_t j0 = aprp[r], n = aprp[r + 1] - j0, q = r & 15u;
+  for (int k = 0; k < AP_SLOTS; k++) {
+    const size_t slot = (size_t)(r >> 4) * AP_SLOTS + k;
+    TS *o = sv + (slot * 16 + q) * 16 + 4 * c;
+    const TA *src = ap + 16 * (size_t)(j0 + k) + 4 * c;
+    for (int i = 0; i < 4; i++) cg_st(o + i, (uint32_t)k < n ? (double)src[i] : 0.0);
+    if (c == 0) sc[slot * 16 + q] = (uint32_t)k < n ? apcol[j0 + k] : 0u;
+  }
+}
+
+__device__ __forceinline__ void cg_ld4(const double *p, double a[4])
+{
+  const double2 lo = reinterpret_cast<const double2 *>(p)[0], hi = reinterpret_cast<const double2 *>(p)[1];
+  a[0] = lo.x; a[1] = lo.y; a[2] = hi.x; a[3] = hi.y;
+}
+__device__ __forceinline__ void cg_ld4(const float *p, double a[4])
+{
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w;
+}
+__device__ __forceinline__ void cg_ld4(const uint16_t *p, double a[4])
+{
+  const uint2 v = *reinterpret_cast<const uint2 *>(p);
+  a[0] = __uint_as_float(v.x << 16);
+  a[1] = __uint_as_float(v.x & 0xffff0000u);
+  a[2] = __uint_as_float(v.y << 16);
+  a[3] = __uint_as_float(v.y & 0xffff0000u);
+}
+
+/* t = b - (A P) z_c over the sliced copy: per row the A P blocks in
+ * ascending aggregate (its slots, then its CSR tail), each component summed
+ * a[c][0] z0 + ... + a[c][3] z3 in FP64, the slots loaded in one batch;
+ * stored as TT (float where the fine sweeps round it to FP32 on reading
+ * anyway) */
+template <typename TS, typename TA, typename TT>
+__global__ __launch_bounds__(CG_WG) void k_coarse_resid_sl(uint32_t N, const uint32_t *__restrict__ aprp,
+                                                           const uint32_t *__restrict__ apcol,
+                                                           const TA *__restrict__ ap, const TS *__restrict__ sv,
+                                                           const uint32_t *__restrict__ sc,
+                                                           const double *__restrict__ zc,
+                                                           const double *__restrict__ b, TT *__restrict__ t)
+{
+  const uint32_t r = kf_xcd_block() * (CG_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;
+  const uint32_t j0 = aprp[r], n = aprp[r + 1] - j0, q = r & 15u;
+  const size_t base = (size_t)(r >> 4) * AP_SLOTS;
+  uint32_t kk[AP_SLOTS];
+#pragma unroll
+  for (int k = 0; k < AP_SLOTS; k++) kk[k] = sc[(base + k) * 16 + q];
+  double a[AP_SLOTS][4];
+#pragma unroll
+  for (int k = 0; k < AP_SLOTS; k++) cg_ld4(sv + ((base + k) * 16 + q) * 16 + 4 * c, a[k]);
+  double z[AP_SLOTS][4];
+#pragma unroll
+  for (int k = 0; k < AP_SLOTS; k++) cg_ld4(zc + 4 * (size_t)kk[k], z[k]);
   double acc = 0.0;
-  for (uint32_t k = aprp[r]; k < aprp[r + 1]; k++) {
+#pragma unroll
+  for (int k = 0; k < AP_SLOTS; k++) {
+    double s = acc + a[k][0] * z[k][0];
+    s = s + a[k][1] * z[k][1];
+    s = s + a[k][2] * z[k][2];
+    s = s + a[k][3] * z[k][3];
+    acc = (uint32_t)k < n ? s : acc;
+  }
+  for (uint32_t k = AP_SLOTS; k < n; k++) {
     double a0, a1, a2, a3;
-    cg_row4<T>(ap, k, c, a0, a1, a2, a3);
-    const double *z = zc + 4 * (size_t)apcol[k];
-    acc = acc + a0 * z[0];
-    acc = acc + a1 * z[1];
-    acc = acc + a2 * z[2];
-    acc = acc + a3 * z[3];
+    cg_row4<TA>(ap, j0 + k, c, a0, a1, a2, a3);
+    const double *zz = zc + 4 * (size_t)apcol[j0 + k];
+    acc = acc + a0 * zz[0];
+    acc = acc + a1 * zz[1];
+    acc = acc + a2 * zz[2];
+    acc = acc + a3 * zz[3];
   }
   const size_t o = 4 * (size_t)r + c;
   t[o] = (TT)(b[o] - acc);
@@ -188,6 +273,35 @@ static kf_coarse *coarse_build(kf_ctx *c)
   return k;
 }
 
+/* the sliced copy of A P in the preconditioner's storage type */
+static void ap_pack(kf_ctx *c, kf_coarse *k)
+{
+  const uint32_t N = c->d.N, nslice = (N + 15) / 16;
+  const size_t es = c->opt.prec == KF_PREC_FP64 ? 8 : c->opt.prec == KF_PREC_FP32 ? 4 : 2;
+  const size_t nv = (size_t)nslice * AP_SLOTS * 256;
+  if (k->apsl && k->apsl_es != es) {   /* the precision changed */
+    KF_CHECK(hipStreamSynchronize(c->stream));
+    KF_CHECK(hipFree(k->apsl));
+    k->apsl = NULL;
+  }
+  if (!k->apsl) {
+    KF_CHECK(hipMalloc(&k->apsl, nv * es));
+    k->apsl_es = es;
+  }
+  if (!k->apsl_col) KF_CHECK(hipMalloc((void **)&k->apsl_col, (size_t)nslice * AP_SLOTS * 16 * sizeof(uint32_t)));
+  const dim3 grid(kf_blocks_for(N, CG_WG / 4)), block(CG_WG);
+  if (es == 8)
+    hipLaunchKernelGGL((k_ap_pack<double, double>), grid, block, 0, c->stream, N, k->aprp, k->apcol,
+                       (const double *)k->ap64, (double *)k->apsl, k->apsl_col);
+  else if (es == 4)
+    hipLaunchKernelGGL((k_ap_pack<float, float>), grid, block, 0, c->stream, N, k->aprp, k->apcol,
+                       (const float *)k->ap32, (float *)k->apsl, k->apsl_col);
+  else
+    hipLaunchKernelGGL((k_ap_pack<uint16_t, float>), grid, block, 0, c->stream, N, k->aprp, k->apcol,
+                       (const float *)k->ap32, (uint16_t *)k->apsl, k->apsl_col);
+  KF_LAUNCH_CHECK();
+}
+
 /* P^T (A P) from A P into a coarse value array */
 template <typename T>
 static void galerkin_from_ap(kf_ctx *c, const kf_coarse *k, const T *ap, T *out)
@@ -239,6 +353,7 @@ void kf_coarse_factor(kf_ctx *c)
     KF_LAUNCH_CHECK();
     galerkin_from_ap<float>(c, k, k->ap32, cc->a32);
   }
+  ap_pack(c, k);
   if (cc->opt.coarse) kf_coarse_factor(cc);   /* the next level, from the Galerkin values */
   kf_launch_ilu(cc);
   kf_prof_end(c, s);
@@ -253,7 +368,8 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
     abort();
   }
   const bool f64 = c->opt.prec == KF_PREC_FP64;
-  if (f64 ? !k->ap64 : !k->ap32) {
+  const size_t es = f64 ? 8 : c->opt.prec == KF_PREC_FP32 ? 4 : 2;
+  if ((f64 ? !k->ap64 : !k->ap32) || !k->apsl || k->apsl_es != es) {
     fprintf(stderr, "kf: two-level preconditioner: the preconditioner's precision changed since "
                     "ComputeNumericalILU\n");
     abort();
@@ -271,16 +387,29 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
   const uint32_t nb = kf_blocks_for(d.N, CG_WG / 4);
   const size_t n4 = 4 * (size_t)d.N;
   const bool sweeps = c->opt.jacobi_sweeps > 0;
+  const bool t32 = sweeps && !f64;
+  if (t32 && !k->t32) KF_CHECK(hipMalloc((void **)&k->t32, n4 * sizeof(float)));
   if (f64) {
-    hipLaunchKernelGGL((k_coarse_resid<double, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
-                       (const double *)k->ap64, k->zc, b, k->t);
-  } else if (sweeps) {
-    if (!k->t32) KF_CHECK(hipMalloc((void **)&k->t32, n4 * sizeof(float)));
-    hipLaunchKernelGGL((k_coarse_resid<float, float>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
-                       (const float *)k->ap32, k->zc, b, k->t32);
+    hipLaunchKernelGGL((k_coarse_resid_sl<double, double, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N,
+                       k->aprp, k->apcol, (const double *)k->ap64, (const double *)k->apsl, k->apsl_col, k->zc, b,
+                       k->t);
+  } else if (k->apsl_es == 4) {
+    if (t32)
+      hipLaunchKernelGGL((k_coarse_resid_sl<float, float, float>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp,
+                         k->apcol, (const float *)k->ap32, (const float *)k->apsl, k->apsl_col, k->zc, b, k->t32);
+    else
+      hipLaunchKernelGGL((k_coarse_resid_sl<float, float, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N,
+                         k->aprp, k->apcol, (const float *)k->ap32, (const float *)k->apsl, k->apsl_col, k->zc, b,
+                         k->t);
   } else {
-    hipLaunchKernelGGL((k_coarse_resid<float, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
-                       (const float *)k->ap32, k->zc, b, k->t);
+    if (t32)
+      hipLaunchKernelGGL((k_coarse_resid_sl<uint16_t, float, float>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N,
+                         k->aprp, k->apcol, (const float *)k->ap32, (const uint16_t *)k->apsl, k->apsl_col, k->zc, b,
+                         k->t32);
+    else
+      hipLaunchKernelGGL((k_coarse_resid_sl<uint16_t, float, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N,
+                         k->aprp, k->apcol, (const float *)k->ap32, (const uint16_t *)k->apsl, k->apsl_col, k->zc, b,
+                         k->t);
   }
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
@@ -301,7 +430,7 @@ void kf_coarse_destroy(kf_ctx *c)
   kf_coarse_destroy(k->cc);
   kf_ctx_destroy_matrix(k->cc);
   void *ptrs[] = {k->agg, k->mptr, k->mlist, k->aprp, k->apcol, k->apgptr, k->apglist, k->gptr, k->glist,
-                  k->ap64, k->ap32, k->rc, k->zc, k->t, k->t32};
+                  k->ap64, k->ap32, k->apsl, k->apsl_col, k->rc, k->zc, k->t, k->t32};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
     if (ptrs[i]) KF_CHECK(hipFree(ptrs[i]));
   delete k;
