@@ -29,8 +29,9 @@
  *   k_gather_sum   a block = the sum of listed blocks in ascending position:
  *                  A P from the fine blocks, then P^T A P from A P
  *   k_restrict     coarse vector = sum of the aggregate's members, ascending
- *   k_coarse_resid_sl  t = b - (A P) z_c over a sliced copy of A P (FP32
- *                  where the fine solve is FP32 sweeps)
+ *   k_coarse_resid t = b - (A P) z_c (FP32 where the fine solve is FP32
+ *                  sweeps); k_coarse_resid_sl the same over a sliced BF16
+ *                  copy of A P with a BF16 preconditioner
  *   k_prolong_add  x += P z_c (with Jacobi sweeps: inside their last sweep)
  * Every sum runs in a fixed order, so the preconditioner is deterministic.
  */
@@ -51,10 +52,9 @@ struct kf_coarse {
   uint32_t *gptr, *glist;   /* A P positions of each coarse block */
   double *ap64;        /* A P values (FP64 preconditioner) ... */
   float *ap32;         /* ... or FP32 */
-  void *apsl;          /* A P packed per 16-row slice and slot for the remainder
-                          product, in the preconditioner's storage type */
+  void *apsl;          /* A P packed per 16-row slice and slot in BF16, for the
+                          remainder product with a BF16 preconditioner */
   uint32_t *apsl_col;  /* ... the slots' aggregates */
-  size_t apsl_es;      /* bytes per packed value */
   double *rc, *zc;     /* coarse right-hand side and solution, 4 nc */
   double *t;           /* the fine remainder b - A z, 4 N */
   float *t32;          /* ... in FP32, where the fine solve is FP32 sweeps */
@@ -114,6 +114,32 @@ __device__ __forceinline__ void cg_row4(const T *a, size_t j, int c, double &a0,
   a3 = (double)p[3];
 }
 
+/* t = b - (A P) z_c; per row the A P blocks in ascending aggregate, each
+ * component summed a[c][0] z0 + ... + a[c][3] z3 in FP64; stored as TT
+ * (float where the fine sweeps round it to FP32 on reading anyway) */
+template <typename T, typename TT>
+__global__ __launch_bounds__(CG_WG) void k_coarse_resid(uint32_t N, const uint32_t *__restrict__ aprp,
+                                                        const uint32_t *__restrict__ apcol, const T *__restrict__ ap,
+                                                        const double *__restrict__ zc,
+                                                        const double *__restrict__ b, TT *__restrict__ t)
+{
+  const uint32_t r = kf_xcd_block() * (CG_WG / 4) + (threadIdx.x >> 2);
+  const int c = threadIdx.x & 3;
+  if (r >= N) return;
+  double acc = 0.0;
+  for (uint32_t k = aprp[r]; k < aprp[r + 1]; k++) {
+    double a0, a1, a2, a3;
+    cg_row4<T>(ap, k, c, a0, a1, a2, a3);
+    const double *z = zc + 4 * (size_t)apcol[k];
+    acc = acc + a0 * z[0];
+    acc = acc + a1 * z[1];
+    acc = acc + a2 * z[2];
+    acc = acc + a3 * z[3];
+  }
+  const size_t o = 4 * (size_t)r + c;
+  t[o] = (TT)(b[o] - acc);
+}
+
 /* ---- A P per 16-row slice, for the remainder product ------------------------
  *
  * As the Jacobi sweeps' factors (jacobi.hip): slot k of a slice's 16 rows is
@@ -121,12 +147,11 @@ __device__ __forceinline__ void cg_row4(const T *a, size_t j, int c, double &a0,
  * contiguous run; AP_SLOTS = 8 slots a row (a lattice row has 5 or 8 A P
  * blocks), a slot past the row's end holds zeros and aggregate 0 and is
  * selected away, entries past the last slot are read from the CSR copy.
- * Values in the preconditioner's storage type: FP64, FP32, or BF16 (rounded
- * to nearest even from the FP32 sums) with a BF16 preconditioner. */
+ * Used with a BF16 preconditioner only, the values rounded to nearest even
+ * from the FP32 sums: padded to 8 slots, an FP32 or FP64 copy would read
+ * more bytes than the CSR form (5.6 blocks a row), and measured slower. */
 #define AP_SLOTS 8
 
-__device__ __forceinline__ void cg_st(double *o, double v) { *o = v; }
-__device__ __forceinline__ void cg_st(float *o, double v) { *o = (float)v; }
 __device__ __forceinline__ void cg_st(uint16_t *o, double v)
 {
   uint32_t u = __float_as_uint((float)v);
@@ -273,32 +298,15 @@ static kf_coarse *coarse_build(kf_ctx *c)
   return k;
 }
 
-/* the sliced copy of A P in the preconditioner's storage type */
+/* the sliced BF16 copy of A P, with a BF16 preconditioner */
 static void ap_pack(kf_ctx *c, kf_coarse *k)
 {
+  if (c->opt.prec != KF_PREC_BF16) return;
   const uint32_t N = c->d.N, nslice = (N + 15) / 16;
-  const size_t es = c->opt.prec == KF_PREC_FP64 ? 8 : c->opt.prec == KF_PREC_FP32 ? 4 : 2;
-  const size_t nv = (size_t)nslice * AP_SLOTS * 256;
-  if (k->apsl && k->apsl_es != es) {   /* the precision changed */
-    KF_CHECK(hipStreamSynchronize(c->stream));
-    KF_CHECK(hipFree(k->apsl));
-    k->apsl = NULL;
-  }
-  if (!k->apsl) {
-    KF_CHECK(hipMalloc(&k->apsl, nv * es));
-    k->apsl_es = es;
-  }
+  if (!k->apsl) KF_CHECK(hipMalloc(&k->apsl, (size_t)nslice * AP_SLOTS * 256 * sizeof(uint16_t)));
   if (!k->apsl_col) KF_CHECK(hipMalloc((void **)&k->apsl_col, (size_t)nslice * AP_SLOTS * 16 * sizeof(uint32_t)));
-  const dim3 grid(kf_blocks_for(N, CG_WG / 4)), block(CG_WG);
-  if (es == 8)
-    hipLaunchKernelGGL((k_ap_pack<double, double>), grid, block, 0, c->stream, N, k->aprp, k->apcol,
-                       (const double *)k->ap64, (double *)k->apsl, k->apsl_col);
-  else if (es == 4)
-    hipLaunchKernelGGL((k_ap_pack<float, float>), grid, block, 0, c->stream, N, k->aprp, k->apcol,
-                       (const float *)k->ap32, (float *)k->apsl, k->apsl_col);
-  else
-    hipLaunchKernelGGL((k_ap_pack<uint16_t, float>), grid, block, 0, c->stream, N, k->aprp, k->apcol,
-                       (const float *)k->ap32, (uint16_t *)k->apsl, k->apsl_col);
+  hipLaunchKernelGGL((k_ap_pack<uint16_t, float>), dim3(kf_blocks_for(N, CG_WG / 4)), dim3(CG_WG), 0, c->stream, N,
+                     k->aprp, k->apcol, (const float *)k->ap32, (uint16_t *)k->apsl, k->apsl_col);
   KF_LAUNCH_CHECK();
 }
 
@@ -367,9 +375,8 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
     fprintf(stderr, "kf: two-level preconditioner: ComputeNumericalILU has not run with coarse = 1\n");
     abort();
   }
-  const bool f64 = c->opt.prec == KF_PREC_FP64;
-  const size_t es = f64 ? 8 : c->opt.prec == KF_PREC_FP32 ? 4 : 2;
-  if ((f64 ? !k->ap64 : !k->ap32) || !k->apsl || k->apsl_es != es) {
+  const bool f64 = c->opt.prec == KF_PREC_FP64, bf = c->opt.prec == KF_PREC_BF16;
+  if ((f64 ? !k->ap64 : !k->ap32) || (bf && !k->apsl)) {
     fprintf(stderr, "kf: two-level preconditioner: the preconditioner's precision changed since "
                     "ComputeNumericalILU\n");
     abort();
@@ -390,17 +397,15 @@ void kf_coarse_apply(kf_ctx *c, const double *b, double *x)
   const bool t32 = sweeps && !f64;
   if (t32 && !k->t32) KF_CHECK(hipMalloc((void **)&k->t32, n4 * sizeof(float)));
   if (f64) {
-    hipLaunchKernelGGL((k_coarse_resid_sl<double, double, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N,
-                       k->aprp, k->apcol, (const double *)k->ap64, (const double *)k->apsl, k->apsl_col, k->zc, b,
-                       k->t);
-  } else if (k->apsl_es == 4) {
+    hipLaunchKernelGGL((k_coarse_resid<double, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+                       (const double *)k->ap64, k->zc, b, k->t);
+  } else if (!bf) {
     if (t32)
-      hipLaunchKernelGGL((k_coarse_resid_sl<float, float, float>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp,
-                         k->apcol, (const float *)k->ap32, (const float *)k->apsl, k->apsl_col, k->zc, b, k->t32);
+      hipLaunchKernelGGL((k_coarse_resid<float, float>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+                         (const float *)k->ap32, k->zc, b, k->t32);
     else
-      hipLaunchKernelGGL((k_coarse_resid_sl<float, float, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N,
-                         k->aprp, k->apcol, (const float *)k->ap32, (const float *)k->apsl, k->apsl_col, k->zc, b,
-                         k->t);
+      hipLaunchKernelGGL((k_coarse_resid<float, double>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N, k->aprp, k->apcol,
+                         (const float *)k->ap32, k->zc, b, k->t);
   } else {
     if (t32)
       hipLaunchKernelGGL((k_coarse_resid_sl<uint16_t, float, float>), dim3(nb), dim3(CG_WG), 0, c->stream, d.N,
