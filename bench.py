@@ -476,15 +476,16 @@ class stdout_to_stderr:
 
 
 def learned(K, mesh, rhs_per_step, train_iters, prec="fp64"):
-    """The learned correction on one mesh: flexible GMRES with ILU(0) alone
-    (the method without the network), then with the network trained online
+    """The learned correction on one mesh: flexible GMRES with the path's
+    preconditioner alone (prec, a path name; fp64 = ILU(0), the method
+    without the network), then with the network trained online
     -- pairs from the first 10 pseudo-time steps (the Newton residual and
     rhs_per_step Gaussian right-hand sides, each solved to 1e-10 or 300
     iterations), Adam with learning rate 1e-3 for train_iters steps -- and
     used from step 11 on with judged steps and the ILU(0) fallback.
     Wall-clock with and without the collection and training."""
     import numpy as np
-    m = K.Mesh(mesh, K.config_options(krylov="fgmres", prec=prec))
+    m = K.Mesh(mesh, path_options(K, prec, krylov="fgmres"))
     log("learned leg: ILU(0)-FGMRES warm-up")
     q0 = m.state()
     m.kernel()
@@ -494,7 +495,7 @@ def learned(K, mesh, rhs_per_step, train_iters, prec="fp64"):
     m.set_state(q0)
     # the solver's per-step lines (and its collection and training lines) are
     # the leg's progress: printed, but on stderr (stdout carries the JSON line)
-    m.set_options(K.config_options(krylov="fgmres", prec=prec, verbose=1))
+    m.set_options(path_options(K, prec, krylov="fgmres", verbose=1))
     net = K.Network(m, K.init_params(seed=0), nslot=11)
     m.attach(net, collect_steps=10, rhs_per_step=rhs_per_step, pair_rtol=1e-10, pair_maxit=300,
              train_iters=train_iters, lr=1e-3, judge_every=5, holdout=0.1, seed=3)
@@ -714,8 +715,9 @@ def main():
     ap.add_argument("--learn-iters", type=int, default=2000, help="Adam steps")
     ap.add_argument("--learn-config", default="c2", choices=["c2", "c5"],
                     help="mesh of the learned leg: c2 (configuration 4) or c5 (the combined form)")
-    ap.add_argument("--learn-prec", default="fp64", choices=["fp64", "fp32", "bf16"],
-                    help="preconditioner precision of the learned leg (fp32: the combined form)")
+    ap.add_argument("--learn-prec", default="fp64",
+                    help="the learned leg's preconditioner path (fp64: ILU(0), the recipe; fp32: the combined form; "
+                         "a multilevel path name puts the network on top of the multilevel preconditioner)")
     ap.add_argument("--traffic-dir", default="gpurun_out/pmc",
                     help="--kernel traffic: the directory holding the counter passes")
     ap.add_argument("--traffic-paths", default="fp64,bf16,bf16_jacobi2_coarse8j3_lag4",
