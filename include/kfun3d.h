@@ -245,6 +245,13 @@ size_t kf_nnzb(const GEOMETRY *g);
  * where the sweeps are throughput-bound; KF_TRSV_FOLD=0/1 forces it), 0 when
  * the combination runs as its own kernel first (for tools and tests) */
 int kf_trsv_fold(const GEOMETRY *g);
+/* 1 when the residual's gradient forms each incidence's LSQ terms from the
+ * end's seven moments (src/weights.c:299-366) and the edge vector, 0 when
+ * it reads the GEOMETRY's per-edge terms (g->e->w).  Setup takes the first
+ * form only once the moments reproduce every term bit for bit;
+ * KF_GRAD_MOMENTS=0 forces the second (for tools and tests).  The gradient
+ * is the same in both. */
+int kf_grad_moments(const GEOMETRY *g);
 /* y = A x (device vectors of 4N doubles, not aliased) with the assembled
  * Jacobian of the last ComputeA -- FP64, or FP32 with a single-precision
  * preconditioner -- which ComputeA keeps when kf_options.bsr_operator is

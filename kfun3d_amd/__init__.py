@@ -137,6 +137,7 @@ def _load():
         "kf_stream": (C.c_void_p, [GP]),
         "kf_nnzb": (C.c_size_t, [GP]),
         "kf_trsv_fold": (C.c_int, [GP]),
+        "kf_grad_moments": (C.c_int, [GP]),
         "kf_bsr_spmv": (None, [GP, DP, DP]),
         "kf_pattern": (None, [GP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "kf_get_blocks": (None, [GP, C.c_void_p]),
@@ -211,7 +212,7 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "ComputeSparseTriangularSolve", "iJacVec", "Kernel", "Compute2ndNorm", "ComputeAXPY",
                "ComputeNewAXPY", "Normalize", "Setup", "kf_default_options", "kf_set_options", "kf_get_options",
                "kf_kernel", "kf_set_state", "kf_get_state", "kf_memcpy_d2h", "kf_memcpy_h2d", "kf_device_alloc",
-               "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_trsv_fold", "kf_bsr_spmv", "kf_pattern", "kf_get_blocks",
+               "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_trsv_fold", "kf_grad_moments", "kf_bsr_spmv", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_sptrsv_jacobi_noalloc", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
                "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
                "kf_ilu_plan_host", "kf_coarse_plan_host", "kf_coarse_matrix", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_init_params", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",

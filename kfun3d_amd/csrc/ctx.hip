@@ -195,6 +195,8 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   d.sell_wy = dupload(wy, ne);
   d.sell_wz = dupload(wz, ne);
   free(sptr); free(deg); free(nbr); free(ent); free(blk); free(wx); free(wy); free(wz);
+  kf_grad_setup(c);
+  kf_phase("LSQ moments (device)");
 
   /* boundary */
   d.sf0 = dupload(g->b->fc->fptr->n0, d.S);
@@ -309,7 +311,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
   }
   kf_dv &d = c->d;
   void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.xyzw, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
-                  d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
+                  d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.lsq, d.sell_nbs, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->rdone,
                   c->trow_l, c->trow_u, c->tlvl_l, c->tlvl_u, c->tdep_l, c->tdep_u, c->eslot, c->ty64, c->ilu_rec, c->ilu_ldiag, c->ilu_moff, c->ilu_mjm, c->ilu_mlo, c->ilu_apub, c->ilu_udpub, c->ticket, c->a64, c->a32,
@@ -500,6 +502,9 @@ extern "C" int kf_trsv_fold(const GEOMETRY *g)
   if (c->opt.jacobi_sweeps > 0) return 0;   /* the combination runs first (every forward sweep reads b) */
   return !(c->trsv_fold == 0 || (c->trsv_fold < 0 && !kf_trsv_capped(c)));
 }
+
+/* whether the gradient forms its LSQ terms from per-vertex moments */
+extern "C" int kf_grad_moments(const GEOMETRY *g) { return kf_ctx_of(g)->grad_mom; }
 
 extern "C" void kf_pattern(const GEOMETRY *g, unsigned int *rowptr, unsigned int *col, unsigned int *diag)
 {

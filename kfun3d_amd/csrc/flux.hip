@@ -54,8 +54,209 @@ __device__ __forceinline__ void tangent_basis(double xn, double yn, double zn, d
   Z2 = xn * Y1; Z2 -= yn * X1;
 }
 
+/* ---- LSQ weights: src/weights.c:9-398 -------------------------------------- */
+
+/* The terms of one incidence of vertex v (src/weights.c:9-100) from v's seven
+ * moments m = (w00, w01, w02, w11, w12, w22, w33) and the edge vector
+ * (dx, dy, dz) = x(n1) - x(n0): side 0 when v is the edge's n0, side 1 when
+ * it is n1.  The same expressions, in the same order, as the host's
+ * per-edge terms, so the same operands give the same bits. */
+__device__ __forceinline__ void lsq_terms(const double m[7], double dx, double dy, double dz, bool side1,
+                                          double &tx, double &ty, double &tz)
+{
+  double c0, c1;
+  if (!side1) {
+    c0 = -dx * m[1] + dy;
+    c1 = -dx * m[2] + dz;
+  } else {
+    c0 = dx * m[1] - dy;
+    c1 = dx * m[2] - dz;
+  }
+  c1 = -m[4] * c0 + c1;
+  tx = m[3] * m[1] * c0;
+  tx = (side1 ? -dx : dx) * m[0] - tx;
+  tx += m[6] * c1;
+  ty = m[4] * m[5] * c1;
+  ty = m[3] * c0 - ty;
+  tz = m[5] * c1;
+}
+
+/* (dx, dy, dz) = x(n1) - x(n0) of the edge between v (at cv) and nb (at cn) */
+__device__ __forceinline__ void edge_vector(const d4 &cv, const d4 &cn, bool side1, double &dx, double &dy,
+                                            double &dz)
+{
+  if (!side1) { dx = cn.p - cv.p; dy = cn.u - cv.u; dz = cn.v - cv.v; }
+  else { dx = cv.p - cn.p; dy = cv.u - cn.u; dz = cv.v - cn.v; }
+}
+
+/* Per-vertex moments (src/weights.c:103-366): three passes over the
+ * incidences in ascending edge id, each reading only the vertex's own sums
+ * of the earlier passes, then the magnitudes.  Also writes sell_nbs. */
+__global__ __launch_bounds__(256) void k_lsq_moments(kf_dv d, double *__restrict__ m, uint32_t *__restrict__ nbs)
+{
+  const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+  if (v >= d.N) return;
+  const d4 cv = ld4(d.xyzw, v);
+  const uint32_t deg = d.deg[v];
+  const size_t base = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+  /* w00, w01, w02: src/weights.c:237-297 (r = x(nb) - x(v) from either end) */
+  for (uint32_t k = 0; k < deg; k++) {
+    const size_t idx = base + 64 * (size_t)k;
+    const uint32_t nb = d.sell_nbr[idx], side = d.sell_ent[idx] & 1u;
+    nbs[idx] = nb | side << 31;
+    const d4 cn = ld4(d.xyzw, nb);
+    const double rx = cn.p - cv.p, ry = cn.u - cv.u, rz = cn.v - cv.v;
+    w0 += rx * rx; w1 += rx * ry; w2 += rx * rz;
+  }
+  w0 = sqrt(w0);
+  w1 /= w0;
+  w2 /= w0;
+  /* w11, w12: src/weights.c:173-232 */
+  for (uint32_t k = 0; k < deg; k++) {
+    const size_t idx = base + 64 * (size_t)k;
+    const bool side1 = d.sell_ent[idx] & 1u;
+    const d4 cn = ld4(d.xyzw, d.sell_nbr[idx]);
+    double dx, dy, dz;
+    edge_vector(cv, cn, side1, dx, dy, dz);
+    const double dd = w1 / w0;
+    if (!side1) {
+      const double d_ = dy - dx * dd;
+      w3 += d_ * d_;
+      w4 += d_ * dz;
+    } else {
+      const double d_ = -dy + dx * dd;
+      w3 += d_ * d_;
+      w4 -= d_ * dz;
+    }
+  }
+  w3 = sqrt(w3);
+  w4 /= w3;
+  /* w22: src/weights.c:103-170 */
+  for (uint32_t k = 0; k < deg; k++) {
+    const size_t idx = base + 64 * (size_t)k;
+    const bool side1 = d.sell_ent[idx] & 1u;
+    const d4 cn = ld4(d.xyzw, d.sell_nbr[idx]);
+    double dx, dy, dz;
+    edge_vector(cv, cn, side1, dx, dy, dz);
+    const double d0 = w1 / w0, d1 = w2 / w0, d2 = w4 / w3;
+    double d0_, d1_;
+    if (!side1) { d0_ = dy - dx * d0; d1_ = dz - dx * d1; }
+    else { d0_ = -dy + dx * d0; d1_ = -dz + dx * d1; }
+    const double d2_ = d1_ - d2 * d0_;
+    w5 += d2_ * d2_;
+  }
+  /* magnitudes: src/weights.c:313-366 */
+  w5 = sqrt(w5);
+  const double sw00 = w0 * w0, sw11 = w3 * w3, sw22 = w5 * w5;
+  double m0 = w1 * w4;
+  m0 -= w2 * w3;
+  const double m1 = w0 * w3 * sw22;
+  const size_t N = d.N;
+  m[v] = 1.0 / sw00;
+  m[N + v] = w1 / w0;
+  m[2 * N + v] = w2 / w0;
+  m[3 * N + v] = 1.0 / sw11;
+  m[4 * N + v] = w4 / w3;
+  m[5 * N + v] = 1.0 / sw22;
+  m[6 * N + v] = m0 / m1;
+}
+
+__device__ __forceinline__ void lsq_load(const double *__restrict__ m, uint32_t N, uint32_t v, double out[7])
+{
+#pragma unroll
+  for (int k = 0; k < 7; k++) out[k] = m[(size_t)k * N + v];
+}
+
+/* every incidence's terms from the moments against the uploaded ones, bit
+ * for bit; any difference sets *bad */
+__global__ __launch_bounds__(256) void k_lsq_check(kf_dv d, const double *__restrict__ m, uint32_t *__restrict__ bad)
+{
+  const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+  if (v >= d.N) return;
+  double mv[7];
+  lsq_load(m, d.N, v, mv);
+  const d4 cv = ld4(d.xyzw, v);
+  const uint32_t deg = d.deg[v];
+  const size_t base = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
+  bool ok = true;
+  for (uint32_t k = 0; k < deg; k++) {
+    const size_t idx = base + 64 * (size_t)k;
+    const bool side1 = d.sell_ent[idx] & 1u;
+    const d4 cn = ld4(d.xyzw, d.sell_nbr[idx]);
+    double dx, dy, dz, tx, ty, tz;
+    edge_vector(cv, cn, side1, dx, dy, dz);
+    lsq_terms(mv, dx, dy, dz, side1, tx, ty, tz);
+    ok = ok && __double_as_longlong(tx) == __double_as_longlong(d.sell_wx[idx]) &&
+         __double_as_longlong(ty) == __double_as_longlong(d.sell_wy[idx]) &&
+         __double_as_longlong(tz) == __double_as_longlong(d.sell_wz[idx]);
+  }
+  if (!ok) atomicOr(bad, 1u);
+}
+
+/* k_grad's LSQ terms from per-vertex moments (56 bytes a vertex, and the
+ * neighbour's coordinates beside its state) rather than 24 bytes per
+ * incidence, once every incidence's terms are shown bit-identical to the
+ * GEOMETRY's (KF_GRAD_MOMENTS=0 keeps the per-incidence terms) */
+void kf_grad_setup(kf_ctx *c)
+{
+  kf_dv &d = c->d;
+  const char *ev = getenv("KF_GRAD_MOMENTS");
+  c->grad_mom = 0;
+  if ((ev && atoi(ev) == 0) || d.N == 0 || d.N > 0x7fffffffu) return;
+  uint32_t nslice_rows;
+  KF_CHECK(hipMemcpy(&nslice_rows, d.sell_ptr + d.nslice, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  KF_CHECK(hipMalloc((void **)&d.lsq, 7 * (size_t)d.N * sizeof(double)));
+  KF_CHECK(hipMalloc((void **)&d.sell_nbs, (size_t)nslice_rows * 64 * sizeof(uint32_t) + sizeof(uint32_t)));
+  uint32_t *bad;
+  KF_CHECK(hipMalloc((void **)&bad, sizeof(uint32_t)));
+  KF_CHECK(hipMemsetAsync(bad, 0, sizeof(uint32_t), c->stream));
+  const dim3 gv(kf_blocks_for(d.N, 256)), b(256);
+  hipLaunchKernelGGL(k_lsq_moments, gv, b, 0, c->stream, d, d.lsq, d.sell_nbs);
+  KF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_lsq_check, gv, b, 0, c->stream, d, (const double *)d.lsq, bad);
+  KF_LAUNCH_CHECK();
+  uint32_t hbad = 1;
+  KF_CHECK(hipMemcpyAsync(&hbad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  KF_CHECK(hipStreamSynchronize(c->stream));
+  KF_CHECK(hipFree(bad));
+  if (hbad) {
+    KF_CHECK(hipFree(d.lsq));
+    KF_CHECK(hipFree(d.sell_nbs));
+    d.lsq = NULL;
+    d.sell_nbs = NULL;
+    return;
+  }
+  KF_CHECK(hipFree(d.sell_wx));
+  KF_CHECK(hipFree(d.sell_wy));
+  KF_CHECK(hipFree(d.sell_wz));
+  d.sell_wx = d.sell_wy = d.sell_wz = NULL;
+  c->grad_mom = 1;
+}
+
 /* ---- LSQ gradient: src/flux.c:60-145 as a per-vertex gather -------------- */
 
+/* a += t (q(nb) - q(v)), component by component */
+__device__ __forceinline__ void grad_add(double a0[4], double a1[4], double a2[4], const double qs[4], const d4 &qn,
+                                         double tx, double ty, double tz)
+{
+  const double qb[4] = {qn.p, qn.u, qn.v, qn.w};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double dq = qb[c] - qs[c];
+    a0[c] += tx * dq;
+    a1[c] += ty * dq;
+    a2[c] += tz * dq;
+  }
+}
+
+/* MOM: the terms from the moments (lsq_terms, the bits of the per-incidence
+ * ones, which kf_grad_setup checked), two incidences a round with all four
+ * gathers issued before the sums, which keep the edge order (288x96x96:
+ * 0.299 ms with the per-incidence terms, 0.264 ms one incidence a round,
+ * 0.243 ms two; four a round at 118 VGPRs, or two capped at 96 VGPRs for a
+ * fifth wave, 0.31-0.32 ms) */
+template <bool MOM>
 __global__ __launch_bounds__(256) void k_grad(kf_dv d, const double *__restrict__ q, double *__restrict__ gx,
                                               double *__restrict__ gy, double *__restrict__ gz, int first_order)
 {
@@ -67,17 +268,35 @@ __global__ __launch_bounds__(256) void k_grad(kf_dv d, const double *__restrict_
     const double qs[4] = {qv.p, qv.u, qv.v, qv.w};
     const uint32_t deg = d.deg[v];
     size_t idx = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
-    for (uint32_t k = 0; k < deg; k++, idx += 64) {
-      const uint32_t nb = d.sell_nbr[idx];
-      const double tx = d.sell_wx[idx], ty = d.sell_wy[idx], tz = d.sell_wz[idx];
-      const d4 qn = ld4(q, nb);
-      const double qb[4] = {qn.p, qn.u, qn.v, qn.w};
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const double dq = qb[c] - qs[c];
-        a0[c] += tx * dq;
-        a1[c] += ty * dq;
-        a2[c] += tz * dq;
+    uint32_t k = 0;
+    if (MOM) {
+      double mv[7];
+      lsq_load(d.lsq, d.N, v, mv);
+      const d4 cv = ld4(d.xyzw, v);
+      double dx, dy, dz, tx, ty, tz;
+      for (; k + 1 < deg; k += 2, idx += 128) {
+        const uint32_t sa = d.sell_nbs[idx], sb = d.sell_nbs[idx + 64];
+        const uint32_t na = sa & 0x7fffffffu, nb = sb & 0x7fffffffu;
+        const d4 ca = ld4(d.xyzw, na), cb = ld4(d.xyzw, nb);
+        const d4 qa = ld4(q, na), qb = ld4(q, nb);
+        edge_vector(cv, ca, sa >> 31, dx, dy, dz);
+        lsq_terms(mv, dx, dy, dz, sa >> 31, tx, ty, tz);
+        grad_add(a0, a1, a2, qs, qa, tx, ty, tz);
+        edge_vector(cv, cb, sb >> 31, dx, dy, dz);
+        lsq_terms(mv, dx, dy, dz, sb >> 31, tx, ty, tz);
+        grad_add(a0, a1, a2, qs, qb, tx, ty, tz);
+      }
+      if (k < deg) {
+        const uint32_t sa = d.sell_nbs[idx], na = sa & 0x7fffffffu;
+        const d4 ca = ld4(d.xyzw, na), qa = ld4(q, na);
+        edge_vector(cv, ca, sa >> 31, dx, dy, dz);
+        lsq_terms(mv, dx, dy, dz, sa >> 31, tx, ty, tz);
+        grad_add(a0, a1, a2, qs, qa, tx, ty, tz);
+      }
+    } else {
+      for (; k < deg; k++, idx += 64) {
+        const uint32_t nb = d.sell_nbr[idx];
+        grad_add(a0, a1, a2, qs, ld4(q, nb), d.sell_wx[idx], d.sell_wy[idx], d.sell_wz[idx]);
       }
     }
   }
@@ -356,7 +575,8 @@ void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *
   const kf_dv &d = c->d;
   const dim3 gv(kf_blocks_for(d.N, 256)), ge(kf_blocks_for(d.E, 256)), b(256);
   int s = kf_prof_begin(c, KP_GRAD);
-  hipLaunchKernelGGL(k_grad, gv, b, 0, c->stream, d, q, gx, gy, gz, c->opt.first_order);
+  if (c->grad_mom) hipLaunchKernelGGL(k_grad<true>, gv, b, 0, c->stream, d, q, gx, gy, gz, c->opt.first_order);
+  else hipLaunchKernelGGL(k_grad<false>, gv, b, 0, c->stream, d, q, gx, gy, gz, c->opt.first_order);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   s = kf_prof_begin(c, KP_EDGE);

@@ -63,7 +63,9 @@ struct kf_dv {
   double *xyzw;                 /* (x, y, z, 0) per vertex: one 32-byte gather for the edge kernel */
   uint32_t *sell_ptr, *deg;
   uint32_t *sell_nbr, *sell_ent, *sell_blk;
-  double *sell_wx, *sell_wy, *sell_wz;
+  double *sell_wx, *sell_wy, *sell_wz;   /* the LSQ terms per incidence (NULL once lsq is verified) */
+  double *lsq;                  /* 7 x N: per-vertex LSQ moments w00 w01 w02 w11 w12 w22 w33 (one array each) */
+  uint32_t *sell_nbs;           /* neighbour | side << 31 per incidence (with lsq) */
   uint32_t *sf0, *sf1, *sf2;
   uint32_t *color, ncolor;    /* solid-face colour offsets */
   uint32_t *sfptr, *sfent;
@@ -118,6 +120,7 @@ struct kf_ctx {
    * tasks of up to trsv_S = 16 / R ranges (one wavefront each) */
   uint32_t trsv_R, trsv_S;
   int trsv_fold;              /* GMRES's combination in the forward sweep: -1 by regime, 0 no, 1 yes */
+  int grad_mom;               /* 1: k_grad forms the LSQ terms from d.lsq, 0: reads them per incidence */
   uint32_t trsv_pf;           /* off-diagonal entries per row held in registers (7 or 8):
                                 7 when no row has more below or above the diagonal */
   uint32_t *trow_l, *trow_u; /* per task (ticket order) and row slot: row, block bounds */
@@ -213,6 +216,7 @@ void kf_prof_end(kf_ctx *c, int slot);
 
 /* launches (defined in the .hip files) */
 /* sr != NULL: also the time-step sums of the state q (see k_residual) */
+void kf_grad_setup(kf_ctx *c);
 void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r, double *sr);
 void kf_launch_timestep(kf_ctx *c, double cfl);
 void kf_launch_timestep_div(kf_ctx *c, const double *sr, double cfl);

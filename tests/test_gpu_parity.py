@@ -56,6 +56,37 @@ def test_flux_bitwise(pair):
         assert _bitwise(a, b), "%s %s: %s" % (name, what, _describe(a, b))
 
 
+@pytest.mark.parametrize("which", ["mesh_a", "mesh_c1"])
+def test_gradient_both_lsq_forms(which, mesh_a, mesh_c1, monkeypatch):
+    """The gradient forms each incidence's LSQ terms from the end's seven
+    moments once setup has checked them against the GEOMETRY's per-edge terms
+    (kf_grad_moments = 1 on these meshes), or reads the per-edge terms
+    (KF_GRAD_MOMENTS=0); both give the oracle's gradient and residual bit for
+    bit."""
+    path = {"mesh_a": mesh_a, "mesh_c1": mesh_c1}[which]
+    hm = K.Mesh(path, device=False)
+    q = perturbed_state(hm.N, 3.0, 21)
+    hm.close()
+    out = {}
+    for form in (1, 0):
+        if form == 0:
+            monkeypatch.setenv("KF_GRAD_MOMENTS", "0")
+        m = K.Mesh(path, K.options())
+        try:
+            assert K.lib().kf_grad_moments(m.ref) == form
+            out[form] = _flux_gpu(m, q)
+            if form == 1:
+                orc = O.Oracle(m)
+                ro, gx, gy, gz = orc.flux(q)
+                orc.close()
+        finally:
+            m.close()
+    for form in (1, 0):
+        r, g = out[form]
+        for a, b, what in [(g[0], gx, "grad x"), (g[1], gy, "grad y"), (g[2], gz, "grad z"), (r, ro, "residual")]:
+            assert _bitwise(a, b), "%s form %d %s: %s" % (which, form, what, _describe(a, b))
+
+
 def test_flux_deterministic(pair):
     name, m, orc = pair
     q = perturbed_state(m.N, 3.0, 2)
