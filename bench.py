@@ -79,6 +79,8 @@ def alg_bytes(kind, N, E, S, Nf, nnzb, s):
         "grad_mom": 8 * E + 216 * N,
         "edge_flux": 72 * E + 152 * N,            # ends 8E, normal 32E, flux out 32E; xyz 24N, q 32N, grad 96N
         "residual_gather": 40 * E + 32 * N,       # ent 8E, flux in 32E; r 32N
+        # with iJacVec's combination (k_residual<false, true>): + base residual, state, base state 96N, cdt 8N
+        "residual_fd": 40 * E + 136 * N,
         "timestep": 40 * E + 40 * N,
         "fill": 40 * E + 40 * N + 16 * s * nnzb,
         "ilu": 2 * 16 * s * nnzb + 4 * nnzb,
@@ -94,7 +96,7 @@ def alg_bytes(kind, N, E, S, Nf, nnzb, s):
     }[kind]
 
 
-PROFILE_TO_KIND = {"grad": "grad_mom", "edge_flux": "edge_flux", "residual": "residual_gather", "timestep": "timestep",
+PROFILE_TO_KIND = {"grad": "grad_mom", "edge_flux": "edge_flux", "residual": "residual_fd", "timestep": "timestep",
                    "fill": "fill", "ilu": "ilu", "trsv_lower": "trsv_lower", "trsv_upper": "trsv_upper",
                    "spmv": "spmv", "jacobi_lower": "jacobi_lower", "jacobi_upper": "jacobi_upper"}
 MATRIX_KINDS = ("fill", "ilu", "trsv_lower", "trsv_upper", "spmv", "jacobi_lower", "jacobi_upper")
@@ -135,7 +137,7 @@ def kernel_name(dom, prec, held, fold):
         return "k_jac_lower_sl<%d, %s, %s, double, double, true>" % (held, tm, tv)
     if dom == "jacobi_upper":
         return "k_jac_upper_sl<%d, %s, %s, double>" % (held, tm, tv)
-    return {"grad": "k_grad<true>", "edge_flux": "k_edge_flux<false>", "residual": "k_residual<false>",
+    return {"grad": "k_grad<true>", "edge_flux": "k_edge_flux<false>", "residual": "k_residual<false, true>",
             "timestep": "k_timestep"}[dom]
 
 
@@ -647,6 +649,8 @@ def traffic_profile(K, config, root, precs):
             kind = kinds[base]
             if k == "k_grad<true>":
                 kind = "grad_mom"
+            if k == "k_residual<false, true>":
+                kind = "residual_fd"
             if multi and kind == "ilu":
                 continue   # the fine and the coarse factorisations share the persistent grid: not separable
             if multi and kind in ("jacobi_lower", "jacobi_upper") and ", %s," % TYPE[parse_path(prec)[0]][0] not in k:

@@ -537,6 +537,22 @@ extern "C" void ComputeFlux(const GEOMETRY *g, const double *q, GRADIENT *grad, 
   kf_launch_flux(c, q, grad->x0, grad->x1, grad->x2, r, NULL);
 }
 
+/* iJacVec's residual and combination in one pass (src/kernel.c:27-44):
+ * y = ((R(w) - r) + cdt (w - q)) |x| / h, the residual's last kernel
+ * forming it before its store; |x| in scal[slot] */
+extern "C" void kfs_flux_fd(GEOMETRY *g, const double *w, GRADIENT *grad, double *y, const double *r, double h,
+                            int slot)
+{
+  kf_ctx *c = kf_ctx_of(g);
+  if (y == w || y == r) {
+    fprintf(stderr, "kf: kfs_flux_fd: y must not alias w or r\n");
+    abort();
+  }
+  kf_check_aligned(r, "r");
+  const kf_fdb fd = {r, w, c->q, c->cdt, c->d.scal + slot, h};
+  kf_launch_flux(c, w, grad->x0, grad->x1, grad->x2, y, NULL, &fd);
+}
+
 extern "C" void ComputeTimeStep(const double cfl, GEOMETRY *g) { kf_launch_timestep(kf_ctx_of(g), cfl); }
 
 /* Driver-internal pair: the residual of the current state together with its
@@ -614,6 +630,17 @@ extern "C" void ComputeSparseTriangularSolve(const GEOMETRY *g, const double *b,
  * ComputeSparseTriangularSolve.  Where the sweeps are throughput-bound the
  * forward sweep forms y as it loads b (one pass less); b may be overwritten
  * with y.  |x| in scal[slot_xnorm] */
+static bool fd_separate(const kf_ctx *c)
+{
+  return c->opt.coarse || c->opt.jacobi_sweeps > 0 || c->trsv_fold == 0 || (c->trsv_fold < 0 && !kf_trsv_capped(c));
+}
+
+/* 1 when kfs_sptrsv_fd would form the combination on its own before the
+ * solve (multilevel, Jacobi sweeps, or latency-bound triangular sweeps):
+ * the residual can then form it instead (kfs_flux_fd, then
+ * ComputeSparseTriangularSolve) */
+extern "C" int kfs_fd_separate(GEOMETRY *g) { return fd_separate(kf_ctx_of(g)); }
+
 extern "C" void kfs_sptrsv_fd(GEOMETRY *g, const double *b, double *x, const double *r, const double *w, double h,
                               int slot_xnorm)
 {

@@ -515,10 +515,14 @@ __device__ __forceinline__ void freestream_term(const kf_dv &d, uint32_t i, cons
 /* SPECTRAL: also the vertex's time-step sum (edge terms in edge order, then
  * the boundary-node terms: k_timestep's order) into sr, before the division
  * by the CFL */
-template <bool SPECTRAL>
+/* FD: q is iJacVec's perturbed state w, and r receives its combination
+ * ((R(w) - fd.r) + cdt (w - q0)) |x| / h (src/kernel.c:33-44) instead of
+ * R(w): the operations of k_fd_combine on the same operands, so the same
+ * bits, without R(w)'s round trip through memory */
+template <bool SPECTRAL, bool FD>
 __global__ __launch_bounds__(256) void k_residual(kf_dv d, const double *__restrict__ q,
                                                   const double *__restrict__ eres, double *__restrict__ r,
-                                                  const double *__restrict__ eterm, double *__restrict__ sr)
+                                                  const double *__restrict__ eterm, double *__restrict__ sr, kf_fdb fd)
 {
   const uint32_t v = kf_xcd_block() * 256u + threadIdx.x;
   if (v >= d.N) return;
@@ -565,10 +569,20 @@ __global__ __launch_bounds__(256) void k_residual(kf_dv d, const double *__restr
       r0 += o[0]; r1 += o[1]; r2 += o[2]; r3 += o[3];
     }
   }
+  if (FD) {
+    const double a = *fd.xnorm / fd.h;
+    const double cd = fd.cdt[v];
+    const d4 rb = ld4(fd.r, v), wv = ld4(q, v), q0 = ld4(fd.q0, v);
+    r0 = (r0 - rb.p + cd * (wv.p - q0.p)) * a;
+    r1 = (r1 - rb.u + cd * (wv.u - q0.u)) * a;
+    r2 = (r2 - rb.v + cd * (wv.v - q0.v)) * a;
+    r3 = (r3 - rb.w + cd * (wv.w - q0.w)) * a;
+  }
   st4(r, v, r0, r1, r2, r3);
 }
 
-void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r, double *sr)
+void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r, double *sr,
+                    const kf_fdb *fd)
 {
   kf_check_aligned(q, "q"); kf_check_aligned(r, "r");
   kf_check_aligned(gx, "grad->x0"); kf_check_aligned(gy, "grad->x1"); kf_check_aligned(gz, "grad->x2");
@@ -586,12 +600,20 @@ void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
   s = kf_prof_begin(c, KP_RES);
+  const kf_fdb none = {};
+  if (sr && fd) {
+    fprintf(stderr, "kf: the residual with time-step sums takes no combination\n");
+    abort();
+  }
   if (sr)
-    hipLaunchKernelGGL(k_residual<true>, dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q, d.eres, r,
-                       c->eterm, sr);
+    hipLaunchKernelGGL((k_residual<true, false>), dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q,
+                       d.eres, r, c->eterm, sr, none);
+  else if (fd)
+    hipLaunchKernelGGL((k_residual<false, true>), dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q,
+                       d.eres, r, (const double *)NULL, (double *)NULL, *fd);
   else
-    hipLaunchKernelGGL(k_residual<false>, dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q, d.eres, r,
-                       (const double *)NULL, (double *)NULL);
+    hipLaunchKernelGGL((k_residual<false, false>), dim3(kf_blocks_for(d.N, 256)), dim3(256), 0, c->stream, d, q,
+                       d.eres, r, (const double *)NULL, (double *)NULL, none);
   KF_LAUNCH_CHECK();
   kf_prof_end(c, s);
 }

@@ -217,7 +217,11 @@ void kf_prof_end(kf_ctx *c, int slot);
 /* launches (defined in the .hip files) */
 /* sr != NULL: also the time-step sums of the state q (see k_residual) */
 void kf_grad_setup(kf_ctx *c);
-void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r, double *sr);
+/* fd: iJacVec's combination formed by the residual's last kernel, whose
+ * state q is then fd->w (k_residual<false, true>) */
+struct kf_fdb;
+void kf_launch_flux(kf_ctx *c, const double *q, double *gx, double *gy, double *gz, double *r, double *sr,
+                    const kf_fdb *fd = NULL);
 void kf_launch_timestep(kf_ctx *c, double cfl);
 void kf_launch_timestep_div(kf_ctx *c, const double *sr, double cfl);
 void kf_launch_forces(kf_ctx *c, double *forces_host);

@@ -39,14 +39,19 @@ static void ijacvec(GEOMETRY *g, const double *x, const double *r, const double 
   if (have < 1) kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
   const double h = FD_EPS * norm;
   if (have < 2) kfs_fd_perturb(g, x, h, KFS_SLOT_XNORM, w);
-  ComputeFlux(g, w, grad, y);
-  kfs_fd_combine(g, r, w, h, KFS_SLOT_XNORM, y);
+  if (y == r || y == w) {   /* aliased output: the two passes, as before */
+    ComputeFlux(g, w, grad, y);
+    kfs_fd_combine(g, r, w, h, KFS_SLOT_XNORM, y);
+    return;
+  }
+  kfs_flux_fd(g, w, grad, y, r, h, KFS_SLOT_XNORM);
 }
 
-/* GMRES's inner step M^-1 (J x): iJacVec up to the residual R(w) in y, then
- * the combination and the triangular solves (kfs_sptrsv_fd: where the sweeps
- * are throughput-bound the forward sweep forms the combination as it loads
- * y, with the operations of kfs_fd_combine); y is scratch afterwards */
+/* GMRES's inner step M^-1 (J x): iJacVec, then the preconditioner.  Where
+ * the triangular sweeps are throughput-bound the forward sweep forms the
+ * combination as it loads R(w) (kfs_sptrsv_fd); otherwise the residual's
+ * last kernel forms it (kfs_flux_fd); both with the operations of
+ * kfs_fd_combine.  y is scratch afterwards */
 static void ijacvec_solve(GEOMETRY *g, const double *x, const double *r, const double norm, GRADIENT *grad,
                           double *w, double *y, int have, double *out)
 {
@@ -54,6 +59,11 @@ static void ijacvec_solve(GEOMETRY *g, const double *x, const double *r, const d
   if (have < 1) kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
   const double h = FD_EPS * norm;
   if (have < 2) kfs_fd_perturb(g, x, h, KFS_SLOT_XNORM, w);
+  if (kfs_fd_separate(g)) {
+    kfs_flux_fd(g, w, grad, y, r, h, KFS_SLOT_XNORM);
+    ComputeSparseTriangularSolve(g, y, out);
+    return;
+  }
   ComputeFlux(g, w, grad, y);
   kfs_sptrsv_fd(g, y, out, r, w, h, KFS_SLOT_XNORM);
 }
