@@ -62,6 +62,7 @@ static void ctx_matrix_setup(kf_ctx *c, const kf_prep *p)
   /* GMRES's combination formed by the forward sweep: -1 where the sweeps are
    * throughput-bound, 0 never, 1 always */
   c->trsv_fold = (ev = getenv("KF_TRSV_FOLD")) ? (atoi(ev) != 0) : -1;
+  c->fd_res = (ev = getenv("KF_FD_RESIDUAL")) ? (atoi(ev) != 0) : 1;
   kf_phase("uploads");
   {
     kf_ilu_plan ip;
@@ -549,6 +550,11 @@ extern "C" void kfs_flux_fd(GEOMETRY *g, const double *w, GRADIENT *grad, double
     abort();
   }
   kf_check_aligned(r, "r");
+  if (!c->fd_res) {   /* KF_FD_RESIDUAL=0: the two passes */
+    kf_launch_flux(c, w, grad->x0, grad->x1, grad->x2, y, NULL);
+    kf_fd_combine(c, r, w, h, slot, y);
+    return;
+  }
   const kf_fdb fd = {r, w, c->q, c->cdt, c->d.scal + slot, h};
   kf_launch_flux(c, w, grad->x0, grad->x1, grad->x2, y, NULL, &fd);
 }
@@ -635,11 +641,15 @@ static bool fd_separate(const kf_ctx *c)
   return c->opt.coarse || c->opt.jacobi_sweeps > 0 || c->trsv_fold == 0 || (c->trsv_fold < 0 && !kf_trsv_capped(c));
 }
 
-/* 1 when kfs_sptrsv_fd would form the combination on its own before the
- * solve (multilevel, Jacobi sweeps, or latency-bound triangular sweeps):
- * the residual can then form it instead (kfs_flux_fd, then
- * ComputeSparseTriangularSolve) */
-extern "C" int kfs_fd_separate(GEOMETRY *g) { return fd_separate(kf_ctx_of(g)); }
+/* 1 when GMRES's inner step has the residual form the combination
+ * (kfs_flux_fd, then ComputeSparseTriangularSolve): where kfs_sptrsv_fd
+ * would form it on its own before the solve (multilevel, Jacobi sweeps, or
+ * latency-bound triangular sweeps), unless KF_FD_RESIDUAL=0 */
+extern "C" int kfs_fd_residual(GEOMETRY *g)
+{
+  const kf_ctx *c = kf_ctx_of(g);
+  return c->fd_res && fd_separate(c);
+}
 
 extern "C" void kfs_sptrsv_fd(GEOMETRY *g, const double *b, double *x, const double *r, const double *w, double h,
                               int slot_xnorm)

@@ -121,6 +121,8 @@ struct kf_ctx {
   uint32_t trsv_R, trsv_S;
   int trsv_fold;              /* GMRES's combination in the forward sweep: -1 by regime, 0 no, 1 yes */
   int grad_mom;               /* 1: k_grad forms the LSQ terms from d.lsq, 0: reads them per incidence */
+  int fd_res;                 /* 1: the residual forms iJacVec's combination where the preconditioner
+                                 would run it on its own (KF_FD_RESIDUAL=0: never) */
   uint32_t trsv_pf;           /* off-diagonal entries per row held in registers (7 or 8):
                                 7 when no row has more below or above the diagonal */
   uint32_t *trow_l, *trow_u; /* per task (ticket order) and row slot: row, block bounds */

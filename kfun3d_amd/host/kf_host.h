@@ -170,8 +170,8 @@ void kfs_fd_perturb(GEOMETRY *g, const double *x, double h, int slot, double *w)
 void kfs_fd_combine(GEOMETRY *g, const double *r, const double *w, double h, int slot, double *y);
 /* ComputeFlux(w) then kfs_fd_combine in one pass (y must not alias w or r) */
 void kfs_flux_fd(GEOMETRY *g, const double *w, GRADIENT *grad, double *y, const double *r, double h, int slot);
-/* whether kfs_sptrsv_fd would form the combination on its own first */
-int kfs_fd_separate(GEOMETRY *g);
+/* whether GMRES's inner step has the residual form the combination */
+int kfs_fd_residual(GEOMETRY *g);
 void kfs_zero(GEOMETRY *g, size_t n, double *x);
 void kfs_cgs_dots(GEOMETRY *g, size_t n, const double *vbase, int nv, const double *w, int slot);
 void kfs_cgs_update(GEOMETRY *g, size_t n, const double *vbase, int nv, int slot, double *w);

@@ -59,7 +59,7 @@ static void ijacvec_solve(GEOMETRY *g, const double *x, const double *r, const d
   if (have < 1) kfs_nrm2(g, sz, x, KFS_SLOT_XNORM);
   const double h = FD_EPS * norm;
   if (have < 2) kfs_fd_perturb(g, x, h, KFS_SLOT_XNORM, w);
-  if (kfs_fd_separate(g)) {
+  if (kfs_fd_residual(g)) {
     kfs_flux_fd(g, w, grad, y, r, h, KFS_SLOT_XNORM);
     ComputeSparseTriangularSolve(g, y, out);
     return;
