@@ -653,8 +653,11 @@ def traffic_profile(K, config, root, precs):
                 kind = "residual_fd"
             if multi and kind == "ilu":
                 continue   # the fine and the coarse factorisations share the persistent grid: not separable
-            if multi and kind in ("jacobi_lower", "jacobi_upper") and ", %s," % TYPE[parse_path(prec)[0]][0] not in k:
-                continue   # a coarse level's sweeps (their factors are FP32 or FP64, not the fine type)
+            if multi and kind in ("jacobi_lower", "jacobi_upper"):
+                held = int(k.split("<")[1].split(",")[0])
+                pp, sw = parse_path(prec)[:2]
+                if k not in jacobi_names(kind, pp, sw, held, True):
+                    continue   # a coarse level's sweeps (other instantiations of the same kernels)
             ss = s if kind in MATRIX_KINDS else 8
             if base in ("k_fill", "k_ilu_rows") and s != 8:
                 ss = 4   # the fill and the factorisation run in FP32 for both single-precision modes
