@@ -692,58 +692,65 @@ __global__ __launch_bounds__(256) void k_force_terms(kf_dv d, const double *__re
 }
 
 /* src/forces.c:33 reduces each colour into a zero-initialised private sum
- * that is then added to the running total.  The sums are sequential in face
- * order, as the reference's; the workgroup stages 256 faces' terms in LDS
- * with coalesced loads and one lane adds them up. */
-__global__ __launch_bounds__(256) void k_force_sum(kf_dv d, const double *__restrict__ ft, double *__restrict__ out)
+ * that is then added to the running total.  The colours' sums are
+ * independent: k_force_part forms colour blockIdx.x's, sequential in face
+ * order as the reference's (the workgroup stages 256 faces' terms in LDS
+ * with coalesced loads and one lane per component adds them up), and
+ * k_force_total adds them to the total in colour order. */
+__global__ __launch_bounds__(256) void k_force_part(kf_dv d, const double *__restrict__ ft, double *__restrict__ parts)
 {
   __shared__ double s[6 * 256];
-  const uint32_t nc = d.ncolor;
-  const uint32_t j0 = nc ? d.color[0] : 0, j1 = nc ? d.color[nc] : 0;
+  const uint32_t cl = blockIdx.x, j0 = d.color[cl], j1 = d.color[cl + 1];
   /* lane c < 3 carries component c (lift, drag, moment) through the faces
    * in order: lift (l - t0) + t1, drag (dr + t2) + t3, moment (m + t4) - t5;
    * a sign of -1 is exact, so (x + (-1) t) == x - t bit for bit */
   const uint32_t c = threadIdx.x;
   const double s0 = c == 0 ? -1.0 : 1.0, s1 = c == 2 ? -1.0 : 1.0;
-  double acc = 0.0, part = 0.0;
-  uint32_t cl = 0, cend = nc ? d.color[1] : 0;
+  double part = 0.0;
   for (uint32_t b = j0; b < j1; b += 256) {
     const uint32_t n = j1 - b < 256u ? j1 - b : 256u;
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < 6 * n; k += 256) s[k] = ft[6 * (size_t)b + k];
     __syncthreads();
     if (c < 3) {
+      const double *o = s + 2 * c;
       uint32_t i = 0;
-      while (i < n) {
-        while (b + i == cend) {   /* colour cl ends before face b + i */
-          acc = acc + part;
-          part = 0.0;
-          cl++;
-          cend = d.color[cl + 1];
-        }
-        const uint32_t e = cend - b < n ? cend - b : n;   /* faces of this colour in the chunk */
-        const double *o = s + 2 * c;
-        for (; i + 8 <= e; i += 8) {
-          double x[16];
+      for (; i + 8 <= n; i += 8) {
+        double x[16];
 #pragma unroll
-          for (int q = 0; q < 8; q++) {
-            x[2 * q] = o[6 * (i + q)];
-            x[2 * q + 1] = o[6 * (i + q) + 1];
-          }
-#pragma unroll
-          for (int q = 0; q < 8; q++) part = (part + s0 * x[2 * q]) + s1 * x[2 * q + 1];
+        for (int q = 0; q < 8; q++) {
+          x[2 * q] = o[6 * (i + q)];
+          x[2 * q + 1] = o[6 * (i + q) + 1];
         }
-        for (; i < e; i++) part = (part + s0 * o[6 * i]) + s1 * o[6 * i + 1];
+#pragma unroll
+        for (int q = 0; q < 8; q++) part = (part + s0 * x[2 * q]) + s1 * x[2 * q + 1];
       }
+      for (; i < n; i++) part = (part + s0 * o[6 * i]) + s1 * o[6 * i + 1];
     }
   }
-  if (c < 3) {
-    for (; cl < nc; cl++) {   /* the last colour, then any empty ones */
-      acc = acc + part;
-      part = 0.0;
-    }
-    out[c] = acc;
+  if (c < 3) parts[3 * (size_t)cl + c] = part;
+}
+
+__global__ __launch_bounds__(64) void k_force_total(uint32_t nc, const double *__restrict__ parts,
+                                                    double *__restrict__ out)
+{
+  const uint32_t c = threadIdx.x;
+  if (c >= 3) return;
+  double acc = 0.0;
+  for (uint32_t cl = 0; cl < nc; cl++) acc = acc + parts[3 * (size_t)cl + c];
+  out[c] = acc;
+}
+
+/* the face terms (ft[6 S]) summed per colour into parts[3 ncolor], then
+ * into out[3] */
+static void force_sums(const kf_dv &d, hipStream_t st, const double *ft, double *parts, double *out)
+{
+  if (d.ncolor) {
+    hipLaunchKernelGGL(k_force_part, dim3(d.ncolor), dim3(256), 0, st, d, ft, parts);
+    KF_LAUNCH_CHECK();
   }
+  hipLaunchKernelGGL(k_force_total, dim3(1), dim3(64), 0, st, d.ncolor, (const double *)parts, out);
+  KF_LAUNCH_CHECK();
 }
 
 void kf_launch_forces(kf_ctx *c, double *forces_host)
@@ -751,7 +758,7 @@ void kf_launch_forces(kf_ctx *c, double *forces_host)
   const kf_dv &d = c->d;
   /* the per-edge flux buffer (4E doubles) is free between residuals */
   double *ft = d.eres;
-  if (6 * (size_t)d.S > 4 * (size_t)d.E) {
+  if (6 * (size_t)d.S + 3 * (size_t)d.ncolor > 4 * (size_t)d.E) {
     fprintf(stderr, "kf: force scratch too small\n");
     abort();
   }
@@ -759,8 +766,7 @@ void kf_launch_forces(kf_ctx *c, double *forces_host)
     hipLaunchKernelGGL(k_force_terms, dim3(kf_blocks_for(d.S, 256)), dim3(256), 0, c->stream, d, c->q, ft);
     KF_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(256), 0, c->stream, d, ft, d.scal + (KF_NSCAL - 3));
-  KF_LAUNCH_CHECK();
+  force_sums(d, c->stream, ft, ft + 6 * (size_t)d.S, d.scal + (KF_NSCAL - 3));
   kf_scal_fetch(c, KF_NSCAL - 3, 3, forces_host);
 }
 
@@ -779,15 +785,14 @@ void kf_forces_async(kf_ctx *c)
     KF_CHECK(hipEventCreateWithFlags(&c->ev_upd, hipEventDisableTiming));
     KF_CHECK(hipEventCreateWithFlags(&c->ev_forces, hipEventDisableTiming));
   }
-  if (!c->ftbuf) KF_CHECK(hipMalloc((void **)&c->ftbuf, (6 * (size_t)d.S + 1) * sizeof(double)));
+  if (!c->ftbuf) KF_CHECK(hipMalloc((void **)&c->ftbuf, (6 * (size_t)d.S + 3 * (size_t)d.ncolor + 1) * sizeof(double)));
   KF_CHECK(hipEventRecord(c->ev_upd, c->stream));
   KF_CHECK(hipStreamWaitEvent(c->fstream, c->ev_upd, 0));
   if (d.S) {
     hipLaunchKernelGGL(k_force_terms, dim3(kf_blocks_for(d.S, 256)), dim3(256), 0, c->fstream, d, c->q, c->ftbuf);
     KF_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(256), 0, c->fstream, d, c->ftbuf, d.scal + KF_SLOT_FASYNC);
-  KF_LAUNCH_CHECK();
+  force_sums(d, c->fstream, c->ftbuf, c->ftbuf + 6 * (size_t)d.S, d.scal + KF_SLOT_FASYNC);
   KF_CHECK(hipMemcpyAsync(c->hscal + KF_SLOT_FASYNC, d.scal + KF_SLOT_FASYNC, 3 * sizeof(double),
                           hipMemcpyDeviceToHost, c->fstream));
   KF_CHECK(hipEventRecord(c->ev_forces, c->fstream));
