@@ -157,12 +157,24 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
   kf_free_stream(o->alpha_deg, &d.U, &d.V);
   c->nnzb = p->nnzb;
 
-  d.n0 = dupload(g->e->eptr->n0, E);
-  d.n1 = dupload(g->e->eptr->n1, E);
-  d.enx = dupload(g->e->xyzn->x0, E);
-  d.eny = dupload(g->e->xyzn->x1, E);
-  d.enz = dupload(g->e->xyzn->x2, E);
-  d.ea = dupload(g->e->xyzn->x3, E);
+  {
+    /* per-edge records: (n0, n1) and (unit normal, area) interleaved */
+    uint32_t *hn = (uint32_t *)kf_xmalloc((2 * E + 2) * sizeof(uint32_t));
+    double *hg = (double *)kf_xmalloc((4 * E + 4) * sizeof(double));
+    const struct xyzn *xn = g->e->xyzn;
+    for (size_t i = 0; i < E; i++) {
+      hn[2 * i] = g->e->eptr->n0[i];
+      hn[2 * i + 1] = g->e->eptr->n1[i];
+      hg[4 * i] = xn->x0[i];
+      hg[4 * i + 1] = xn->x1[i];
+      hg[4 * i + 2] = xn->x2[i];
+      hg[4 * i + 3] = xn->x3[i];
+    }
+    d.en = dupload(hn, 2 * E);
+    d.egeo = dupload(hg, 4 * E);
+    free(hn);
+    free(hg);
+  }
   d.x = dupload(g->n->xyz->x0, N);
   d.y = dupload(g->n->xyz->x1, N);
   d.z = dupload(g->n->xyz->x2, N);
@@ -311,7 +323,7 @@ extern "C" void kf_ctx_destroy(GEOMETRY *g)
     KF_CHECK(hipStreamDestroy(c->fstream));
   }
   kf_dv &d = c->d;
-  void *ptrs[] = {d.n0, d.n1, d.enx, d.eny, d.enz, d.ea, d.x, d.y, d.z, d.xyzw, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
+  void *ptrs[] = {d.en, d.egeo, d.x, d.y, d.z, d.xyzw, d.sell_ptr, d.deg, d.sell_nbr, d.sell_ent,
                   d.sell_blk, d.sell_wx, d.sell_wy, d.sell_wz, d.lsq, d.sell_nbs, d.sf0, d.sf1, d.sf2, d.color, d.sfptr, d.sfent, d.snx, d.sny,
                   d.snz, d.snptr, d.snent, d.fnx, d.fny, d.fnz, d.fnptr, d.fnent, d.rowptr, d.col, d.diag, d.eres,
                   d.red, d.scal, d.err, c->lrows, c->rdone,

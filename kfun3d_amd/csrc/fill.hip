@@ -223,12 +223,13 @@ __global__ __launch_bounds__(128) void k_fill(kf_dv d, const double *__restrict_
     const double *qn = q + 4 * (size_t)nb;
     const T un = (T)qn[1], vn = (T)qn[2], wn = (T)qn[3];
     T v0[16], v1[16], off[16];
+    const kf_egeo eg = kf_edge_geo(d.egeo, e);
     if (ent & 1u) {   /* r is n1: (n1,n0) -= v0, (n1,n1) -= v1 */
-      edge_blocks<T>((T)d.enx[e], (T)d.eny[e], (T)d.enz[e], (T)d.ea[e], un, vn, wn, ur, vr, wr, v0, v1);
+      edge_blocks<T>((T)eg.xn, (T)eg.yn, (T)eg.zn, (T)eg.ln, un, vn, wn, ur, vr, wr, v0, v1);
 #pragma unroll
       for (int j = 0; j < 16; j++) { off[j] = zero + mone * v0[j]; dg[j] += mone * v1[j]; }
     } else {          /* r is n0: (n0,n0) += v0, (n0,n1) += v1 */
-      edge_blocks<T>((T)d.enx[e], (T)d.eny[e], (T)d.enz[e], (T)d.ea[e], ur, vr, wr, un, vn, wn, v0, v1);
+      edge_blocks<T>((T)eg.xn, (T)eg.yn, (T)eg.zn, (T)eg.ln, ur, vr, wr, un, vn, wn, v0, v1);
 #pragma unroll
       for (int j = 0; j < 16; j++) { off[j] = zero + v1[j]; dg[j] += v0[j]; }
     }

@@ -357,8 +357,10 @@ __global__ __launch_bounds__(256) void k_edge_flux(kf_dv d, const double *__rest
 {
   const uint32_t i = kf_xcd_block() * 256u + threadIdx.x;
   if (i >= d.E) return;
-  const uint32_t node0 = d.n0[i], node1 = d.n1[i];
-  const double xn = d.enx[i], yn = d.eny[i], zn = d.enz[i], ln = d.ea[i];
+  const uint2 nn = reinterpret_cast<const uint2 *>(d.en)[i];
+  const uint32_t node0 = nn.x, node1 = nn.y;
+  const kf_egeo eg = kf_edge_geo(d.egeo, i);
+  const double xn = eg.xn, yn = eg.yn, zn = eg.zn, ln = eg.ln;
   const d4 c0 = ld4(d.xyzw, node0), c1 = ld4(d.xyzw, node1);
   const double x0 = c0.p, y0 = c0.u, z0 = c0.v;
   const double x1 = c1.p, y1 = c1.u, z1 = c1.v;
@@ -633,7 +635,8 @@ __global__ __launch_bounds__(256) void k_timestep(kf_dv d, const double *__restr
     const uint32_t ent = d.sell_ent[idx], e = ent >> 1;
     const d4 qn = ld4(q, d.sell_nbr[idx]);
     const d4 qa = (ent & 1u) ? qn : qv, qb = (ent & 1u) ? qv : qn;   /* (n0, n1) */
-    acc = acc + spectral_term(qa, qb, d.enx[e], d.eny[e], d.enz[e], d.ea[e]);
+    const kf_egeo eg = kf_edge_geo(d.egeo, e);
+    acc = acc + spectral_term(qa, qb, eg.xn, eg.yn, eg.zn, eg.ln);
   }
   cdt[v] = spectral_boundary(d, v, qv, acc) / cfl;
 }

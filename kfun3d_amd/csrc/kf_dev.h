@@ -53,12 +53,21 @@
 #define KF_ERR_SPIN_LIMIT 2u
 
 /* everything a kernel reads, passed by value */
+/* an edge's (unit normal, area) record */
+struct kf_egeo { double xn, yn, zn, ln; };
+__device__ __forceinline__ kf_egeo kf_edge_geo(const double *__restrict__ egeo, uint32_t e)
+{
+  const double2 *p = reinterpret_cast<const double2 *>(egeo + 4 * (size_t)e);
+  const double2 lo = p[0], hi = p[1];
+  return {lo.x, lo.y, hi.x, hi.y};
+}
+
 struct kf_dv {
   uint32_t N, E, S, NS, NF, nslice;
   uint32_t spin_sleep;          /* s_sleep(1) repeats between two polls */
   double U, V;
-  uint32_t *n0, *n1;
-  double *enx, *eny, *enz, *ea;
+  uint32_t *en;                 /* (n0, n1) per edge */
+  double *egeo;                 /* (unit normal x, y, z, dual-face area) per edge: one 32-byte load */
   double *x, *y, *z;
   double *xyzw;                 /* (x, y, z, 0) per vertex: one 32-byte gather for the edge kernel */
   uint32_t *sell_ptr, *deg;

@@ -287,8 +287,9 @@ __global__ __launch_bounds__(MP_RB) void k_mp_edge_vectors(kf_dv d, float4 *__re
   size_t idx = (size_t)d.sell_ptr[v >> 6] * 64u + (v & 63u);
   for (uint32_t k = 0; k < d.deg[v]; k++, idx += 64) {
     const uint32_t ent = d.sell_ent[idx], e = ent >> 1;
-    const double ar = (ent & 1u) ? -d.ea[e] : d.ea[e];
-    ev[idx] = make_float4((float)(d.enx[e] * ar), (float)(d.eny[e] * ar), (float)(d.enz[e] * ar), 0.0f);
+    const kf_egeo eg = kf_edge_geo(d.egeo, e);
+    const double ar = (ent & 1u) ? -eg.ln : eg.ln;
+    ev[idx] = make_float4((float)(eg.xn * ar), (float)(eg.yn * ar), (float)(eg.zn * ar), 0.0f);
   }
 }
 
