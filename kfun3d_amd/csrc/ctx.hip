@@ -161,15 +161,7 @@ extern "C" void kf_ctx_create(GEOMETRY *g, const kf_prep *p, const kf_options *o
     /* per-edge records: (n0, n1) and (unit normal, area) interleaved */
     uint32_t *hn = (uint32_t *)kf_xmalloc((2 * E + 2) * sizeof(uint32_t));
     double *hg = (double *)kf_xmalloc((4 * E + 4) * sizeof(double));
-    const struct xyzn *xn = g->e->xyzn;
-    for (size_t i = 0; i < E; i++) {
-      hn[2 * i] = g->e->eptr->n0[i];
-      hn[2 * i + 1] = g->e->eptr->n1[i];
-      hg[4 * i] = xn->x0[i];
-      hg[4 * i + 1] = xn->x1[i];
-      hg[4 * i + 2] = xn->x2[i];
-      hg[4 * i + 3] = xn->x3[i];
-    }
+    kf_edge_records(g, hn, hg);
     d.en = dupload(hn, 2 * E);
     d.egeo = dupload(hg, 4 * E);
     free(hn);

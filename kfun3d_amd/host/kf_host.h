@@ -86,6 +86,8 @@ typedef struct kf_sell {
   double *wx, *wy, *wz;             /* LSQ term of the edge seen from v */
 } kf_sell;
 void kf_build_sell(const GEOMETRY *g, const kf_prep *p, kf_sell *out);
+/* the device's per-edge records: (n0, n1) pairs and (unit normal, area) */
+void kf_edge_records(const GEOMETRY *g, uint32_t *en, double *egeo);
 
 /* .msh reader (src/mesh.c:22-154 without the device part) */
 void kf_read_msh(const char *fpath, GEOMETRY *g);

@@ -464,6 +464,24 @@ void kf_build_prep(const GEOMETRY *g, kf_prep *p)
   kf_phase("boundary incidences");
 }
 
+/* the device's per-edge records: en[2e..2e+1] = (n0, n1), egeo[4e..4e+3] =
+ * (unit normal, dual-face area) */
+void kf_edge_records(const GEOMETRY *g, uint32_t *en, double *egeo)
+{
+  const size_t E = g->e->sz;
+  const uint32_t *n0 = g->e->eptr->n0, *n1 = g->e->eptr->n1;
+  const struct xyzn *xn = g->e->xyzn;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < E; i++) {
+    en[2 * i] = n0[i];
+    en[2 * i + 1] = n1[i];
+    egeo[4 * i] = xn->x0[i];
+    egeo[4 * i + 1] = xn->x1[i];
+    egeo[4 * i + 2] = xn->x2[i];
+    egeo[4 * i + 3] = xn->x3[i];
+  }
+}
+
 void kf_build_sell(const GEOMETRY *g, const kf_prep *p, kf_sell *out)
 {
   const size_t N = p->N;
