@@ -272,6 +272,24 @@ struct jsl_view {
   const uint32_t *col;    /* L, U */
 };
 
+/* row r of one forward sweep (lane c of its quad) */
+template <int K, typename TM, typename TV, typename TB, typename TX, bool DIAG>
+__device__ __forceinline__ void jac_lower_row(uint32_t r, int c, const uint32_t *__restrict__ rowptr,
+                                              const uint32_t *__restrict__ col, const uint32_t *__restrict__ diag,
+                                              const TM *__restrict__ a, const jsl_view &v, const TB *__restrict__ b,
+                                              const TX *__restrict__ xin, TV *__restrict__ y, TV *__restrict__ x0)
+{
+  const uint32_t sl = r >> 4, q = r & 15u, j0 = rowptr[r], jd = diag[r];
+  const TM *sv = static_cast<const TM *>(v.val);
+  const TV s = jsum_sl<K, TM, TV, TX>(sv, v.col, sl * K, q, c, jd - j0, col, a, j0, xin);
+  const TV u = (-s) + (TV)b[4 * (size_t)r + c];
+  y[4 * (size_t)r + c] = u;
+  if (DIAG) {
+    const TM *dv = sv + (size_t)2 * v.nslice * K * 256;
+    x0[4 * (size_t)r + c] = jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
+  }
+}
+
 template <int K, typename TM, typename TV, typename TB, typename TX, bool DIAG>
 __global__ __launch_bounds__(JAC_WG) void k_jac_lower_sl(uint32_t N, const uint32_t *__restrict__ rowptr,
                                                          const uint32_t *__restrict__ col,
@@ -283,15 +301,29 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_lower_sl(uint32_t N, const uint3
   const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
   const int c = threadIdx.x & 3;
   if (r >= N) return;
-  const uint32_t sl = r >> 4, q = r & 15u, j0 = rowptr[r], jd = diag[r];
+  jac_lower_row<K, TM, TV, TB, TX, DIAG>(r, c, rowptr, col, diag, a, v, b, xin, y, x0);
+}
+
+/* row r of one backward sweep (lane c of its quad) */
+template <int K, typename TM, typename TV, typename TO>
+__device__ __forceinline__ void jac_upper_row(uint32_t r, int c, const uint32_t *__restrict__ rowptr,
+                                              const uint32_t *__restrict__ col, const uint32_t *__restrict__ diag,
+                                              const TM *__restrict__ a, const jsl_view &v, const TV *__restrict__ y,
+                                              const TV *__restrict__ xin, TO *__restrict__ x,
+                                              const uint32_t *__restrict__ pagg, const double *__restrict__ pzc)
+{
+  const uint32_t sl = r >> 4, q = r & 15u, jd = diag[r];
   const TM *sv = static_cast<const TM *>(v.val);
-  const TV s = jsum_sl<K, TM, TV, TX>(sv, v.col, sl * K, q, c, jd - j0, col, a, j0, xin);
-  const TV u = (-s) + (TV)b[4 * (size_t)r + c];
-  y[4 * (size_t)r + c] = u;
-  if (DIAG) {
-    const TM *dv = sv + (size_t)2 * v.nslice * K * 256;
-    x0[4 * (size_t)r + c] = jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
-  }
+  const size_t nk = (size_t)v.nslice * K;
+  const TV s = jsum_sl<K, TM, TV, TV>(sv + nk * 256, v.col + nk * 16, sl * K, q, c, rowptr[r + 1] - jd - 1, col, a,
+                                      jd + 1, xin);
+  const TV u = (-s) + y[4 * (size_t)r + c];
+  const TM *dv = sv + 2 * nk * 256;
+  TO xo = (TO)jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
+  /* the multilevel preconditioner's prolongation x += P z_c, fused into the
+   * last sweep (TO = double; the same add as k_prolong_add) */
+  if (pagg) xo = (TO)((double)xo + pzc[4 * (size_t)pagg[r] + c]);
+  x[4 * (size_t)r + c] = xo;
 }
 
 template <int K, typename TM, typename TV, typename TO>
@@ -306,18 +338,57 @@ __global__ __launch_bounds__(JAC_WG) void k_jac_upper_sl(uint32_t N, const uint3
   const uint32_t r = kf_xcd_block() * (JAC_WG / 4) + (threadIdx.x >> 2);
   const int c = threadIdx.x & 3;
   if (r >= N) return;
-  const uint32_t sl = r >> 4, q = r & 15u, jd = diag[r];
-  const TM *sv = static_cast<const TM *>(v.val);
-  const size_t nk = (size_t)v.nslice * K;
-  const TV s = jsum_sl<K, TM, TV, TV>(sv + nk * 256, v.col + nk * 16, sl * K, q, c, rowptr[r + 1] - jd - 1, col, a,
-                                      jd + 1, xin);
-  const TV u = (-s) + y[4 * (size_t)r + c];
-  const TM *dv = sv + 2 * nk * 256;
-  TO xo = (TO)jdinv_sl<TM, TV>(dv, ((size_t)sl * 16 + q) * 16 + 4 * c, u);
-  /* the multilevel preconditioner's prolongation x += P z_c, fused into the
-   * last sweep (TO = double; the same add as k_prolong_add) */
-  if (pagg) xo = (TO)((double)xo + pzc[4 * (size_t)pagg[r] + c]);
-  x[4 * (size_t)r + c] = xo;
+  jac_upper_row<K, TM, TV, TO>(r, c, rowptr, col, diag, a, v, y, xin, x, pagg, pzc);
+}
+
+/* A whole solve of jacobi_solve_sl in one workgroup, for systems of at most
+ * JAC_SMALL_ROWS rows (the small levels of the multilevel preconditioner,
+ * where each sweep's launch costs more than its work): the same sweeps,
+ * rows, buffers and operations, a barrier between two sweeps instead of a
+ * launch. */
+#define JAC_SMALL_WG 512   /* 2 waves per SIMD: registers for a whole row, no spills */
+#define JAC_SMALL_ROWS (JAC_SMALL_WG / 4)
+template <int K, typename TM, typename TV, typename TB>
+__global__ __launch_bounds__(JAC_SMALL_WG) void k_jac_small(uint32_t N, const uint32_t *__restrict__ rowptr,
+                                                            const uint32_t *__restrict__ col,
+                                                            const uint32_t *__restrict__ diag,
+                                                            const TM *__restrict__ a, jsl_view v,
+                                                            const TB *__restrict__ b, double *__restrict__ x,
+                                                            TV *__restrict__ y, TV *__restrict__ p0,
+                                                            TV *__restrict__ p1, int k,
+                                                            const uint32_t *__restrict__ pagg,
+                                                            const double *__restrict__ pzc)
+{
+  const int c = threadIdx.x & 3;
+  const uint32_t r = threadIdx.x >> 2;   /* one row per quad: N <= JAC_SMALL_ROWS */
+  const bool on = r < N;
+  TV *const xread = k >= 2 ? (((k - 1) & 1) ? p0 : p1) : NULL;
+  TV *cur = xread == p0 ? p1 : p0;
+  const TV *prev = NULL;
+  for (int sw = 1; sw <= k; sw++) {
+    TV *out = sw == k ? y : ((sw & 1) ? p0 : p1);
+    if (on) {
+      if (sw == 1 && sw == k) jac_lower_row<K, TM, TV, TB, TB, true>(r, c, rowptr, col, diag, a, v, b, b, out, cur);
+      else if (sw == 1) jac_lower_row<K, TM, TV, TB, TB, false>(r, c, rowptr, col, diag, a, v, b, b, out, (TV *)NULL);
+      else if (sw == k) jac_lower_row<K, TM, TV, TB, TV, true>(r, c, rowptr, col, diag, a, v, b, prev, out, cur);
+      else jac_lower_row<K, TM, TV, TB, TV, false>(r, c, rowptr, col, diag, a, v, b, prev, out, (TV *)NULL);
+    }
+    prev = out;
+    __syncthreads();
+  }
+  TV *other = cur == p0 ? p1 : p0;
+  for (int sw = 1; sw <= k; sw++) {
+    if (sw == k) {
+      if (on) jac_upper_row<K, TM, TV, double>(r, c, rowptr, col, diag, a, v, y, cur, x, pagg, pzc);
+    } else {
+      if (on) jac_upper_row<K, TM, TV, TV>(r, c, rowptr, col, diag, a, v, y, cur, other, (const uint32_t *)NULL,
+                                           (const double *)NULL);
+      TV *t = cur;
+      cur = other;
+      other = t;
+      __syncthreads();
+    }
+  }
 }
 
 /* the sliced copy from the CSR factors (a quad per row, lane c copying
@@ -461,6 +532,16 @@ template <int K, typename TM, typename TV, typename TB>
 static void jacobi_solve_sl(kf_ctx *c, const TM *a, const TB *b, double *x, TV *y, TV *p0, TV *p1, int k,
                             const jsl_view &v, const uint32_t *pagg, const double *pzc)
 {
+  static const bool small_on = !(getenv("KF_JAC_SMALL") && atoi(getenv("KF_JAC_SMALL")) == 0);
+  if (small_on && c->d.N <= JAC_SMALL_ROWS) {   /* the whole solve in one workgroup (KF_JAC_SMALL=0: never) */
+    const kf_dv &d = c->d;
+    const int s = kf_prof_begin(c, KP_JAC_L);
+    hipLaunchKernelGGL((k_jac_small<K, TM, TV, TB>), dim3(1), dim3(JAC_SMALL_WG), 0, c->stream, d.N, d.rowptr, d.col,
+                       d.diag, a, v, b, x, y, p0, p1, k, pagg, pzc);
+    KF_LAUNCH_CHECK();
+    kf_prof_end(c, s);
+    return;
+  }
   TV *const xread = k >= 2 ? (((k - 1) & 1) ? p0 : p1) : NULL;
   TV *cur = xread == p0 ? p1 : p0;
   int s = kf_prof_begin(c, KP_JAC_L);
