@@ -182,11 +182,6 @@ __device__ __forceinline__ void cg_ld4(const double *p, double a[4])
   const double2 lo = reinterpret_cast<const double2 *>(p)[0], hi = reinterpret_cast<const double2 *>(p)[1];
   a[0] = lo.x; a[1] = lo.y; a[2] = hi.x; a[3] = hi.y;
 }
-__device__ __forceinline__ void cg_ld4(const float *p, double a[4])
-{
-  const float4 v = *reinterpret_cast<const float4 *>(p);
-  a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w;
-}
 __device__ __forceinline__ void cg_ld4(const uint16_t *p, double a[4])
 {
   const uint2 v = *reinterpret_cast<const uint2 *>(p);
