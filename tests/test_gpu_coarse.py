@@ -18,6 +18,10 @@ coarse level adds:
   * on the 144x48x48 lattice, exact FP64 levels and the BF16 path with
     Jacobi sweeps (9 levels) reach the plain FP64 path's forces within 1e-8
     in under half its Krylov iterations;
+  * the small levels' Jacobi solves in one workgroup (KF_JAC_SMALL, at
+    most 128 block rows) give the same solve, bit for bit, as one launch
+    per sweep: the whole history of a 5-level BF16 solve with sweeps, in
+    two child processes (the switch is read once per process);
   * switching the number of levels on one context rebuilds or drops the
     deeper levels: a three-level, then a two-level, then a three-level
     solve on one mesh each repeat their own fresh-context history (with the
@@ -187,3 +191,35 @@ def test_multilevel_at_144x48x48(mesh_dir):
             assert 2 * k2 < k1
     finally:
         m.close()
+
+
+_SMALL_CHILD = """
+import sys
+sys.path.insert(0, sys.argv[2])
+import kfun3d_amd as K
+m = K.Mesh(sys.argv[1], K.config_options(prec="bf16", coarse=4, jacobi_sweeps=2, coarse_sweeps=3))
+h = m.kernel()
+assert m.last_error() == 0
+m.close()
+for s in h:
+    print(s["krylov"], s["fnorm"].hex(), " ".join(f.hex() for f in s["forces"]))
+"""
+
+
+@pytest.mark.parametrize("which", ["mesh_a", "mesh_c1"])
+def test_small_levels_in_one_workgroup(which, mesh_a, mesh_c1):
+    import os
+    import subprocess
+    import sys
+    path = {"mesh_a": mesh_a, "mesh_c1": mesh_c1}[which]
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = {}
+    for flag in ("1", "0"):
+        env = dict(os.environ, KF_JAC_SMALL=flag)
+        r = subprocess.run([sys.executable, "-c", _SMALL_CHILD, path, root], env=env, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        out[flag] = r.stdout.split("\n")
+    assert len(out["1"]) > 3
+    assert out["1"] == out["0"]
+
