@@ -344,9 +344,13 @@ int  kf_last_error(const GEOMETRY *g);
 /* Per-kernel timing with HIP events on the context stream.  kf_profile(g, 1)
  * clears and starts it; kf_profile_get returns the total milliseconds and the
  * launch count of one kernel class ("grad", "edge_flux", "residual",
- * "timestep", "fill", "ilu", "trsv_lower", "trsv_upper", "spmv"); -1 if
- * unknown. */
+ * "timestep", "fill", "ilu", "trsv_lower", "trsv_upper", "spmv",
+ * "jacobi_lower", "jacobi_upper", "coarse_factor", "coarse"); -1 if
+ * unknown.  kf_profile_only(g, name) clears and starts it for that one class
+ * (fewer events in the timed work); -1 if unknown.  No counterpart in the
+ * reference. */
 void kf_profile(GEOMETRY *g, int on);
+int  kf_profile_only(GEOMETRY *g, const char *name);
 int  kf_profile_get(GEOMETRY *g, const char *name, double *total_ms, long *count);
 
 /* ---- learned preconditioner correction (no counterpart in the reference) ----

@@ -145,6 +145,7 @@ def _load():
         "kf_last_error": (C.c_int, [GP]),
         "kf_bind": (None, [GP]),
         "kf_profile": (None, [GP, C.c_int]),
+        "kf_profile_only": (C.c_int, [GP, C.c_char_p]),
         "kf_profile_get": (C.c_int, [GP, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
         "kf_trsv_plan_host": (C.c_size_t, [GP, C.c_int, C.c_uint, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -214,7 +215,7 @@ ABI_SYMBOLS = ["ComputeA", "ComputeTimeStep", "ComputeFlux", "ComputeForces", "C
                "kf_kernel", "kf_set_state", "kf_get_state", "kf_memcpy_d2h", "kf_memcpy_h2d", "kf_device_alloc",
                "kf_device_free", "kf_synchronize", "kf_stream", "kf_nnzb", "kf_trsv_fold", "kf_grad_moments", "kf_bsr_spmv", "kf_pattern", "kf_get_blocks",
                "kf_write_lattice_msh", "kf_sptrsv_jacobi_noalloc", "kf_geometry_free", "kf_last_error", "kf_setup_ex", "kf_setup_host",
-               "kf_bind", "kf_profile", "kf_profile_get", "kf_trsv_plan_host",
+               "kf_bind", "kf_profile", "kf_profile_only", "kf_profile_get", "kf_trsv_plan_host",
                "kf_ilu_plan_host", "kf_coarse_plan_host", "kf_coarse_matrix", "kf_mpnn_nparam", "kf_mpnn_nhead", "kf_mpnn_init_params", "kf_mpnn_prepare", "kf_mpnn_create", "kf_mpnn_destroy", "kf_mpnn_set_params",
                "kf_mpnn_get_params", "kf_mpnn_take_diag", "kf_mpnn_set_diag", "kf_mpnn_forward",
                "kf_mpnn_loss_grad", "kf_mpnn_adam", "kf_mpnn_train_step", "kf_mpnn_losses", "kf_mpnn_reset_losses",
@@ -506,6 +507,11 @@ class Mesh:
 
     def profile(self, on):
         lib().kf_profile(self.ref, 1 if on else 0)
+
+    def profile_only(self, name):
+        """profile one kernel class only (the others run without events)"""
+        if lib().kf_profile_only(self.ref, name.encode()) != 0:
+            raise KeyError(name)
 
     def profile_get(self, name):
         """(total milliseconds, launches) of one kernel class since profile(True)"""

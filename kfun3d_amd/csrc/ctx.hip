@@ -760,7 +760,7 @@ static void prof_drain(kf_ctx *c)
 int kf_prof_begin(kf_ctx *c, int id)
 {
   kf_prof &p = c->prof;
-  if (!p.on) return -1;
+  if (!p.on || (p.on > 1 && id != p.on - 2)) return -1;   /* on = 2 + id: that class only */
   if (p.used == KF_PROF_POOL) prof_drain(c);
   const int s = p.used++;
   p.id[s] = id;
@@ -785,6 +785,19 @@ extern "C" void kf_profile(GEOMETRY *g, int on)
   if (on)
     for (int k = 0; k < KP_N; k++) { p.total_ms[k] = 0.0; p.count[k] = 0; }
   p.on = on;
+}
+
+/* events for one kernel class only (the others run unrecorded): -1 for an
+ * unknown name */
+extern "C" int kf_profile_only(GEOMETRY *g, const char *name)
+{
+  for (int k = 0; k < KP_N; k++)
+    if (!strcmp(name, kf_prof_names[k])) {
+      kf_profile(g, 1);
+      kf_ctx_of(g)->prof.on = 2 + k;
+      return 0;
+    }
+  return -1;
 }
 
 /* total milliseconds and launch count of one kernel class since kf_profile(g, 1) */
